@@ -1,0 +1,126 @@
+/*
+ * pelagos_relaybp.h -- C ABI of the MI355X Relay-BP-S decoder library
+ * (libpelagos_relaybp.so, built from pelagos_amd/csrc/relaybp.hip).
+ *
+ * Plain C: raw pointers, sizes and an opaque graph handle; no torch types.
+ * Device pointers are HIP device addresses on the graph's device; `stream`
+ * is a hipStream_t passed as void* (NULL = default stream).
+ *
+ * Ownership: the caller owns every per-decode buffer (syndromes, priors,
+ * gamma tables, outputs, workspace); the library owns only the graph layout
+ * it builds in pel_graph_create.  All entry points return 0 on success and a
+ * negative code on failure, with a message from pel_last_error() (per thread).
+ *
+ * The reference (quantum-supercomputing/pelagos) has no C ABI: its boundary is
+ * the Python RelayBPDecoder launching Triton kernels.  Each entry point below
+ * names the reference code it replaces.
+ */
+#ifndef PELAGOS_RELAYBP_H
+#define PELAGOS_RELAYBP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PEL_ABI_VERSION 1
+
+/* status codes */
+#define PEL_OK 0
+#define PEL_EINVAL (-1)    /* bad argument: maps to the reference's ValueError */
+#define PEL_EDEVICE (-2)   /* HIP runtime error: maps to RuntimeError */
+#define PEL_ENOMEM (-3)
+
+typedef struct pel_graph pel_graph;
+
+/* Relay-BP-S schedule parameters (RelayBPDecoder.__init__, decoder.py:271-307;
+ * per-call overrides of decode(), decoder.py:2317-2361). */
+typedef struct {
+    int32_t pre_iter;        /* T0 (or decode(pre_iter_override=...)) */
+    int32_t num_sets;        /* R: relay legs */
+    int32_t set_max_iter;    /* Tr: iterations per relay leg */
+    int32_t stop_nconv;      /* S: solutions to collect */
+    int32_t plain;           /* algo == "plain": gamma 0 in the pre-iter leg */
+    float gamma0;            /* pre-iter gamma for algo == "relay" */
+    int32_t use_alpha;       /* normalized_min_sum_alpha is not None */
+    int32_t use_beta;        /* offset_min_sum_beta is not None */
+    float beta;              /* offset */
+    const float *alpha_t;    /* device [max(T0,Tr)] alpha per in-leg iteration, or NULL */
+    int32_t last_leg;        /* unconverged_correction == "last_leg" */
+    int32_t skip_relay_legs; /* decode(skip_relay_legs=True) */
+} pel_params;
+
+/* One batch of shots.  Shapes: B shots, C checks, V variables, R = num_sets. */
+typedef struct {
+    int32_t B;
+    const uint8_t *syndromes;    /* [B,C] 0/1 */
+    const float *prior_llr;      /* [V], or [B,V] when prior_batched (decode(prior_llr=...)) */
+    const float *wj;             /* selection weights, same shape as prior_llr */
+    int32_t prior_batched;
+    const float *gamma_legs;     /* [R,V], or [R,B,V] when gamma_per_row; row l = relay leg l */
+    int32_t gamma_per_row;
+    const float *init_gamma;     /* [B,V] pre-iter gamma (decode(init_gamma=...)) or NULL */
+    const float *init_posterior; /* [B,V] (decode(init_posterior=...)) or NULL */
+    uint8_t *errors;             /* out [B,V] */
+    float *weights;              /* out [B]: best solution weight, +inf if none */
+    int32_t *iterations;         /* out [B]: BP iterations run */
+    int32_t *found_count;        /* out [B]: valid solutions found */
+    float *last_posterior;       /* out [B,V] or NULL (decode(return_last_posterior=True)) */
+    void *workspace;             /* device scratch of pel_plan.workspace_bytes */
+} pel_batch;
+
+/* Launch plan for one (graph, B) pair: chosen by pel_plan_decode. */
+typedef struct {
+    int32_t block;           /* threads per workgroup (one shot per workgroup at a time) */
+    int32_t grid;            /* persistent workgroups */
+    int32_t placement;       /* bit0: variable state in LDS, bit1: check state in LDS,
+                                bit2: sign words in LDS; a clear bit places it in workspace */
+    int32_t lds_bytes;       /* dynamic LDS per workgroup */
+    int64_t spill_stride;    /* workspace bytes per workgroup for state not in LDS */
+    int64_t workspace_bytes; /* total workspace the caller must provide */
+} pel_plan;
+
+/* ABI version (PEL_ABI_VERSION). */
+int pel_abi_version(void);
+
+/* Last error message of the calling thread ("" if none). */
+const char *pel_last_error(void);
+
+/* Build the device layout of a check matrix on the current HIP device.
+ * chk_ptr [C+1], chk_var [E] are host arrays of H in CSR form; edge ids are
+ * CSR positions.  Replaces CSRGraph._build_edge_representation
+ * (src/pelagos/graph.py:101-211). */
+int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, int32_t V,
+                     int32_t E, pel_graph **out);
+
+/* Free a graph (device layout included). */
+int pel_graph_destroy(pel_graph *g);
+
+/* Sizes of a graph: C, V, E and the per-shot state bytes (variables, checks,
+ * sign words).  No counterpart in the reference beyond CSRGraph.get_stats
+ * (graph.py:227-245). */
+int pel_graph_info(const pel_graph *g, int32_t *C, int32_t *V, int32_t *E,
+                   int64_t *var_state_bytes, int64_t *chk_state_bytes, int64_t *sign_bytes);
+
+/* Choose the launch plan for B shots.  block_override > 0 forces the block
+ * size; placement_override >= 0 forces the placement bits (a state array that
+ * does not fit the LDS budget is never placed there).  Replaces the
+ * launch-configuration resolution of RelayBPDecoder._resolve_configs /
+ * _get_tuned_meta (decoder.py:905-1222). */
+int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override,
+                    int32_t placement_override, pel_plan *out);
+
+/* Decode a batch with one persistent launch: every shot runs the full
+ * Relay-BP-S schedule (pre-iter leg, then relay legs until stop_nconv
+ * solutions) inside one workgroup, and idle workgroups take the next shot.
+ * Replaces RelayBPDecoder._decode_loop_single_gpu (decoder.py:3394-3873) and
+ * the kernels it launches (kernels.py:446-1350). */
+int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, const pel_plan *plan,
+               void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PELAGOS_RELAYBP_H */

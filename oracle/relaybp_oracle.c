@@ -1,0 +1,349 @@
+/*
+ * relaybp_oracle.c -- CPU restatement of Relay-BP-S min-sum decoding.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP decoder in
+ * pelagos_amd/csrc: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product path never calls it.
+ *
+ * It restates, per shot and in plain fp32 C, the algorithm of the reference
+ * (quantum-supercomputing/pelagos) throughput/flood-fill path:
+ *
+ *   schedule       src/pelagos/decoder.py:3394-3873  (_decode_loop_single_gpu)
+ *   one iteration  src/pelagos/decoder.py:2610-2875  (_run_bp_iteration)
+ *   C2V min-sum    src/pelagos/kernels.py:1051-1215  (c2v_min_sum_btile_kernel)
+ *   V2C + gamma    src/pelagos/kernels.py:1219-1350  (v2c_and_gamma_btile_kernel)
+ *   parity         src/pelagos/kernels.py:509-569    (parity_from_hard_compact_with_zero_kernel)
+ *   select         src/pelagos/kernels.py:608-741    (check_and_select_kernel)
+ *   freeze         src/pelagos/kernels.py:745-847    (freeze_finished_lanes_kernel)
+ *   message init   src/pelagos/kernels.py:903-1000   (init_messages_with_priors_EB_kernel)
+ *
+ * It deliberately keeps the reference's explicit per-edge message arrays
+ * (mu, nu) and its min1/min2/tie-count formulation of the check update, so it
+ * is an independent formulation from the GPU kernel (which keeps a compressed
+ * per-check state).  Each shot is decoded independently: the reference's lane
+ * freezing makes every lane's result independent of the batch
+ * (tests/test_decoder_fixes.py:184-268), so a per-shot loop is the same
+ * algorithm.
+ *
+ * Arithmetic conventions shared with the HIP kernel (both built with
+ * -ffp-contract=off):
+ *   - lam_mixed = (1-g)*lam0 + g*post_prev, two roundings, no FMA
+ *     (kernels.py:1309; the relay-bp Rust crate does not contract either);
+ *   - the variable sum runs over a variable's edges in ascending edge id
+ *     (= ascending check index), starting from 0.0f (kernels.py:1317-1325
+ *     with BLOCK_SIZE fixed at 1, autotune.py:635);
+ *   - the solution weight sum_j wj*[post_j <= 0] is accumulated in double and
+ *     rounded once to float (exact, hence order-free, for these inputs).
+ *
+ * Parity status: pinned by tests/test_oracle_pins.py (closed forms, tree-code
+ * exactness, brute-force ML, schedule invariants, planted-bug mutants).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Compile-time planted mistakes, used only by the mutation test that shows the
+ * pinning checks reject a wrong oracle.  0 = the real oracle. */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+
+typedef struct {
+    int32_t pre_iter;        /* T0 */
+    int32_t num_sets;        /* R  */
+    int32_t set_max_iter;    /* Tr */
+    int32_t stop_nconv;      /* S  */
+    int32_t plain;           /* algo == "plain": gamma = 0 in the pre-iter leg (decoder.py:3489-3491) */
+    float gamma0;            /* pre-iter gamma for algo == "relay" (decoder.py:3496-3500) */
+    int32_t use_alpha;       /* normalized_min_sum_alpha is not None (decoder.py:1712) */
+    int32_t use_beta;        /* offset_min_sum_beta is not None (decoder.py:1713) */
+    float beta;
+    const float *alpha_t;    /* [max(T0,Tr)] alpha for in-leg iteration t (decoder.py:2649-2660) */
+    int32_t last_leg;        /* unconverged_correction == "last_leg" (decoder.py:2877-2888) */
+    int32_t skip_relay_legs; /* decode(skip_relay_legs=True) (decoder.py:3625) */
+} orc_params;
+
+typedef struct {
+    int32_t B;
+    const uint8_t *syndromes;     /* [B,C] */
+    const float *prior_llr;       /* [V] or [B,V] when prior_batched */
+    int32_t prior_batched;
+    const float *gamma_legs;      /* [R,V] or [R,B,V] when gamma_per_row */
+    int32_t gamma_per_row;
+    const float *init_gamma;      /* [B,V] or NULL */
+    const float *init_posterior;  /* [B,V] or NULL */
+    uint8_t *errors;              /* [B,V] out */
+    float *weights;               /* [B]   out */
+    int32_t *iterations;          /* [B]   out */
+    int32_t *found_count;         /* [B]   out */
+    float *last_posterior;        /* [B,V] out or NULL */
+} orc_batch;
+
+typedef struct {
+    int32_t C, V, E;
+    const int32_t *row_ptr;   /* [C+1] CSR over checks; edge id = CSR position (graph.py:112-165) */
+    const int32_t *row_var;   /* [E]   variable of each edge */
+    int32_t *var_ptr;         /* [V+1] */
+    int32_t *var_edges;       /* [E]   edge ids grouped by variable, ascending */
+} orc_graph;
+
+/* Variable-major edge lists in ascending edge id (a stable version of the
+ * argsort at graph.py:144-154). */
+static int build_var_lists(orc_graph *g)
+{
+    g->var_ptr = (int32_t *)calloc((size_t)g->V + 1, sizeof(int32_t));
+    g->var_edges = (int32_t *)malloc(sizeof(int32_t) * (size_t)(g->E > 0 ? g->E : 1));
+    int32_t *fill = (int32_t *)calloc((size_t)g->V + 1, sizeof(int32_t));
+    if (!g->var_ptr || !g->var_edges || !fill) { free(fill); return -1; }
+    for (int32_t e = 0; e < g->E; ++e) g->var_ptr[g->row_var[e] + 1]++;
+    for (int32_t j = 0; j < g->V; ++j) g->var_ptr[j + 1] += g->var_ptr[j];
+    for (int32_t e = 0; e < g->E; ++e) {
+        int32_t j = g->row_var[e];
+        g->var_edges[g->var_ptr[j] + fill[j]++] = e;
+    }
+    free(fill);
+    return 0;
+}
+
+typedef struct {
+    float *mu, *nu;          /* [E] variable->check and check->variable messages */
+    float *post;             /* [V] posterior_ratios */
+    uint8_t *hard;           /* [V] hard_dec */
+    uint8_t *best;           /* [V] best_errors */
+    uint8_t *first_hd;       /* [V] first_leg_hard_dec */
+    const float *gamma;      /* [V] current leg's memory strengths (or NULL with gamma_const) */
+    float gamma_const;
+} orc_lane;
+
+/* C2V, kernels.py:1128-1215: per check, the parity of negative incoming
+ * messages XOR the syndrome bit, and the minimum |mu| over the other edges. */
+static void check_update(const orc_graph *g, const orc_params *p, const uint8_t *syn,
+                         float alpha, orc_lane *L)
+{
+    for (int32_t i = 0; i < g->C; ++i) {
+        int32_t r0 = g->row_ptr[i], r1 = g->row_ptr[i + 1];
+        if (r1 <= r0) continue;
+        int deg_is_one = (r1 - r0) == 1;
+        int neg = 0, cnt1 = 0;
+        float min1 = 1e30f, min2 = 1e30f;
+        for (int32_t e = r0; e < r1; ++e) {
+            float m = L->mu[e];
+            float a = fabsf(m);
+            neg ^= (m < 0.0f);
+            if (a < min1) { min2 = min1; min1 = a; cnt1 = 1; }
+            else if (a == min1) { cnt1++; min2 = a; }
+            else if (a < min2) { min2 = a; }
+        }
+        int par = (neg & 1) ^ (syn[i] & 1);
+#if ORC_PLANT == 1
+        par = neg & 1;                            /* planted: syndrome sign dropped */
+#endif
+        float sign_prod = par == 0 ? 1.0f : -1.0f;
+        for (int32_t e = r0; e < r1; ++e) {
+            float m = L->mu[e];
+            float a = fabsf(m);
+            float out = (a == min1 && cnt1 == 1) ? min2 : min1;
+#if ORC_PLANT == 2
+            out = min1;                           /* planted: own edge not excluded */
+#endif
+            if (deg_is_one) out = 0.0f;
+            if (p->use_alpha) out = alpha * out;
+            if (p->use_beta) out = fmaxf(out - p->beta, 0.0f);
+            float sgn = m >= 0.0f ? 1.0f : -1.0f;
+            L->nu[e] = (sign_prod * sgn) * out;
+        }
+    }
+}
+
+/* V2C with memory mixing, kernels.py:1299-1350. */
+static void variable_update(const orc_graph *g, const float *lam0, orc_lane *L)
+{
+    for (int32_t j = 0; j < g->V; ++j) {
+        float gj = L->gamma ? L->gamma[j] : L->gamma_const;
+        float lm = (1.0f - gj) * lam0[j] + gj * L->post[j];
+#if ORC_PLANT == 3
+        lm = gj * lam0[j] + (1.0f - gj) * L->post[j];   /* planted: memory weights swapped */
+#endif
+        float s = 0.0f;
+        for (int32_t q = g->var_ptr[j]; q < g->var_ptr[j + 1]; ++q) s += L->nu[g->var_edges[q]];
+        float post = lm + s;
+        L->post[j] = post;
+        L->hard[j] = post <= 0.0f;
+        for (int32_t q = g->var_ptr[j]; q < g->var_ptr[j + 1]; ++q) {
+            int32_t e = g->var_edges[q];
+#if ORC_PLANT == 4
+            L->mu[e] = lm + s;                    /* planted: extrinsic subtraction dropped */
+#else
+            L->mu[e] = lm + (s - L->nu[e]);
+#endif
+        }
+    }
+}
+
+/* parity_from_hard_*: every check XOR of hard decisions equals its syndrome bit. */
+static int all_checks_ok(const orc_graph *g, const uint8_t *syn, const uint8_t *hard)
+{
+    for (int32_t i = 0; i < g->C; ++i) {
+        int par = 0;
+        for (int32_t e = g->row_ptr[i]; e < g->row_ptr[i + 1]; ++e) par ^= hard[g->row_var[e]];
+        if (par != (syn[i] & 1)) return 0;
+    }
+    return 1;
+}
+
+/* init_messages_with_priors_EB_kernel: nu = 0, mu[e] = lam0[var(e)]. */
+static void init_messages(const orc_graph *g, const float *lam0, orc_lane *L)
+{
+    for (int32_t e = 0; e < g->E; ++e) { L->mu[e] = lam0[g->row_var[e]]; L->nu[e] = 0.0f; }
+}
+
+typedef struct {
+    float best_w;
+    int32_t found;
+    int32_t iters;
+} orc_shot;
+
+/* One leg: up to T flood iterations; the lane freezes at its first valid
+ * solution of the leg (freeze_finished_lanes_kernel with in_leg=True and
+ * sought = found + 1).  Returns 1 if the lane converged in this leg. */
+static int run_leg(const orc_graph *g, const orc_params *p, const uint8_t *syn,
+                   const float *lam0, const float *wj, int32_t T, orc_lane *L, orc_shot *st)
+{
+    for (int32_t t = 0; t < T; ++t) {
+        float alpha = p->use_alpha ? p->alpha_t[t] : 1.0f;
+        check_update(g, p, syn, alpha, L);
+        variable_update(g, lam0, L);
+        st->iters++;
+        if (all_checks_ok(g, syn, L->hard)) {
+            /* check_and_select_kernel: weight over bits (post <= 0) with wj. */
+            double acc = 0.0;
+            for (int32_t j = 0; j < g->V; ++j)
+                if (L->post[j] <= 0.0f) acc += (double)wj[j];
+            float w = (float)acc;
+            if (st->best_w >= 1e20f || w < st->best_w) {
+                st->best_w = w;
+                for (int32_t j = 0; j < g->V; ++j) L->best[j] = L->post[j] <= 0.0f;
+            }
+            st->found++;
+            /* freeze: hard_dec <- best_errors (kernels.py:806-817) */
+            memcpy(L->hard, L->best, (size_t)g->V);
+            return 1;
+        }
+    }
+    return 0;
+}
+
+static void decode_shot(const orc_graph *g, const orc_params *p, const orc_batch *bt,
+                        int32_t b, orc_lane *L, float *wj_buf)
+{
+    const int32_t V = g->V;
+    const uint8_t *syn = bt->syndromes + (size_t)b * g->C;
+    const float *lam0 = bt->prior_llr + (bt->prior_batched ? (size_t)b * V : 0);
+    /* wj = lam0 with non-finite entries zeroed (decoder.py:736-753) */
+    for (int32_t j = 0; j < V; ++j) wj_buf[j] = isfinite(lam0[j]) ? lam0[j] : 0.0f;
+
+    /* state reset (decoder.py:3430-3464; _reset_reused_tensors 1557-1592) */
+    for (int32_t j = 0; j < V; ++j)
+        L->post[j] = bt->init_posterior ? bt->init_posterior[(size_t)b * V + j] : lam0[j];
+    memset(L->hard, 0, (size_t)V);
+    memset(L->best, 0, (size_t)V);
+    memset(L->first_hd, 0, (size_t)V);
+    orc_shot st = {INFINITY, 0, 0};
+    const int32_t S = p->stop_nconv;
+
+    /* pre-iter leg (decoder.py:3470-3593) */
+    init_messages(g, lam0, L);
+    if (p->plain) { L->gamma = NULL; L->gamma_const = 0.0f; }
+    else if (bt->init_gamma) { L->gamma = bt->init_gamma + (size_t)b * V; }
+    else { L->gamma = NULL; L->gamma_const = p->gamma0; }
+    if (st.found < S) run_leg(g, p, syn, lam0, wj_buf, p->pre_iter, L, &st);
+    memcpy(L->first_hd, L->hard, (size_t)V);            /* decoder.py:3597-3598 */
+
+    if (!p->skip_relay_legs) {
+        for (int32_t leg = 0; leg < p->num_sets; ++leg) { /* decoder.py:3672 */
+            if (st.found >= S) break;                     /* lane stays frozen */
+            L->gamma = bt->gamma_legs +
+                       (bt->gamma_per_row ? ((size_t)leg * bt->B + b) * V : (size_t)leg * V);
+            init_messages(g, lam0, L);                    /* decoder.py:3729-3739 */
+            run_leg(g, p, syn, lam0, wj_buf, p->set_max_iter, L, &st);
+            if (leg == 0 && p->pre_iter == 0)             /* decoder.py:3809-3810 */
+                memcpy(L->first_hd, L->hard, (size_t)V);
+        }
+    }
+
+    /* output assembly (decoder.py:3841-3873) */
+    uint8_t *out = bt->errors + (size_t)b * V;
+    const uint8_t *src = isfinite(st.best_w) ? L->best : (p->last_leg ? L->hard : L->first_hd);
+    memcpy(out, src, (size_t)V);
+    bt->weights[b] = st.best_w;
+    bt->iterations[b] = st.iters;
+    bt->found_count[b] = st.found;
+    if (bt->last_posterior) memcpy(bt->last_posterior + (size_t)b * V, L->post, sizeof(float) * V);
+}
+
+/* Decode a batch.  Returns 0 on success, -1 on allocation failure.
+ * nthreads <= 0 uses the OpenMP default. */
+int orc_decode(int32_t C, int32_t V, int32_t E, const int32_t *row_ptr, const int32_t *row_var,
+               const orc_params *p, const orc_batch *bt, int32_t nthreads)
+{
+    orc_graph g = {C, V, E, row_ptr, row_var, NULL, NULL};
+    if (build_var_lists(&g)) return -1;
+    int rc = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel reduction(| : rc)
+    {
+        orc_lane L;
+        size_t e1 = (size_t)(E > 0 ? E : 1);
+        L.mu = (float *)malloc(sizeof(float) * e1);
+        L.nu = (float *)malloc(sizeof(float) * e1);
+        L.post = (float *)malloc(sizeof(float) * (size_t)V);
+        L.hard = (uint8_t *)malloc((size_t)V);
+        L.best = (uint8_t *)malloc((size_t)V);
+        L.first_hd = (uint8_t *)malloc((size_t)V);
+        float *wj = (float *)malloc(sizeof(float) * (size_t)V);
+        L.gamma = NULL;
+        L.gamma_const = 0.0f;
+        if (!L.mu || !L.nu || !L.post || !L.hard || !L.best || !L.first_hd || !wj) {
+            rc |= 1;
+        } else {
+#pragma omp for schedule(dynamic, 1)
+            for (int32_t b = 0; b < bt->B; ++b) decode_shot(&g, p, bt, b, &L, wj);
+        }
+        free(L.mu); free(L.nu); free(L.post); free(L.hard); free(L.best); free(L.first_hd); free(wj);
+    }
+    free(g.var_ptr);
+    free(g.var_edges);
+    return rc ? -1 : 0;
+}
+
+/* A single check update in isolation, for closed-form tests of C2V:
+ * mu[deg] in, nu[deg] out (kernels.py:1128-1215). */
+void orc_check_update_row(int32_t deg, const float *mu, uint8_t syn, int32_t use_alpha,
+                          float alpha, int32_t use_beta, float beta, float *nu)
+{
+    int32_t row_ptr[2] = {0, deg};
+    int32_t *row_var = (int32_t *)malloc(sizeof(int32_t) * (size_t)(deg > 0 ? deg : 1));
+    for (int32_t k = 0; k < deg; ++k) row_var[k] = k;
+    orc_graph g = {1, deg, deg, row_ptr, row_var, NULL, NULL};
+    orc_params p;
+    memset(&p, 0, sizeof p);
+    p.use_alpha = use_alpha;
+    p.use_beta = use_beta;
+    p.beta = beta;
+    orc_lane L;
+    memset(&L, 0, sizeof L);
+    L.mu = (float *)mu;
+    L.nu = nu;
+    check_update(&g, &p, &syn, alpha, &L);
+    free(row_var);
+}
+
+int orc_planted(void) { return ORC_PLANT; }

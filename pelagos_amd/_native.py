@@ -1,0 +1,126 @@
+"""ctypes binding of the C ABI in include/pelagos_relaybp.h.
+
+The library is built in-tree (``pelagos_amd/libpelagos_relaybp.so``) by
+``pelagos_amd/build.py``.  There is no fallback: if the library is missing or
+fails to load, :func:`lib` raises, and so does every decoder entry point.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+from typing import Optional
+
+LIB_PATH = Path(__file__).resolve().parent / "libpelagos_relaybp.so"
+
+PEL_ABI_VERSION = 1
+PEL_OK = 0
+PEL_EINVAL = -1
+PEL_EDEVICE = -2
+PEL_ENOMEM = -3
+
+_vp = ctypes.c_void_p
+
+
+class PelParams(ctypes.Structure):
+    _fields_ = [
+        ("pre_iter", ctypes.c_int32),
+        ("num_sets", ctypes.c_int32),
+        ("set_max_iter", ctypes.c_int32),
+        ("stop_nconv", ctypes.c_int32),
+        ("plain", ctypes.c_int32),
+        ("gamma0", ctypes.c_float),
+        ("use_alpha", ctypes.c_int32),
+        ("use_beta", ctypes.c_int32),
+        ("beta", ctypes.c_float),
+        ("alpha_t", _vp),
+        ("last_leg", ctypes.c_int32),
+        ("skip_relay_legs", ctypes.c_int32),
+    ]
+
+
+class PelBatch(ctypes.Structure):
+    _fields_ = [
+        ("B", ctypes.c_int32),
+        ("syndromes", _vp),
+        ("prior_llr", _vp),
+        ("wj", _vp),
+        ("prior_batched", ctypes.c_int32),
+        ("gamma_legs", _vp),
+        ("gamma_per_row", ctypes.c_int32),
+        ("init_gamma", _vp),
+        ("init_posterior", _vp),
+        ("errors", _vp),
+        ("weights", _vp),
+        ("iterations", _vp),
+        ("found_count", _vp),
+        ("last_posterior", _vp),
+        ("workspace", _vp),
+    ]
+
+
+class PelPlan(ctypes.Structure):
+    _fields_ = [
+        ("block", ctypes.c_int32),
+        ("grid", ctypes.c_int32),
+        ("placement", ctypes.c_int32),
+        ("lds_bytes", ctypes.c_int32),
+        ("spill_stride", ctypes.c_int64),
+        ("workspace_bytes", ctypes.c_int64),
+    ]
+
+
+# Every symbol the header declares, with its ctypes signature.
+SIGNATURES = {
+    "pel_abi_version": (ctypes.c_int, []),
+    "pel_last_error": (ctypes.c_char_p, []),
+    "pel_graph_create": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.POINTER(_vp)]),
+    "pel_graph_destroy": (ctypes.c_int, [_vp]),
+    "pel_graph_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
+                                      ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    "pel_plan_decode": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.POINTER(PelPlan)]),
+    "pel_decode": (ctypes.c_int, [_vp, ctypes.POINTER(PelParams), ctypes.POINTER(PelBatch),
+                                  ctypes.POINTER(PelPlan), _vp]),
+}
+
+_LIB: Optional[ctypes.CDLL] = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib() -> ctypes.CDLL:
+    """Load libpelagos_relaybp.so (raises if it is not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise NativeError(
+            f"HIP decoder library not found at {LIB_PATH}; build it with "
+            "`python -m pelagos_amd.build` (the decoder has no CPU fallback)"
+        )
+    L = ctypes.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    if L.pel_abi_version() != PEL_ABI_VERSION:
+        raise NativeError("libpelagos_relaybp.so ABI version mismatch; rebuild it")
+    _LIB = L
+    return L
+
+
+def check(rc: int, what: str) -> None:
+    """Map a status code to the reference's exception types."""
+    if rc == PEL_OK:
+        return
+    msg = lib().pel_last_error().decode(errors="replace")
+    if rc == PEL_EINVAL:
+        raise ValueError(f"{what}: {msg}")
+    if rc == PEL_ENOMEM:
+        raise MemoryError(f"{what}: {msg}")
+    raise RuntimeError(f"{what}: {msg}")

@@ -1,0 +1,663 @@
+// relaybp.hip -- MI355X (gfx950) Relay-BP-S min-sum decoder.
+//
+// Design (one persistent launch per decode):
+//   * A workgroup owns one shot at a time and runs that shot's whole Relay-BP-S
+//     schedule -- pre-iter leg, then relay legs until stop_nconv solutions --
+//     with no host round trip.  When it finishes it takes the next shot from a
+//     device work counter, so the heavy tail of slow shots is absorbed by
+//     whichever workgroups are free (the reference's per-lane freezing,
+//     kernels.py:745-847, becomes "this workgroup moves on").
+//   * Per-shot state is kept in compressed min-sum form instead of per-edge
+//     messages (the reference's muT/nuT [E,B], decoder.py:1527-1529):
+//       variables  LS[j] = {lam_mixed_j, S_j}        (8 B)  S_j = sum of incoming nu
+//       checks     CS[i] = {m1, m2, argbit, parity}  (16 B) outgoing magnitudes
+//       sign words SB     one bit per edge: sign of the variable->check message
+//     Every message is a pure function of this state, bit for bit:
+//       nu_e = (parity_i ^ sign_e ? -1 : +1) * (bit(e) == argbit_i ? m2_i : m1_i)
+//       mu_e = lam_mixed_j + (S_j - nu_e)              (kernels.py:1339-1350)
+//       post_j = lam_mixed_j + S_j                       (kernels.py:1328)
+//     At 936 x 8785 this is 90 KB per shot, which fits the 160 KB LDS of a CU,
+//     so the iteration loop touches HBM only for the graph indices (L2-resident,
+//     shared by every shot) and the shot's priors.
+//   * One iteration = check pass (a thread per check row, sequential over the
+//     row: min1/min2/argmin/sign parity, plus the parity test of the previous
+//     iteration's hard decision) and variable pass (a thread per variable,
+//     sequential sum over its edges in ascending edge id), separated by
+//     workgroup barriers.  No cross-lane reductions are needed.
+//
+// Floating point follows the reference kernels exactly (fp32, no FMA
+// contraction -- the build passes -ffp-contract=off and the memory mix uses
+// explicit round-to-nearest intrinsics).
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "pelagos_relaybp.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define PEL_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t _e = (call);                                                            \
+        if (_e != hipSuccess)                                                              \
+            return fail(PEL_EDEVICE, std::string(#call) + ": " + hipGetErrorString(_e));   \
+    } while (0)
+
+struct GraphDev {
+    int32_t C, V, E, n_words;
+    const int32_t *row_ptr;   // [C+1]
+    const int32_t *row_var;   // [E]   variable of each edge (CSR order)
+    const int32_t *row_wofs;  // [C]   first sign word of each row
+    const int32_t *var_ptr;   // [V+1]
+    const int2 *var_edge;     // [E]   {check, sign bit index}, per variable in ascending edge id
+};
+
+struct KArgs {
+    GraphDev g;
+    int32_t B;
+    int32_t pre_iter, num_sets, set_max_iter, stop_nconv;
+    int32_t plain, use_alpha, use_beta, last_leg, skip_relay_legs;
+    float gamma0, beta;
+    const float *alpha_t;
+    const uint8_t *syn;
+    const float *prior;
+    const float *wj;
+    int32_t prior_batched;
+    const float *gamma_legs;
+    int32_t gamma_per_row;
+    const float *init_gamma;
+    const float *init_post;
+    uint8_t *errors;
+    float *weights;
+    int32_t *iters;
+    int32_t *found;
+    float *last_post;
+    int32_t *queue;
+    uint8_t *spill;
+    int64_t spill_stride;
+    uint32_t ls_off, cs_off, sb_off;  // byte offsets (LDS or spill slot, by placement)
+};
+
+// Memory strength source for one leg: a row of V values or a constant.
+struct GammaSrc {
+    const float *row;
+    float konst;
+    __device__ float operator()(int j) const { return row ? row[j] : konst; }
+};
+
+struct ShotState {
+    float best_w;
+    int32_t found;
+    int32_t iters;
+    int32_t any_iter;
+};
+
+template <int BLOCK>
+__device__ double block_sum(double v, double *s_red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) s_red[wid] = v;
+    __syncthreads();
+    double tot = 0.0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) tot += s_red[w];
+    __syncthreads();
+    return tot;
+}
+
+// Check pass for row i.  first: messages were just reset (mu_e = lam0_j,
+// kernels.py:981-1000), so nothing is reconstructed and no parity is tested.
+// Otherwise reconstructs mu_e from the state of the previous iteration and also
+// returns 1 if the previous iteration's hard decision violates check i
+// (kernels.py:509-569 fused into the check pass).
+__device__ __forceinline__ int check_row(const KArgs &a, int i, bool first, float alpha,
+                                         const float *__restrict__ l0, const uint8_t *__restrict__ syn,
+                                         const float2 *LS, float4 *CS, uint32_t *SB) {
+    const int r0 = a.g.row_ptr[i];
+    const int deg = a.g.row_ptr[i + 1] - r0;
+    const uint32_t s_i = syn[i] & 1u;
+    if (deg == 0) return first ? 0 : (int)s_i;
+    const int wofs = a.g.row_wofs[i];
+    float om1 = 0.f, om2 = 0.f;
+    uint32_t oarg = 0xffffffffu, opar = 0;
+    if (!first) {
+        const float4 cs = CS[i];
+        om1 = cs.x;
+        om2 = cs.y;
+        oarg = __float_as_uint(cs.z);
+        opar = __float_as_uint(cs.w);
+    }
+    float m1 = INFINITY, m2 = INFINITY;
+    uint32_t arg = 0, neg = 0, hpar = 0;
+    const int32_t *__restrict__ vars = a.g.row_var + r0;
+    for (int k0 = 0; k0 < deg; k0 += 32) {
+        const int n = min(32, deg - k0);
+        const uint32_t wbase = (uint32_t)(wofs + (k0 >> 5));
+        const uint32_t wold = first ? 0u : SB[wbase];
+        uint32_t wnew = 0;
+        for (int kk = 0; kk < n; ++kk) {
+            const int j = vars[k0 + kk];
+            float mu;
+            if (first) {
+                mu = l0[j];
+            } else {
+                const float2 ls = LS[j];
+                hpar ^= (ls.x + ls.y <= 0.f) ? 1u : 0u;
+                const uint32_t bit = (wbase << 5) + (uint32_t)kk;
+                const float mag = (bit == oarg) ? om2 : om1;
+                const float nu = (((wold >> kk) ^ opar) & 1u) ? -mag : mag;
+                mu = ls.x + (ls.y - nu);
+            }
+            const uint32_t isneg = mu < 0.f ? 1u : 0u;
+            neg ^= isneg;
+            wnew |= isneg << kk;
+            const float am = fabsf(mu);
+            if (am < m1) {
+                m2 = m1;
+                m1 = am;
+                arg = (wbase << 5) + (uint32_t)kk;
+            } else if (am < m2) {
+                m2 = am;
+            }
+        }
+        SB[wbase] = wnew;
+    }
+    float o1 = m1, o2 = m2;
+    if (deg == 1) o1 = o2 = 0.f;  // degree-1 checks send zero (kernels.py:1192-1193)
+    if (a.use_alpha) {
+        o1 = alpha * o1;
+        o2 = alpha * o2;
+    }
+    if (a.use_beta) {
+        o1 = fmaxf(o1 - a.beta, 0.f);
+        o2 = fmaxf(o2 - a.beta, 0.f);
+    }
+    CS[i] = make_float4(o1, o2, __uint_as_float(arg), __uint_as_float((neg ^ s_i) & 1u));
+    return first ? 0 : (int)((hpar ^ s_i) & 1u);
+}
+
+// Parity of the current hard decision on row i (after the last iteration of a leg).
+__device__ __forceinline__ int parity_row(const KArgs &a, int i, const uint8_t *__restrict__ syn,
+                                          const float2 *LS) {
+    const int r0 = a.g.row_ptr[i], r1 = a.g.row_ptr[i + 1];
+    uint32_t hpar = syn[i] & 1u;
+    for (int e = r0; e < r1; ++e) {
+        const float2 ls = LS[a.g.row_var[e]];
+        hpar ^= (ls.x + ls.y <= 0.f) ? 1u : 0u;
+    }
+    return (int)hpar;
+}
+
+// Variable pass for variable j (kernels.py:1299-1350).
+__device__ __forceinline__ void var_update(const KArgs &a, int j, const GammaSrc &gs,
+                                           const float *__restrict__ l0, float2 *LS,
+                                           const float4 *CS, const uint32_t *SB) {
+    const float g = gs(j);
+    const float2 ls = LS[j];
+    const float post_prev = ls.x + ls.y;
+    const float lm = __fadd_rn(__fmul_rn(1.0f - g, l0[j]), __fmul_rn(g, post_prev));
+    float s = 0.f;
+    const int q1 = a.g.var_ptr[j + 1];
+    for (int q = a.g.var_ptr[j]; q < q1; ++q) {
+        const int2 ve = a.g.var_edge[q];
+        const float4 cs = CS[ve.x];
+        const uint32_t w = SB[(uint32_t)ve.y >> 5];
+        const float mag = ((uint32_t)ve.y == __float_as_uint(cs.z)) ? cs.y : cs.x;
+        const float nu = (((w >> (ve.y & 31)) ^ __float_as_uint(cs.w)) & 1u) ? -mag : mag;
+        s += nu;
+    }
+    LS[j] = make_float2(lm, s);
+}
+
+// check_and_select_kernel (kernels.py:657-741) for a shot whose hard decision
+// satisfies every check: weight = sum of wj over bits with post <= 0 (double
+// accumulation, rounded once), strict improvement or first solution wins.
+template <int BLOCK>
+__device__ void select_solution(const KArgs &a, const float *__restrict__ wj, const float2 *LS,
+                                uint8_t *err, ShotState &st, double *s_red) {
+    const int V = a.g.V;
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < V; j += BLOCK) {
+        const float2 ls = LS[j];
+        if (ls.x + ls.y <= 0.f) acc += (double)wj[j];
+    }
+    const float w = (float)block_sum<BLOCK>(acc, s_red);
+    if (st.best_w >= 1e20f || w < st.best_w) {
+        st.best_w = w;
+        for (int j = threadIdx.x; j < V; j += BLOCK) {
+            const float2 ls = LS[j];
+            err[j] = (ls.x + ls.y <= 0.f) ? 1 : 0;
+        }
+    }
+    st.found += 1;
+}
+
+// One leg of up to T iterations.  The shot leaves the leg at its first valid
+// solution (freeze_finished_lanes_kernel with sought = found + 1).
+template <int BLOCK>
+__device__ bool run_leg(const KArgs &a, int T, const GammaSrc &gs, const float *__restrict__ l0,
+                        const float *__restrict__ wj, const uint8_t *__restrict__ syn, uint8_t *err,
+                        float2 *LS, float4 *CS, uint32_t *SB, ShotState &st, double *s_red) {
+    const int C = a.g.C, V = a.g.V;
+    for (int t = 0; t < T; ++t) {
+        const float alpha = a.use_alpha ? a.alpha_t[t] : 1.0f;
+        const bool first = (t == 0);
+        int bad = 0;
+        for (int i = threadIdx.x; i < C; i += BLOCK) bad |= check_row(a, i, first, alpha, l0, syn, LS, CS, SB);
+        bad = __syncthreads_or(bad);
+        if (!first && !bad) {  // iteration t-1 converged
+            select_solution<BLOCK>(a, wj, LS, err, st, s_red);
+            return true;
+        }
+        for (int j = threadIdx.x; j < V; j += BLOCK) var_update(a, j, gs, l0, LS, CS, SB);
+        __syncthreads();
+        st.iters += 1;
+        st.any_iter = 1;
+    }
+    if (T > 0) {
+        int bad = 0;
+        for (int i = threadIdx.x; i < C; i += BLOCK) bad |= parity_row(a, i, syn, LS);
+        bad = __syncthreads_or(bad);
+        if (!bad) {
+            select_solution<BLOCK>(a, wj, LS, err, st, s_red);
+            return true;
+        }
+    }
+    return false;
+}
+
+template <int BLOCK>
+__device__ void write_hard(const KArgs &a, const float2 *LS, uint8_t *err, bool any_iter) {
+    for (int j = threadIdx.x; j < a.g.V; j += BLOCK) {
+        uint8_t h = 0;
+        if (any_iter) {
+            const float2 ls = LS[j];
+            h = (ls.x + ls.y <= 0.f) ? 1 : 0;
+        }
+        err[j] = h;
+    }
+}
+
+template <int BLOCK>
+__device__ void decode_shot(const KArgs &a, int b, float2 *LS, float4 *CS, uint32_t *SB,
+                            double *s_red) {
+    const int C = a.g.C, V = a.g.V;
+    const uint8_t *syn = a.syn + (size_t)b * C;
+    const size_t pofs = a.prior_batched ? (size_t)b * V : 0;
+    const float *l0 = a.prior + pofs;
+    const float *wj = a.wj + pofs;
+    uint8_t *err = a.errors + (size_t)b * V;
+
+    // State reset (decoder.py:3430-3464): post = prior (or init_posterior),
+    // stored as lam_mixed = post, S = 0 so that lam_mixed + S == post exactly.
+    for (int j = threadIdx.x; j < V; j += BLOCK) {
+        const float p0 = a.init_post ? a.init_post[(size_t)b * V + j] : l0[j];
+        LS[j] = make_float2(p0, 0.f);
+        err[j] = 0;
+    }
+    __syncthreads();
+
+    ShotState st{INFINITY, 0, 0, 0};
+    const int S = a.stop_nconv;
+
+    // Pre-iter leg (decoder.py:3470-3593).
+    GammaSrc g0{nullptr, a.plain ? 0.f : a.gamma0};
+    if (!a.plain && a.init_gamma) g0.row = a.init_gamma + (size_t)b * V;
+    if (st.found < S) run_leg<BLOCK>(a, a.pre_iter, g0, l0, wj, syn, err, LS, CS, SB, st, s_red);
+    // first_leg_hard_dec (decoder.py:3597-3598): kept in err while no solution exists.
+    if (!a.last_leg && st.found == 0) write_hard<BLOCK>(a, LS, err, st.any_iter);
+
+    if (!a.skip_relay_legs) {
+        for (int leg = 0; leg < a.num_sets && st.found < S; ++leg) {
+            GammaSrc gl{a.gamma_legs + (a.gamma_per_row ? ((size_t)leg * a.B + b) * V : (size_t)leg * V),
+                        0.f};
+            run_leg<BLOCK>(a, a.set_max_iter, gl, l0, wj, syn, err, LS, CS, SB, st, s_red);
+            if (leg == 0 && a.pre_iter == 0 && !a.last_leg && st.found == 0)
+                write_hard<BLOCK>(a, LS, err, st.any_iter);  // decoder.py:3809-3810
+        }
+    }
+    // last_leg correction: the live hard decision for shots with no solution.
+    if (a.last_leg && st.found == 0) write_hard<BLOCK>(a, LS, err, st.any_iter);
+
+    if (a.last_post) {
+        float *lp = a.last_post + (size_t)b * V;
+        for (int j = threadIdx.x; j < V; j += BLOCK) {
+            const float2 ls = LS[j];
+            lp[j] = ls.x + ls.y;
+        }
+    }
+    if (threadIdx.x == 0) {
+        a.weights[b] = st.best_w;
+        a.iters[b] = st.iters;
+        a.found[b] = st.found;
+    }
+}
+
+// PLACE bit0: LS in LDS; bit1: CS in LDS; bit2: SB in LDS.
+template <int BLOCK, int PLACE>
+__global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    __shared__ double s_red[BLOCK / 64];
+    __shared__ int s_shot;
+    uint8_t *slot = a.spill + (int64_t)blockIdx.x * a.spill_stride;
+    float2 *LS = reinterpret_cast<float2 *>(((PLACE & 1) ? smem : slot) + a.ls_off);
+    float4 *CS = reinterpret_cast<float4 *>(((PLACE & 2) ? smem : slot) + a.cs_off);
+    uint32_t *SB = reinterpret_cast<uint32_t *>(((PLACE & 4) ? smem : slot) + a.sb_off);
+    for (;;) {
+        if (threadIdx.x == 0) s_shot = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int b = s_shot;
+        __syncthreads();
+        if (b >= a.B) break;
+        decode_shot<BLOCK>(a, b, LS, CS, SB, s_red);
+    }
+}
+
+__global__ void reset_queue_kernel(int32_t *q) { *q = 0; }
+
+typedef void (*KernelFn)(KArgs);
+
+template <int BLOCK>
+KernelFn pick_place(int place) {
+    switch (place) {
+        case 7: return relay_decode_kernel<BLOCK, 7>;
+        case 3: return relay_decode_kernel<BLOCK, 3>;
+        case 1: return relay_decode_kernel<BLOCK, 1>;
+        default: return relay_decode_kernel<BLOCK, 0>;
+    }
+}
+
+KernelFn pick_kernel(int block, int place) {
+    switch (block) {
+        case 64: return pick_place<64>(place);
+        case 128: return pick_place<128>(place);
+        case 256: return pick_place<256>(place);
+        case 512: return pick_place<512>(place);
+        default: return pick_place<1024>(place);
+    }
+}
+
+int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+struct pel_graph {
+    int device;
+    int32_t C, V, E, n_words;
+    int32_t max_row_deg;
+    void *dev_mem;  // one allocation holding every array below
+    GraphDev d;
+};
+
+extern "C" {
+
+int pel_abi_version(void) { return PEL_ABI_VERSION; }
+
+const char *pel_last_error(void) { return g_last_error.c_str(); }
+
+int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, int32_t V, int32_t E,
+                     pel_graph **out) {
+    if (!out) return fail(PEL_EINVAL, "pel_graph_create: out is NULL");
+    *out = nullptr;
+    if (!chk_ptr || (!chk_var && E > 0)) return fail(PEL_EINVAL, "pel_graph_create: NULL CSR array");
+    if (C <= 0 || V <= 0 || E < 0) return fail(PEL_EINVAL, "pel_graph_create: empty matrix");
+    if (chk_ptr[0] != 0 || chk_ptr[C] != E)
+        return fail(PEL_EINVAL, "pel_graph_create: chk_ptr must start at 0 and end at E");
+    std::vector<int32_t> wofs(C);
+    int32_t nw = 0, maxdeg = 0;
+    for (int32_t i = 0; i < C; ++i) {
+        const int32_t d = chk_ptr[i + 1] - chk_ptr[i];
+        if (d < 0) return fail(PEL_EINVAL, "pel_graph_create: chk_ptr not non-decreasing");
+        wofs[i] = nw;
+        nw += (d + 31) / 32;
+        maxdeg = std::max(maxdeg, d);
+    }
+    std::vector<int32_t> var_ptr(V + 1, 0);
+    for (int32_t e = 0; e < E; ++e) {
+        const int32_t j = chk_var[e];
+        if (j < 0 || j >= V) return fail(PEL_EINVAL, "pel_graph_create: column index out of range");
+        var_ptr[j + 1]++;
+    }
+    for (int32_t j = 0; j < V; ++j) var_ptr[j + 1] += var_ptr[j];
+    std::vector<int32_t> fill(V, 0);
+    std::vector<int2> var_edge(std::max(E, 1));
+    for (int32_t i = 0; i < C; ++i) {
+        for (int32_t e = chk_ptr[i]; e < chk_ptr[i + 1]; ++e) {
+            const int32_t j = chk_var[e];
+            const int32_t k = e - chk_ptr[i];
+            var_edge[var_ptr[j] + fill[j]++] = make_int2(i, wofs[i] * 32 + k);
+        }
+    }
+    // One device allocation: row_ptr | row_var | row_wofs | var_ptr | var_edge
+    const int64_t o_rp = 0;
+    const int64_t o_rv = align_up(o_rp + 4LL * (C + 1), 256);
+    const int64_t o_rw = align_up(o_rv + 4LL * std::max(E, 1), 256);
+    const int64_t o_vp = align_up(o_rw + 4LL * C, 256);
+    const int64_t o_ve = align_up(o_vp + 4LL * (V + 1), 256);
+    const int64_t total = align_up(o_ve + 8LL * std::max(E, 1), 256);
+    pel_graph *g = new pel_graph();
+    PEL_HIP(hipGetDevice(&g->device));
+    hipError_t e = hipMalloc(&g->dev_mem, (size_t)total);
+    if (e != hipSuccess) {
+        delete g;
+        return fail(PEL_ENOMEM, std::string("pel_graph_create: hipMalloc: ") + hipGetErrorString(e));
+    }
+    uint8_t *base = static_cast<uint8_t *>(g->dev_mem);
+    std::vector<uint8_t> host((size_t)total, 0);
+    memcpy(host.data() + o_rp, chk_ptr, 4ULL * (C + 1));
+    if (E > 0) memcpy(host.data() + o_rv, chk_var, 4ULL * E);
+    memcpy(host.data() + o_rw, wofs.data(), 4ULL * C);
+    memcpy(host.data() + o_vp, var_ptr.data(), 4ULL * (V + 1));
+    memcpy(host.data() + o_ve, var_edge.data(), 8ULL * std::max(E, 1));
+    e = hipMemcpy(base, host.data(), (size_t)total, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(g->dev_mem);
+        delete g;
+        return fail(PEL_EDEVICE, std::string("pel_graph_create: hipMemcpy: ") + hipGetErrorString(e));
+    }
+    g->C = C;
+    g->V = V;
+    g->E = E;
+    g->n_words = nw;
+    g->max_row_deg = maxdeg;
+    g->d.C = C;
+    g->d.V = V;
+    g->d.E = E;
+    g->d.n_words = nw;
+    g->d.row_ptr = reinterpret_cast<const int32_t *>(base + o_rp);
+    g->d.row_var = reinterpret_cast<const int32_t *>(base + o_rv);
+    g->d.row_wofs = reinterpret_cast<const int32_t *>(base + o_rw);
+    g->d.var_ptr = reinterpret_cast<const int32_t *>(base + o_vp);
+    g->d.var_edge = reinterpret_cast<const int2 *>(base + o_ve);
+    *out = g;
+    return PEL_OK;
+}
+
+int pel_graph_destroy(pel_graph *g) {
+    if (!g) return PEL_OK;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    if (cur != g->device) (void)hipSetDevice(g->device);
+    hipError_t e = hipFree(g->dev_mem);
+    if (cur != g->device) (void)hipSetDevice(cur);
+    delete g;
+    if (e != hipSuccess) return fail(PEL_EDEVICE, std::string("pel_graph_destroy: ") + hipGetErrorString(e));
+    return PEL_OK;
+}
+
+int pel_graph_info(const pel_graph *g, int32_t *C, int32_t *V, int32_t *E, int64_t *var_state_bytes,
+                   int64_t *chk_state_bytes, int64_t *sign_bytes) {
+    if (!g) return fail(PEL_EINVAL, "pel_graph_info: NULL graph");
+    if (C) *C = g->C;
+    if (V) *V = g->V;
+    if (E) *E = g->E;
+    if (var_state_bytes) *var_state_bytes = 8LL * g->V;
+    if (chk_state_bytes) *chk_state_bytes = 16LL * g->C;
+    if (sign_bytes) *sign_bytes = 4LL * g->n_words;
+    return PEL_OK;
+}
+
+int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32_t placement_override,
+                    pel_plan *out) {
+    if (!g || !out) return fail(PEL_EINVAL, "pel_plan_decode: NULL argument");
+    if (B < 0) return fail(PEL_EINVAL, "pel_plan_decode: negative batch");
+    int dev = 0;
+    PEL_HIP(hipGetDevice(&dev));
+    if (dev != g->device) return fail(PEL_EINVAL, "pel_plan_decode: current device differs from the graph's");
+    int max_lds = 0, n_cu = 0;
+    PEL_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    PEL_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    int block = block_override;
+    if (block <= 0) {
+        const int want = std::max(g->C, g->V / 8);
+        block = 64;
+        while (block < want && block < 1024) block *= 2;
+    }
+    if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
+        return fail(PEL_EINVAL, "pel_plan_decode: block must be 64, 128, 256, 512 or 1024");
+    const int64_t ls = align_up(8LL * g->V, 16), cs = align_up(16LL * g->C, 16),
+                  sb = align_up(4LL * std::max(g->n_words, 1), 16);
+    const int64_t static_lds = 8LL * (block / 64) + 64;
+    const int64_t budget = (int64_t)max_lds - static_lds;
+    int place;
+    if (ls + cs + sb <= budget) place = 7;
+    else if (ls + cs <= budget) place = 3;
+    else if (ls <= budget) place = 1;
+    else place = 0;
+    if (placement_override >= 0) {
+        if (placement_override != 0 && placement_override != 1 && placement_override != 3 &&
+            placement_override != 7)
+            return fail(PEL_EINVAL, "pel_plan_decode: placement must be 0, 1, 3 or 7");
+        if (placement_override > place)
+            return fail(PEL_EINVAL, "pel_plan_decode: requested placement exceeds the LDS budget");
+        place = placement_override;
+    }
+    int64_t lds = 0, spill = 0;
+    uint32_t offs[3];
+    const int64_t sz[3] = {ls, cs, sb};
+    for (int k = 0; k < 3; ++k) {
+        if (place & (1 << k)) {
+            offs[k] = (uint32_t)lds;
+            lds += sz[k];
+        } else {
+            offs[k] = (uint32_t)spill;
+            spill += sz[k];
+        }
+    }
+    (void)offs;
+    KernelFn fn = pick_kernel(block, place);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)e;
+        (void)hipGetLastError();
+    }
+    int per_cu = 0;
+    PEL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(fn), block,
+                                                         (size_t)lds));
+    if (per_cu <= 0) return fail(PEL_EDEVICE, "pel_plan_decode: kernel cannot be resident (LDS/registers)");
+    int64_t grid = (int64_t)per_cu * n_cu;
+    grid = std::min<int64_t>(grid, std::max<int32_t>(B, 1));
+    out->block = block;
+    out->grid = (int32_t)grid;
+    out->placement = place;
+    out->lds_bytes = (int32_t)lds;
+    out->spill_stride = align_up(spill, 256);
+    out->workspace_bytes = 256 + out->spill_stride * grid;
+    return PEL_OK;
+}
+
+int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, const pel_plan *plan, void *stream) {
+    if (!g || !p || !b || !plan) return fail(PEL_EINVAL, "pel_decode: NULL argument");
+    if (b->B < 0) return fail(PEL_EINVAL, "pel_decode: negative batch");
+    if (b->B == 0) return PEL_OK;
+    if (p->pre_iter < 0 || p->num_sets < 0 || p->set_max_iter < 0)
+        return fail(PEL_EINVAL, "pel_decode: iteration counts must be >= 0");
+    if (p->stop_nconv < 1) return fail(PEL_EINVAL, "pel_decode: stop_nconv must be >= 1");
+    if (!b->syndromes || !b->prior_llr || !b->wj || !b->errors || !b->weights || !b->iterations ||
+        !b->found_count || !b->workspace)
+        return fail(PEL_EINVAL, "pel_decode: NULL buffer");
+    if (p->num_sets > 0 && !p->skip_relay_legs && !b->gamma_legs)
+        return fail(PEL_EINVAL, "pel_decode: gamma_legs is NULL with num_sets > 0");
+    if (p->use_alpha && !p->alpha_t) return fail(PEL_EINVAL, "pel_decode: alpha_t is NULL with use_alpha");
+    int dev = 0;
+    PEL_HIP(hipGetDevice(&dev));
+    if (dev != g->device) return fail(PEL_EINVAL, "pel_decode: current device differs from the graph's");
+    KArgs a;
+    memset(&a, 0, sizeof a);
+    a.g = g->d;
+    a.B = b->B;
+    a.pre_iter = p->pre_iter;
+    a.num_sets = p->num_sets;
+    a.set_max_iter = p->set_max_iter;
+    a.stop_nconv = p->stop_nconv;
+    a.plain = p->plain;
+    a.use_alpha = p->use_alpha;
+    a.use_beta = p->use_beta;
+    a.last_leg = p->last_leg;
+    a.skip_relay_legs = p->skip_relay_legs;
+    a.gamma0 = p->gamma0;
+    a.beta = p->beta;
+    a.alpha_t = p->alpha_t;
+    a.syn = b->syndromes;
+    a.prior = b->prior_llr;
+    a.wj = b->wj;
+    a.prior_batched = b->prior_batched;
+    a.gamma_legs = b->gamma_legs;
+    a.gamma_per_row = b->gamma_per_row;
+    a.init_gamma = b->init_gamma;
+    a.init_post = b->init_posterior;
+    a.errors = b->errors;
+    a.weights = b->weights;
+    a.iters = b->iterations;
+    a.found = b->found_count;
+    a.last_post = b->last_posterior;
+    uint8_t *ws = static_cast<uint8_t *>(b->workspace);
+    a.queue = reinterpret_cast<int32_t *>(ws);
+    a.spill = ws + 256;
+    a.spill_stride = plan->spill_stride;
+    const int64_t ls = align_up(8LL * g->V, 16), cs = align_up(16LL * g->C, 16),
+                  sb = align_up(4LL * std::max(g->n_words, 1), 16);
+    const int64_t sz[3] = {ls, cs, sb};
+    int64_t lds = 0, spill = 0;
+    uint32_t offs[3];
+    for (int k = 0; k < 3; ++k) {
+        if (plan->placement & (1 << k)) {
+            offs[k] = (uint32_t)lds;
+            lds += sz[k];
+        } else {
+            offs[k] = (uint32_t)spill;
+            spill += sz[k];
+        }
+    }
+    if (lds != plan->lds_bytes || align_up(spill, 256) != plan->spill_stride)
+        return fail(PEL_EINVAL, "pel_decode: plan does not belong to this graph");
+    a.ls_off = offs[0];
+    a.cs_off = offs[1];
+    a.sb_off = offs[2];
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    KernelFn fn = pick_kernel(plan->block, plan->placement);
+    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, s, a.queue);
+    PEL_HIP(hipGetLastError());
+    hipLaunchKernelGGL(fn, dim3(plan->grid), dim3(plan->block), (size_t)plan->lds_bytes, s, a);
+    PEL_HIP(hipGetLastError());
+    return PEL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,544 @@
+"""RelayBPDecoder: the reference's Relay-BP-S decoder surface on MI355X.
+
+Same constructor (src/pelagos/decoder.py:271-307), same ``decode`` contract
+(decoder.py:2317-2449) and the same exception types and messages for the
+validations the reference performs.  Underneath, one call of ``decode`` is a
+single persistent HIP launch (``pel_decode``, include/pelagos_relaybp.h):
+every shot runs its whole pre-iter leg and relay legs inside one workgroup,
+so there is no per-iteration host synchronisation.
+
+PyTorch supplies device memory, streams and the per-leg memory-strength draws
+(the reference's exact ``torch.rand`` calls on the decoding device,
+utils.py:128-225); all decoding arithmetic runs in the HIP library.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, List, Optional, Tuple, Union
+
+import numpy as np
+import torch
+from scipy.sparse import csr_matrix
+
+from . import _native
+from .graph import CSRGraph
+from .utils import (
+    GAMMA_DRAW_MODES,
+    UNCONVERGED_CORRECTIONS,
+    bitpack_errors,
+    compute_log_prior_ratios,
+    sample_gamma_per_variable,
+    sample_gamma_uniform,
+    validate_csr_matrix,
+    validate_error_priors,
+)
+
+_DTYPE_ALIASES = {
+    "float": "float32",
+    "half": "float16",
+    "double": "float64",
+    "fp16": "float16",
+    "fp32": "float32",
+    "fp64": "float64",
+}
+_PERF_ALIASES = {
+    "default": "default",
+    "row": "default",
+    "rowwise": "default",
+    "throughput": "throughput",
+    "btile": "throughput",
+}
+
+
+def alpha_schedule(alpha: Optional[float], scaling: float, n: int) -> np.ndarray:
+    """alpha for in-leg iteration t = 0..n-1, as the float32 kernel scalar
+    (decoder.py:2649-2660): 1 - 2^{-(t+1)/s} when alpha == 0, 1 when alpha < 0
+    or None, else alpha."""
+    out = np.ones(max(n, 1), dtype=np.float32)
+    if alpha is None:
+        return out
+    a = float(alpha)
+    for t in range(n):
+        if a == 0.0:
+            s = float(scaling or 1.0)
+            out[t] = np.float32(1.0 - (2.0 ** (-((t + 1) / s))))
+        elif a < 0.0:
+            out[t] = 1.0
+        else:
+            out[t] = np.float32(a)
+    return out
+
+
+class _GraphHandle:
+    """Owns a pel_graph on one device."""
+
+    def __init__(self, graph: CSRGraph, device: torch.device):
+        self.device = device
+        L = _native.lib()
+        h = ctypes.c_void_p()
+        with torch.cuda.device(device):
+            _native.check(
+                L.pel_graph_create(graph.chk_ptr.ctypes.data, graph.edge_var.ctypes.data,
+                                   graph.C, graph.V, graph.E, ctypes.byref(h)),
+                "pel_graph_create",
+            )
+        self.h = h
+
+    def plan(self, B: int, block: int = 0, placement: int = -1) -> _native.PelPlan:
+        p = _native.PelPlan()
+        with torch.cuda.device(self.device):
+            _native.check(_native.lib().pel_plan_decode(self.h, int(B), int(block), int(placement),
+                                                        ctypes.byref(p)),
+                          "pel_plan_decode")
+        return p
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                _native.lib().pel_graph_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+
+def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
+    return None if t is None else t.data_ptr()
+
+
+class RelayBPDecoder:
+    """Relay-BP-S min-sum decoder with disordered memory strengths on one MI355X.
+
+    Arguments and defaults follow the reference constructor
+    (decoder.py:271-307).  Options the MI355X path does not implement raise
+    ``NotImplementedError`` at construction instead of being ignored:
+    ``data_type`` other than float32, layered scheduling, more than one GPU.
+    ``block_size`` and ``state_placement`` (additions) force the workgroup size
+    and the LDS placement bits of the decode kernel (see ``pel_plan_decode``).
+    """
+
+    def __init__(
+        self,
+        H_csr: csr_matrix,
+        error_priors: np.ndarray,
+        pre_iter: int = 80,
+        num_sets: int = 100,
+        set_max_iter: int = 60,
+        gamma0: float = 0.65,
+        gamma_dist_interval: Tuple[float, float] = (-0.24, 0.66),
+        stop_nconv: int = 5,
+        normalized_min_sum_alpha: Optional[float] = None,
+        offset_min_sum_beta: Optional[float] = None,
+        data_type: str = "float32",
+        device: Union[str, List[int], int] = [0],
+        seed: int = 1234,
+        gamma_draw: str = "broadcast",
+        bitpack_output: bool = False,
+        algo: Optional[str] = None,
+        perf: Optional[str] = None,
+        alpha_iteration_scaling_factor: Optional[float] = 1.0,
+        explicit_gammas: Optional[np.ndarray] = None,
+        gamma_dist_interval_by_error: Optional[np.ndarray] = None,
+        fast_max_unconverged_pct: Optional[float] = None,
+        scheduling: str = "flood-fill",
+        layered_k_iters: int = 0,
+        layered_block_policy: str = "contiguous",
+        layered_num_blocks: Optional[int] = None,
+        layered_target_edges_per_block: Optional[int] = None,
+        layered_block_seed: int = 42,
+        unconverged_correction: str = "first_leg",
+        block_size: Optional[int] = None,
+        state_placement: Optional[int] = None,
+    ):
+        # Validation in the reference's order and wording (decoder.py:362-512).
+        if unconverged_correction not in UNCONVERGED_CORRECTIONS:
+            raise ValueError(
+                f"unconverged_correction must be one of {UNCONVERGED_CORRECTIONS} "
+                f"(got {unconverged_correction!r})"
+            )
+        self.unconverged_correction = unconverged_correction
+        validate_csr_matrix(H_csr)
+        error_priors = validate_error_priors(error_priors)
+        if normalized_min_sum_alpha is not None and offset_min_sum_beta is not None:
+            raise ValueError("Cannot specify both normalized_min_sum_alpha and offset_min_sum_beta")
+        if normalized_min_sum_alpha is not None and not (0 <= normalized_min_sum_alpha <= 1):
+            raise ValueError("normalized_min_sum_alpha must be in [0, 1]")
+        if offset_min_sum_beta is not None and offset_min_sum_beta < 0:
+            raise ValueError("offset_min_sum_beta must be >= 0")
+        self.H_csr = H_csr
+        self.check_permutation = None
+        self.error_priors = error_priors
+        self.pre_iter = int(pre_iter)
+        self.num_sets = int(num_sets)
+        self.set_max_iter = int(set_max_iter)
+        self.gamma0 = gamma0
+        self.gamma_dist_interval = gamma_dist_interval
+        self.stop_nconv = int(stop_nconv)
+        if self.stop_nconv < 1:
+            raise ValueError(f"stop_nconv must be >= 1, got {stop_nconv}")
+        if min(self.pre_iter, self.num_sets, self.set_max_iter) < 0:
+            raise ValueError("pre_iter, num_sets and set_max_iter must be >= 0")
+        self.fast_max_unconverged_pct = fast_max_unconverged_pct
+        if fast_max_unconverged_pct is not None:
+            if not (0.0 <= fast_max_unconverged_pct <= 1.0):
+                raise ValueError(
+                    "fast_max_unconverged_pct must be between 0.0 and 1.0, got "
+                    f"{fast_max_unconverged_pct}"
+                )
+        self.normalized_min_sum_alpha = normalized_min_sum_alpha
+        self.offset_min_sum_beta = offset_min_sum_beta
+
+        data_type = _DTYPE_ALIASES.get(data_type, data_type)
+        if data_type not in ("float16", "float32", "float64"):
+            raise ValueError(
+                f"data_type must be 'float16', 'float32', or 'float64', got {data_type}"
+            )
+        if data_type != "float32":
+            raise NotImplementedError(
+                f"data_type={data_type!r}: the MI355X decoder computes in float32 only"
+            )
+        self.dtype_str = data_type
+        self.dtype = torch.float32
+
+        self.device_ids = self._parse_device_spec(device)
+        self.num_gpus = len(self.device_ids)
+        if not torch.cuda.is_available():
+            raise RuntimeError(
+                "GPU not available (install a GPU-enabled PyTorch build: CUDA or ROCm)"
+            )
+        if self.num_gpus != 1:
+            raise NotImplementedError(
+                "one decoder drives one GPU; shard shots across decoders, one per GPU"
+            )
+        self._backend = "rocm"
+        self.device_type = "cuda"
+        self.device = torch.device(self.device_type, self.device_ids[0])
+
+        self.seed = seed
+        if gamma_draw not in GAMMA_DRAW_MODES:
+            raise ValueError(f"gamma_draw must be one of {GAMMA_DRAW_MODES}; got {gamma_draw!r}")
+        self.gamma_draw = gamma_draw
+        self.bitpack_output = bitpack_output
+        if algo not in ("relay", "plain"):
+            raise ValueError(f"Unknown algo '{algo}'. Use 'relay' or 'plain'.")
+        self.algo = algo
+        perf_norm = (perf or "throughput").strip().lower()
+        if perf_norm not in _PERF_ALIASES:
+            raise ValueError(f"Unknown perf '{perf_norm}'. Use 'default'/'throughput'.")
+        # Both reference modes compute the same per-shot results; one kernel serves both.
+        self.perf = _PERF_ALIASES[perf_norm]
+        self.alpha_iteration_scaling_factor = float(alpha_iteration_scaling_factor or 1.0)
+        scheduling_norm = (scheduling or "flood-fill").strip().lower()
+        if scheduling_norm not in ("flood-fill", "layered"):
+            raise ValueError(
+                f"Unknown scheduling '{scheduling_norm}'. Use 'flood-fill' or 'layered'."
+            )
+        if scheduling_norm == "layered" and self.perf != "throughput":
+            raise ValueError("scheduling='layered' requires perf='throughput'")
+        if scheduling_norm == "layered":
+            raise NotImplementedError("scheduling='layered' is not implemented on MI355X")
+        self.scheduling = scheduling_norm
+        self.layered_k_iters = int(layered_k_iters) if layered_k_iters is not None else 0
+
+        self.C, self.V = (int(x) for x in H_csr.shape)
+        self._explicit_gammas = None
+        if explicit_gammas is not None:
+            eg = np.asarray(explicit_gammas, dtype=np.float32)
+            if eg.ndim != 2 or eg.shape[1] != self.V:
+                raise ValueError(f"explicit_gammas must have shape (K, V={self.V})")
+            self._explicit_gammas = torch.from_numpy(eg).to(self.device, dtype=self.dtype)
+        self._gamma_by_error = None
+        if gamma_dist_interval_by_error is not None:
+            gd = np.asarray(gamma_dist_interval_by_error, dtype=np.float32)
+            if gd.ndim != 2 or gd.shape[0] != self.V or gd.shape[1] != 2:
+                raise ValueError(
+                    f"gamma_dist_interval_by_error must have shape (V={self.V}, 2), got "
+                    f"{tuple(gd.shape)}"
+                )
+            if np.any(gd[:, 0] > gd[:, 1]):
+                raise ValueError(
+                    "gamma_dist_interval_by_error: lower bounds must be <= upper bounds"
+                )
+            self._gamma_by_error = torch.from_numpy(gd).to(self.device, dtype=self.dtype)
+
+        self.graph = CSRGraph.from_matrix(H_csr)
+        self.E = self.graph.E
+        log_ratios = compute_log_prior_ratios(error_priors)
+        lpr = torch.from_numpy(log_ratios).to(self.device, dtype=self.dtype).unsqueeze(0)
+        self.log_prior_ratios_per_gpu = [lpr]
+        wj = lpr.squeeze(0)
+        self.wj = torch.where(torch.isfinite(wj), wj, torch.zeros_like(wj)).contiguous()
+        self.wj_per_gpu = {self.device_ids[0]: self.wj}
+        torch.manual_seed(seed)  # the reference's global side effect (decoder.py:774)
+
+        self._block_size = int(block_size) if block_size else 0
+        self._placement = -1 if state_placement is None else int(state_placement)
+        self._handle = _GraphHandle(self.graph, self.device)
+        self._plans: Dict[int, _native.PelPlan] = {}
+        self._workspace: Optional[torch.Tensor] = None
+        self._gamma_cache: Dict[Tuple, torch.Tensor] = {}
+        n_alpha = max(self.pre_iter, self.set_max_iter, 1)
+        self._alpha_cache: Dict[int, torch.Tensor] = {}
+        self._alpha_n = n_alpha
+        # When a list, decode() appends a (start, end) pair of HIP events
+        # recorded on the decode stream around each kernel launch.
+        self.kernel_events: Optional[list] = None
+
+    # ------------------------------------------------------------------ helpers
+    @staticmethod
+    def _parse_device_spec(device) -> List[int]:
+        """Device list parsing of decoder.py:1403-1452."""
+        if isinstance(device, bool):
+            raise ValueError(f"Invalid device type: {type(device)}. Expected List[int], int, or str.")
+        if isinstance(device, int):
+            if device < 1:
+                raise ValueError(f"Number of GPUs must be >= 1, got {device}")
+            available = torch.cuda.device_count()
+            if device > available:
+                raise ValueError(f"Requested {device} GPUs but only {available} available")
+            return list(range(device))
+        if isinstance(device, list):
+            if not device:
+                raise ValueError("Device list cannot be empty")
+            available = torch.cuda.device_count()
+            for gpu_id in device:
+                if not isinstance(gpu_id, int):
+                    raise ValueError(f"Device IDs must be integers, got {type(gpu_id)}")
+                if gpu_id < 0 or gpu_id >= available:
+                    raise ValueError(f"Device ID {gpu_id} out of range [0, {available - 1}]")
+            return device
+        if isinstance(device, str):
+            if device.strip().lower() in ("auto", "cuda", "rocm"):
+                return [0]
+            raise ValueError(
+                f"Unsupported device string '{device}'. Use List[int], int, or 'auto'/'cuda'."
+            )
+        raise ValueError(f"Invalid device type: {type(device)}. Expected List[int], int, or str.")
+
+    def gamma_table(self, B: int) -> torch.Tensor:
+        """Relay-leg memory strengths: [R, V] (broadcast) or [R, B, V] (per_row).
+
+        Row l is what the reference writes into ``gamma`` at relay leg l
+        (decoder.py:3684-3700): explicit row (l+1) % K, else a draw seeded
+        ``seed + l`` on the decoding device.
+        """
+        R, V = self.num_sets, self.V
+        per_row = self._explicit_gammas is None and self.gamma_draw == "per_row"
+        key = ("per_row", B) if per_row else ("broadcast",)
+        cached = self._gamma_cache.get(key)
+        if cached is not None:
+            return cached
+        dev = self.device
+        if R == 0:
+            table = torch.zeros(1, V, dtype=self.dtype, device=dev)
+        elif self._explicit_gammas is not None:
+            K = int(self._explicit_gammas.shape[0])
+            idx = torch.tensor([(l + 1) % K for l in range(R)], device=dev)
+            table = self._explicit_gammas.index_select(0, idx).contiguous()
+        else:
+            rows = B if per_row else 1
+            draws = []
+            for leg in range(R):
+                if self._gamma_by_error is not None:
+                    g = sample_gamma_per_variable(rows, V, self._gamma_by_error, dev, self.seed + leg)
+                else:
+                    gmin, gmax = self.gamma_dist_interval
+                    g = sample_gamma_uniform(rows, V, gmin, gmax, dev, self.seed + leg)
+                draws.append(g.to(self.dtype))
+            table = torch.stack(draws, 0).contiguous()
+            if not per_row:
+                table = table.reshape(R, V)
+        if per_row:
+            self._gamma_cache = {k: v for k, v in self._gamma_cache.items() if k[0] != "per_row"}
+        self._gamma_cache[key] = table
+        return table
+
+    def _alpha_table(self, n: int) -> torch.Tensor:
+        n = max(n, 1)
+        t = self._alpha_cache.get(n)
+        if t is None:
+            a = alpha_schedule(self.normalized_min_sum_alpha, self.alpha_iteration_scaling_factor, n)
+            t = torch.from_numpy(a).to(self.device)
+            self._alpha_cache[n] = t
+        return t
+
+    def plan(self, B: int) -> _native.PelPlan:
+        p = self._plans.get(B)
+        if p is None:
+            p = self._handle.plan(B, self._block_size, self._placement)
+            self._plans[B] = p
+        return p
+
+    def _workspace_for(self, nbytes: int) -> torch.Tensor:
+        if self._workspace is None or self._workspace.numel() < nbytes:
+            self._workspace = torch.empty(int(nbytes), dtype=torch.uint8, device=self.device)
+        return self._workspace
+
+    def override_llr_columns(self, cols, value):
+        """Write prior LLRs (and selection weights) of ``cols`` directly, bypassing
+        the prior clip; returns a restore callable (decoder.py:786-835)."""
+        cols = np.asarray(cols, dtype=np.int64)
+        lpr = self.log_prior_ratios_per_gpu[0]
+        saved_lpr = lpr[0, cols].clone()
+        saved_wj = self.wj[cols].clone()
+        lpr[0, cols] = torch.as_tensor(value, dtype=lpr.dtype, device=lpr.device)
+        self.wj[cols] = torch.as_tensor(value, dtype=self.wj.dtype, device=self.wj.device)
+
+        def _restore():
+            lpr[0, cols] = saved_lpr
+            self.wj[cols] = saved_wj
+
+        return _restore
+
+    # ------------------------------------------------------------------ decode
+    def decode(
+        self,
+        syndromes: torch.Tensor,
+        *,
+        init_posterior: Optional[torch.Tensor] = None,
+        init_gamma: Optional[torch.Tensor] = None,
+        skip_relay_legs: bool = False,
+        pre_iter_override: Optional[int] = None,
+        return_last_posterior: bool = False,
+        prior_llr: Optional[torch.Tensor] = None,
+    ) -> Dict[str, torch.Tensor]:
+        """Decode ``syndromes [B, C]`` (uint8); returns the reference's result dict.
+
+        Keys: ``errors`` [B,V] uint8 (or [B, ceil(V/32)] int32 when
+        bitpack_output), ``weights`` [B] float32 (+inf without a solution),
+        ``valid_mask`` [B] bool (found_count >= stop_nconv), ``iterations`` [B]
+        int32, ``found_count`` [B] int32, ``unconverged_indices`` [U] int32 and,
+        with ``return_last_posterior``, ``last_posterior`` [B,V].
+        """
+        if syndromes.dim() != 2:
+            raise ValueError(f"syndromes must be 2-D [B, C], got shape {tuple(syndromes.shape)}")
+        B = int(syndromes.shape[0])
+        if syndromes.shape[1] != self.C:
+            raise ValueError(f"Expected {self.C} syndrome bits, got {syndromes.shape[1]}")
+        V = self.V
+        if init_posterior is not None and tuple(init_posterior.shape) != (B, V):
+            raise ValueError(f"init_posterior shape {tuple(init_posterior.shape)} != ({B}, {V})")
+        if init_gamma is not None and tuple(init_gamma.shape) != (B, V):
+            raise ValueError(f"init_gamma shape {tuple(init_gamma.shape)} != ({B}, {V})")
+        if prior_llr is not None and tuple(prior_llr.shape) != (B, V):
+            raise ValueError(f"prior_llr shape {tuple(prior_llr.shape)} != ({B}, {V})")
+        if self.fast_max_unconverged_pct is not None:
+            return self._decode_legwise(
+                syndromes, init_posterior=init_posterior, init_gamma=init_gamma,
+                skip_relay_legs=skip_relay_legs, pre_iter_override=pre_iter_override,
+                return_last_posterior=return_last_posterior, prior_llr=prior_llr,
+            )
+        dev = self.device
+        pre_iter = self.pre_iter if pre_iter_override is None else int(pre_iter_override)
+        if pre_iter < 0:
+            raise ValueError("pre_iter_override must be >= 0")
+        with torch.cuda.device(dev):
+            syn = syndromes.to(device=dev, dtype=torch.uint8).contiguous()
+            if prior_llr is not None:
+                prior = prior_llr.to(device=dev, dtype=self.dtype).contiguous()
+                wj = torch.where(torch.isfinite(prior), prior, torch.zeros_like(prior)).contiguous()
+            else:
+                prior = self.log_prior_ratios_per_gpu[0].reshape(V)
+                wj = self.wj
+            ig = None
+            if init_gamma is not None:
+                ig = init_gamma.to(device=dev, dtype=self.dtype).contiguous()
+            ip = None
+            if init_posterior is not None:
+                ip = init_posterior.to(device=dev, dtype=self.dtype).contiguous()
+            gtab = self.gamma_table(B)
+            alpha = self._alpha_table(max(pre_iter, self.set_max_iter))
+            errors = torch.empty(B, V, dtype=torch.uint8, device=dev)
+            weights = torch.empty(B, dtype=self.dtype, device=dev)
+            iters = torch.empty(B, dtype=torch.int32, device=dev)
+            found = torch.empty(B, dtype=torch.int32, device=dev)
+            last_post = torch.empty(B, V, dtype=self.dtype, device=dev) if return_last_posterior else None
+            if B > 0:
+                self._launch(syn, prior, wj, prior_llr is not None, gtab, ig, ip, errors, weights,
+                             iters, found, last_post, pre_iter, skip_relay_legs, B)
+            return self._assemble(errors, weights, iters, found, last_post)
+
+    def _launch(self, syn, prior, wj, prior_batched, gtab, ig, ip, errors, weights, iters, found,
+                last_post, pre_iter, skip_relay_legs, B) -> None:
+        plan = self.plan(B)
+        ws = self._workspace_for(plan.workspace_bytes)
+        per_row = gtab.dim() == 3
+        params = _native.PelParams(
+            pre_iter=pre_iter,
+            num_sets=self.num_sets,
+            set_max_iter=self.set_max_iter,
+            stop_nconv=self.stop_nconv,
+            plain=int(self.algo == "plain"),
+            gamma0=float(self.gamma0),
+            use_alpha=int(self.normalized_min_sum_alpha is not None),
+            use_beta=int(self.offset_min_sum_beta is not None),
+            beta=float(self.offset_min_sum_beta or 0.0),
+            alpha_t=self._alpha_table(max(pre_iter, self.set_max_iter)).data_ptr(),
+            last_leg=int(self.unconverged_correction == "last_leg"),
+            skip_relay_legs=int(bool(skip_relay_legs)),
+        )
+        batch = _native.PelBatch(
+            B=B,
+            syndromes=syn.data_ptr(),
+            prior_llr=prior.data_ptr(),
+            wj=wj.data_ptr(),
+            prior_batched=int(prior_batched),
+            gamma_legs=gtab.data_ptr(),
+            gamma_per_row=int(per_row),
+            init_gamma=_ptr(ig),
+            init_posterior=_ptr(ip),
+            errors=errors.data_ptr(),
+            weights=weights.data_ptr(),
+            iterations=iters.data_ptr(),
+            found_count=found.data_ptr(),
+            last_posterior=_ptr(last_post),
+            workspace=ws.data_ptr(),
+        )
+        cur = torch.cuda.current_stream(self.device)
+        ev = self.kernel_events
+        if ev is not None:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(cur)
+        _native.check(
+            _native.lib().pel_decode(self._handle.h, ctypes.byref(params), ctypes.byref(batch),
+                                     ctypes.byref(plan), ctypes.c_void_p(cur.cuda_stream)),
+            "pel_decode",
+        )
+        if ev is not None:
+            e1.record(cur)
+            ev.append((e0, e1))
+
+    def _assemble(self, errors, weights, iters, found, last_post) -> Dict[str, torch.Tensor]:
+        unconverged = torch.nonzero(found < self.stop_nconv, as_tuple=False).flatten().to(torch.int32)
+        out = {
+            "errors": bitpack_errors(errors) if self.bitpack_output else errors,
+            "weights": weights,
+            "valid_mask": found >= self.stop_nconv,
+            "iterations": iters,
+            "found_count": found,
+            "unconverged_indices": unconverged,
+        }
+        if last_post is not None:
+            out["last_posterior"] = last_post
+        return out
+
+    def _decode_legwise(self, syndromes, **kw):
+        raise NotImplementedError(
+            "fast_max_unconverged_pct needs the leg-synchronous schedule, which is not built yet"
+        )
+
+    def get_stats(self) -> Dict:
+        return {
+            "graph": self.graph.get_stats(),
+            "parameters": {
+                "pre_iter": self.pre_iter,
+                "num_sets": self.num_sets,
+                "set_max_iter": self.set_max_iter,
+                "stop_nconv": self.stop_nconv,
+                "gamma0": self.gamma0,
+                "gamma_dist_interval": self.gamma_dist_interval,
+                "algo": self.algo,
+            },
+        }

@@ -1,0 +1,249 @@
+"""HIP decoder vs the CPU oracle, through the C ABI, on seeded inputs.
+
+Bar: bit-exact errors, iterations, found_count and weights, and bit-exact
+posteriors (both sides compute the same fp32 operations in the same order).
+Cases follow the reference's own tests (tests/test_decoder_fixes.py,
+tests/test_unconverged_correction.py, tests/test_triton_against_rust.py)
+plus the two decoding workloads: the [[72,12,6]] bivariate bicycle code and
+the 936 x 8785 banded circuit-shaped matrix.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from pelagos_amd import RelayBPDecoder, codes, utils
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("errors", "iterations", "found_count", "weights")
+
+
+def _decoder(H, p, **kw):
+    args = dict(pre_iter=80, num_sets=100, set_max_iter=60, gamma0=0.65,
+                gamma_dist_interval=(-0.24, 0.66), stop_nconv=5, data_type="float32",
+                algo="relay", seed=0, device=[0])
+    args.update(kw)
+    return RelayBPDecoder(H, p, **args)
+
+
+def _oracle_for(dec: RelayBPDecoder, syn: np.ndarray, *, prior=None, init_gamma=None,
+                init_posterior=None, pre_iter=None, skip_relay_legs=False, want_posterior=True):
+    B = syn.shape[0]
+    lam = prior if prior is not None else dec.log_prior_ratios_per_gpu[0].reshape(-1).cpu().numpy()
+    gtab = dec.gamma_table(B).cpu().numpy()
+    return oracle.decode(
+        dec.graph.chk_ptr, dec.graph.edge_var, dec.V, syn, lam, gtab,
+        pre_iter=dec.pre_iter if pre_iter is None else pre_iter,
+        num_sets=dec.num_sets, set_max_iter=dec.set_max_iter, stop_nconv=dec.stop_nconv,
+        gamma0=dec.gamma0, plain=dec.algo == "plain", alpha=dec.normalized_min_sum_alpha,
+        alpha_scaling=dec.alpha_iteration_scaling_factor, beta=dec.offset_min_sum_beta,
+        last_leg=dec.unconverged_correction == "last_leg", skip_relay_legs=skip_relay_legs,
+        gamma_per_row=gtab.ndim == 3, init_gamma=init_gamma, init_posterior=init_posterior,
+        want_posterior=want_posterior,
+    )
+
+
+def _compare(out, ref, keys=KEYS + ("last_posterior",)):
+    for k in keys:
+        if k not in ref:
+            continue
+        got = out[k].cpu().numpy()
+        exp = ref[k]
+        if not np.array_equal(got, exp):
+            bad = np.nonzero((got != exp).reshape(got.shape[0], -1).any(1))[0]
+            raise AssertionError(f"{k}: {bad.size} shots differ, first {bad[:8].tolist()}")
+
+
+def _run(dec, syn, **kw):
+    print(f"[gpu] decode B={syn.shape[0]} C={dec.C} V={dec.V} plan={dec.plan(syn.shape[0]).block}",
+          flush=True)
+    out = dec.decode(torch.from_numpy(syn).to("cuda:0"), return_last_posterior=True, **kw)
+    torch.cuda.synchronize()
+    return out
+
+
+def _syndrome_ok(H, errors, syn):
+    return (codes.syndromes_of(H, errors) == syn).all(1)
+
+
+# --- the two decoding workloads ----------------------------------------------------------
+
+
+def test_bb_code_capacity_matches_oracle():
+    """[[72,12,6]] BB code, p = 0.03, 1024 shots, the full Relay-BP-S schedule."""
+    H = codes.bb_72_12_6()
+    p = np.full(72, 0.03)
+    err = codes.sample_errors(p, 1024, seed=0)
+    syn = codes.syndromes_of(H, err)
+    dec = _decoder(H, p)
+    out = _run(dec, syn)
+    ref = _oracle_for(dec, syn)
+    _compare(out, ref)
+    found = out["found_count"].cpu().numpy()
+    solved = np.isfinite(out["weights"].cpu().numpy())
+    assert _syndrome_ok(H, out["errors"].cpu().numpy()[solved], syn[solved]).all()
+    assert np.array_equal(out["valid_mask"].cpu().numpy(), found >= 5)
+
+
+@pytest.mark.parametrize("placement", [None, 1, 0])
+def test_banded_circuit_code_matches_oracle(placement):
+    """936 x 8785 banded matrix, log-uniform priors; all-LDS and spilled state."""
+    H, p = codes.banded_circuit_code(seed=0)
+    err = codes.sample_errors(p, 192, seed=0)
+    syn = codes.syndromes_of(H, err)
+    dec = _decoder(H, p, state_placement=placement)
+    out = _run(dec, syn)
+    ref = _oracle_for(dec, syn)
+    _compare(out, ref)
+
+
+# --- schedule options ------------------------------------------------------------------------
+
+
+@pytest.fixture(scope="module")
+def regular_case():
+    """(3,6)-regular code, non-uniform priors, 128 syndromes of increasing difficulty
+    (tests/test_decoder_fixes.py:89-97)."""
+    H = codes.regular_code(3)
+    V = H.shape[1]
+    priors = np.random.default_rng(4).uniform(0.01, 0.03, V)
+    scale = np.repeat([0.5, 1.0, 1.5, 3.0], 32)[:, None]
+    E = (np.random.default_rng(5).random((128, V)) < priors * scale).astype(np.uint8)
+    S = codes.syndromes_of(H, E)
+    return H, priors, S
+
+
+OPTION_CASES = {
+    "relay_default": dict(pre_iter=10, num_sets=7, set_max_iter=15, stop_nconv=3),
+    "plain": dict(algo="plain", pre_iter=30, num_sets=0, set_max_iter=20, stop_nconv=1),
+    "alpha_fixed": dict(normalized_min_sum_alpha=0.8, pre_iter=10, num_sets=5, set_max_iter=10),
+    "alpha_ramp": dict(normalized_min_sum_alpha=0.0, alpha_iteration_scaling_factor=2.0,
+                       pre_iter=10, num_sets=5, set_max_iter=10),
+    "beta": dict(offset_min_sum_beta=0.3, pre_iter=10, num_sets=5, set_max_iter=10),
+    "last_leg": dict(unconverged_correction="last_leg", pre_iter=5, num_sets=4, set_max_iter=5),
+    "no_pre_iter": dict(pre_iter=0, num_sets=6, set_max_iter=12, stop_nconv=2),
+    "per_row": dict(gamma_draw="per_row", pre_iter=5, num_sets=8, set_max_iter=10, stop_nconv=2),
+    "stop1": dict(pre_iter=30, num_sets=5, set_max_iter=20, stop_nconv=1),
+    "zero_legs": dict(pre_iter=30, num_sets=0, set_max_iter=20, stop_nconv=1),
+    "set_max_iter_0": dict(pre_iter=200, num_sets=1, set_max_iter=0, stop_nconv=1, gamma0=0.0),
+}
+
+
+@pytest.mark.parametrize("case", sorted(OPTION_CASES))
+def test_regular_code_options_match_oracle(regular_case, case):
+    H, priors, S = regular_case
+    dec = _decoder(H, priors, seed=7, **OPTION_CASES[case])
+    out = _run(dec, S)
+    ref = _oracle_for(dec, S)
+    _compare(out, ref)
+
+
+def test_explicit_gammas_and_by_error_match_oracle(regular_case):
+    H, priors, S = regular_case
+    V = H.shape[1]
+    gammas = np.random.default_rng(6).uniform(-0.24, 0.66, (3, V))
+    dec = _decoder(H, priors, pre_iter=10, num_sets=7, set_max_iter=15, stop_nconv=3,
+                   explicit_gammas=gammas, seed=7)
+    _compare(_run(dec, S), _oracle_for(dec, S))
+    intervals = np.stack([np.full(V, -0.3), np.linspace(0.3, 0.8, V)], axis=1)
+    dec2 = _decoder(H, priors, pre_iter=5, num_sets=10, set_max_iter=10, stop_nconv=2,
+                    gamma_dist_interval_by_error=intervals, seed=7)
+    _compare(_run(dec2, S), _oracle_for(dec2, S))
+
+
+def test_decode_overrides_match_oracle(regular_case):
+    """init_gamma, init_posterior, prior_llr, pre_iter_override, skip_relay_legs."""
+    H, priors, S = regular_case
+    B, V = S.shape[0], H.shape[1]
+    rng = np.random.default_rng(11)
+    dec = _decoder(H, priors, pre_iter=10, num_sets=6, set_max_iter=10, stop_nconv=2, seed=7)
+    ig = rng.uniform(-0.2, 0.7, (B, V)).astype(np.float32)
+    ip = (utils.compute_log_prior_ratios(priors)[None, :] * rng.uniform(0.5, 1.5, (B, V))).astype(np.float32)
+    pl = utils.compute_log_prior_ratios(priors[None, :] * rng.uniform(0.5, 2.0, (B, V)))
+    dev = "cuda:0"
+    out = dec.decode(torch.from_numpy(S).to(dev), init_gamma=torch.from_numpy(ig).to(dev),
+                     init_posterior=torch.from_numpy(ip).to(dev), prior_llr=torch.from_numpy(pl).to(dev),
+                     pre_iter_override=7, return_last_posterior=True)
+    ref = _oracle_for(dec, S, prior=pl, init_gamma=ig, init_posterior=ip, pre_iter=7)
+    _compare(out, ref)
+    out2 = dec.decode(torch.from_numpy(S).to(dev), skip_relay_legs=True, return_last_posterior=True)
+    ref2 = _oracle_for(dec, S, skip_relay_legs=True)
+    _compare(out2, ref2)
+
+
+def test_toy_codes_match_oracle():
+    """Triangle, high-degree, a 100-wide check, degree-1 and empty checks."""
+    dense = np.zeros((5, 120), np.uint8)
+    dense[0, :100] = 1          # 100-wide row: four sign words
+    dense[1, 100] = 1           # degree-1 check
+    dense[3, [5, 50, 110]] = 1  # row 2 stays empty
+    dense[4, 100:120] = 1
+    from scipy.sparse import csr_matrix
+
+    cases = [
+        (codes.triangle_code(), 0.01),
+        (codes.high_degree_code(), 0.01),
+        (csr_matrix(dense), 0.02),
+        (codes.repetition_code(9), 0.1),
+    ]
+    for H, pval in cases:
+        V = H.shape[1]
+        p = np.full(V, pval)
+        err = codes.sample_errors(p, 64, seed=3, scale=3.0)
+        syn = codes.syndromes_of(H, err)
+        dec = _decoder(H, p, pre_iter=20, num_sets=5, set_max_iter=10, stop_nconv=2)
+        _compare(_run(dec, syn), _oracle_for(dec, syn))
+
+
+def test_block_sizes_match_oracle():
+    """The same decode with every workgroup size."""
+    H, p = codes.banded_circuit_code(n_blocks=4, V=2700, seed=2)
+    err = codes.sample_errors(p, 64, seed=1, scale=2.0)
+    syn = codes.syndromes_of(H, err)
+    ref = None
+    for block in (64, 128, 256, 512, 1024):
+        dec = _decoder(H, p, block_size=block)
+        out = _run(dec, syn)
+        if ref is None:
+            ref = _oracle_for(dec, syn)
+        _compare(out, ref)
+
+
+# --- invariants of the reference tests -------------------------------------------------------
+
+
+def test_batch_subset_and_single_shot_invariance(regular_case):
+    """A shot decodes the same alone, in a subset, and in the full batch
+    (tests/test_decoder_fixes.py:184-268)."""
+    H, priors, S = regular_case
+    dec = _decoder(H, priors, pre_iter=10, num_sets=20, set_max_iter=20, stop_nconv=2, seed=7)
+    full = _run(dec, S)
+    rows = np.random.default_rng(4).permutation(S.shape[0])[:10]
+    part = _run(dec, S[rows])
+    for k in ("errors", "found_count", "iterations", "weights"):
+        assert torch.equal(full[k][torch.from_numpy(rows).to(full[k].device)], part[k]), k
+    one = _run(dec, S[rows[:1]])
+    for k in ("errors", "found_count", "iterations"):
+        assert torch.equal(full[k][int(rows[0])], one[k][0]), k
+
+
+def test_never_converged_counts_every_iteration():
+    """A syndrome no error pattern explains never converges and reports
+    pre_iter + num_sets * set_max_iter (tests/test_decoder_fixes.py:367-374)."""
+    dense = np.zeros((3, 6), np.uint8)
+    dense[0, :3] = 1
+    dense[1, 3:] = 1      # row 2 is empty: syndrome bit 1 there is unsatisfiable
+    from scipy.sparse import csr_matrix
+
+    H = csr_matrix(dense)
+    syn = np.array([[1, 0, 1], [0, 1, 1]], np.uint8)
+    dec = _decoder(H, np.full(6, 0.05), pre_iter=30, num_sets=5, set_max_iter=20, stop_nconv=1)
+    out = _run(dec, syn)
+    assert out["iterations"].cpu().tolist() == [130, 130]
+    assert out["found_count"].cpu().tolist() == [0, 0]
+    assert torch.isinf(out["weights"]).all()
+    _compare(out, _oracle_for(dec, syn))
