@@ -107,18 +107,44 @@ struct ShotState {
     int32_t any_iter;
 };
 
+// Values every thread of the workgroup holds identically (LDS broadcasts,
+// workgroup reductions) are passed through readfirstlane so the compiler
+// treats the branches they control as uniform.  Without this, a 64-thread
+// workgroup (whose barriers compile to nothing) can be structurized into
+// per-lane loops in which lanes stop moving in step.
+__device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ double uniform_f64(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(u & 0xffffffffu));
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// An opaque copy of the thread index.  Branches on a particular thread (the
+// workgroup leader, lane 0 of a wave) test this value so that the optimizer
+// cannot carry the outcome of one such test to the next one: jump-threading
+// "thread 0 stores results" into "thread 0 fetches the next shot" gives the
+// persistent loop a second back edge that other lanes take without lane 0,
+// which then never fetches a new shot.
+__device__ __forceinline__ unsigned opaque_tid() {
+    unsigned v;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "v"((unsigned)threadIdx.x));
+    return v;
+}
+
 template <int BLOCK>
 __device__ double block_sum(double v, double *s_red) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) s_red[wid] = v;
+    const int wid = threadIdx.x >> 6;
+    if ((opaque_tid() & 63u) == 0) s_red[wid] = v;
     __syncthreads();
     double tot = 0.0;
 #pragma unroll
     for (int w = 0; w < BLOCK / 64; ++w) tot += s_red[w];
     __syncthreads();
-    return tot;
+    return uniform_f64(tot);
 }
 
 // Check pass for row i.  first: messages were just reset (mu_e = lam0_j,
@@ -260,7 +286,7 @@ __device__ bool run_leg(const KArgs &a, int T, const GammaSrc &gs, const float *
         const bool first = (t == 0);
         int bad = 0;
         for (int i = threadIdx.x; i < C; i += BLOCK) bad |= check_row(a, i, first, alpha, l0, syn, LS, CS, SB);
-        bad = __syncthreads_or(bad);
+        bad = uniform_i32(__syncthreads_or(bad));
         if (!first && !bad) {  // iteration t-1 converged
             select_solution<BLOCK>(a, wj, LS, err, st, s_red);
             return true;
@@ -273,7 +299,7 @@ __device__ bool run_leg(const KArgs &a, int T, const GammaSrc &gs, const float *
     if (T > 0) {
         int bad = 0;
         for (int i = threadIdx.x; i < C; i += BLOCK) bad |= parity_row(a, i, syn, LS);
-        bad = __syncthreads_or(bad);
+        bad = uniform_i32(__syncthreads_or(bad));
         if (!bad) {
             select_solution<BLOCK>(a, wj, LS, err, st, s_red);
             return true;
@@ -295,8 +321,8 @@ __device__ void write_hard(const KArgs &a, const float2 *LS, uint8_t *err, bool 
 }
 
 template <int BLOCK>
-__device__ void decode_shot(const KArgs &a, int b, float2 *LS, float4 *CS, uint32_t *SB,
-                            double *s_red) {
+__device__ ShotState decode_shot(const KArgs &a, int b, float2 *LS, float4 *CS, uint32_t *SB,
+                                 double *s_red) {
     const int C = a.g.C, V = a.g.V;
     const uint8_t *syn = a.syn + (size_t)b * C;
     const size_t pofs = a.prior_batched ? (size_t)b * V : 0;
@@ -342,11 +368,7 @@ __device__ void decode_shot(const KArgs &a, int b, float2 *LS, float4 *CS, uint3
             lp[j] = ls.x + ls.y;
         }
     }
-    if (threadIdx.x == 0) {
-        a.weights[b] = st.best_w;
-        a.iters[b] = st.iters;
-        a.found[b] = st.found;
-    }
+    return st;
 }
 
 // PLACE bit0: LS in LDS; bit1: CS in LDS; bit2: SB in LDS.
@@ -359,13 +381,25 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
     float2 *LS = reinterpret_cast<float2 *>(((PLACE & 1) ? smem : slot) + a.ls_off);
     float4 *CS = reinterpret_cast<float4 *>(((PLACE & 2) ? smem : slot) + a.cs_off);
     uint32_t *SB = reinterpret_cast<uint32_t *>(((PLACE & 4) ? smem : slot) + a.sb_off);
+    // The leader writes the finished shot's scalars and takes the next ticket in
+    // one branch, so the loop has a single leader-only region.
+    int prev = -1;
+    ShotState st{0.f, 0, 0, 0};
     for (;;) {
-        if (threadIdx.x == 0) s_shot = atomicAdd(a.queue, 1);
+        if (opaque_tid() == 0) {
+            if (prev >= 0) {
+                a.weights[prev] = st.best_w;
+                a.iters[prev] = st.iters;
+                a.found[prev] = st.found;
+            }
+            s_shot = atomicAdd(a.queue, 1);
+        }
         __syncthreads();
-        const int b = s_shot;
+        const int b = uniform_i32(s_shot);
         __syncthreads();
         if (b >= a.B) break;
-        decode_shot<BLOCK>(a, b, LS, CS, SB, s_red);
+        st = decode_shot<BLOCK>(a, b, LS, CS, SB, s_red);
+        prev = b;
     }
 }
 
