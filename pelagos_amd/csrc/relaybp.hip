@@ -4,16 +4,17 @@
 //   * A workgroup owns one shot at a time and runs that shot's whole Relay-BP-S
 //     schedule -- pre-iter leg, then relay legs until stop_nconv solutions --
 //     with no host round trip.  When it finishes it takes the next shot from a
-//     device work counter, so the heavy tail of slow shots is absorbed by
+//     device work queue, so the heavy tail of slow shots is absorbed by
 //     whichever workgroups are free (the reference's per-lane freezing,
-//     kernels.py:745-847, becomes "this workgroup moves on").
+//     kernels.py:745-847, becomes "this workgroup moves on").  The queue hands
+//     out shots in decreasing syndrome weight, so the slow shots start early.
 //   * Per-shot state is kept in compressed min-sum form instead of per-edge
 //     messages (the reference's muT/nuT [E,B], decoder.py:1527-1529):
 //       variables  LS[j] = {lam_mixed_j, S_j}        (8 B)  S_j = sum of incoming nu
-//       checks     CS[i] = {m1, m2, argbit, parity}  (16 B) outgoing magnitudes
+//       checks     CS[r] = {m1, m2, argbit, parity}  (16 B) outgoing magnitudes
 //       sign words SB     one bit per edge: sign of the variable->check message
 //     Every message is a pure function of this state, bit for bit:
-//       nu_e = (parity_i ^ sign_e ? -1 : +1) * (bit(e) == argbit_i ? m2_i : m1_i)
+//       nu_e = (parity_r ^ sign_e ? -1 : +1) * (bit(e) == argbit_r ? m2_r : m1_r)
 //       mu_e = lam_mixed_j + (S_j - nu_e)              (kernels.py:1339-1350)
 //       post_j = lam_mixed_j + S_j                       (kernels.py:1328)
 //     At 936 x 8785 this is 90 KB per shot, which fits the 160 KB LDS of a CU,
@@ -24,6 +25,11 @@
 //     iteration's hard decision) and variable pass (a thread per variable,
 //     sequential sum over its edges in ascending edge id), separated by
 //     workgroup barriers.  No cross-lane reductions are needed.
+//   * Graph indices are stored as 64-wide ELL slices (one slice per wave):
+//     check rows sorted by degree, so a wave's rows have similar lengths, and
+//     element k of every row of a slice stored contiguously, so the wave's index
+//     load for step k is one coalesced 256-byte read.  Variables keep their
+//     natural order (per-variable priors and gammas load coalesced).
 //
 // Floating point follows the reference kernels exactly (fp32, no FMA
 // contraction -- the build passes -ffp-contract=off and the memory mix uses
@@ -36,7 +42,7 @@
 #include <string.h>
 
 #include <algorithm>
-#include <mutex>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -58,13 +64,21 @@ int fail(int code, const std::string &msg) {
             return fail(PEL_EDEVICE, std::string(#call) + ": " + hipGetErrorString(_e));   \
     } while (0)
 
+constexpr int kSlice = 64;  // rows / variables per ELL slice (one wave)
+
 struct GraphDev {
-    int32_t C, V, E, n_words;
-    const int32_t *row_ptr;   // [C+1]
-    const int32_t *row_var;   // [E]   variable of each edge (CSR order)
-    const int32_t *row_wofs;  // [C]   first sign word of each row
-    const int32_t *var_ptr;   // [V+1]
-    const int2 *var_edge;     // [E]   {check, sign bit index}, per variable in ascending edge id
+    int32_t C, V, E;
+    int32_t sb_words;           // sign words per shot (slice-padded)
+    // check side, by row position (rows sorted by decreasing degree)
+    const int32_t *row_orig;    // [C]  original check index
+    const int32_t *row_deg;     // [C]
+    const int32_t *rslice_idx;  // [C/64] first element of the slice in row_var
+    const int32_t *rslice_sb;   // [C/64] first sign word of the slice
+    const int32_t *row_var;     // ELL: element (k, lane) of slice s at rslice_idx[s] + 64k + lane
+    // variable side, natural order
+    const int32_t *var_deg;     // [V]
+    const int32_t *vslice_idx;  // [V/64]
+    const int2 *var_edge;       // ELL of {row position, sign bit address}, ascending edge id
 };
 
 struct KArgs {
@@ -88,6 +102,7 @@ struct KArgs {
     int32_t *found;
     float *last_post;
     int32_t *queue;
+    const int32_t *order;  // [B] shot taken at each queue position
     uint8_t *spill;
     int64_t spill_stride;
     uint32_t ls_off, cs_off, sb_off;  // byte offsets (LDS or spill slot, by placement)
@@ -109,9 +124,7 @@ struct ShotState {
 
 // Values every thread of the workgroup holds identically (LDS broadcasts,
 // workgroup reductions) are passed through readfirstlane so the compiler
-// treats the branches they control as uniform.  Without this, a 64-thread
-// workgroup (whose barriers compile to nothing) can be structurized into
-// per-lane loops in which lanes stop moving in step.
+// treats the branches they control as uniform.
 __device__ __forceinline__ int uniform_i32(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __device__ __forceinline__ double uniform_f64(double v) {
@@ -147,23 +160,24 @@ __device__ double block_sum(double v, double *s_red) {
     return uniform_f64(tot);
 }
 
-// Check pass for row i.  first: messages were just reset (mu_e = lam0_j,
-// kernels.py:981-1000), so nothing is reconstructed and no parity is tested.
-// Otherwise reconstructs mu_e from the state of the previous iteration and also
-// returns 1 if the previous iteration's hard decision violates check i
-// (kernels.py:509-569 fused into the check pass).
-__device__ __forceinline__ int check_row(const KArgs &a, int i, bool first, float alpha,
+// Check pass for row position pos.  first: messages were just reset
+// (mu_e = lam0_j, kernels.py:981-1000), so nothing is reconstructed and no
+// parity is tested.  Otherwise reconstructs mu_e from the state of the
+// previous iteration and also returns 1 if the previous iteration's hard
+// decision violates the check (kernels.py:509-569 fused into the check pass).
+__device__ __forceinline__ int check_row(const KArgs &a, int pos, bool first, float alpha,
                                          const float *__restrict__ l0, const uint8_t *__restrict__ syn,
                                          const float2 *LS, float4 *CS, uint32_t *SB) {
-    const int r0 = a.g.row_ptr[i];
-    const int deg = a.g.row_ptr[i + 1] - r0;
-    const uint32_t s_i = syn[i] & 1u;
+    const int deg = a.g.row_deg[pos];
+    const uint32_t s_i = syn[a.g.row_orig[pos]] & 1u;
     if (deg == 0) return first ? 0 : (int)s_i;
-    const int wofs = a.g.row_wofs[i];
+    const int slice = pos >> 6, lane = pos & 63;
+    const int32_t *__restrict__ vars = a.g.row_var + a.g.rslice_idx[slice] + lane;
+    const int sbase = a.g.rslice_sb[slice] + lane;
     float om1 = 0.f, om2 = 0.f;
     uint32_t oarg = 0xffffffffu, opar = 0;
     if (!first) {
-        const float4 cs = CS[i];
+        const float4 cs = CS[pos];
         om1 = cs.x;
         om2 = cs.y;
         oarg = __float_as_uint(cs.z);
@@ -171,21 +185,21 @@ __device__ __forceinline__ int check_row(const KArgs &a, int i, bool first, floa
     }
     float m1 = INFINITY, m2 = INFINITY;
     uint32_t arg = 0, neg = 0, hpar = 0;
-    const int32_t *__restrict__ vars = a.g.row_var + r0;
     for (int k0 = 0; k0 < deg; k0 += 32) {
         const int n = min(32, deg - k0);
-        const uint32_t wbase = (uint32_t)(wofs + (k0 >> 5));
-        const uint32_t wold = first ? 0u : SB[wbase];
+        const uint32_t word = (uint32_t)(sbase + (k0 >> 5) * kSlice);
+        const uint32_t wold = first ? 0u : SB[word];
         uint32_t wnew = 0;
+#pragma unroll 4
         for (int kk = 0; kk < n; ++kk) {
-            const int j = vars[k0 + kk];
+            const int j = vars[(k0 + kk) * kSlice];
             float mu;
             if (first) {
                 mu = l0[j];
             } else {
                 const float2 ls = LS[j];
                 hpar ^= (ls.x + ls.y <= 0.f) ? 1u : 0u;
-                const uint32_t bit = (wbase << 5) + (uint32_t)kk;
+                const uint32_t bit = (word << 5) | (uint32_t)kk;
                 const float mag = (bit == oarg) ? om2 : om1;
                 const float nu = (((wold >> kk) ^ opar) & 1u) ? -mag : mag;
                 mu = ls.x + (ls.y - nu);
@@ -197,12 +211,12 @@ __device__ __forceinline__ int check_row(const KArgs &a, int i, bool first, floa
             if (am < m1) {
                 m2 = m1;
                 m1 = am;
-                arg = (wbase << 5) + (uint32_t)kk;
+                arg = (word << 5) | (uint32_t)kk;
             } else if (am < m2) {
                 m2 = am;
             }
         }
-        SB[wbase] = wnew;
+        SB[word] = wnew;
     }
     float o1 = m1, o2 = m2;
     if (deg == 1) o1 = o2 = 0.f;  // degree-1 checks send zero (kernels.py:1192-1193)
@@ -214,17 +228,20 @@ __device__ __forceinline__ int check_row(const KArgs &a, int i, bool first, floa
         o1 = fmaxf(o1 - a.beta, 0.f);
         o2 = fmaxf(o2 - a.beta, 0.f);
     }
-    CS[i] = make_float4(o1, o2, __uint_as_float(arg), __uint_as_float((neg ^ s_i) & 1u));
+    CS[pos] = make_float4(o1, o2, __uint_as_float(arg), __uint_as_float((neg ^ s_i) & 1u));
     return first ? 0 : (int)((hpar ^ s_i) & 1u);
 }
 
-// Parity of the current hard decision on row i (after the last iteration of a leg).
-__device__ __forceinline__ int parity_row(const KArgs &a, int i, const uint8_t *__restrict__ syn,
+// Parity of the current hard decision on row position pos (after the last
+// iteration of a leg).
+__device__ __forceinline__ int parity_row(const KArgs &a, int pos, const uint8_t *__restrict__ syn,
                                           const float2 *LS) {
-    const int r0 = a.g.row_ptr[i], r1 = a.g.row_ptr[i + 1];
-    uint32_t hpar = syn[i] & 1u;
-    for (int e = r0; e < r1; ++e) {
-        const float2 ls = LS[a.g.row_var[e]];
+    const int deg = a.g.row_deg[pos];
+    uint32_t hpar = syn[a.g.row_orig[pos]] & 1u;
+    const int32_t *__restrict__ vars = a.g.row_var + a.g.rslice_idx[pos >> 6] + (pos & 63);
+#pragma unroll 4
+    for (int k = 0; k < deg; ++k) {
+        const float2 ls = LS[vars[k * kSlice]];
         hpar ^= (ls.x + ls.y <= 0.f) ? 1u : 0u;
     }
     return (int)hpar;
@@ -238,14 +255,16 @@ __device__ __forceinline__ void var_update(const KArgs &a, int j, const GammaSrc
     const float2 ls = LS[j];
     const float post_prev = ls.x + ls.y;
     const float lm = __fadd_rn(__fmul_rn(1.0f - g, l0[j]), __fmul_rn(g, post_prev));
+    const int deg = a.g.var_deg[j];
+    const int2 *__restrict__ ve = a.g.var_edge + a.g.vslice_idx[j >> 6] + (j & 63);
     float s = 0.f;
-    const int q1 = a.g.var_ptr[j + 1];
-    for (int q = a.g.var_ptr[j]; q < q1; ++q) {
-        const int2 ve = a.g.var_edge[q];
-        const float4 cs = CS[ve.x];
-        const uint32_t w = SB[(uint32_t)ve.y >> 5];
-        const float mag = ((uint32_t)ve.y == __float_as_uint(cs.z)) ? cs.y : cs.x;
-        const float nu = (((w >> (ve.y & 31)) ^ __float_as_uint(cs.w)) & 1u) ? -mag : mag;
+#pragma unroll 4
+    for (int q = 0; q < deg; ++q) {
+        const int2 e = ve[q * kSlice];
+        const float4 cs = CS[e.x];
+        const uint32_t w = SB[(uint32_t)e.y >> 5];
+        const float mag = ((uint32_t)e.y == __float_as_uint(cs.z)) ? cs.y : cs.x;
+        const float nu = (((w >> (e.y & 31)) ^ __float_as_uint(cs.w)) & 1u) ? -mag : mag;
         s += nu;
     }
     LS[j] = make_float2(lm, s);
@@ -392,7 +411,8 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
                 a.iters[prev] = st.iters;
                 a.found[prev] = st.found;
             }
-            s_shot = atomicAdd(a.queue, 1);
+            const int t = atomicAdd(a.queue, 1);
+            s_shot = t < a.B ? a.order[t] : a.B;
         }
         __syncthreads();
         const int b = uniform_i32(s_shot);
@@ -403,7 +423,38 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
     }
 }
 
-__global__ void reset_queue_kernel(int32_t *q) { *q = 0; }
+// Work-queue order: shots by decreasing syndrome weight (a counting sort in
+// one workgroup over C + 1 weight buckets).  Heavier syndromes tend to need
+// more iterations, so starting them first shortens the tail of the launch.
+// Also resets the queue counter.
+constexpr int kOrderBlock = 1024;
+
+__global__ void __launch_bounds__(kOrderBlock) order_shots_kernel(const uint8_t *syn, int32_t B, int32_t C,
+                                                                  int32_t *bucket, int32_t *weight,
+                                                                  int32_t *order, int32_t *queue) {
+    const int tid = threadIdx.x;
+    for (int k = tid; k <= C; k += kOrderBlock) bucket[k] = 0;
+    __syncthreads();
+    for (int b = tid; b < B; b += kOrderBlock) {
+        const uint8_t *s = syn + (size_t)b * C;
+        int w = 0;
+        for (int i = 0; i < C; ++i) w += s[i] & 1;
+        weight[b] = C - w;  // bucket 0 = heaviest
+        atomicAdd(&bucket[C - w], 1);
+    }
+    __syncthreads();
+    if (opaque_tid() == 0) {
+        int run = 0;
+        for (int k = 0; k <= C; ++k) {
+            const int c = bucket[k];
+            bucket[k] = run;
+            run += c;
+        }
+        *queue = 0;
+    }
+    __syncthreads();
+    for (int b = tid; b < B; b += kOrderBlock) order[atomicAdd(&bucket[weight[b]], 1)] = b;
+}
 
 typedef void (*KernelFn)(KArgs);
 
@@ -429,15 +480,40 @@ KernelFn pick_kernel(int block, int place) {
 
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
+struct StateSizes {
+    int64_t sz[3];  // LS, CS, SB bytes per shot
+};
+
 }  // namespace
 
 struct pel_graph {
     int device;
-    int32_t C, V, E, n_words;
-    int32_t max_row_deg;
-    void *dev_mem;  // one allocation holding every array below
+    int32_t C, V, E, sb_words, max_row_deg;
+    void *dev_mem;  // one allocation holding every array of d
     GraphDev d;
+    StateSizes state() const {
+        return {{align_up(8LL * V, 16), align_up(16LL * C, 16), align_up(4LL * std::max(sb_words, 1), 16)}};
+    }
 };
+
+namespace {
+
+// Workspace layout: [0,256) queue | order [B] | weight [B] | bucket [C+1] | spill slots
+struct WsLayout {
+    int64_t order, weight, bucket, spill, total_fixed;
+};
+
+WsLayout ws_layout(const pel_graph *g, int64_t B) {
+    WsLayout w;
+    w.order = 256;
+    w.weight = align_up(w.order + 4 * std::max<int64_t>(B, 1), 256);
+    w.bucket = align_up(w.weight + 4 * std::max<int64_t>(B, 1), 256);
+    w.spill = align_up(w.bucket + 4LL * (g->C + 1), 256);
+    w.total_fixed = w.spill;
+    return w;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -453,71 +529,119 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
     if (C <= 0 || V <= 0 || E < 0) return fail(PEL_EINVAL, "pel_graph_create: empty matrix");
     if (chk_ptr[0] != 0 || chk_ptr[C] != E)
         return fail(PEL_EINVAL, "pel_graph_create: chk_ptr must start at 0 and end at E");
-    std::vector<int32_t> wofs(C);
-    int32_t nw = 0, maxdeg = 0;
+    int32_t maxdeg = 0;
     for (int32_t i = 0; i < C; ++i) {
         const int32_t d = chk_ptr[i + 1] - chk_ptr[i];
         if (d < 0) return fail(PEL_EINVAL, "pel_graph_create: chk_ptr not non-decreasing");
-        wofs[i] = nw;
-        nw += (d + 31) / 32;
         maxdeg = std::max(maxdeg, d);
     }
-    std::vector<int32_t> var_ptr(V + 1, 0);
-    for (int32_t e = 0; e < E; ++e) {
-        const int32_t j = chk_var[e];
-        if (j < 0 || j >= V) return fail(PEL_EINVAL, "pel_graph_create: column index out of range");
-        var_ptr[j + 1]++;
+    for (int32_t e = 0; e < E; ++e)
+        if (chk_var[e] < 0 || chk_var[e] >= V)
+            return fail(PEL_EINVAL, "pel_graph_create: column index out of range");
+
+    // Row positions: checks sorted by decreasing degree (stable).
+    std::vector<int32_t> rorig(C), rpos(C), rdeg(C);
+    std::iota(rorig.begin(), rorig.end(), 0);
+    std::stable_sort(rorig.begin(), rorig.end(), [&](int32_t x, int32_t y) {
+        return (chk_ptr[x + 1] - chk_ptr[x]) > (chk_ptr[y + 1] - chk_ptr[y]);
+    });
+    for (int32_t p = 0; p < C; ++p) {
+        rpos[rorig[p]] = p;
+        rdeg[p] = chk_ptr[rorig[p] + 1] - chk_ptr[rorig[p]];
     }
-    for (int32_t j = 0; j < V; ++j) var_ptr[j + 1] += var_ptr[j];
+    const int32_t nrs = (C + kSlice - 1) / kSlice;
+    std::vector<int32_t> rs_idx(nrs), rs_sb(nrs);
+    int64_t nidx = 0, nword = 0;
+    for (int32_t s = 0; s < nrs; ++s) {
+        const int32_t w = rdeg[s * kSlice];  // widest row of the slice comes first
+        rs_idx[s] = (int32_t)nidx;
+        rs_sb[s] = (int32_t)nword;
+        nidx += (int64_t)w * kSlice;
+        nword += (int64_t)((w + 31) / 32) * kSlice;
+    }
+    if (nidx > INT32_MAX / 2 || nword * 32 > INT32_MAX)
+        return fail(PEL_EINVAL, "pel_graph_create: graph too large for 32-bit indexing");
+    std::vector<int32_t> row_var(std::max<int64_t>(nidx, 1), 0);
+    for (int32_t p = 0; p < C; ++p) {
+        const int32_t s = p / kSlice, lane = p % kSlice, i = rorig[p];
+        for (int32_t k = 0; k < rdeg[p]; ++k)
+            row_var[rs_idx[s] + (int64_t)k * kSlice + lane] = chk_var[chk_ptr[i] + k];
+    }
+    // Variable side: natural order, edges in ascending edge id (= ascending check).
+    std::vector<int32_t> vdeg(V, 0);
+    for (int32_t e = 0; e < E; ++e) vdeg[chk_var[e]]++;
+    const int32_t nvs = (V + kSlice - 1) / kSlice;
+    std::vector<int32_t> vs_idx(nvs);
+    int64_t nve = 0;
+    for (int32_t s = 0; s < nvs; ++s) {
+        int32_t w = 0;
+        for (int32_t j = s * kSlice; j < std::min(V, (s + 1) * kSlice); ++j) w = std::max(w, vdeg[j]);
+        vs_idx[s] = (int32_t)nve;
+        nve += (int64_t)w * kSlice;
+    }
+    if (nve > INT32_MAX / 2) return fail(PEL_EINVAL, "pel_graph_create: graph too large for 32-bit indexing");
+    std::vector<int2> var_edge(std::max<int64_t>(nve, 1), make_int2(0, 0));
     std::vector<int32_t> fill(V, 0);
-    std::vector<int2> var_edge(std::max(E, 1));
     for (int32_t i = 0; i < C; ++i) {
+        const int32_t p = rpos[i], s = p / kSlice, lane = p % kSlice;
         for (int32_t e = chk_ptr[i]; e < chk_ptr[i + 1]; ++e) {
-            const int32_t j = chk_var[e];
-            const int32_t k = e - chk_ptr[i];
-            var_edge[var_ptr[j] + fill[j]++] = make_int2(i, wofs[i] * 32 + k);
+            const int32_t j = chk_var[e], k = e - chk_ptr[i];
+            const int32_t word = rs_sb[s] + (k / 32) * kSlice + lane;
+            var_edge[vs_idx[j / kSlice] + (int64_t)fill[j]++ * kSlice + (j % kSlice)] =
+                make_int2(p, word * 32 + (k % 32));
         }
     }
-    // One device allocation: row_ptr | row_var | row_wofs | var_ptr | var_edge
-    const int64_t o_rp = 0;
-    const int64_t o_rv = align_up(o_rp + 4LL * (C + 1), 256);
-    const int64_t o_rw = align_up(o_rv + 4LL * std::max(E, 1), 256);
-    const int64_t o_vp = align_up(o_rw + 4LL * C, 256);
-    const int64_t o_ve = align_up(o_vp + 4LL * (V + 1), 256);
-    const int64_t total = align_up(o_ve + 8LL * std::max(E, 1), 256);
+    // One device allocation for every array.
+    int64_t off = 0;
+    auto take = [&](int64_t bytes) {
+        const int64_t o = off;
+        off = align_up(off + std::max<int64_t>(bytes, 4), 256);
+        return o;
+    };
+    const int64_t o_ro = take(4LL * C), o_rd = take(4LL * C), o_rsi = take(4LL * nrs), o_rsb = take(4LL * nrs),
+                  o_rv = take(4LL * (int64_t)row_var.size()), o_vd = take(4LL * V), o_vsi = take(4LL * nvs),
+                  o_ve = take(8LL * (int64_t)var_edge.size());
+    const int64_t total = off;
+    std::vector<uint8_t> host((size_t)total, 0);
+    memcpy(host.data() + o_ro, rorig.data(), 4ULL * C);
+    memcpy(host.data() + o_rd, rdeg.data(), 4ULL * C);
+    memcpy(host.data() + o_rsi, rs_idx.data(), 4ULL * nrs);
+    memcpy(host.data() + o_rsb, rs_sb.data(), 4ULL * nrs);
+    memcpy(host.data() + o_rv, row_var.data(), 4ULL * row_var.size());
+    memcpy(host.data() + o_vd, vdeg.data(), 4ULL * V);
+    memcpy(host.data() + o_vsi, vs_idx.data(), 4ULL * nvs);
+    memcpy(host.data() + o_ve, var_edge.data(), 8ULL * var_edge.size());
+
     pel_graph *g = new pel_graph();
-    PEL_HIP(hipGetDevice(&g->device));
-    hipError_t e = hipMalloc(&g->dev_mem, (size_t)total);
+    hipError_t e = hipGetDevice(&g->device);
+    if (e == hipSuccess) e = hipMalloc(&g->dev_mem, (size_t)total);
     if (e != hipSuccess) {
         delete g;
-        return fail(PEL_ENOMEM, std::string("pel_graph_create: hipMalloc: ") + hipGetErrorString(e));
+        return fail(PEL_ENOMEM, std::string("pel_graph_create: ") + hipGetErrorString(e));
     }
-    uint8_t *base = static_cast<uint8_t *>(g->dev_mem);
-    std::vector<uint8_t> host((size_t)total, 0);
-    memcpy(host.data() + o_rp, chk_ptr, 4ULL * (C + 1));
-    if (E > 0) memcpy(host.data() + o_rv, chk_var, 4ULL * E);
-    memcpy(host.data() + o_rw, wofs.data(), 4ULL * C);
-    memcpy(host.data() + o_vp, var_ptr.data(), 4ULL * (V + 1));
-    memcpy(host.data() + o_ve, var_edge.data(), 8ULL * std::max(E, 1));
-    e = hipMemcpy(base, host.data(), (size_t)total, hipMemcpyHostToDevice);
+    e = hipMemcpy(g->dev_mem, host.data(), (size_t)total, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         (void)hipFree(g->dev_mem);
         delete g;
         return fail(PEL_EDEVICE, std::string("pel_graph_create: hipMemcpy: ") + hipGetErrorString(e));
     }
+    const uint8_t *base = static_cast<const uint8_t *>(g->dev_mem);
     g->C = C;
     g->V = V;
     g->E = E;
-    g->n_words = nw;
+    g->sb_words = (int32_t)nword;
     g->max_row_deg = maxdeg;
     g->d.C = C;
     g->d.V = V;
     g->d.E = E;
-    g->d.n_words = nw;
-    g->d.row_ptr = reinterpret_cast<const int32_t *>(base + o_rp);
+    g->d.sb_words = (int32_t)nword;
+    g->d.row_orig = reinterpret_cast<const int32_t *>(base + o_ro);
+    g->d.row_deg = reinterpret_cast<const int32_t *>(base + o_rd);
+    g->d.rslice_idx = reinterpret_cast<const int32_t *>(base + o_rsi);
+    g->d.rslice_sb = reinterpret_cast<const int32_t *>(base + o_rsb);
     g->d.row_var = reinterpret_cast<const int32_t *>(base + o_rv);
-    g->d.row_wofs = reinterpret_cast<const int32_t *>(base + o_rw);
-    g->d.var_ptr = reinterpret_cast<const int32_t *>(base + o_vp);
+    g->d.var_deg = reinterpret_cast<const int32_t *>(base + o_vd);
+    g->d.vslice_idx = reinterpret_cast<const int32_t *>(base + o_vsi);
     g->d.var_edge = reinterpret_cast<const int2 *>(base + o_ve);
     *out = g;
     return PEL_OK;
@@ -538,12 +662,13 @@ int pel_graph_destroy(pel_graph *g) {
 int pel_graph_info(const pel_graph *g, int32_t *C, int32_t *V, int32_t *E, int64_t *var_state_bytes,
                    int64_t *chk_state_bytes, int64_t *sign_bytes) {
     if (!g) return fail(PEL_EINVAL, "pel_graph_info: NULL graph");
+    const StateSizes s = g->state();
     if (C) *C = g->C;
     if (V) *V = g->V;
     if (E) *E = g->E;
-    if (var_state_bytes) *var_state_bytes = 8LL * g->V;
-    if (chk_state_bytes) *chk_state_bytes = 16LL * g->C;
-    if (sign_bytes) *sign_bytes = 4LL * g->n_words;
+    if (var_state_bytes) *var_state_bytes = s.sz[0];
+    if (chk_state_bytes) *chk_state_bytes = s.sz[1];
+    if (sign_bytes) *sign_bytes = s.sz[2];
     return PEL_OK;
 }
 
@@ -565,8 +690,8 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32
     }
     if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
         return fail(PEL_EINVAL, "pel_plan_decode: block must be 64, 128, 256, 512 or 1024");
-    const int64_t ls = align_up(8LL * g->V, 16), cs = align_up(16LL * g->C, 16),
-                  sb = align_up(4LL * std::max(g->n_words, 1), 16);
+    const StateSizes st = g->state();
+    const int64_t ls = st.sz[0], cs = st.sz[1], sb = st.sz[2];
     const int64_t static_lds = 8LL * (block / 64) + 64;
     const int64_t budget = (int64_t)max_lds - static_lds;
     int place;
@@ -583,23 +708,11 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32
         place = placement_override;
     }
     int64_t lds = 0, spill = 0;
-    uint32_t offs[3];
-    const int64_t sz[3] = {ls, cs, sb};
-    for (int k = 0; k < 3; ++k) {
-        if (place & (1 << k)) {
-            offs[k] = (uint32_t)lds;
-            lds += sz[k];
-        } else {
-            offs[k] = (uint32_t)spill;
-            spill += sz[k];
-        }
-    }
-    (void)offs;
+    for (int k = 0; k < 3; ++k) (place & (1 << k) ? lds : spill) += st.sz[k];
     KernelFn fn = pick_kernel(block, place);
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        (void)e;
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
         (void)hipGetLastError();
     }
     int per_cu = 0;
@@ -608,12 +721,13 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32
     if (per_cu <= 0) return fail(PEL_EDEVICE, "pel_plan_decode: kernel cannot be resident (LDS/registers)");
     int64_t grid = (int64_t)per_cu * n_cu;
     grid = std::min<int64_t>(grid, std::max<int32_t>(B, 1));
+    const WsLayout w = ws_layout(g, B);
     out->block = block;
     out->grid = (int32_t)grid;
     out->placement = place;
     out->lds_bytes = (int32_t)lds;
     out->spill_stride = align_up(spill, 256);
-    out->workspace_bytes = 256 + out->spill_stride * grid;
+    out->workspace_bytes = w.total_fixed + out->spill_stride * grid;
     return PEL_OK;
 }
 
@@ -633,6 +747,18 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     int dev = 0;
     PEL_HIP(hipGetDevice(&dev));
     if (dev != g->device) return fail(PEL_EINVAL, "pel_decode: current device differs from the graph's");
+    const StateSizes st = g->state();
+    int64_t lds = 0, spill = 0;
+    uint32_t offs[3];
+    for (int k = 0; k < 3; ++k) {
+        int64_t &acc = (plan->placement & (1 << k)) ? lds : spill;
+        offs[k] = (uint32_t)acc;
+        acc += st.sz[k];
+    }
+    const WsLayout w = ws_layout(g, b->B);
+    if (lds != plan->lds_bytes || align_up(spill, 256) != plan->spill_stride || plan->grid <= 0 ||
+        plan->workspace_bytes < w.total_fixed + plan->spill_stride * plan->grid)
+        return fail(PEL_EINVAL, "pel_decode: plan does not match this graph and batch size");
     KArgs a;
     memset(&a, 0, sizeof a);
     a.g = g->d;
@@ -664,31 +790,19 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     a.last_post = b->last_posterior;
     uint8_t *ws = static_cast<uint8_t *>(b->workspace);
     a.queue = reinterpret_cast<int32_t *>(ws);
-    a.spill = ws + 256;
+    int32_t *order = reinterpret_cast<int32_t *>(ws + w.order);
+    a.order = order;
+    a.spill = ws + w.spill;
     a.spill_stride = plan->spill_stride;
-    const int64_t ls = align_up(8LL * g->V, 16), cs = align_up(16LL * g->C, 16),
-                  sb = align_up(4LL * std::max(g->n_words, 1), 16);
-    const int64_t sz[3] = {ls, cs, sb};
-    int64_t lds = 0, spill = 0;
-    uint32_t offs[3];
-    for (int k = 0; k < 3; ++k) {
-        if (plan->placement & (1 << k)) {
-            offs[k] = (uint32_t)lds;
-            lds += sz[k];
-        } else {
-            offs[k] = (uint32_t)spill;
-            spill += sz[k];
-        }
-    }
-    if (lds != plan->lds_bytes || align_up(spill, 256) != plan->spill_stride)
-        return fail(PEL_EINVAL, "pel_decode: plan does not belong to this graph");
     a.ls_off = offs[0];
     a.cs_off = offs[1];
     a.sb_off = offs[2];
     hipStream_t s = static_cast<hipStream_t>(stream);
-    KernelFn fn = pick_kernel(plan->block, plan->placement);
-    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, s, a.queue);
+    hipLaunchKernelGGL(order_shots_kernel, dim3(1), dim3(kOrderBlock), 0, s, b->syndromes, b->B, g->C,
+                       reinterpret_cast<int32_t *>(ws + w.bucket), reinterpret_cast<int32_t *>(ws + w.weight),
+                       order, a.queue);
     PEL_HIP(hipGetLastError());
+    KernelFn fn = pick_kernel(plan->block, plan->placement);
     hipLaunchKernelGGL(fn, dim3(plan->grid), dim3(plan->block), (size_t)plan->lds_bytes, s, a);
     PEL_HIP(hipGetLastError());
     return PEL_OK;
