@@ -68,6 +68,8 @@ typedef struct {
     int32_t *found_count;        /* out [B]: valid solutions found */
     float *last_posterior;       /* out [B,V] or NULL (decode(return_last_posterior=True)) */
     void *workspace;             /* device scratch of pel_plan.workspace_bytes */
+    int64_t *shot_clock;         /* out [B,3] or NULL: device wall clock (100 MHz ticks) when the
+                                    shot started and ended, and the workgroup that ran it */
 } pel_batch;
 
 /* Launch plan for one (graph, B) pair: chosen by pel_plan_decode. */

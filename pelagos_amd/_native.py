@@ -56,6 +56,7 @@ class PelBatch(ctypes.Structure):
         ("found_count", _vp),
         ("last_posterior", _vp),
         ("workspace", _vp),
+        ("shot_clock", _vp),
     ]
 
 

@@ -10,16 +10,18 @@
 //     out shots in decreasing syndrome weight, so the slow shots start early.
 //   * Per-shot state is kept in compressed min-sum form instead of per-edge
 //     messages (the reference's muT/nuT [E,B], decoder.py:1527-1529):
-//       variables  LS[j] = {lam_mixed_j, S_j}        (8 B)  S_j = sum of incoming nu
-//       checks     CS[r] = {m1, m2, argbit, parity}  (16 B) outgoing magnitudes
-//       sign words SB     one bit per edge: sign of the variable->check message
+//       variables  LS[j] = {lam_mixed_j, S_j}   (8 B)  S_j = sum of incoming nu
+//       checks     CS[r] = {P1_r, P2_r}         (8 B)  outgoing magnitudes m1, m2
+//                                                      with the check's sign applied
+//       edge codes SB    2 bits per edge: sign of the variable->check message and
+//                         "this edge holds the check's minimum"
 //     Every message is a pure function of this state, bit for bit:
-//       nu_e = (parity_r ^ sign_e ? -1 : +1) * (bit(e) == argbit_r ? m2_r : m1_r)
+//       nu_e = sign_e ? -P : P,  P = argmin_e ? P2_r : P1_r
 //       mu_e = lam_mixed_j + (S_j - nu_e)              (kernels.py:1339-1350)
 //       post_j = lam_mixed_j + S_j                       (kernels.py:1328)
-//     At 936 x 8785 this is 90 KB per shot, which fits the 160 KB LDS of a CU,
-//     so the iteration loop touches HBM only for the graph indices (L2-resident,
-//     shared by every shot) and the shot's priors.
+//     With the shot's prior LLRs this is 126 KB at 936 x 8785, which fits the
+//     160 KB LDS of a CU, so the iteration loop touches HBM/L2 only for the graph
+//     indices (shared by every shot).
 //   * One iteration = check pass (a thread per check row, sequential over the
 //     row: min1/min2/argmin/sign parity, plus the parity test of the previous
 //     iteration's hard decision) and variable pass (a thread per variable,
@@ -28,8 +30,9 @@
 //   * Graph indices are stored as 64-wide ELL slices (one slice per wave):
 //     check rows sorted by degree, so a wave's rows have similar lengths, and
 //     element k of every row of a slice stored contiguously, so the wave's index
-//     load for step k is one coalesced 256-byte read.  Variables keep their
-//     natural order (per-variable priors and gammas load coalesced).
+//     load for step k is one coalesced 256-byte read.  Slices are padded to the
+//     chunk width, so every chunk load stays inside the slice.  Variables keep
+//     their natural order (per-variable priors and gammas load coalesced).
 //
 // Floating point follows the reference kernels exactly (fp32, no FMA
 // contraction -- the build passes -ffp-contract=off and the memory mix uses
@@ -64,21 +67,27 @@ int fail(int code, const std::string &msg) {
             return fail(PEL_EDEVICE, std::string(#call) + ": " + hipGetErrorString(_e));   \
     } while (0)
 
-constexpr int kSlice = 64;  // rows / variables per ELL slice (one wave)
+constexpr int kSlice = 64;     // rows / variables per ELL slice (one wave)
+constexpr int kChunk = 8;      // check-row edges per chunk (two chunks per edge-code word)
+constexpr int kVChunk = 4;     // variable edges per chunk (column weights are small)
+constexpr int kEdgesPerWord = 16;  // 2-bit edge codes
+
+// State arrays, in the order they are given LDS (most accessed first).
+enum { kLS = 0, kCS = 1, kSB = 2, kL0 = 3, kNumArrays = 4 };
 
 struct GraphDev {
     int32_t C, V, E;
-    int32_t sb_words;           // sign words per shot (slice-padded)
+    int32_t sb_words;           // edge-code words per shot (slice-padded)
     // check side, by row position (rows sorted by decreasing degree)
     const int32_t *row_orig;    // [C]  original check index
     const int32_t *row_deg;     // [C]
     const int32_t *rslice_idx;  // [C/64] first element of the slice in row_var
-    const int32_t *rslice_sb;   // [C/64] first sign word of the slice
+    const int32_t *rslice_sb;   // [C/64] first edge-code word of the slice
     const int32_t *row_var;     // ELL: element (k, lane) of slice s at rslice_idx[s] + 64k + lane
     // variable side, natural order
     const int32_t *var_deg;     // [V]
     const int32_t *vslice_idx;  // [V/64]
-    const int2 *var_edge;       // ELL of {row position, sign bit address}, ascending edge id
+    const int2 *var_edge;       // ELL of {row position, code bit address}, ascending edge id
 };
 
 struct KArgs {
@@ -103,10 +112,34 @@ struct KArgs {
     float *last_post;
     int32_t *queue;
     const int32_t *order;  // [B] shot taken at each queue position
+    int64_t *shot_clock;   // [B,3] start/end wall clock and workgroup, or null
     uint8_t *spill;
     int64_t spill_stride;
-    uint32_t ls_off, cs_off, sb_off;  // byte offsets (LDS or spill slot, by placement)
+    uint32_t off[kNumArrays];  // byte offsets of the state arrays (LDS or spill slot)
 };
+
+// One shot's working memory (LDS or a workspace slot, by placement).
+struct ShotMem {
+    float2 *LS;
+    float2 *CS;
+    uint32_t *SB;
+    float *L0s;        // the shot's prior LLRs copied to LDS (placement bit kL0)
+    const float *L0g;  // or read in place from the global prior
+};
+
+// The shot's prior LLR of variable j, from LDS or global memory as PLACE says.
+template <int PLACE>
+__device__ __forceinline__ float prior_at(const ShotMem &m, int j) {
+    if (PLACE & (1 << kL0)) return m.L0s[j];
+    return m.L0g[j];
+}
+
+// Load base[idx] with a 32-bit byte offset from a uniform base, so the load
+// uses the scalar-base + vector-offset addressing form.
+template <typename T>
+__device__ __forceinline__ T load_idx(const T *__restrict__ base, uint32_t idx) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (idx * (uint32_t)sizeof(T)));
+}
 
 // Memory strength source for one leg: a row of V values or a constant.
 struct GammaSrc {
@@ -160,64 +193,102 @@ __device__ double block_sum(double v, double *s_red) {
     return uniform_f64(tot);
 }
 
-// Check pass for row position pos.  first: messages were just reset
-// (mu_e = lam0_j, kernels.py:981-1000), so nothing is reconstructed and no
-// parity is tested.  Otherwise reconstructs mu_e from the state of the
-// previous iteration and also returns 1 if the previous iteration's hard
+// OR of a predicate over the workgroup with one barrier: a ballot per wave, one
+// flag per wave in LDS.  Each wave rewrites its own flag on every call; two
+// calls are always separated by another barrier, so no flag is overwritten
+// while a slower wave still reads the previous round.
+template <int BLOCK>
+__device__ __forceinline__ int block_or(int pred, int *s_flag) {
+    const bool any = __ballot(pred) != 0ull;
+    if ((opaque_tid() & 63u) == 0) s_flag[threadIdx.x >> 6] = any ? 1 : 0;
+    __syncthreads();
+    int r = 0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) r |= s_flag[w];
+    return uniform_i32(r);
+}
+
+// nu of one edge from its 2-bit code (sign, argmin) and its check's {P1, P2}.
+__device__ __forceinline__ float edge_message(uint32_t code, float p1, float p2) {
+    const float p = (code & 2u) ? p2 : p1;
+    return (code & 1u) ? -p : p;
+}
+
+// Check pass for row position pos (kernels.py:1128-1215).  FIRST: messages were
+// just reset (mu_e = lam0_j, kernels.py:981-1000), so nothing is reconstructed
+// and no parity is tested.  Otherwise reconstructs mu_e from the previous
+// iteration's state and also returns 1 if the previous iteration's hard
 // decision violates the check (kernels.py:509-569 fused into the check pass).
-__device__ __forceinline__ int check_row(const KArgs &a, int pos, bool first, float alpha,
-                                         const float *__restrict__ l0, const uint8_t *__restrict__ syn,
-                                         const float2 *LS, float4 *CS, uint32_t *SB) {
+//
+// The row is walked in chunks of kChunk edges: the chunk's index loads are
+// issued together (slices are padded to whole chunks), then the LDS gathers,
+// then branch-free arithmetic in which a padding slot contributes
+// |mu| = +inf, no sign and no hard bit.  The min update keeps the reference's
+// tie semantics: m2 = min(m2, max(|mu|, m1)) equals m1 when two edges share the
+// minimum, and the argmin edge is the first to attain it.
+template <bool FIRST, int PLACE>
+__device__ __forceinline__ int check_row(const KArgs &a, const ShotMem &m, int pos, float alpha,
+                                         const uint8_t *__restrict__ syn) {
     const int deg = a.g.row_deg[pos];
     const uint32_t s_i = syn[a.g.row_orig[pos]] & 1u;
-    if (deg == 0) return first ? 0 : (int)s_i;
+    if (deg == 0) return FIRST ? 0 : (int)s_i;
     const int slice = pos >> 6, lane = pos & 63;
-    const int32_t *__restrict__ vars = a.g.row_var + a.g.rslice_idx[slice] + lane;
-    const int sbase = a.g.rslice_sb[slice] + lane;
-    float om1 = 0.f, om2 = 0.f;
-    uint32_t oarg = 0xffffffffu, opar = 0;
-    if (!first) {
-        const float4 cs = CS[pos];
-        om1 = cs.x;
-        om2 = cs.y;
-        oarg = __float_as_uint(cs.z);
-        opar = __float_as_uint(cs.w);
+    const uint32_t ioff = (uint32_t)(a.g.rslice_idx[slice] + lane);
+    const uint32_t woff = (uint32_t)(a.g.rslice_sb[slice] + lane);
+    const int32_t *__restrict__ rv = a.g.row_var;
+    float p1 = 0.f, p2 = 0.f;
+    if (!FIRST) {
+        const float2 cs = m.CS[pos];
+        p1 = cs.x;
+        p2 = cs.y;
     }
     float m1 = INFINITY, m2 = INFINITY;
-    uint32_t arg = 0, neg = 0, hpar = 0;
-    for (int k0 = 0; k0 < deg; k0 += 32) {
-        const int n = min(32, deg - k0);
-        const uint32_t word = (uint32_t)(sbase + (k0 >> 5) * kSlice);
-        const uint32_t wold = first ? 0u : SB[word];
-        uint32_t wnew = 0;
-#pragma unroll 4
-        for (int kk = 0; kk < n; ++kk) {
-            const int j = vars[(k0 + kk) * kSlice];
-            float mu;
-            if (first) {
-                mu = l0[j];
+    uint32_t argk = 0, neg = 0, hpar = 0;
+    uint32_t wold = 0, wnew = 0;
+    for (int k0 = 0; k0 < deg; k0 += kChunk) {
+        const uint32_t word = woff + (uint32_t)(k0 >> 4) * kSlice;
+        if ((k0 & 15) == 0) {
+            if (!FIRST) wold = m.SB[word];
+            wnew = 0;
+        }
+        int jv[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) jv[u] = load_idx(rv, ioff + (uint32_t)(k0 + u) * kSlice);
+        float x[kChunk], y[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+            if (FIRST) {
+                x[u] = prior_at<PLACE>(m, jv[u]);
+                y[u] = 0.f;
             } else {
-                const float2 ls = LS[j];
-                hpar ^= (ls.x + ls.y <= 0.f) ? 1u : 0u;
-                const uint32_t bit = (word << 5) | (uint32_t)kk;
-                const float mag = (bit == oarg) ? om2 : om1;
-                const float nu = (((wold >> kk) ^ opar) & 1u) ? -mag : mag;
-                mu = ls.x + (ls.y - nu);
-            }
-            const uint32_t isneg = mu < 0.f ? 1u : 0u;
-            neg ^= isneg;
-            wnew |= isneg << kk;
-            const float am = fabsf(mu);
-            if (am < m1) {
-                m2 = m1;
-                m1 = am;
-                arg = (word << 5) | (uint32_t)kk;
-            } else if (am < m2) {
-                m2 = am;
+                const float2 ls = m.LS[jv[u]];
+                x[u] = ls.x;
+                y[u] = ls.y;
             }
         }
-        SB[word] = wnew;
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+            const bool valid = (k0 + u) < deg;
+            const uint32_t sh = (uint32_t)(2 * ((k0 + u) & 15));
+            float mu;
+            if (FIRST) {
+                mu = x[u];
+            } else {
+                hpar ^= (valid && (x[u] + y[u] <= 0.f)) ? 1u : 0u;
+                mu = x[u] + (y[u] - edge_message(wold >> sh, p1, p2));
+            }
+            const uint32_t isneg = (valid && mu < 0.f) ? 1u : 0u;
+            neg ^= isneg;
+            wnew |= isneg << sh;
+            const float am = valid ? fabsf(mu) : INFINITY;
+            argk = (am < m1) ? (uint32_t)(k0 + u) : argk;
+            m2 = fminf(m2, fmaxf(am, m1));
+            m1 = fminf(m1, am);
+        }
+        if (((k0 + kChunk) & 15) == 0 || k0 + kChunk >= deg) m.SB[word] = wnew;
     }
+    // Mark the edge holding the minimum (written after the sign words above).
+    m.SB[woff + (argk >> 4) * kSlice] |= 2u << (2 * (argk & 15));
     float o1 = m1, o2 = m2;
     if (deg == 1) o1 = o2 = 0.f;  // degree-1 checks send zero (kernels.py:1192-1193)
     if (a.use_alpha) {
@@ -228,65 +299,83 @@ __device__ __forceinline__ int check_row(const KArgs &a, int pos, bool first, fl
         o1 = fmaxf(o1 - a.beta, 0.f);
         o2 = fmaxf(o2 - a.beta, 0.f);
     }
-    CS[pos] = make_float4(o1, o2, __uint_as_float(arg), __uint_as_float((neg ^ s_i) & 1u));
-    return first ? 0 : (int)((hpar ^ s_i) & 1u);
+    const bool par = (neg ^ s_i) & 1u;
+    m.CS[pos] = make_float2(par ? -o1 : o1, par ? -o2 : o2);
+    return FIRST ? 0 : (int)((hpar ^ s_i) & 1u);
 }
 
 // Parity of the current hard decision on row position pos (after the last
 // iteration of a leg).
-__device__ __forceinline__ int parity_row(const KArgs &a, int pos, const uint8_t *__restrict__ syn,
-                                          const float2 *LS) {
+__device__ __forceinline__ int parity_row(const KArgs &a, const ShotMem &m, int pos,
+                                          const uint8_t *__restrict__ syn) {
     const int deg = a.g.row_deg[pos];
     uint32_t hpar = syn[a.g.row_orig[pos]] & 1u;
-    const int32_t *__restrict__ vars = a.g.row_var + a.g.rslice_idx[pos >> 6] + (pos & 63);
-#pragma unroll 4
-    for (int k = 0; k < deg; ++k) {
-        const float2 ls = LS[vars[k * kSlice]];
-        hpar ^= (ls.x + ls.y <= 0.f) ? 1u : 0u;
+    const uint32_t ioff = (uint32_t)(a.g.rslice_idx[pos >> 6] + (pos & 63));
+    const int32_t *__restrict__ rv = a.g.row_var;
+    for (int k0 = 0; k0 < deg; k0 += kChunk) {
+        int jv[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) jv[u] = load_idx(rv, ioff + (uint32_t)(k0 + u) * kSlice);
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+            const float2 ls = m.LS[jv[u]];
+            hpar ^= ((k0 + u) < deg && (ls.x + ls.y <= 0.f)) ? 1u : 0u;
+        }
     }
     return (int)hpar;
 }
 
-// Variable pass for variable j (kernels.py:1299-1350).
-__device__ __forceinline__ void var_update(const KArgs &a, int j, const GammaSrc &gs,
-                                           const float *__restrict__ l0, float2 *LS,
-                                           const float4 *CS, const uint32_t *SB) {
+__device__ __forceinline__ float var_edge_message(const ShotMem &m, int2 e) {
+    const float2 cs = m.CS[e.x];
+    const uint32_t w = m.SB[(uint32_t)e.y >> 5];
+    return edge_message(w >> (e.y & 31), cs.x, cs.y);
+}
+
+// Variable pass for variable j (kernels.py:1299-1350).  The incoming messages
+// are summed one at a time in ascending edge id; whole chunks of edge loads are
+// issued together, the remainder one by one.
+template <int PLACE>
+__device__ __forceinline__ void var_update(const KArgs &a, const ShotMem &m, int j, const GammaSrc &gs) {
     const float g = gs(j);
-    const float2 ls = LS[j];
+    const float2 ls = m.LS[j];
     const float post_prev = ls.x + ls.y;
-    const float lm = __fadd_rn(__fmul_rn(1.0f - g, l0[j]), __fmul_rn(g, post_prev));
+    const float lm = __fadd_rn(__fmul_rn(1.0f - g, prior_at<PLACE>(m, j)), __fmul_rn(g, post_prev));
     const int deg = a.g.var_deg[j];
-    const int2 *__restrict__ ve = a.g.var_edge + a.g.vslice_idx[j >> 6] + (j & 63);
+    const uint32_t eoff = (uint32_t)(a.g.vslice_idx[j >> 6] + (j & 63));
+    const int2 *__restrict__ ve = a.g.var_edge;
     float s = 0.f;
-#pragma unroll 4
-    for (int q = 0; q < deg; ++q) {
-        const int2 e = ve[q * kSlice];
-        const float4 cs = CS[e.x];
-        const uint32_t w = SB[(uint32_t)e.y >> 5];
-        const float mag = ((uint32_t)e.y == __float_as_uint(cs.z)) ? cs.y : cs.x;
-        const float nu = (((w >> (e.y & 31)) ^ __float_as_uint(cs.w)) & 1u) ? -mag : mag;
-        s += nu;
+    int q = 0;
+    for (; q + kVChunk <= deg; q += kVChunk) {
+        int2 e[kVChunk];
+#pragma unroll
+        for (int u = 0; u < kVChunk; ++u) e[u] = load_idx(ve, eoff + (uint32_t)(q + u) * kSlice);
+        float nu[kVChunk];
+#pragma unroll
+        for (int u = 0; u < kVChunk; ++u) nu[u] = var_edge_message(m, e[u]);
+#pragma unroll
+        for (int u = 0; u < kVChunk; ++u) s += nu[u];
     }
-    LS[j] = make_float2(lm, s);
+    for (; q < deg; ++q) s += var_edge_message(m, load_idx(ve, eoff + (uint32_t)q * kSlice));
+    m.LS[j] = make_float2(lm, s);
 }
 
 // check_and_select_kernel (kernels.py:657-741) for a shot whose hard decision
 // satisfies every check: weight = sum of wj over bits with post <= 0 (double
 // accumulation, rounded once), strict improvement or first solution wins.
 template <int BLOCK>
-__device__ void select_solution(const KArgs &a, const float *__restrict__ wj, const float2 *LS,
+__device__ void select_solution(const KArgs &a, const ShotMem &m, const float *__restrict__ wj,
                                 uint8_t *err, ShotState &st, double *s_red) {
     const int V = a.g.V;
     double acc = 0.0;
     for (int j = threadIdx.x; j < V; j += BLOCK) {
-        const float2 ls = LS[j];
+        const float2 ls = m.LS[j];
         if (ls.x + ls.y <= 0.f) acc += (double)wj[j];
     }
     const float w = (float)block_sum<BLOCK>(acc, s_red);
     if (st.best_w >= 1e20f || w < st.best_w) {
         st.best_w = w;
         for (int j = threadIdx.x; j < V; j += BLOCK) {
-            const float2 ls = LS[j];
+            const float2 ls = m.LS[j];
             err[j] = (ls.x + ls.y <= 0.f) ? 1 : 0;
         }
     }
@@ -295,32 +384,35 @@ __device__ void select_solution(const KArgs &a, const float *__restrict__ wj, co
 
 // One leg of up to T iterations.  The shot leaves the leg at its first valid
 // solution (freeze_finished_lanes_kernel with sought = found + 1).
-template <int BLOCK>
-__device__ bool run_leg(const KArgs &a, int T, const GammaSrc &gs, const float *__restrict__ l0,
+template <int BLOCK, int PLACE>
+__device__ bool run_leg(const KArgs &a, const ShotMem &m, int T, const GammaSrc &gs,
                         const float *__restrict__ wj, const uint8_t *__restrict__ syn, uint8_t *err,
-                        float2 *LS, float4 *CS, uint32_t *SB, ShotState &st, double *s_red) {
+                        ShotState &st, double *s_red, int *s_flag) {
     const int C = a.g.C, V = a.g.V;
     for (int t = 0; t < T; ++t) {
         const float alpha = a.use_alpha ? a.alpha_t[t] : 1.0f;
-        const bool first = (t == 0);
         int bad = 0;
-        for (int i = threadIdx.x; i < C; i += BLOCK) bad |= check_row(a, i, first, alpha, l0, syn, LS, CS, SB);
-        bad = uniform_i32(__syncthreads_or(bad));
-        if (!first && !bad) {  // iteration t-1 converged
-            select_solution<BLOCK>(a, wj, LS, err, st, s_red);
+        if (t == 0) {
+            for (int i = threadIdx.x; i < C; i += BLOCK) check_row<true, PLACE>(a, m, i, alpha, syn);
+        } else {
+            for (int i = threadIdx.x; i < C; i += BLOCK) bad |= check_row<false, PLACE>(a, m, i, alpha, syn);
+        }
+        bad = block_or<BLOCK>(bad, s_flag);
+        if (t > 0 && !bad) {  // iteration t-1 converged
+            select_solution<BLOCK>(a, m, wj, err, st, s_red);
             return true;
         }
-        for (int j = threadIdx.x; j < V; j += BLOCK) var_update(a, j, gs, l0, LS, CS, SB);
+        for (int j = threadIdx.x; j < V; j += BLOCK) var_update<PLACE>(a, m, j, gs);
         __syncthreads();
         st.iters += 1;
         st.any_iter = 1;
     }
     if (T > 0) {
         int bad = 0;
-        for (int i = threadIdx.x; i < C; i += BLOCK) bad |= parity_row(a, i, syn, LS);
-        bad = uniform_i32(__syncthreads_or(bad));
+        for (int i = threadIdx.x; i < C; i += BLOCK) bad |= parity_row(a, m, i, syn);
+        bad = block_or<BLOCK>(bad, s_flag);
         if (!bad) {
-            select_solution<BLOCK>(a, wj, LS, err, st, s_red);
+            select_solution<BLOCK>(a, m, wj, err, st, s_red);
             return true;
         }
     }
@@ -328,20 +420,19 @@ __device__ bool run_leg(const KArgs &a, int T, const GammaSrc &gs, const float *
 }
 
 template <int BLOCK>
-__device__ void write_hard(const KArgs &a, const float2 *LS, uint8_t *err, bool any_iter) {
+__device__ void write_hard(const KArgs &a, const ShotMem &m, uint8_t *err, bool any_iter) {
     for (int j = threadIdx.x; j < a.g.V; j += BLOCK) {
         uint8_t h = 0;
         if (any_iter) {
-            const float2 ls = LS[j];
+            const float2 ls = m.LS[j];
             h = (ls.x + ls.y <= 0.f) ? 1 : 0;
         }
         err[j] = h;
     }
 }
 
-template <int BLOCK>
-__device__ ShotState decode_shot(const KArgs &a, int b, float2 *LS, float4 *CS, uint32_t *SB,
-                                 double *s_red) {
+template <int BLOCK, int PLACE>
+__device__ ShotState decode_shot(const KArgs &a, int b, ShotMem m, double *s_red, int *s_flag) {
     const int C = a.g.C, V = a.g.V;
     const uint8_t *syn = a.syn + (size_t)b * C;
     const size_t pofs = a.prior_batched ? (size_t)b * V : 0;
@@ -352,10 +443,13 @@ __device__ ShotState decode_shot(const KArgs &a, int b, float2 *LS, float4 *CS, 
     // State reset (decoder.py:3430-3464): post = prior (or init_posterior),
     // stored as lam_mixed = post, S = 0 so that lam_mixed + S == post exactly.
     for (int j = threadIdx.x; j < V; j += BLOCK) {
-        const float p0 = a.init_post ? a.init_post[(size_t)b * V + j] : l0[j];
-        LS[j] = make_float2(p0, 0.f);
+        const float lj = l0[j];
+        const float p0 = a.init_post ? a.init_post[(size_t)b * V + j] : lj;
+        m.LS[j] = make_float2(p0, 0.f);
+        if (PLACE & (1 << kL0)) m.L0s[j] = lj;
         err[j] = 0;
     }
+    m.L0g = l0;
     __syncthreads();
 
     ShotState st{INFINITY, 0, 0, 0};
@@ -364,61 +458,73 @@ __device__ ShotState decode_shot(const KArgs &a, int b, float2 *LS, float4 *CS, 
     // Pre-iter leg (decoder.py:3470-3593).
     GammaSrc g0{nullptr, a.plain ? 0.f : a.gamma0};
     if (!a.plain && a.init_gamma) g0.row = a.init_gamma + (size_t)b * V;
-    if (st.found < S) run_leg<BLOCK>(a, a.pre_iter, g0, l0, wj, syn, err, LS, CS, SB, st, s_red);
+    if (st.found < S) run_leg<BLOCK, PLACE>(a, m, a.pre_iter, g0, wj, syn, err, st, s_red, s_flag);
     // first_leg_hard_dec (decoder.py:3597-3598): kept in err while no solution exists.
-    if (!a.last_leg && st.found == 0) write_hard<BLOCK>(a, LS, err, st.any_iter);
+    if (!a.last_leg && st.found == 0) write_hard<BLOCK>(a, m, err, st.any_iter);
 
     if (!a.skip_relay_legs) {
         for (int leg = 0; leg < a.num_sets && st.found < S; ++leg) {
             GammaSrc gl{a.gamma_legs + (a.gamma_per_row ? ((size_t)leg * a.B + b) * V : (size_t)leg * V),
                         0.f};
-            run_leg<BLOCK>(a, a.set_max_iter, gl, l0, wj, syn, err, LS, CS, SB, st, s_red);
+            run_leg<BLOCK, PLACE>(a, m, a.set_max_iter, gl, wj, syn, err, st, s_red, s_flag);
             if (leg == 0 && a.pre_iter == 0 && !a.last_leg && st.found == 0)
-                write_hard<BLOCK>(a, LS, err, st.any_iter);  // decoder.py:3809-3810
+                write_hard<BLOCK>(a, m, err, st.any_iter);  // decoder.py:3809-3810
         }
     }
     // last_leg correction: the live hard decision for shots with no solution.
-    if (a.last_leg && st.found == 0) write_hard<BLOCK>(a, LS, err, st.any_iter);
+    if (a.last_leg && st.found == 0) write_hard<BLOCK>(a, m, err, st.any_iter);
 
     if (a.last_post) {
         float *lp = a.last_post + (size_t)b * V;
         for (int j = threadIdx.x; j < V; j += BLOCK) {
-            const float2 ls = LS[j];
+            const float2 ls = m.LS[j];
             lp[j] = ls.x + ls.y;
         }
     }
     return st;
 }
 
-// PLACE bit0: LS in LDS; bit1: CS in LDS; bit2: SB in LDS.
+// PLACE bit k: state array k (kLS, kCS, kSB, kL0) in LDS; otherwise the
+// workgroup's workspace slot (kL0: the global prior is read in place).
 template <int BLOCK, int PLACE>
 __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ double s_red[BLOCK / 64];
+    __shared__ int s_flag[BLOCK / 64];
     __shared__ int s_shot;
     uint8_t *slot = a.spill + (int64_t)blockIdx.x * a.spill_stride;
-    float2 *LS = reinterpret_cast<float2 *>(((PLACE & 1) ? smem : slot) + a.ls_off);
-    float4 *CS = reinterpret_cast<float4 *>(((PLACE & 2) ? smem : slot) + a.cs_off);
-    uint32_t *SB = reinterpret_cast<uint32_t *>(((PLACE & 4) ? smem : slot) + a.sb_off);
+    ShotMem m;
+    m.LS = reinterpret_cast<float2 *>(((PLACE & (1 << kLS)) ? smem : slot) + a.off[kLS]);
+    m.CS = reinterpret_cast<float2 *>(((PLACE & (1 << kCS)) ? smem : slot) + a.off[kCS]);
+    m.SB = reinterpret_cast<uint32_t *>(((PLACE & (1 << kSB)) ? smem : slot) + a.off[kSB]);
+    m.L0s = reinterpret_cast<float *>(smem + a.off[kL0]);
+    m.L0g = nullptr;
     // The leader writes the finished shot's scalars and takes the next ticket in
     // one branch, so the loop has a single leader-only region.
     int prev = -1;
     ShotState st{0.f, 0, 0, 0};
     for (;;) {
         if (opaque_tid() == 0) {
+            const int64_t now = a.shot_clock ? (int64_t)wall_clock64() : 0;
             if (prev >= 0) {
                 a.weights[prev] = st.best_w;
                 a.iters[prev] = st.iters;
                 a.found[prev] = st.found;
+                if (a.shot_clock) a.shot_clock[3 * prev + 1] = now;
             }
             const int t = atomicAdd(a.queue, 1);
-            s_shot = t < a.B ? a.order[t] : a.B;
+            const int nb = t < a.B ? a.order[t] : a.B;
+            s_shot = nb;
+            if (a.shot_clock && nb < a.B) {
+                a.shot_clock[3 * nb] = now;
+                a.shot_clock[3 * nb + 2] = blockIdx.x;
+            }
         }
         __syncthreads();
         const int b = uniform_i32(s_shot);
         __syncthreads();
         if (b >= a.B) break;
-        st = decode_shot<BLOCK>(a, b, LS, CS, SB, s_red);
+        st = decode_shot<BLOCK, PLACE>(a, b, m, s_red, s_flag);
         prev = b;
     }
 }
@@ -458,9 +564,13 @@ __global__ void __launch_bounds__(kOrderBlock) order_shots_kernel(const uint8_t 
 
 typedef void (*KernelFn)(KArgs);
 
+// Placements are prefixes of the array order: LS, +CS, +SB, +L0.
+constexpr int kPlacements[] = {0, 1, 3, 7, 15};
+
 template <int BLOCK>
 KernelFn pick_place(int place) {
     switch (place) {
+        case 15: return relay_decode_kernel<BLOCK, 15>;
         case 7: return relay_decode_kernel<BLOCK, 7>;
         case 3: return relay_decode_kernel<BLOCK, 3>;
         case 1: return relay_decode_kernel<BLOCK, 1>;
@@ -481,7 +591,7 @@ KernelFn pick_kernel(int block, int place) {
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 struct StateSizes {
-    int64_t sz[3];  // LS, CS, SB bytes per shot
+    int64_t sz[kNumArrays];  // LS, CS, SB, L0 bytes per shot
 };
 
 }  // namespace
@@ -492,7 +602,8 @@ struct pel_graph {
     void *dev_mem;  // one allocation holding every array of d
     GraphDev d;
     StateSizes state() const {
-        return {{align_up(8LL * V, 16), align_up(16LL * C, 16), align_up(4LL * std::max(sb_words, 1), 16)}};
+        return {{align_up(8LL * V, 16), align_up(8LL * C, 16), align_up(4LL * std::max(sb_words, 1), 16),
+                 align_up(4LL * V, 16)}};
     }
 };
 
@@ -511,6 +622,23 @@ WsLayout ws_layout(const pel_graph *g, int64_t B) {
     w.spill = align_up(w.bucket + 4LL * (g->C + 1), 256);
     w.total_fixed = w.spill;
     return w;
+}
+
+// Byte offsets of the state arrays for a placement; LDS and spill totals.
+void place_arrays(const StateSizes &st, int place, uint32_t off[kNumArrays], int64_t *lds, int64_t *spill) {
+    *lds = 0;
+    *spill = 0;
+    for (int k = 0; k < kNumArrays; ++k) {
+        if (place & (1 << k)) {
+            off[k] = (uint32_t)*lds;
+            *lds += st.sz[k];
+        } else if (k == kL0) {
+            off[k] = 0;  // read from the global prior, no copy
+        } else {
+            off[k] = (uint32_t)*spill;
+            *spill += st.sz[k];
+        }
+    }
 }
 
 }  // namespace
@@ -549,15 +677,17 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
         rpos[rorig[p]] = p;
         rdeg[p] = chk_ptr[rorig[p] + 1] - chk_ptr[rorig[p]];
     }
+    // Row slices: width padded to whole chunks; edge-code words of 16 edges.
     const int32_t nrs = (C + kSlice - 1) / kSlice;
     std::vector<int32_t> rs_idx(nrs), rs_sb(nrs);
     int64_t nidx = 0, nword = 0;
     for (int32_t s = 0; s < nrs; ++s) {
         const int32_t w = rdeg[s * kSlice];  // widest row of the slice comes first
+        const int32_t wpad = (w + kChunk - 1) / kChunk * kChunk;
         rs_idx[s] = (int32_t)nidx;
         rs_sb[s] = (int32_t)nword;
-        nidx += (int64_t)w * kSlice;
-        nword += (int64_t)((w + 31) / 32) * kSlice;
+        nidx += (int64_t)wpad * kSlice;
+        nword += (int64_t)((w + kEdgesPerWord - 1) / kEdgesPerWord) * kSlice;
     }
     if (nidx > INT32_MAX / 2 || nword * 32 > INT32_MAX)
         return fail(PEL_EINVAL, "pel_graph_create: graph too large for 32-bit indexing");
@@ -586,9 +716,9 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
         const int32_t p = rpos[i], s = p / kSlice, lane = p % kSlice;
         for (int32_t e = chk_ptr[i]; e < chk_ptr[i + 1]; ++e) {
             const int32_t j = chk_var[e], k = e - chk_ptr[i];
-            const int32_t word = rs_sb[s] + (k / 32) * kSlice + lane;
+            const int32_t word = rs_sb[s] + (k / kEdgesPerWord) * kSlice + lane;
             var_edge[vs_idx[j / kSlice] + (int64_t)fill[j]++ * kSlice + (j % kSlice)] =
-                make_int2(p, word * 32 + (k % 32));
+                make_int2(p, word * 32 + 2 * (k % kEdgesPerWord));
         }
     }
     // One device allocation for every array.
@@ -666,9 +796,9 @@ int pel_graph_info(const pel_graph *g, int32_t *C, int32_t *V, int32_t *E, int64
     if (C) *C = g->C;
     if (V) *V = g->V;
     if (E) *E = g->E;
-    if (var_state_bytes) *var_state_bytes = s.sz[0];
-    if (chk_state_bytes) *chk_state_bytes = s.sz[1];
-    if (sign_bytes) *sign_bytes = s.sz[2];
+    if (var_state_bytes) *var_state_bytes = s.sz[kLS] + s.sz[kL0];
+    if (chk_state_bytes) *chk_state_bytes = s.sz[kCS];
+    if (sign_bytes) *sign_bytes = s.sz[kSB];
     return PEL_OK;
 }
 
@@ -691,24 +821,28 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32
     if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
         return fail(PEL_EINVAL, "pel_plan_decode: block must be 64, 128, 256, 512 or 1024");
     const StateSizes st = g->state();
-    const int64_t ls = st.sz[0], cs = st.sz[1], sb = st.sz[2];
-    const int64_t static_lds = 8LL * (block / 64) + 64;
+    const int64_t static_lds = 12LL * (block / 64) + 64;
     const int64_t budget = (int64_t)max_lds - static_lds;
-    int place;
-    if (ls + cs + sb <= budget) place = 7;
-    else if (ls + cs <= budget) place = 3;
-    else if (ls <= budget) place = 1;
-    else place = 0;
+    int place = 0;
+    {
+        int64_t used = 0;
+        for (int k = 0; k < kNumArrays; ++k) {
+            if (used + st.sz[k] > budget) break;
+            used += st.sz[k];
+            place |= 1 << k;
+        }
+    }
     if (placement_override >= 0) {
-        if (placement_override != 0 && placement_override != 1 && placement_override != 3 &&
-            placement_override != 7)
-            return fail(PEL_EINVAL, "pel_plan_decode: placement must be 0, 1, 3 or 7");
+        bool known = false;
+        for (int p : kPlacements) known |= (p == placement_override);
+        if (!known) return fail(PEL_EINVAL, "pel_plan_decode: placement must be 0, 1, 3, 7 or 15");
         if (placement_override > place)
             return fail(PEL_EINVAL, "pel_plan_decode: requested placement exceeds the LDS budget");
         place = placement_override;
     }
+    uint32_t offs[kNumArrays];
     int64_t lds = 0, spill = 0;
-    for (int k = 0; k < 3; ++k) (place & (1 << k) ? lds : spill) += st.sz[k];
+    place_arrays(st, place, offs, &lds, &spill);
     KernelFn fn = pick_kernel(block, place);
     if (lds > 65536) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -747,16 +881,14 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     int dev = 0;
     PEL_HIP(hipGetDevice(&dev));
     if (dev != g->device) return fail(PEL_EINVAL, "pel_decode: current device differs from the graph's");
+    bool known = false;
+    for (int pl : kPlacements) known |= (pl == plan->placement);
     const StateSizes st = g->state();
+    uint32_t offs[kNumArrays];
     int64_t lds = 0, spill = 0;
-    uint32_t offs[3];
-    for (int k = 0; k < 3; ++k) {
-        int64_t &acc = (plan->placement & (1 << k)) ? lds : spill;
-        offs[k] = (uint32_t)acc;
-        acc += st.sz[k];
-    }
+    place_arrays(st, plan->placement, offs, &lds, &spill);
     const WsLayout w = ws_layout(g, b->B);
-    if (lds != plan->lds_bytes || align_up(spill, 256) != plan->spill_stride || plan->grid <= 0 ||
+    if (!known || lds != plan->lds_bytes || align_up(spill, 256) != plan->spill_stride || plan->grid <= 0 ||
         plan->workspace_bytes < w.total_fixed + plan->spill_stride * plan->grid)
         return fail(PEL_EINVAL, "pel_decode: plan does not match this graph and batch size");
     KArgs a;
@@ -788,15 +920,14 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     a.iters = b->iterations;
     a.found = b->found_count;
     a.last_post = b->last_posterior;
+    a.shot_clock = b->shot_clock;
     uint8_t *ws = static_cast<uint8_t *>(b->workspace);
     a.queue = reinterpret_cast<int32_t *>(ws);
     int32_t *order = reinterpret_cast<int32_t *>(ws + w.order);
     a.order = order;
     a.spill = ws + w.spill;
     a.spill_stride = plan->spill_stride;
-    a.ls_off = offs[0];
-    a.cs_off = offs[1];
-    a.sb_off = offs[2];
+    for (int k = 0; k < kNumArrays; ++k) a.off[k] = offs[k];
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(order_shots_kernel, dim3(1), dim3(kOrderBlock), 0, s, b->syndromes, b->B, g->C,
                        reinterpret_cast<int32_t *>(ws + w.bucket), reinterpret_cast<int32_t *>(ws + w.weight),

@@ -281,6 +281,10 @@ class RelayBPDecoder:
         n_alpha = max(self.pre_iter, self.set_max_iter, 1)
         self._alpha_cache: Dict[int, torch.Tensor] = {}
         self._alpha_n = n_alpha
+        # When True, decode() records per-shot device timestamps in
+        # self.shot_clock ([B, 3] int64: start, end in 100 MHz ticks, workgroup).
+        self.record_shot_clock = False
+        self.shot_clock: Optional[torch.Tensor] = None
         # When a list, decode() appends a (start, end) pair of HIP events
         # recorded on the decode stream around each kernel launch.
         self.kernel_events: Optional[list] = None
@@ -369,6 +373,12 @@ class RelayBPDecoder:
             p = self._handle.plan(B, self._block_size, self._placement)
             self._plans[B] = p
         return p
+
+    def _shot_clock_for(self, B: int) -> Optional[torch.Tensor]:
+        if not self.record_shot_clock:
+            return None
+        self.shot_clock = torch.zeros(B, 3, dtype=torch.int64, device=self.device)
+        return self.shot_clock
 
     def _workspace_for(self, nbytes: int) -> torch.Tensor:
         if self._workspace is None or self._workspace.numel() < nbytes:
@@ -494,6 +504,7 @@ class RelayBPDecoder:
             found_count=found.data_ptr(),
             last_posterior=_ptr(last_post),
             workspace=ws.data_ptr(),
+            shot_clock=_ptr(self._shot_clock_for(B)),
         )
         cur = torch.cuda.current_stream(self.device)
         ev = self.kernel_events

@@ -89,7 +89,7 @@ def test_bb_code_capacity_matches_oracle():
     assert np.array_equal(out["valid_mask"].cpu().numpy(), found >= 5)
 
 
-@pytest.mark.parametrize("placement", [None, 1, 0])
+@pytest.mark.parametrize("placement", [None, 7, 3, 1, 0])
 def test_banded_circuit_code_matches_oracle(placement):
     """936 x 8785 banded matrix, log-uniform priors; all-LDS and spilled state."""
     H, p = codes.banded_circuit_code(seed=0)
