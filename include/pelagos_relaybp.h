@@ -60,6 +60,8 @@ typedef struct {
     int32_t prior_batched;
     const float *gamma_legs;     /* [R,V], or [R,B,V] when gamma_per_row; row l = relay leg l */
     int32_t gamma_per_row;
+    int32_t gamma_by_position;   /* [R,V] rows already in the graph's variable order
+                                    (pel_graph_permute_vars); 0 = caller's order */
     const float *init_gamma;     /* [B,V] pre-iter gamma (decode(init_gamma=...)) or NULL */
     const float *init_posterior; /* [B,V] (decode(init_posterior=...)) or NULL */
     uint8_t *errors;             /* out [B,V] */
@@ -104,6 +106,12 @@ int pel_graph_destroy(pel_graph *g);
  * (graph.py:227-245). */
 int pel_graph_info(const pel_graph *g, int32_t *C, int32_t *V, int32_t *E,
                    int64_t *var_state_bytes, int64_t *chk_state_bytes, int64_t *sign_bytes);
+
+/* dst[r][v] = src[r][perm[v]] for r < rows: per-variable rows (e.g. a [R,V]
+ * gamma table) into the graph's internal variable order, on `stream`.  No
+ * counterpart in the reference; lets pel_decode read broadcast gamma rows
+ * without a gather (pel_batch.gamma_by_position = 1). */
+int pel_graph_permute_vars(const pel_graph *g, const float *src, float *dst, int64_t rows, void *stream);
 
 /* Choose the launch plan for B shots.  block_override > 0 forces the block
  * size; placement_override >= 0 forces the placement bits (a state array that

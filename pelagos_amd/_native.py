@@ -48,6 +48,7 @@ class PelBatch(ctypes.Structure):
         ("prior_batched", ctypes.c_int32),
         ("gamma_legs", _vp),
         ("gamma_per_row", ctypes.c_int32),
+        ("gamma_by_position", ctypes.c_int32),
         ("init_gamma", _vp),
         ("init_posterior", _vp),
         ("errors", _vp),
@@ -81,6 +82,7 @@ SIGNATURES = {
     "pel_graph_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
                                       ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    "pel_graph_permute_vars": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, _vp]),
     "pel_plan_decode": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.POINTER(PelPlan)]),
     "pel_decode": (ctypes.c_int, [_vp, ctypes.POINTER(PelParams), ctypes.POINTER(PelBatch),

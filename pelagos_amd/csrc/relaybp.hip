@@ -10,15 +10,15 @@
 //     out shots in decreasing syndrome weight, so the slow shots start early.
 //   * Per-shot state is kept in compressed min-sum form instead of per-edge
 //     messages (the reference's muT/nuT [E,B], decoder.py:1527-1529):
-//       variables  LS[j] = {lam_mixed_j, S_j}   (8 B)  S_j = sum of incoming nu
+//       variables  LS[v] = {lam_mixed_v, S_v}   (8 B)  S_v = sum of incoming nu
 //       checks     CS[r] = {P1_r, P2_r}         (8 B)  outgoing magnitudes m1, m2
 //                                                      with the check's sign applied
 //       edge codes SB    2 bits per edge: sign of the variable->check message and
 //                         "this edge holds the check's minimum"
 //     Every message is a pure function of this state, bit for bit:
 //       nu_e = sign_e ? -P : P,  P = argmin_e ? P2_r : P1_r
-//       mu_e = lam_mixed_j + (S_j - nu_e)              (kernels.py:1339-1350)
-//       post_j = lam_mixed_j + S_j                       (kernels.py:1328)
+//       mu_e = lam_mixed_v + (S_v - nu_e)              (kernels.py:1339-1350)
+//       post_v = lam_mixed_v + S_v                       (kernels.py:1328)
 //     With the shot's prior LLRs this is 126 KB at 936 x 8785, which fits the
 //     160 KB LDS of a CU, so the iteration loop touches HBM/L2 only for the graph
 //     indices (shared by every shot).
@@ -27,12 +27,13 @@
 //     iteration's hard decision) and variable pass (a thread per variable,
 //     sequential sum over its edges in ascending edge id), separated by
 //     workgroup barriers.  No cross-lane reductions are needed.
-//   * Graph indices are stored as 64-wide ELL slices (one slice per wave):
-//     check rows sorted by degree, so a wave's rows have similar lengths, and
-//     element k of every row of a slice stored contiguously, so the wave's index
-//     load for step k is one coalesced 256-byte read.  Slices are padded to the
-//     chunk width, so every chunk load stays inside the slice.  Variables keep
-//     their natural order (per-variable priors and gammas load coalesced).
+//   * Rows and variables are renumbered by decreasing degree and stored as
+//     64-wide ELL slices (one slice per wave), so the lanes of a wave do equal
+//     work and element k of every row of a slice is one coalesced 256-byte
+//     read.  The index entries are the LDS byte addresses of the state they
+//     select, so a gather needs no address arithmetic.  Per-variable inputs
+//     and outputs (priors, gammas, error bits) stay in the caller's order and
+//     are mapped through the permutation.
 //
 // Floating point follows the reference kernels exactly (fp32, no FMA
 // contraction -- the build passes -ffp-contract=off and the memory mix uses
@@ -67,27 +68,30 @@ int fail(int code, const std::string &msg) {
             return fail(PEL_EDEVICE, std::string(#call) + ": " + hipGetErrorString(_e));   \
     } while (0)
 
-constexpr int kSlice = 64;     // rows / variables per ELL slice (one wave)
-constexpr int kChunk = 8;      // check-row edges per chunk (two chunks per edge-code word)
-constexpr int kVChunk = 4;     // variable edges per chunk (column weights are small)
+constexpr int kSlice = 64;         // rows / variables per ELL slice (one wave)
+constexpr int kChunk = 8;          // check-row edges per chunk (two chunks per edge-code word)
+constexpr int kVChunk = 4;         // variable edges per chunk (column weights are small)
 constexpr int kEdgesPerWord = 16;  // 2-bit edge codes
 
-// State arrays, in the order they are given LDS (most accessed first).
+// State arrays.  Placement bit k puts array k in LDS.  With LS, CS and SB all in
+// LDS ("fast" placements 7 and 15) the layout is the canonical one the index
+// entries are baked for: CS at byte 0, SB at sbb, LS at lsb, L0 after LS.
 enum { kLS = 0, kCS = 1, kSB = 2, kL0 = 3, kNumArrays = 4 };
 
 struct GraphDev {
     int32_t C, V, E;
-    int32_t sb_words;           // edge-code words per shot (slice-padded)
+    uint32_t sbb, lsb;          // canonical LDS byte offsets of SB and LS
     // check side, by row position (rows sorted by decreasing degree)
     const int32_t *row_orig;    // [C]  original check index
     const int32_t *row_deg;     // [C]
-    const int32_t *rslice_idx;  // [C/64] first element of the slice in row_var
-    const int32_t *rslice_sb;   // [C/64] first edge-code word of the slice
-    const int32_t *row_var;     // ELL: element (k, lane) of slice s at rslice_idx[s] + 64k + lane
-    // variable side, natural order
+    const int4 *rslice;         // [C/64] {first ELL element, first code word, max degree, min degree}
+    const uint32_t *row_ent;    // ELL (k, lane) at rslice.x + 64k + lane: lsb + 8 * (variable position)
+    // variable side, by variable position (variables sorted by decreasing degree)
+    const int32_t *vperm;       // [V]  position -> caller's variable index
+    const int32_t *vpos;        // [V]  caller's variable index -> position
     const int32_t *var_deg;     // [V]
-    const int32_t *vslice_idx;  // [V/64]
-    const int2 *var_edge;       // ELL of {row position, code bit address}, ascending edge id
+    const int4 *vslice;         // [V/64] {first ELL element, max degree, min degree, 0}
+    const uint2 *var_ent;       // ELL {8 * row position, ((sbb + 4 * word) << 5) | bit}, ascending edge id
 };
 
 struct KArgs {
@@ -103,6 +107,7 @@ struct KArgs {
     int32_t prior_batched;
     const float *gamma_legs;
     int32_t gamma_per_row;
+    int32_t gamma_by_position;  // gamma_legs rows are in variable-position order
     const float *init_gamma;
     const float *init_post;
     uint8_t *errors;
@@ -118,34 +123,55 @@ struct KArgs {
     uint32_t off[kNumArrays];  // byte offsets of the state arrays (LDS or spill slot)
 };
 
-// One shot's working memory (LDS or a workspace slot, by placement).
-struct ShotMem {
-    float2 *LS;
-    float2 *CS;
-    uint32_t *SB;
-    float *L0s;        // the shot's prior LLRs copied to LDS (placement bit kL0)
-    const float *L0g;  // or read in place from the global prior
+template <int PLACE>
+struct IsFast {
+    static constexpr bool value = (PLACE & 7) == 7;
 };
 
-// The shot's prior LLR of variable j, from LDS or global memory as PLACE says.
+// One shot's working memory.  In the fast placements every access goes through
+// the LDS base and a baked byte offset; otherwise through per-array pointers.
 template <int PLACE>
-__device__ __forceinline__ float prior_at(const ShotMem &m, int j) {
-    if (PLACE & (1 << kL0)) return m.L0s[j];
-    return m.L0g[j];
-}
+struct ShotMem {
+    uint8_t *lds;       // dynamic LDS base
+    uint8_t *ls, *cs, *sb;  // array bases (LDS or workspace slot)
+    float *l0s;         // prior LLRs by position in LDS (placement bit kL0)
+    const float *l0g;   // or the caller's prior array (caller's order)
+    uint32_t lsb, sbb;
 
-// Load base[idx] with a 32-bit byte offset from a uniform base, so the load
-// uses the scalar-base + vector-offset addressing form.
-template <typename T>
-__device__ __forceinline__ T load_idx(const T *__restrict__ base, uint32_t idx) {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (idx * (uint32_t)sizeof(T)));
-}
+    static constexpr bool kFast = IsFast<PLACE>::value;
 
-// Memory strength source for one leg: a row of V values or a constant.
+    __device__ __forceinline__ float2 ls_ent(uint32_t ent) const {  // ent = lsb + 8 * position
+        if (kFast) return *reinterpret_cast<const float2 *>(lds + ent);
+        return *reinterpret_cast<const float2 *>(ls + (ent - lsb));
+    }
+    __device__ __forceinline__ float2 &ls_pos(int v) const {
+        return *reinterpret_cast<float2 *>((kFast ? lds + lsb : ls) + 8u * (uint32_t)v);
+    }
+    __device__ __forceinline__ float2 &cs_pos(int r) const {
+        return *reinterpret_cast<float2 *>((kFast ? lds : cs) + 8u * (uint32_t)r);
+    }
+    __device__ __forceinline__ float2 cs_ent(uint32_t e) const {  // e = 8 * row position
+        return *reinterpret_cast<const float2 *>((kFast ? lds : cs) + e);
+    }
+    __device__ __forceinline__ uint32_t &sb_word(uint32_t w) const {
+        return *reinterpret_cast<uint32_t *>((kFast ? lds + sbb : sb) + 4u * w);
+    }
+    __device__ __forceinline__ uint32_t sb_ent(uint32_t e) const {  // e = ((sbb + 4 word) << 5) | bit
+        if (kFast) return *reinterpret_cast<const uint32_t *>(lds + (e >> 5));
+        return *reinterpret_cast<const uint32_t *>(sb + ((e >> 5) - sbb));
+    }
+};
+
+// Memory strength of the variable at position v for one leg: a row of V values
+// (in position order or the caller's order) or a constant.
 struct GammaSrc {
     const float *row;
+    const int32_t *perm;  // non-null: row is in the caller's order
     float konst;
-    __device__ float operator()(int j) const { return row ? row[j] : konst; }
+    __device__ __forceinline__ float operator()(int v) const {
+        if (!row) return konst;
+        return perm ? row[perm[v]] : row[v];
+    }
 };
 
 struct ShotState {
@@ -167,6 +193,10 @@ __device__ __forceinline__ double uniform_f64(double v) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+__device__ __forceinline__ int4 uniform_i4(int4 v) {
+    return make_int4(uniform_i32(v.x), uniform_i32(v.y), uniform_i32(v.z), uniform_i32(v.w));
+}
+
 // An opaque copy of the thread index.  Branches on a particular thread (the
 // workgroup leader, lane 0 of a wave) test this value so that the optimizer
 // cannot carry the outcome of one such test to the next one: jump-threading
@@ -177,6 +207,13 @@ __device__ __forceinline__ unsigned opaque_tid() {
     unsigned v;
     asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "v"((unsigned)threadIdx.x));
     return v;
+}
+
+// Load base[idx] with a 32-bit byte offset from a uniform base, so the load
+// uses the scalar-base + vector-offset addressing form.
+template <typename T>
+__device__ __forceinline__ T load_idx(const T *__restrict__ base, uint32_t idx) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (idx * (uint32_t)sizeof(T)));
 }
 
 template <int BLOCK>
@@ -208,174 +245,195 @@ __device__ __forceinline__ int block_or(int pred, int *s_flag) {
     return uniform_i32(r);
 }
 
-// nu of one edge from its 2-bit code (sign, argmin) and its check's {P1, P2}.
+// nu of one edge from its 2-bit code (bit 0: sign, bit 1: argmin) and its
+// check's {P1, P2}: select, then flip the sign bit.
 __device__ __forceinline__ float edge_message(uint32_t code, float p1, float p2) {
-    const float p = (code & 2u) ? p2 : p1;
-    return (code & 1u) ? -p : p;
+    const float p = ((code & 2u) != 0u) ? p2 : p1;
+    return __uint_as_float(__float_as_uint(p) ^ (code << 31));
 }
 
+// Running min-sum reduction of one check row.  The reference's tie semantics
+// (kernels.py:1160-1178) are kept: the argmin edge is the first to attain the
+// minimum and m2 equals m1 when two edges share it; med3(m1, a, m2) is the new
+// second minimum for every order of a against m1 <= m2.
+struct RowMin {
+    float m1, m2;
+    uint32_t argk;
+    __device__ __forceinline__ void add(float a, uint32_t k) {
+        const bool lt = a < m1;
+        m2 = __builtin_amdgcn_fmed3f(m1, a, m2);
+        argk = lt ? k : argk;
+        m1 = lt ? a : m1;
+    }
+};
+
 // Check pass for row position pos (kernels.py:1128-1215).  FIRST: messages were
-// just reset (mu_e = lam0_j, kernels.py:981-1000), so nothing is reconstructed
+// just reset (mu_e = lam0_v, kernels.py:981-1000), so nothing is reconstructed
 // and no parity is tested.  Otherwise reconstructs mu_e from the previous
 // iteration's state and also returns 1 if the previous iteration's hard
 // decision violates the check (kernels.py:509-569 fused into the check pass).
 //
-// The row is walked in chunks of kChunk edges: the chunk's index loads are
-// issued together (slices are padded to whole chunks), then the LDS gathers,
-// then branch-free arithmetic in which a padding slot contributes
-// |mu| = +inf, no sign and no hard bit.  The min update keeps the reference's
-// tie semantics: m2 = min(m2, max(|mu|, m1)) equals m1 when two edges share the
-// minimum, and the argmin edge is the first to attain it.
+// The wave walks its slice in chunks of kChunk edges up to the slice's widest
+// row: the chunk's index loads are issued together, then the LDS gathers, then
+// the arithmetic.  Chunks below the slice's narrowest row need no per-edge
+// bounds test; in the others a slot past the row's end contributes nothing.
 template <bool FIRST, int PLACE>
-__device__ __forceinline__ int check_row(const KArgs &a, const ShotMem &m, int pos, float alpha,
+__device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m, int pos, float alpha,
                                          const uint8_t *__restrict__ syn) {
+    const int4 rs = uniform_i4(a.g.rslice[pos >> 6]);  // {ent, word, max deg, min deg}
+    const int lane = pos & 63;
     const int deg = a.g.row_deg[pos];
     const uint32_t s_i = syn[a.g.row_orig[pos]] & 1u;
-    if (deg == 0) return FIRST ? 0 : (int)s_i;
-    const int slice = pos >> 6, lane = pos & 63;
-    const uint32_t ioff = (uint32_t)(a.g.rslice_idx[slice] + lane);
-    const uint32_t woff = (uint32_t)(a.g.rslice_sb[slice] + lane);
-    const int32_t *__restrict__ rv = a.g.row_var;
+    const uint32_t ebase = (uint32_t)(rs.x + lane);
+    const uint32_t wbase = (uint32_t)(rs.y + lane);
+    const uint32_t *__restrict__ re = a.g.row_ent;
     float p1 = 0.f, p2 = 0.f;
-    if (!FIRST) {
-        const float2 cs = m.CS[pos];
+    if (!FIRST && deg > 0) {
+        const float2 cs = m.cs_pos(pos);
         p1 = cs.x;
         p2 = cs.y;
     }
-    float m1 = INFINITY, m2 = INFINITY;
-    uint32_t argk = 0, neg = 0, hpar = 0;
-    uint32_t wold = 0, wnew = 0;
-    for (int k0 = 0; k0 < deg; k0 += kChunk) {
-        const uint32_t word = woff + (uint32_t)(k0 >> 4) * kSlice;
+    RowMin mn{INFINITY, INFINITY, 0u};
+    bool hpar = false;
+    uint32_t neg = 0, wold = 0, wnew = 0;
+    for (int k0 = 0; k0 < rs.z; k0 += kChunk) {
+        const uint32_t word = wbase + (uint32_t)(k0 >> 4) * kSlice;
         if ((k0 & 15) == 0) {
-            if (!FIRST) wold = m.SB[word];
+            if (!FIRST) wold = m.sb_word(word);
             wnew = 0;
         }
-        int jv[kChunk];
+        uint32_t e[kChunk];
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u) jv[u] = load_idx(rv, ioff + (uint32_t)(k0 + u) * kSlice);
+        for (int u = 0; u < kChunk; ++u) e[u] = load_idx(re, ebase + (uint32_t)(k0 + u) * kSlice);
         float x[kChunk], y[kChunk];
 #pragma unroll
         for (int u = 0; u < kChunk; ++u) {
             if (FIRST) {
-                x[u] = prior_at<PLACE>(m, jv[u]);
+                const int v = (int)((e[u] - m.lsb) >> 3);
+                x[u] = (PLACE & (1 << kL0)) ? m.l0s[v] : m.l0g[a.g.vperm[v]];
                 y[u] = 0.f;
             } else {
-                const float2 ls = m.LS[jv[u]];
+                const float2 ls = m.ls_ent(e[u]);
                 x[u] = ls.x;
                 y[u] = ls.y;
             }
         }
+        const bool full = k0 + kChunk <= rs.w;
 #pragma unroll
         for (int u = 0; u < kChunk; ++u) {
-            const bool valid = (k0 + u) < deg;
-            const uint32_t sh = (uint32_t)(2 * ((k0 + u) & 15));
-            float mu;
-            if (FIRST) {
-                mu = x[u];
-            } else {
-                hpar ^= (valid && (x[u] + y[u] <= 0.f)) ? 1u : 0u;
+            const uint32_t k = (uint32_t)(k0 + u);
+            const uint32_t sh = 2u * (k & 15u);
+            const bool valid = full || (int)k < deg;
+            float mu = x[u];
+            if (!FIRST) {
+                hpar ^= valid && (x[u] + y[u] <= 0.f);
                 mu = x[u] + (y[u] - edge_message(wold >> sh, p1, p2));
             }
-            const uint32_t isneg = (valid && mu < 0.f) ? 1u : 0u;
-            neg ^= isneg;
-            wnew |= isneg << sh;
-            const float am = valid ? fabsf(mu) : INFINITY;
-            argk = (am < m1) ? (uint32_t)(k0 + u) : argk;
-            m2 = fminf(m2, fmaxf(am, m1));
-            m1 = fminf(m1, am);
+            wnew |= (valid && mu < 0.f) ? (1u << sh) : 0u;
+            mn.add(valid ? fabsf(mu) : INFINITY, k);
         }
-        if (((k0 + kChunk) & 15) == 0 || k0 + kChunk >= deg) m.SB[word] = wnew;
+        if (((k0 + kChunk) & 15) == 0 || k0 + kChunk >= rs.z) {
+            m.sb_word(word) = wnew;
+            neg ^= __popc(wnew);
+        }
     }
-    // Mark the edge holding the minimum (written after the sign words above).
-    m.SB[woff + (argk >> 4) * kSlice] |= 2u << (2 * (argk & 15));
-    float o1 = m1, o2 = m2;
-    if (deg == 1) o1 = o2 = 0.f;  // degree-1 checks send zero (kernels.py:1192-1193)
-    if (a.use_alpha) {
-        o1 = alpha * o1;
-        o2 = alpha * o2;
+    if (deg > 0) {
+        // Mark the edge holding the minimum (after the sign words above).
+        m.sb_word(wbase + (mn.argk >> 4) * kSlice) |= 2u << (2u * (mn.argk & 15u));
+        float o1 = mn.m1, o2 = mn.m2;
+        if (deg == 1) o1 = o2 = 0.f;  // degree-1 checks send zero (kernels.py:1192-1193)
+        if (a.use_alpha) {
+            o1 = alpha * o1;
+            o2 = alpha * o2;
+        }
+        if (a.use_beta) {
+            o1 = fmaxf(o1 - a.beta, 0.f);
+            o2 = fmaxf(o2 - a.beta, 0.f);
+        }
+        const bool par = (neg ^ s_i) & 1u;
+        m.cs_pos(pos) = make_float2(par ? -o1 : o1, par ? -o2 : o2);
     }
-    if (a.use_beta) {
-        o1 = fmaxf(o1 - a.beta, 0.f);
-        o2 = fmaxf(o2 - a.beta, 0.f);
-    }
-    const bool par = (neg ^ s_i) & 1u;
-    m.CS[pos] = make_float2(par ? -o1 : o1, par ? -o2 : o2);
-    return FIRST ? 0 : (int)((hpar ^ s_i) & 1u);
+    return FIRST ? 0 : (int)(hpar ^ (s_i != 0u));
 }
 
 // Parity of the current hard decision on row position pos (after the last
 // iteration of a leg).
-__device__ __forceinline__ int parity_row(const KArgs &a, const ShotMem &m, int pos,
+template <int PLACE>
+__device__ __forceinline__ int parity_row(const KArgs &a, const ShotMem<PLACE> &m, int pos,
                                           const uint8_t *__restrict__ syn) {
+    const int4 rs = uniform_i4(a.g.rslice[pos >> 6]);
     const int deg = a.g.row_deg[pos];
-    uint32_t hpar = syn[a.g.row_orig[pos]] & 1u;
-    const uint32_t ioff = (uint32_t)(a.g.rslice_idx[pos >> 6] + (pos & 63));
-    const int32_t *__restrict__ rv = a.g.row_var;
-    for (int k0 = 0; k0 < deg; k0 += kChunk) {
-        int jv[kChunk];
+    bool hpar = (syn[a.g.row_orig[pos]] & 1u) != 0u;
+    const uint32_t ebase = (uint32_t)(rs.x + (pos & 63));
+    const uint32_t *__restrict__ re = a.g.row_ent;
+    for (int k0 = 0; k0 < rs.z; k0 += kChunk) {
+        uint32_t e[kChunk];
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u) jv[u] = load_idx(rv, ioff + (uint32_t)(k0 + u) * kSlice);
+        for (int u = 0; u < kChunk; ++u) e[u] = load_idx(re, ebase + (uint32_t)(k0 + u) * kSlice);
 #pragma unroll
         for (int u = 0; u < kChunk; ++u) {
-            const float2 ls = m.LS[jv[u]];
-            hpar ^= ((k0 + u) < deg && (ls.x + ls.y <= 0.f)) ? 1u : 0u;
+            const float2 ls = m.ls_ent(e[u]);
+            hpar ^= (k0 + u) < deg && (ls.x + ls.y <= 0.f);
         }
     }
     return (int)hpar;
 }
 
-__device__ __forceinline__ float var_edge_message(const ShotMem &m, int2 e) {
-    const float2 cs = m.CS[e.x];
-    const uint32_t w = m.SB[(uint32_t)e.y >> 5];
-    return edge_message(w >> (e.y & 31), cs.x, cs.y);
-}
-
-// Variable pass for variable j (kernels.py:1299-1350).  The incoming messages
-// are summed one at a time in ascending edge id; whole chunks of edge loads are
-// issued together, the remainder one by one.
+// Variable pass for variable position v (kernels.py:1299-1350).  The incoming
+// messages are added one at a time in ascending edge id.
 template <int PLACE>
-__device__ __forceinline__ void var_update(const KArgs &a, const ShotMem &m, int j, const GammaSrc &gs) {
-    const float g = gs(j);
-    const float2 ls = m.LS[j];
+__device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> &m, int v, const GammaSrc &gs) {
+    const int4 vs = uniform_i4(a.g.vslice[v >> 6]);  // {ent, max deg, min deg, 0}
+    const int deg = a.g.var_deg[v];
+    const float g = gs(v);
+    const float l0 = (PLACE & (1 << kL0)) ? m.l0s[v] : m.l0g[a.g.vperm[v]];
+    float2 &lsr = m.ls_pos(v);
+    const float2 ls = lsr;
     const float post_prev = ls.x + ls.y;
-    const float lm = __fadd_rn(__fmul_rn(1.0f - g, prior_at<PLACE>(m, j)), __fmul_rn(g, post_prev));
-    const int deg = a.g.var_deg[j];
-    const uint32_t eoff = (uint32_t)(a.g.vslice_idx[j >> 6] + (j & 63));
-    const int2 *__restrict__ ve = a.g.var_edge;
+    const float lm = __fadd_rn(__fmul_rn(1.0f - g, l0), __fmul_rn(g, post_prev));
+    const uint32_t ebase = (uint32_t)(vs.x + (v & 63));
+    const uint2 *__restrict__ ve = a.g.var_ent;
     float s = 0.f;
-    int q = 0;
-    for (; q + kVChunk <= deg; q += kVChunk) {
-        int2 e[kVChunk];
+    for (int q0 = 0; q0 < vs.y; q0 += kVChunk) {
+        uint2 e[kVChunk];
 #pragma unroll
-        for (int u = 0; u < kVChunk; ++u) e[u] = load_idx(ve, eoff + (uint32_t)(q + u) * kSlice);
+        for (int u = 0; u < kVChunk; ++u) e[u] = load_idx(ve, ebase + (uint32_t)(q0 + u) * kSlice);
         float nu[kVChunk];
 #pragma unroll
-        for (int u = 0; u < kVChunk; ++u) nu[u] = var_edge_message(m, e[u]);
+        for (int u = 0; u < kVChunk; ++u) {
+            const float2 cs = m.cs_ent(e[u].x);
+            const uint32_t w = m.sb_ent(e[u].y);
+            nu[u] = edge_message(__builtin_amdgcn_ubfe(w, e[u].y & 31u, 2), cs.x, cs.y);
+        }
+        if (q0 + kVChunk <= vs.z) {
 #pragma unroll
-        for (int u = 0; u < kVChunk; ++u) s += nu[u];
+            for (int u = 0; u < kVChunk; ++u) s += nu[u];
+        } else {
+#pragma unroll
+            for (int u = 0; u < kVChunk; ++u)
+                if (q0 + u < deg) s += nu[u];
+        }
     }
-    for (; q < deg; ++q) s += var_edge_message(m, load_idx(ve, eoff + (uint32_t)q * kSlice));
-    m.LS[j] = make_float2(lm, s);
+    lsr = make_float2(lm, s);
 }
 
 // check_and_select_kernel (kernels.py:657-741) for a shot whose hard decision
 // satisfies every check: weight = sum of wj over bits with post <= 0 (double
 // accumulation, rounded once), strict improvement or first solution wins.
-template <int BLOCK>
-__device__ void select_solution(const KArgs &a, const ShotMem &m, const float *__restrict__ wj,
+template <int BLOCK, int PLACE>
+__device__ void select_solution(const KArgs &a, const ShotMem<PLACE> &m, const float *__restrict__ wj,
                                 uint8_t *err, ShotState &st, double *s_red) {
     const int V = a.g.V;
     double acc = 0.0;
     for (int j = threadIdx.x; j < V; j += BLOCK) {
-        const float2 ls = m.LS[j];
+        const float2 ls = m.ls_pos(a.g.vpos[j]);
         if (ls.x + ls.y <= 0.f) acc += (double)wj[j];
     }
     const float w = (float)block_sum<BLOCK>(acc, s_red);
     if (st.best_w >= 1e20f || w < st.best_w) {
         st.best_w = w;
         for (int j = threadIdx.x; j < V; j += BLOCK) {
-            const float2 ls = m.LS[j];
+            const float2 ls = m.ls_pos(a.g.vpos[j]);
             err[j] = (ls.x + ls.y <= 0.f) ? 1 : 0;
         }
     }
@@ -385,7 +443,7 @@ __device__ void select_solution(const KArgs &a, const ShotMem &m, const float *_
 // One leg of up to T iterations.  The shot leaves the leg at its first valid
 // solution (freeze_finished_lanes_kernel with sought = found + 1).
 template <int BLOCK, int PLACE>
-__device__ bool run_leg(const KArgs &a, const ShotMem &m, int T, const GammaSrc &gs,
+__device__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m, int T, const GammaSrc &gs,
                         const float *__restrict__ wj, const uint8_t *__restrict__ syn, uint8_t *err,
                         ShotState &st, double *s_red, int *s_flag) {
     const int C = a.g.C, V = a.g.V;
@@ -399,32 +457,32 @@ __device__ bool run_leg(const KArgs &a, const ShotMem &m, int T, const GammaSrc 
         }
         bad = block_or<BLOCK>(bad, s_flag);
         if (t > 0 && !bad) {  // iteration t-1 converged
-            select_solution<BLOCK>(a, m, wj, err, st, s_red);
+            select_solution<BLOCK, PLACE>(a, m, wj, err, st, s_red);
             return true;
         }
-        for (int j = threadIdx.x; j < V; j += BLOCK) var_update<PLACE>(a, m, j, gs);
+        for (int v = threadIdx.x; v < V; v += BLOCK) var_update<PLACE>(a, m, v, gs);
         __syncthreads();
         st.iters += 1;
         st.any_iter = 1;
     }
     if (T > 0) {
         int bad = 0;
-        for (int i = threadIdx.x; i < C; i += BLOCK) bad |= parity_row(a, m, i, syn);
+        for (int i = threadIdx.x; i < C; i += BLOCK) bad |= parity_row<PLACE>(a, m, i, syn);
         bad = block_or<BLOCK>(bad, s_flag);
         if (!bad) {
-            select_solution<BLOCK>(a, m, wj, err, st, s_red);
+            select_solution<BLOCK, PLACE>(a, m, wj, err, st, s_red);
             return true;
         }
     }
     return false;
 }
 
-template <int BLOCK>
-__device__ void write_hard(const KArgs &a, const ShotMem &m, uint8_t *err, bool any_iter) {
+template <int BLOCK, int PLACE>
+__device__ void write_hard(const KArgs &a, const ShotMem<PLACE> &m, uint8_t *err, bool any_iter) {
     for (int j = threadIdx.x; j < a.g.V; j += BLOCK) {
         uint8_t h = 0;
         if (any_iter) {
-            const float2 ls = m.LS[j];
+            const float2 ls = m.ls_pos(a.g.vpos[j]);
             h = (ls.x + ls.y <= 0.f) ? 1 : 0;
         }
         err[j] = h;
@@ -432,7 +490,7 @@ __device__ void write_hard(const KArgs &a, const ShotMem &m, uint8_t *err, bool 
 }
 
 template <int BLOCK, int PLACE>
-__device__ ShotState decode_shot(const KArgs &a, int b, ShotMem m, double *s_red, int *s_flag) {
+__device__ ShotState decode_shot(const KArgs &a, int b, ShotMem<PLACE> m, double *s_red, int *s_flag) {
     const int C = a.g.C, V = a.g.V;
     const uint8_t *syn = a.syn + (size_t)b * C;
     const size_t pofs = a.prior_batched ? (size_t)b * V : 0;
@@ -443,49 +501,57 @@ __device__ ShotState decode_shot(const KArgs &a, int b, ShotMem m, double *s_red
     // State reset (decoder.py:3430-3464): post = prior (or init_posterior),
     // stored as lam_mixed = post, S = 0 so that lam_mixed + S == post exactly.
     for (int j = threadIdx.x; j < V; j += BLOCK) {
+        const int v = a.g.vpos[j];
         const float lj = l0[j];
         const float p0 = a.init_post ? a.init_post[(size_t)b * V + j] : lj;
-        m.LS[j] = make_float2(p0, 0.f);
-        if (PLACE & (1 << kL0)) m.L0s[j] = lj;
+        m.ls_pos(v) = make_float2(p0, 0.f);
+        if (PLACE & (1 << kL0)) m.l0s[v] = lj;
         err[j] = 0;
     }
-    m.L0g = l0;
+    m.l0g = l0;
     __syncthreads();
 
     ShotState st{INFINITY, 0, 0, 0};
     const int S = a.stop_nconv;
 
     // Pre-iter leg (decoder.py:3470-3593).
-    GammaSrc g0{nullptr, a.plain ? 0.f : a.gamma0};
-    if (!a.plain && a.init_gamma) g0.row = a.init_gamma + (size_t)b * V;
+    GammaSrc g0{nullptr, nullptr, a.plain ? 0.f : a.gamma0};
+    if (!a.plain && a.init_gamma) {
+        g0.row = a.init_gamma + (size_t)b * V;
+        g0.perm = a.g.vperm;
+    }
     if (st.found < S) run_leg<BLOCK, PLACE>(a, m, a.pre_iter, g0, wj, syn, err, st, s_red, s_flag);
     // first_leg_hard_dec (decoder.py:3597-3598): kept in err while no solution exists.
-    if (!a.last_leg && st.found == 0) write_hard<BLOCK>(a, m, err, st.any_iter);
+    if (!a.last_leg && st.found == 0) write_hard<BLOCK, PLACE>(a, m, err, st.any_iter);
 
     if (!a.skip_relay_legs) {
         for (int leg = 0; leg < a.num_sets && st.found < S; ++leg) {
-            GammaSrc gl{a.gamma_legs + (a.gamma_per_row ? ((size_t)leg * a.B + b) * V : (size_t)leg * V),
-                        0.f};
+            GammaSrc gl{nullptr, nullptr, 0.f};
+            if (a.gamma_per_row) {
+                gl.row = a.gamma_legs + ((size_t)leg * a.B + b) * V;
+                gl.perm = a.g.vperm;
+            } else {
+                gl.row = a.gamma_legs + (size_t)leg * V;
+                gl.perm = a.gamma_by_position ? nullptr : a.g.vperm;
+            }
             run_leg<BLOCK, PLACE>(a, m, a.set_max_iter, gl, wj, syn, err, st, s_red, s_flag);
             if (leg == 0 && a.pre_iter == 0 && !a.last_leg && st.found == 0)
-                write_hard<BLOCK>(a, m, err, st.any_iter);  // decoder.py:3809-3810
+                write_hard<BLOCK, PLACE>(a, m, err, st.any_iter);  // decoder.py:3809-3810
         }
     }
     // last_leg correction: the live hard decision for shots with no solution.
-    if (a.last_leg && st.found == 0) write_hard<BLOCK>(a, m, err, st.any_iter);
+    if (a.last_leg && st.found == 0) write_hard<BLOCK, PLACE>(a, m, err, st.any_iter);
 
     if (a.last_post) {
         float *lp = a.last_post + (size_t)b * V;
         for (int j = threadIdx.x; j < V; j += BLOCK) {
-            const float2 ls = m.LS[j];
+            const float2 ls = m.ls_pos(a.g.vpos[j]);
             lp[j] = ls.x + ls.y;
         }
     }
     return st;
 }
 
-// PLACE bit k: state array k (kLS, kCS, kSB, kL0) in LDS; otherwise the
-// workgroup's workspace slot (kL0: the global prior is read in place).
 template <int BLOCK, int PLACE>
 __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -493,12 +559,15 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
     __shared__ int s_flag[BLOCK / 64];
     __shared__ int s_shot;
     uint8_t *slot = a.spill + (int64_t)blockIdx.x * a.spill_stride;
-    ShotMem m;
-    m.LS = reinterpret_cast<float2 *>(((PLACE & (1 << kLS)) ? smem : slot) + a.off[kLS]);
-    m.CS = reinterpret_cast<float2 *>(((PLACE & (1 << kCS)) ? smem : slot) + a.off[kCS]);
-    m.SB = reinterpret_cast<uint32_t *>(((PLACE & (1 << kSB)) ? smem : slot) + a.off[kSB]);
-    m.L0s = reinterpret_cast<float *>(smem + a.off[kL0]);
-    m.L0g = nullptr;
+    ShotMem<PLACE> m;
+    m.lds = smem;
+    m.ls = ((PLACE & (1 << kLS)) ? smem : slot) + a.off[kLS];
+    m.cs = ((PLACE & (1 << kCS)) ? smem : slot) + a.off[kCS];
+    m.sb = ((PLACE & (1 << kSB)) ? smem : slot) + a.off[kSB];
+    m.l0s = reinterpret_cast<float *>(smem + a.off[kL0]);
+    m.l0g = nullptr;
+    m.lsb = a.g.lsb;
+    m.sbb = a.g.sbb;
     // The leader writes the finished shot's scalars and takes the next ticket in
     // one branch, so the loop has a single leader-only region.
     int prev = -1;
@@ -562,9 +631,20 @@ __global__ void __launch_bounds__(kOrderBlock) order_shots_kernel(const uint8_t 
     for (int b = tid; b < B; b += kOrderBlock) order[atomicAdd(&bucket[weight[b]], 1)] = b;
 }
 
+// dst[r][v] = src[r][vperm[v]] for rows r: per-variable data into position order.
+__global__ void permute_vars_kernel(const float *src, float *dst, const int32_t *vperm, int32_t V, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i / V;
+        const int32_t v = (int32_t)(i - r * V);
+        dst[i] = src[r * V + vperm[v]];
+    }
+}
+
 typedef void (*KernelFn)(KArgs);
 
-// Placements are prefixes of the array order: LS, +CS, +SB, +L0.
+// Placements: 15 and 7 keep every state array but the priors (7) in LDS in the
+// canonical layout; 3 and 1 keep the variable state (and 3 also the check
+// state) in LDS; 0 keeps nothing there.
 constexpr int kPlacements[] = {0, 1, 3, 7, 15};
 
 template <int BLOCK>
@@ -628,12 +708,21 @@ WsLayout ws_layout(const pel_graph *g, int64_t B) {
 void place_arrays(const StateSizes &st, int place, uint32_t off[kNumArrays], int64_t *lds, int64_t *spill) {
     *lds = 0;
     *spill = 0;
+    if ((place & 7) == 7) {  // canonical LDS layout: CS | SB | LS | L0
+        off[kCS] = 0;
+        off[kSB] = (uint32_t)st.sz[kCS];
+        off[kLS] = (uint32_t)(st.sz[kCS] + st.sz[kSB]);
+        off[kL0] = (uint32_t)(st.sz[kCS] + st.sz[kSB] + st.sz[kLS]);
+        *lds = st.sz[kCS] + st.sz[kSB] + st.sz[kLS] + ((place & (1 << kL0)) ? st.sz[kL0] : 0);
+        if (!(place & (1 << kL0))) off[kL0] = 0;
+        return;
+    }
     for (int k = 0; k < kNumArrays; ++k) {
         if (place & (1 << k)) {
             off[k] = (uint32_t)*lds;
             *lds += st.sz[k];
         } else if (k == kL0) {
-            off[k] = 0;  // read from the global prior, no copy
+            off[k] = 0;  // read from the caller's prior, no copy
         } else {
             off[k] = (uint32_t)*spill;
             *spill += st.sz[k];
@@ -677,48 +766,62 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
         rpos[rorig[p]] = p;
         rdeg[p] = chk_ptr[rorig[p] + 1] - chk_ptr[rorig[p]];
     }
+    // Variable positions: variables sorted by decreasing degree (stable).
+    std::vector<int32_t> vdeg0(V, 0);
+    for (int32_t e = 0; e < E; ++e) vdeg0[chk_var[e]]++;
+    std::vector<int32_t> vperm(V), vposv(V), vdeg(V);
+    std::iota(vperm.begin(), vperm.end(), 0);
+    std::stable_sort(vperm.begin(), vperm.end(), [&](int32_t x, int32_t y) { return vdeg0[x] > vdeg0[y]; });
+    for (int32_t v = 0; v < V; ++v) {
+        vposv[vperm[v]] = v;
+        vdeg[v] = vdeg0[vperm[v]];
+    }
     // Row slices: width padded to whole chunks; edge-code words of 16 edges.
     const int32_t nrs = (C + kSlice - 1) / kSlice;
-    std::vector<int32_t> rs_idx(nrs), rs_sb(nrs);
+    std::vector<int4> rs(nrs);
     int64_t nidx = 0, nword = 0;
     for (int32_t s = 0; s < nrs; ++s) {
-        const int32_t w = rdeg[s * kSlice];  // widest row of the slice comes first
+        const int32_t w = rdeg[s * kSlice];                         // widest row comes first
+        const int32_t wmin = rdeg[std::min(C, (s + 1) * kSlice) - 1];  // narrowest comes last
         const int32_t wpad = (w + kChunk - 1) / kChunk * kChunk;
-        rs_idx[s] = (int32_t)nidx;
-        rs_sb[s] = (int32_t)nword;
+        rs[s] = make_int4((int32_t)nidx, (int32_t)nword, wpad, (std::min(C, (s + 1) * kSlice) - s * kSlice) == kSlice ? wmin : 0);
         nidx += (int64_t)wpad * kSlice;
-        nword += (int64_t)((w + kEdgesPerWord - 1) / kEdgesPerWord) * kSlice;
+        nword += (int64_t)((wpad + kEdgesPerWord - 1) / kEdgesPerWord) * kSlice;
     }
-    if (nidx > INT32_MAX / 2 || nword * 32 > INT32_MAX)
-        return fail(PEL_EINVAL, "pel_graph_create: graph too large for 32-bit indexing");
-    std::vector<int32_t> row_var(std::max<int64_t>(nidx, 1), 0);
+    const int64_t cs_bytes = align_up(8LL * C, 16);
+    const int64_t sb_bytes = align_up(4LL * std::max<int64_t>(nword, 1), 16);
+    const int64_t lsb = cs_bytes + sb_bytes;
+    if (nidx > INT32_MAX / 2 || lsb + 8LL * V > (1LL << 26))
+        return fail(PEL_EINVAL, "pel_graph_create: graph too large for 32-bit state addressing");
+    std::vector<uint32_t> row_ent(std::max<int64_t>(nidx, 1), (uint32_t)lsb);  // padding -> LS[0]
     for (int32_t p = 0; p < C; ++p) {
         const int32_t s = p / kSlice, lane = p % kSlice, i = rorig[p];
         for (int32_t k = 0; k < rdeg[p]; ++k)
-            row_var[rs_idx[s] + (int64_t)k * kSlice + lane] = chk_var[chk_ptr[i] + k];
+            row_ent[rs[s].x + (int64_t)k * kSlice + lane] = (uint32_t)(lsb + 8LL * vposv[chk_var[chk_ptr[i] + k]]);
     }
-    // Variable side: natural order, edges in ascending edge id (= ascending check).
-    std::vector<int32_t> vdeg(V, 0);
-    for (int32_t e = 0; e < E; ++e) vdeg[chk_var[e]]++;
+    // Variable slices (position order); entries in ascending edge id (= ascending check).
     const int32_t nvs = (V + kSlice - 1) / kSlice;
-    std::vector<int32_t> vs_idx(nvs);
+    std::vector<int4> vs(nvs);
     int64_t nve = 0;
     for (int32_t s = 0; s < nvs; ++s) {
-        int32_t w = 0;
-        for (int32_t j = s * kSlice; j < std::min(V, (s + 1) * kSlice); ++j) w = std::max(w, vdeg[j]);
-        vs_idx[s] = (int32_t)nve;
-        nve += (int64_t)w * kSlice;
+        const int32_t w = vdeg[s * kSlice];
+        const int32_t last = std::min(V, (s + 1) * kSlice) - 1;
+        const int32_t wmin = (last - s * kSlice + 1) == kSlice ? vdeg[last] : 0;
+        const int32_t wpad = (w + kVChunk - 1) / kVChunk * kVChunk;
+        vs[s] = make_int4((int32_t)nve, wpad, wmin, 0);
+        nve += (int64_t)wpad * kSlice;
     }
     if (nve > INT32_MAX / 2) return fail(PEL_EINVAL, "pel_graph_create: graph too large for 32-bit indexing");
-    std::vector<int2> var_edge(std::max<int64_t>(nve, 1), make_int2(0, 0));
+    const uint32_t pad_sb = (uint32_t)(cs_bytes << 5);  // word 0 of SB, bit 0
+    std::vector<uint2> var_ent(std::max<int64_t>(nve, 1), make_uint2(0u, pad_sb));
     std::vector<int32_t> fill(V, 0);
     for (int32_t i = 0; i < C; ++i) {
         const int32_t p = rpos[i], s = p / kSlice, lane = p % kSlice;
         for (int32_t e = chk_ptr[i]; e < chk_ptr[i + 1]; ++e) {
-            const int32_t j = chk_var[e], k = e - chk_ptr[i];
-            const int32_t word = rs_sb[s] + (k / kEdgesPerWord) * kSlice + lane;
-            var_edge[vs_idx[j / kSlice] + (int64_t)fill[j]++ * kSlice + (j % kSlice)] =
-                make_int2(p, word * 32 + 2 * (k % kEdgesPerWord));
+            const int32_t v = vposv[chk_var[e]], k = e - chk_ptr[i];
+            const int64_t word = (int64_t)rs[s].y + (k / kEdgesPerWord) * kSlice + lane;
+            const uint32_t sbaddr = (uint32_t)((cs_bytes + 4 * word) << 5) | (uint32_t)(2 * (k % kEdgesPerWord));
+            var_ent[vs[v / kSlice].x + (int64_t)fill[v]++ * kSlice + (v % kSlice)] = make_uint2(8u * (uint32_t)p, sbaddr);
         }
     }
     // One device allocation for every array.
@@ -728,19 +831,20 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
         off = align_up(off + std::max<int64_t>(bytes, 4), 256);
         return o;
     };
-    const int64_t o_ro = take(4LL * C), o_rd = take(4LL * C), o_rsi = take(4LL * nrs), o_rsb = take(4LL * nrs),
-                  o_rv = take(4LL * (int64_t)row_var.size()), o_vd = take(4LL * V), o_vsi = take(4LL * nvs),
-                  o_ve = take(8LL * (int64_t)var_edge.size());
+    const int64_t o_ro = take(4LL * C), o_rd = take(4LL * C), o_rs = take(16LL * nrs),
+                  o_re = take(4LL * (int64_t)row_ent.size()), o_vp = take(4LL * V), o_vq = take(4LL * V),
+                  o_vd = take(4LL * V), o_vs = take(16LL * nvs), o_ve = take(8LL * (int64_t)var_ent.size());
     const int64_t total = off;
     std::vector<uint8_t> host((size_t)total, 0);
     memcpy(host.data() + o_ro, rorig.data(), 4ULL * C);
     memcpy(host.data() + o_rd, rdeg.data(), 4ULL * C);
-    memcpy(host.data() + o_rsi, rs_idx.data(), 4ULL * nrs);
-    memcpy(host.data() + o_rsb, rs_sb.data(), 4ULL * nrs);
-    memcpy(host.data() + o_rv, row_var.data(), 4ULL * row_var.size());
+    memcpy(host.data() + o_rs, rs.data(), 16ULL * nrs);
+    memcpy(host.data() + o_re, row_ent.data(), 4ULL * row_ent.size());
+    memcpy(host.data() + o_vp, vperm.data(), 4ULL * V);
+    memcpy(host.data() + o_vq, vposv.data(), 4ULL * V);
     memcpy(host.data() + o_vd, vdeg.data(), 4ULL * V);
-    memcpy(host.data() + o_vsi, vs_idx.data(), 4ULL * nvs);
-    memcpy(host.data() + o_ve, var_edge.data(), 8ULL * var_edge.size());
+    memcpy(host.data() + o_vs, vs.data(), 16ULL * nvs);
+    memcpy(host.data() + o_ve, var_ent.data(), 8ULL * var_ent.size());
 
     pel_graph *g = new pel_graph();
     hipError_t e = hipGetDevice(&g->device);
@@ -764,15 +868,17 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
     g->d.C = C;
     g->d.V = V;
     g->d.E = E;
-    g->d.sb_words = (int32_t)nword;
+    g->d.sbb = (uint32_t)cs_bytes;
+    g->d.lsb = (uint32_t)lsb;
     g->d.row_orig = reinterpret_cast<const int32_t *>(base + o_ro);
     g->d.row_deg = reinterpret_cast<const int32_t *>(base + o_rd);
-    g->d.rslice_idx = reinterpret_cast<const int32_t *>(base + o_rsi);
-    g->d.rslice_sb = reinterpret_cast<const int32_t *>(base + o_rsb);
-    g->d.row_var = reinterpret_cast<const int32_t *>(base + o_rv);
+    g->d.rslice = reinterpret_cast<const int4 *>(base + o_rs);
+    g->d.row_ent = reinterpret_cast<const uint32_t *>(base + o_re);
+    g->d.vperm = reinterpret_cast<const int32_t *>(base + o_vp);
+    g->d.vpos = reinterpret_cast<const int32_t *>(base + o_vq);
     g->d.var_deg = reinterpret_cast<const int32_t *>(base + o_vd);
-    g->d.vslice_idx = reinterpret_cast<const int32_t *>(base + o_vsi);
-    g->d.var_edge = reinterpret_cast<const int2 *>(base + o_ve);
+    g->d.vslice = reinterpret_cast<const int4 *>(base + o_vs);
+    g->d.var_ent = reinterpret_cast<const uint2 *>(base + o_ve);
     *out = g;
     return PEL_OK;
 }
@@ -802,6 +908,18 @@ int pel_graph_info(const pel_graph *g, int32_t *C, int32_t *V, int32_t *E, int64
     return PEL_OK;
 }
 
+int pel_graph_permute_vars(const pel_graph *g, const float *src, float *dst, int64_t rows, void *stream) {
+    if (!g || (!src && rows > 0) || (!dst && rows > 0) || rows < 0)
+        return fail(PEL_EINVAL, "pel_graph_permute_vars: bad argument");
+    if (rows == 0) return PEL_OK;
+    const int64_t n = rows * (int64_t)g->V;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(permute_vars_kernel, dim3((unsigned)blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       src, dst, g->d.vperm, g->V, n);
+    PEL_HIP(hipGetLastError());
+    return PEL_OK;
+}
+
 int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32_t placement_override,
                     pel_plan *out) {
     if (!g || !out) return fail(PEL_EINVAL, "pel_plan_decode: NULL argument");
@@ -824,13 +942,11 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32
     const int64_t static_lds = 12LL * (block / 64) + 64;
     const int64_t budget = (int64_t)max_lds - static_lds;
     int place = 0;
-    {
-        int64_t used = 0;
-        for (int k = 0; k < kNumArrays; ++k) {
-            if (used + st.sz[k] > budget) break;
-            used += st.sz[k];
-            place |= 1 << k;
-        }
+    for (int p : kPlacements) {
+        uint32_t o[kNumArrays];
+        int64_t l = 0, s = 0;
+        place_arrays(st, p, o, &l, &s);
+        if (l <= budget) place = p;
     }
     if (placement_override >= 0) {
         bool known = false;
@@ -877,6 +993,8 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
         return fail(PEL_EINVAL, "pel_decode: NULL buffer");
     if (p->num_sets > 0 && !p->skip_relay_legs && !b->gamma_legs)
         return fail(PEL_EINVAL, "pel_decode: gamma_legs is NULL with num_sets > 0");
+    if (b->gamma_per_row && b->gamma_by_position)
+        return fail(PEL_EINVAL, "pel_decode: per-shot gamma rows are taken in the caller's order");
     if (p->use_alpha && !p->alpha_t) return fail(PEL_EINVAL, "pel_decode: alpha_t is NULL with use_alpha");
     int dev = 0;
     PEL_HIP(hipGetDevice(&dev));
@@ -913,6 +1031,7 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     a.prior_batched = b->prior_batched;
     a.gamma_legs = b->gamma_legs;
     a.gamma_per_row = b->gamma_per_row;
+    a.gamma_by_position = b->gamma_by_position;
     a.init_gamma = b->init_gamma;
     a.init_post = b->init_posterior;
     a.errors = b->errors;

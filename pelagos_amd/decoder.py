@@ -278,6 +278,7 @@ class RelayBPDecoder:
         self._plans: Dict[int, _native.PelPlan] = {}
         self._workspace: Optional[torch.Tensor] = None
         self._gamma_cache: Dict[Tuple, torch.Tensor] = {}
+        self._gamma_perm_cache: Dict[Tuple, Tuple[torch.Tensor, torch.Tensor]] = {}
         n_alpha = max(self.pre_iter, self.set_max_iter, 1)
         self._alpha_cache: Dict[int, torch.Tensor] = {}
         self._alpha_n = n_alpha
@@ -357,6 +358,22 @@ class RelayBPDecoder:
             self._gamma_cache = {k: v for k, v in self._gamma_cache.items() if k[0] != "per_row"}
         self._gamma_cache[key] = table
         return table
+
+    def _gamma_by_position(self, table: torch.Tensor) -> torch.Tensor:
+        """The [R, V] gamma table in the graph's internal variable order (cached
+        per source table; pel_graph_permute_vars)."""
+        key = (table.data_ptr(), tuple(table.shape))
+        hit = self._gamma_perm_cache.get(key)
+        if hit is not None and hit[0] is table:
+            return hit[1]
+        out = torch.empty_like(table)
+        with torch.cuda.device(self.device):
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+            _native.check(_native.lib().pel_graph_permute_vars(
+                self._handle.h, table.data_ptr(), out.data_ptr(), int(table.shape[0]),
+                ctypes.c_void_p(stream)), "pel_graph_permute_vars")
+        self._gamma_perm_cache = {key: (table, out)}
+        return out
 
     def _alpha_table(self, n: int) -> torch.Tensor:
         n = max(n, 1)
@@ -474,6 +491,8 @@ class RelayBPDecoder:
         plan = self.plan(B)
         ws = self._workspace_for(plan.workspace_bytes)
         per_row = gtab.dim() == 3
+        if not per_row:
+            gtab = self._gamma_by_position(gtab)
         params = _native.PelParams(
             pre_iter=pre_iter,
             num_sets=self.num_sets,
@@ -496,6 +515,7 @@ class RelayBPDecoder:
             prior_batched=int(prior_batched),
             gamma_legs=gtab.data_ptr(),
             gamma_per_row=int(per_row),
+            gamma_by_position=int(not per_row),
             init_gamma=_ptr(ig),
             init_posterior=_ptr(ip),
             errors=errors.data_ptr(),
