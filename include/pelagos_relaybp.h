@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define PEL_ABI_VERSION 1
+#define PEL_ABI_VERSION 2
 
 /* status codes */
 #define PEL_OK 0
@@ -71,7 +71,9 @@ typedef struct {
     float *last_posterior;       /* out [B,V] or NULL (decode(return_last_posterior=True)) */
     void *workspace;             /* device scratch of pel_plan.workspace_bytes */
     int64_t *shot_clock;         /* out [B,3] or NULL: device wall clock (100 MHz ticks) when the
-                                    shot started and ended, and the workgroup that ran it */
+                                    shot started and ended, and the workgroup that finished it */
+    void *ev_begin;              /* hipEvent_t or NULL: recorded on the stream right before the */
+    void *ev_end;                /* decode kernel and right after it (kernel-only timing) */
 } pel_batch;
 
 /* Launch plan for one (graph, B) pair: chosen by pel_plan_decode. */
@@ -83,6 +85,10 @@ typedef struct {
     int32_t lds_bytes;       /* dynamic LDS per workgroup */
     int64_t spill_stride;    /* workspace bytes per workgroup for state not in LDS */
     int64_t workspace_bytes; /* total workspace the caller must provide */
+    int32_t requeue_below;   /* shots whose first leg converged in fewer iterations are
+                                re-queued behind all first legs (their posterior is parked in
+                                the workspace); 0 = never.  Lowering it to 0 after planning is
+                                allowed; raising it needs a plan made with it. */
 } pel_plan;
 
 /* ABI version (PEL_ABI_VERSION). */
@@ -123,7 +129,10 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override,
 
 /* Decode a batch with one persistent launch: every shot runs the full
  * Relay-BP-S schedule (pre-iter leg, then relay legs until stop_nconv
- * solutions) inside one workgroup, and idle workgroups take the next shot.
+ * solutions) on one workgroup at a time, and idle workgroups take the next
+ * shot from a device queue: first every shot's pre-iter leg (heaviest
+ * syndromes first), then the relay legs of the shots re-queued after a quick
+ * first leg (pel_plan.requeue_below).  Results do not depend on the order.
  * Replaces RelayBPDecoder._decode_loop_single_gpu (decoder.py:3394-3873) and
  * the kernels it launches (kernels.py:446-1350). */
 int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, const pel_plan *plan,

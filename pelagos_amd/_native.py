@@ -13,7 +13,7 @@ from typing import Optional
 
 LIB_PATH = Path(__file__).resolve().parent / "libpelagos_relaybp.so"
 
-PEL_ABI_VERSION = 1
+PEL_ABI_VERSION = 2
 PEL_OK = 0
 PEL_EINVAL = -1
 PEL_EDEVICE = -2
@@ -58,6 +58,8 @@ class PelBatch(ctypes.Structure):
         ("last_posterior", _vp),
         ("workspace", _vp),
         ("shot_clock", _vp),
+        ("ev_begin", _vp),
+        ("ev_end", _vp),
     ]
 
 
@@ -69,6 +71,7 @@ class PelPlan(ctypes.Structure):
         ("lds_bytes", ctypes.c_int32),
         ("spill_stride", ctypes.c_int64),
         ("workspace_bytes", ctypes.c_int64),
+        ("requeue_below", ctypes.c_int32),
     ]
 
 

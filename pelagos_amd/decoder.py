@@ -289,6 +289,9 @@ class RelayBPDecoder:
         # When a list, decode() appends a (start, end) pair of HIP events
         # recorded on the decode stream around each kernel launch.
         self.kernel_events: Optional[list] = None
+        # Re-queue threshold of the launch plan (None: the library's default; 0:
+        # every shot runs to completion in one go).
+        self.requeue_below: Optional[int] = None
 
     # ------------------------------------------------------------------ helpers
     @staticmethod
@@ -388,6 +391,12 @@ class RelayBPDecoder:
         p = self._plans.get(B)
         if p is None:
             p = self._handle.plan(B, self._block_size, self._placement)
+            if self.requeue_below is not None:
+                if self.requeue_below < 0:
+                    raise ValueError("requeue_below must be >= 0")
+                if self.requeue_below > 0 and p.requeue_below == 0:
+                    raise ValueError("this plan has no workspace for re-queued shots")
+                p.requeue_below = int(self.requeue_below)
             self._plans[B] = p
         return p
 
@@ -529,16 +538,20 @@ class RelayBPDecoder:
         cur = torch.cuda.current_stream(self.device)
         ev = self.kernel_events
         if ev is not None:
+            # The library records the pair right around the decode kernel on this
+            # stream; recording them once here creates the underlying HIP events.
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record(cur)
+            e1.record(cur)
+            batch.ev_begin = e0.cuda_event
+            batch.ev_end = e1.cuda_event
         _native.check(
             _native.lib().pel_decode(self._handle.h, ctypes.byref(params), ctypes.byref(batch),
                                      ctypes.byref(plan), ctypes.c_void_p(cur.cuda_stream)),
             "pel_decode",
         )
         if ev is not None:
-            e1.record(cur)
             ev.append((e0, e1))
 
     def _assemble(self, errors, weights, iters, found, last_post) -> Dict[str, torch.Tensor]:

@@ -101,6 +101,21 @@ def test_banded_circuit_code_matches_oracle(placement):
     _compare(out, ref)
 
 
+@pytest.mark.parametrize("requeue_below", [0, 1000])
+def test_requeue_threshold_does_not_change_results(requeue_below):
+    """Shots parked between their first leg and their relay legs (every shot whose
+    first leg converged, with 1000) decode exactly as shots run in one go (0)."""
+    for H, p, B in ((codes.bb_72_12_6(), np.full(72, 0.03), 512),
+                    (codes.banded_circuit_code(seed=0)[0], codes.banded_circuit_code(seed=0)[1], 128)):
+        err = codes.sample_errors(p, B, seed=5)
+        syn = codes.syndromes_of(H, err)
+        dec = _decoder(H, p)
+        dec.requeue_below = requeue_below
+        out = _run(dec, syn)
+        assert dec.plan(B).requeue_below == requeue_below
+        _compare(out, _oracle_for(dec, syn))
+
+
 # --- schedule options ------------------------------------------------------------------------
 
 
