@@ -11,15 +11,15 @@
 //   * Per-shot state is kept in compressed min-sum form instead of per-edge
 //     messages (the reference's muT/nuT [E,B], decoder.py:1527-1529):
 //       variables  LS[v] = {lam_mixed_v, S_v}   (8 B)  S_v = sum of incoming nu
-//       checks     CS[r] = {P1_r, P2_r}         (8 B)  outgoing magnitudes m1, m2
-//                                                      with the check's sign applied
+//       checks     CS[r] = {P1, -P1, P2, -P2}   (16 B) outgoing magnitudes m1, m2 with
+//                                                      the check's sign applied, and negated
 //       edge codes SB    2 bits per edge: sign of the variable->check message and
 //                         "this edge holds the check's minimum"
 //     Every message is a pure function of this state, bit for bit:
-//       nu_e = sign_e ? -P : P,  P = argmin_e ? P2_r : P1_r
+//       nu_e = CS[r][code_e]  (code = sign + 2 argmin; "-" flips the sign bit)
 //       mu_e = lam_mixed_v + (S_v - nu_e)              (kernels.py:1339-1350)
 //       post_v = lam_mixed_v + S_v                       (kernels.py:1328)
-//     With the shot's prior LLRs this is 126 KB at 936 x 8785, which fits the
+//     With the shot's prior LLRs this is 133 KB at 936 x 8785, which fits the
 //     160 KB LDS of a CU, so the iteration loop touches HBM/L2 only for the graph
 //     indices (shared by every shot).
 //   * One iteration = check pass (a thread per check row, sequential over the
@@ -33,7 +33,10 @@
 //     read.  The index entries are the LDS byte addresses of the state they
 //     select, so a gather needs no address arithmetic.  Per-variable inputs
 //     and outputs (priors, gammas, error bits) stay in the caller's order and
-//     are mapped through the permutation.
+//     are mapped through the permutation.  Slots past a row's or a variable's
+//     degree point at sentinel state (a variable with LLR +inf, a check whose
+//     messages are +0), which leaves every result unchanged, so no lane tests
+//     its own degree.
 //
 // Floating point follows the reference kernels exactly (fp32, no FMA
 // contraction -- the build passes -ffp-contract=off and the memory mix uses
@@ -88,14 +91,16 @@ struct GraphDev {
     // check side, by row position (rows sorted by decreasing degree)
     const int32_t *row_orig;    // [C]  original check index
     const int32_t *row_deg;     // [C]
-    const int4 *rslice;         // [C/64] {first ELL element, first code word, max degree, min degree}
-    const uint32_t *row_ent;    // ELL (k, lane) at rslice.x + 64k + lane: lsb + 8 * (variable position)
+    const int4 *rslice;         // [C/64] {first ELL element, first code word, width (whole chunks), 0}
+    const uint32_t *row_ent;    // ELL (k, lane) at rslice.x + 64k + lane: lsb + 8 * (variable position);
+                                // padding: the sentinel variable at position V
     // variable side, by variable position (variables sorted by decreasing degree)
     const int32_t *vperm;       // [V]  position -> caller's variable index
     const int32_t *vpos;        // [V]  caller's variable index -> position
     const int32_t *var_deg;     // [V]
-    const int4 *vslice;         // [V/64] {first ELL element, max degree, min degree, 0}
-    const uint2 *var_ent;       // ELL {8 * row position, ((sbb + 4 * word) << 5) | bit}, ascending edge id
+    const int4 *vslice;         // [V/64] {first ELL element, max degree, 0, 0}
+    const uint2 *var_ent;       // ELL {16 * row position, ((sbb + 4 * word) << 5) | bit}, ascending edge id;
+                                // padding: the sentinel check at position C
 };
 
 struct KArgs {
@@ -129,6 +134,7 @@ struct KArgs {
     uint8_t *spill;
     int64_t spill_stride;
     uint32_t off[kNumArrays];  // byte offsets of the state arrays (LDS or spill slot)
+    uint32_t scratch_off;      // LDS byte offset of the workgroup's reduction scratch
 };
 
 template <int PLACE>
@@ -136,36 +142,69 @@ struct IsFast {
     static constexpr bool value = (PLACE & 7) == 7;
 };
 
-// One shot's working memory.  In the fast placements every access goes through
-// the LDS base and a baked byte offset; otherwise through per-array pointers.
+// LDS addresses as plain integers: the kernel has no static LDS, so the
+// dynamic region starts at LDS address 0 and a baked byte offset is the
+// address itself (checked on the host when a fast placement is planned).
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(3))) T *lds_at(uint32_t addr) {
+    return reinterpret_cast<__attribute__((address_space(3))) T *>(addr);
+}
+typedef float vec2f __attribute__((ext_vector_type(2)));
+typedef float vec4f __attribute__((ext_vector_type(4)));
+
+// One shot's working memory.  In the fast placements every access is an LDS
+// access at a baked address; otherwise it goes through per-array pointers.
 template <int PLACE>
 struct ShotMem {
-    uint8_t *lds;       // dynamic LDS base
     uint8_t *ls, *cs, *sb;  // array bases (LDS or workspace slot)
-    float *l0s;         // prior LLRs by position in LDS (placement bit kL0)
-    const float *l0g;   // or the caller's prior array (caller's order)
-    uint32_t lsb, sbb;
+    const float *l0g;   // the caller's prior array (caller's order) when kL0 is clear
+    uint32_t lsb, sbb, l0b;  // canonical LDS addresses of LS, SB and L0
 
     static constexpr bool kFast = IsFast<PLACE>::value;
 
     __device__ __forceinline__ float2 ls_ent(uint32_t ent) const {  // ent = lsb + 8 * position
-        if (kFast) return *reinterpret_cast<const float2 *>(lds + ent);
+        if (kFast) {
+            const vec2f r = *lds_at<vec2f>(ent);
+            return make_float2(r.x, r.y);
+        }
         return *reinterpret_cast<const float2 *>(ls + (ent - lsb));
     }
-    __device__ __forceinline__ float2 &ls_pos(int v) const {
-        return *reinterpret_cast<float2 *>((kFast ? lds + lsb : ls) + 8u * (uint32_t)v);
+    // Prior of the variable whose LS entry is ent (placement bit kL0, which
+    // only the fast placement 15 has): LS and L0 are parallel arrays.
+    __device__ __forceinline__ float l0_ent(uint32_t ent) const {
+        return *lds_at<float>((ent >> 1) + (l0b - (lsb >> 1)));
     }
-    __device__ __forceinline__ float2 &cs_pos(int r) const {
-        return *reinterpret_cast<float2 *>((kFast ? lds : cs) + 8u * (uint32_t)r);
+    __device__ __forceinline__ void ls_set(int v, float2 x) const {
+        if (kFast) *lds_at<vec2f>(lsb + 8u * (uint32_t)v) = vec2f{x.x, x.y};
+        else *reinterpret_cast<float2 *>(ls + 8u * (uint32_t)v) = x;
     }
-    __device__ __forceinline__ float2 cs_ent(uint32_t e) const {  // e = 8 * row position
-        return *reinterpret_cast<const float2 *>((kFast ? lds : cs) + e);
+    __device__ __forceinline__ float l0_pos(int v) const { return *lds_at<float>(l0b + 4u * (uint32_t)v); }
+    __device__ __forceinline__ void l0_set(int v, float x) const { *lds_at<float>(l0b + 4u * (uint32_t)v) = x; }
+    __device__ __forceinline__ float2 ls_pos(int v) const {
+        if (kFast) {
+            const vec2f r = *lds_at<vec2f>(lsb + 8u * (uint32_t)v);
+            return make_float2(r.x, r.y);
+        }
+        return *reinterpret_cast<const float2 *>(ls + 8u * (uint32_t)v);
     }
-    __device__ __forceinline__ uint32_t &sb_word(uint32_t w) const {
-        return *reinterpret_cast<uint32_t *>((kFast ? lds + sbb : sb) + 4u * w);
+    __device__ __forceinline__ void cs_set(int r, float4 x) const {
+        if (kFast) *lds_at<vec4f>(16u * (uint32_t)r) = vec4f{x.x, x.y, x.z, x.w};
+        else *reinterpret_cast<float4 *>(cs + 16u * (uint32_t)r) = x;
+    }
+    __device__ __forceinline__ float cs_msg(uint32_t e) const {  // e = 16 * row position + 4 * code
+        if (kFast) return *lds_at<float>(e);
+        return *reinterpret_cast<const float *>(cs + e);
+    }
+    __device__ __forceinline__ uint32_t sb_get(uint32_t w) const {
+        if (kFast) return *lds_at<uint32_t>(sbb + 4u * w);
+        return *reinterpret_cast<const uint32_t *>(sb + 4u * w);
+    }
+    __device__ __forceinline__ void sb_set(uint32_t w, uint32_t x) const {
+        if (kFast) *lds_at<uint32_t>(sbb + 4u * w) = x;
+        else *reinterpret_cast<uint32_t *>(sb + 4u * w) = x;
     }
     __device__ __forceinline__ uint32_t sb_ent(uint32_t e) const {  // e = ((sbb + 4 word) << 5) | bit
-        if (kFast) return *reinterpret_cast<const uint32_t *>(lds + (e >> 5));
+        if (kFast) return *lds_at<uint32_t>(e >> 5);
         return *reinterpret_cast<const uint32_t *>(sb + ((e >> 5) - sbb));
     }
 };
@@ -217,11 +256,13 @@ __device__ __forceinline__ unsigned opaque_tid() {
     return v;
 }
 
-// Load base[idx] with a 32-bit byte offset from a uniform base, so the load
-// uses the scalar-base + vector-offset addressing form.
+// Element u of one ELL chunk: the chunk base is uniform (scalar registers),
+// the lane's byte offset is a loop-invariant 32-bit register and u's offset a
+// compile-time immediate, so the load needs no per-element address arithmetic.
 template <typename T>
-__device__ __forceinline__ T load_idx(const T *__restrict__ base, uint32_t idx) {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (idx * (uint32_t)sizeof(T)));
+__device__ __forceinline__ T load_ell(const T *__restrict__ chunk, uint32_t lane_off, int u) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(chunk) + (size_t)lane_off +
+                                        (size_t)(u * kSlice * (int)sizeof(T)));
 }
 
 template <int BLOCK>
@@ -253,102 +294,97 @@ __device__ __forceinline__ int block_or(int pred, int *s_flag) {
     return uniform_i32(r);
 }
 
-// nu of one edge from its 2-bit code (bit 0: sign, bit 1: argmin) and its
-// check's {P1, P2}: select, then flip the sign bit.
-__device__ __forceinline__ float edge_message(uint32_t code, float p1, float p2) {
-    const float p = ((code & 2u) != 0u) ? p2 : p1;
-    return __uint_as_float(__float_as_uint(p) ^ (code << 31));
-}
-
-// Running min-sum reduction of one check row.  The reference's tie semantics
-// (kernels.py:1160-1178) are kept: the argmin edge is the first to attain the
-// minimum and m2 equals m1 when two edges share it; med3(m1, a, m2) is the new
-// second minimum for every order of a against m1 <= m2.
-struct RowMin {
-    float m1, m2;
-    uint32_t argk;
-    __device__ __forceinline__ void add(float a, uint32_t k) {
-        const bool lt = a < m1;
-        m2 = __builtin_amdgcn_fmed3f(m1, a, m2);
-        argk = lt ? k : argk;
-        m1 = lt ? a : m1;
-    }
-};
-
 // Check pass for row position pos (kernels.py:1128-1215).  FIRST: messages were
 // just reset (mu_e = lam0_v, kernels.py:981-1000), so nothing is reconstructed
-// and no parity is tested.  Otherwise reconstructs mu_e from the previous
-// iteration's state and also returns 1 if the previous iteration's hard
-// decision violates the check (kernels.py:509-569 fused into the check pass).
+// and no parity is tested.  Otherwise reconstructs mu_e = lam_mixed + (S - nu_e)
+// from the previous iteration's state and also returns 1 if the previous
+// iteration's hard decision violates the check (kernels.py:509-569 fused into
+// the check pass).
 //
 // The wave walks its slice in chunks of kChunk edges up to the slice's widest
 // row: the chunk's index loads are issued together, then the LDS gathers, then
-// the arithmetic.  Chunks below the slice's narrowest row need no per-edge
-// bounds test; in the others a slot past the row's end contributes nothing.
+// the arithmetic.  Slots past the row's end hold the sentinel variable (LLR
+// +inf), whose message is +inf: positive, never below the running minimum and
+// never a hard-decision 1, so it changes nothing.
+//
+// The running reduction keeps the reference's tie semantics (kernels.py:
+// 1160-1178): the argmin edge is the first to attain the minimum and m2 equals
+// m1 when two edges share it; med3(m1, a, m2) is the new second minimum for
+// every order of a against m1 <= m2.
 template <bool FIRST, int PLACE>
 __device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m, int pos, float alpha,
                                          const uint8_t *__restrict__ syn) {
-    const int4 rs = uniform_i4(a.g.rslice[pos >> 6]);  // {ent, word, max deg, min deg}
+    const int4 rs = uniform_i4(a.g.rslice[pos >> 6]);  // {ent, word, width, 0}
     const int lane = pos & 63;
     const int deg = a.g.row_deg[pos];
     const uint32_t s_i = syn[a.g.row_orig[pos]] & 1u;
-    const uint32_t ebase = (uint32_t)(rs.x + lane);
     const uint32_t wbase = (uint32_t)(rs.y + lane);
-    const uint32_t *__restrict__ re = a.g.row_ent;
-    float p1 = 0.f, p2 = 0.f;
-    if (!FIRST && deg > 0) {
-        const float2 cs = m.cs_pos(pos);
-        p1 = cs.x;
-        p2 = cs.y;
-    }
-    RowMin mn{INFINITY, INFINITY, 0u};
-    bool hpar = false;
+    const uint32_t lane_off = 4u * (uint32_t)lane;
+    const uint32_t qrow = 16u * (uint32_t)pos;  // this row's messages CS[pos][0..3]
+    float m1 = INFINITY, m2 = INFINITY;
+    uint32_t argk = 0, argu = 0;
+    uint64_t hpar = 0;  // lane mask: parity of hard decisions so far
     uint32_t neg = 0, wold = 0, wnew = 0;
     for (int k0 = 0; k0 < rs.z; k0 += kChunk) {
         const uint32_t word = wbase + (uint32_t)(k0 >> 4) * kSlice;
+        const uint32_t half = (uint32_t)(k0 & 8) * 2u;  // bit offset of this chunk in its word
         if ((k0 & 15) == 0) {
-            if (!FIRST) wold = m.sb_word(word);
+            if (!FIRST) wold = m.sb_get(word);
             wnew = 0;
         }
+        const uint32_t *__restrict__ chunk = a.g.row_ent + rs.x + (int64_t)k0 * kSlice;
         uint32_t e[kChunk];
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u) e[u] = load_idx(re, ebase + (uint32_t)(k0 + u) * kSlice);
-        float x[kChunk], y[kChunk];
+        for (int u = 0; u < kChunk; ++u) e[u] = load_ell(chunk, lane_off, u);
+        float x[kChunk], y[kChunk], nuo[kChunk];
+        const uint32_t w4 = FIRST ? 0u : ((wold >> half) << 2);  // slot u's code times 4 at bit 2u
 #pragma unroll
         for (int u = 0; u < kChunk; ++u) {
             if (FIRST) {
-                const int v = (int)((e[u] - m.lsb) >> 3);
-                x[u] = (PLACE & (1 << kL0)) ? m.l0s[v] : m.l0g[a.g.vperm[v]];
+                if (PLACE & (1 << kL0)) {
+                    x[u] = m.l0_ent(e[u]);
+                } else {
+                    const int v = (int)((e[u] - m.lsb) >> 3);
+                    x[u] = v < a.g.V ? m.l0g[a.g.vperm[v]] : INFINITY;
+                }
                 y[u] = 0.f;
+                nuo[u] = 0.f;
             } else {
                 const float2 ls = m.ls_ent(e[u]);
                 x[u] = ls.x;
                 y[u] = ls.y;
+                nuo[u] = m.cs_msg(((w4 >> (2u * (uint32_t)u)) & 12u) | qrow);
             }
         }
-        const bool full = k0 + kChunk <= rs.w;
+        uint32_t bits = 0;
+        bool imp = false;
 #pragma unroll
         for (int u = 0; u < kChunk; ++u) {
-            const uint32_t k = (uint32_t)(k0 + u);
-            const uint32_t sh = 2u * (k & 15u);
-            const bool valid = full || (int)k < deg;
             float mu = x[u];
             if (!FIRST) {
-                hpar ^= valid && (x[u] + y[u] <= 0.f);
-                mu = x[u] + (y[u] - edge_message(wold >> sh, p1, p2));
+                hpar ^= __ballot(x[u] + y[u] <= 0.f);
+                mu = x[u] + (y[u] - nuo[u]);
             }
-            wnew |= (valid && mu < 0.f) ? (1u << sh) : 0u;
-            mn.add(valid ? fabsf(mu) : INFINITY, k);
+            bits |= (mu < 0.f ? 1u : 0u) << (2u * (uint32_t)u);
+            const float am = fabsf(mu);
+            const bool lt = am < m1;
+            m2 = __builtin_amdgcn_fmed3f(m1, am, m2);
+            argu = lt ? (uint32_t)u : argu;
+            imp |= lt;
+            m1 = lt ? am : m1;
         }
+        if (imp) argk = (uint32_t)k0 + argu;
+        wnew |= bits << half;
         if (((k0 + kChunk) & 15) == 0 || k0 + kChunk >= rs.z) {
-            m.sb_word(word) = wnew;
+            m.sb_set(word, wnew);
             neg ^= __popc(wnew);
         }
     }
     if (deg > 0) {
         // Mark the edge holding the minimum (after the sign words above).
-        m.sb_word(wbase + (mn.argk >> 4) * kSlice) |= 2u << (2u * (mn.argk & 15u));
-        float o1 = mn.m1, o2 = mn.m2;
+        const uint32_t aw = wbase + (argk >> 4) * kSlice;
+        m.sb_set(aw, m.sb_get(aw) | (2u << (2u * (argk & 15u))));
+        float o1 = m1, o2 = m2;
         if (deg == 1) o1 = o2 = 0.f;  // degree-1 checks send zero (kernels.py:1192-1193)
         if (a.use_alpha) {
             o1 = alpha * o1;
@@ -358,71 +394,78 @@ __device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m
             o1 = fmaxf(o1 - a.beta, 0.f);
             o2 = fmaxf(o2 - a.beta, 0.f);
         }
-        const bool par = (neg ^ s_i) & 1u;
-        m.cs_pos(pos) = make_float2(par ? -o1 : o1, par ? -o2 : o2);
+        const uint32_t par = ((neg ^ s_i) & 1u) << 31;
+        const uint32_t p1 = __float_as_uint(o1) ^ par, p2 = __float_as_uint(o2) ^ par;
+        m.cs_set(pos, make_float4(__uint_as_float(p1), __uint_as_float(p1 ^ 0x80000000u), __uint_as_float(p2),
+                                      __uint_as_float(p2 ^ 0x80000000u)));
     }
-    return FIRST ? 0 : (int)(hpar ^ (s_i != 0u));
+    const bool hp = (hpar >> (threadIdx.x & 63)) & 1ull;
+    return FIRST ? 0 : (int)(hp ^ (s_i != 0u));
 }
 
 // Parity of the current hard decision on row position pos (after the last
-// iteration of a leg).
+// iteration of a leg); padding slots read the sentinel variable (+inf, a 0).
 template <int PLACE>
 __device__ __forceinline__ int parity_row(const KArgs &a, const ShotMem<PLACE> &m, int pos,
                                           const uint8_t *__restrict__ syn) {
     const int4 rs = uniform_i4(a.g.rslice[pos >> 6]);
-    const int deg = a.g.row_deg[pos];
     bool hpar = (syn[a.g.row_orig[pos]] & 1u) != 0u;
-    const uint32_t ebase = (uint32_t)(rs.x + (pos & 63));
-    const uint32_t *__restrict__ re = a.g.row_ent;
+    const uint32_t lane_off = 4u * (uint32_t)(pos & 63);
     for (int k0 = 0; k0 < rs.z; k0 += kChunk) {
+        const uint32_t *__restrict__ chunk = a.g.row_ent + rs.x + (int64_t)k0 * kSlice;
         uint32_t e[kChunk];
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u) e[u] = load_idx(re, ebase + (uint32_t)(k0 + u) * kSlice);
+        for (int u = 0; u < kChunk; ++u) e[u] = load_ell(chunk, lane_off, u);
 #pragma unroll
         for (int u = 0; u < kChunk; ++u) {
             const float2 ls = m.ls_ent(e[u]);
-            hpar ^= (k0 + u) < deg && (ls.x + ls.y <= 0.f);
+            hpar ^= (ls.x + ls.y <= 0.f);
         }
     }
     return (int)hpar;
+}
+
+// W consecutive incoming messages of one variable, added to s in slot order
+// (ascending edge id).  Entry {16 * row position, ((sbb + 4 word) << 5) | bit}:
+// the 2-bit code is read at the bit offset held in the entry's low 5 bits
+// (v_bfe takes its offset modulo 32) and selects CS[row][code].  Padding slots
+// select the sentinel check's +0, which leaves s unchanged (s is never -0).
+template <int W, int PLACE>
+__device__ __forceinline__ void var_messages(const ShotMem<PLACE> &m, const uint2 *__restrict__ chunk,
+                                             uint32_t lane_off, float &s) {
+    uint2 e[W];
+#pragma unroll
+    for (int u = 0; u < W; ++u) e[u] = load_ell(chunk, lane_off, u);
+    float nu[W];
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+        const uint32_t w = m.sb_ent(e[u].y);
+        nu[u] = m.cs_msg(e[u].x + 4u * __builtin_amdgcn_ubfe(w, e[u].y, 2u));
+    }
+#pragma unroll
+    for (int u = 0; u < W; ++u) s += nu[u];
 }
 
 // Variable pass for variable position v (kernels.py:1299-1350).  The incoming
 // messages are added one at a time in ascending edge id.
 template <int PLACE>
 __device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> &m, int v, const GammaSrc &gs) {
-    const int4 vs = uniform_i4(a.g.vslice[v >> 6]);  // {ent, max deg, min deg, 0}
-    const int deg = a.g.var_deg[v];
+    const int4 vs = uniform_i4(a.g.vslice[v >> 6]);  // {ent, width, 0, 0}
     const float g = gs(v);
-    const float l0 = (PLACE & (1 << kL0)) ? m.l0s[v] : m.l0g[a.g.vperm[v]];
-    float2 &lsr = m.ls_pos(v);
-    const float2 ls = lsr;
+    const float l0 = (PLACE & (1 << kL0)) ? m.l0_pos(v) : m.l0g[a.g.vperm[v]];
+    const float2 ls = m.ls_pos(v);
     const float post_prev = ls.x + ls.y;
     const float lm = __fadd_rn(__fmul_rn(1.0f - g, l0), __fmul_rn(g, post_prev));
-    const uint32_t ebase = (uint32_t)(vs.x + (v & 63));
-    const uint2 *__restrict__ ve = a.g.var_ent;
+    const uint32_t lane_off = 8u * (uint32_t)(v & 63);
+    const uint2 *__restrict__ base = a.g.var_ent + vs.x;
     float s = 0.f;
-    for (int q0 = 0; q0 < vs.y; q0 += kVChunk) {
-        uint2 e[kVChunk];
-#pragma unroll
-        for (int u = 0; u < kVChunk; ++u) e[u] = load_idx(ve, ebase + (uint32_t)(q0 + u) * kSlice);
-        float nu[kVChunk];
-#pragma unroll
-        for (int u = 0; u < kVChunk; ++u) {
-            const float2 cs = m.cs_ent(e[u].x);
-            const uint32_t w = m.sb_ent(e[u].y);
-            nu[u] = edge_message(__builtin_amdgcn_ubfe(w, e[u].y & 31u, 2), cs.x, cs.y);
-        }
-        if (q0 + kVChunk <= vs.z) {
-#pragma unroll
-            for (int u = 0; u < kVChunk; ++u) s += nu[u];
-        } else {
-#pragma unroll
-            for (int u = 0; u < kVChunk; ++u)
-                if (q0 + u < deg) s += nu[u];
-        }
-    }
-    lsr = make_float2(lm, s);
+    int q = 0;
+    for (; q + kVChunk <= vs.y; q += kVChunk) var_messages<kVChunk, PLACE>(m, base + (int64_t)q * kSlice, lane_off, s);
+    const int rest = vs.y - q;
+    if (rest == 3) var_messages<3, PLACE>(m, base + (int64_t)q * kSlice, lane_off, s);
+    else if (rest == 2) var_messages<2, PLACE>(m, base + (int64_t)q * kSlice, lane_off, s);
+    else if (rest == 1) var_messages<1, PLACE>(m, base + (int64_t)q * kSlice, lane_off, s);
+    m.ls_set(v, make_float2(lm, s));
 }
 
 // check_and_select_kernel (kernels.py:657-741) for a shot whose hard decision
@@ -523,8 +566,8 @@ __device__ bool first_leg(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &s
         const int v = a.g.vpos[j];
         const float lj = io.l0[j];
         const float p0 = a.init_post ? a.init_post[(size_t)b * V + j] : lj;
-        m.ls_pos(v) = make_float2(p0, 0.f);
-        if (PLACE & (1 << kL0)) m.l0s[v] = lj;
+        m.ls_set(v, make_float2(p0, 0.f));
+        if (PLACE & (1 << kL0)) m.l0_set(v, lj);
         io.err[j] = 0;
     }
     m.l0g = io.l0;
@@ -606,8 +649,8 @@ __device__ void resume_shot(const KArgs &a, int b, ShotMem<PLACE> &m) {
     const ShotIO io = shot_io(a, b);
     const float *post = a.rq_post + (size_t)b * V;
     for (int v = threadIdx.x; v < V; v += BLOCK) {
-        m.ls_pos(v) = make_float2(post[v], 0.f);
-        if (PLACE & (1 << kL0)) m.l0s[v] = io.l0[a.g.vperm[v]];
+        m.ls_set(v, make_float2(post[v], 0.f));
+        if (PLACE & (1 << kL0)) m.l0_set(v, io.l0[a.g.vperm[v]]);
     }
     m.l0g = io.l0;
     __syncthreads();
@@ -634,20 +677,27 @@ __device__ int wait_requeued(const KArgs &a, int t2) {
 
 template <int BLOCK, int PLACE>
 __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
+    // All LDS is dynamic (state arrays, then the reduction scratch), so the
+    // baked state addresses are absolute LDS addresses.
     extern __shared__ __align__(16) uint8_t smem[];
-    __shared__ double s_red[BLOCK / 64];
-    __shared__ int s_flag[BLOCK / 64];
-    __shared__ int s_next[2];
+    double *s_red = reinterpret_cast<double *>(smem + a.scratch_off);
+    int *s_flag = reinterpret_cast<int *>(smem + a.scratch_off + 8u * (BLOCK / 64));
+    int *s_next = s_flag + BLOCK / 64;
     uint8_t *slot = a.spill + (int64_t)blockIdx.x * a.spill_stride;
     ShotMem<PLACE> m;
-    m.lds = smem;
     m.ls = ((PLACE & (1 << kLS)) ? smem : slot) + a.off[kLS];
     m.cs = ((PLACE & (1 << kCS)) ? smem : slot) + a.off[kCS];
     m.sb = ((PLACE & (1 << kSB)) ? smem : slot) + a.off[kSB];
-    m.l0s = reinterpret_cast<float *>(smem + a.off[kL0]);
     m.l0g = nullptr;
     m.lsb = a.g.lsb;
     m.sbb = a.g.sbb;
+    m.l0b = a.off[kL0];
+    // Sentinels that padding slots point at; no pass ever writes them.
+    if (opaque_tid() == 0) {
+        m.ls_set(a.g.V, make_float2(INFINITY, 0.f));
+        if (PLACE & (1 << kL0)) m.l0_set(a.g.V, INFINITY);
+        m.cs_set(a.g.C, make_float4(0.f, 0.f, 0.f, 0.f));
+    }
     // The leader writes the finished shot's scalars and takes the next ticket in
     // one branch: first-leg tickets in queue order, then re-queued shots.
     int prev = -1;
@@ -823,8 +873,9 @@ struct pel_graph {
     void *dev_mem;  // one allocation holding every array of d
     GraphDev d;
     StateSizes state() const {
-        return {{align_up(8LL * V, 16), align_up(8LL * C, 16), align_up(4LL * std::max(sb_words, 1), 16),
-                 align_up(4LL * V, 16)}};
+        // each array has one sentinel entry past its last variable or check
+        return {{align_up(8LL * (V + 1), 16), align_up(16LL * (C + 1), 16), align_up(4LL * std::max(sb_words, 1), 16),
+                 align_up(4LL * (V + 1), 16)}};
     }
 };
 
@@ -849,8 +900,10 @@ WsLayout ws_layout(const pel_graph *g, int64_t B, bool requeue) {
     return w;
 }
 
-// Byte offsets of the state arrays for a placement; LDS and spill totals.
-void place_arrays(const StateSizes &st, int place, uint32_t off[kNumArrays], int64_t *lds, int64_t *spill) {
+// Byte offsets of the state arrays for a placement and of the workgroup's
+// reduction scratch (always in LDS, after the arrays); LDS and spill totals.
+void place_arrays(const StateSizes &st, int place, int block, uint32_t off[kNumArrays], uint32_t *scratch,
+                  int64_t *lds, int64_t *spill) {
     *lds = 0;
     *spill = 0;
     if ((place & 7) == 7) {  // canonical LDS layout: CS | SB | LS | L0
@@ -860,19 +913,21 @@ void place_arrays(const StateSizes &st, int place, uint32_t off[kNumArrays], int
         off[kL0] = (uint32_t)(st.sz[kCS] + st.sz[kSB] + st.sz[kLS]);
         *lds = st.sz[kCS] + st.sz[kSB] + st.sz[kLS] + ((place & (1 << kL0)) ? st.sz[kL0] : 0);
         if (!(place & (1 << kL0))) off[kL0] = 0;
-        return;
-    }
-    for (int k = 0; k < kNumArrays; ++k) {
-        if (place & (1 << k)) {
-            off[k] = (uint32_t)*lds;
-            *lds += st.sz[k];
-        } else if (k == kL0) {
-            off[k] = 0;  // read from the caller's prior, no copy
-        } else {
-            off[k] = (uint32_t)*spill;
-            *spill += st.sz[k];
+    } else {
+        for (int k = 0; k < kNumArrays; ++k) {
+            if (place & (1 << k)) {
+                off[k] = (uint32_t)*lds;
+                *lds += st.sz[k];
+            } else if (k == kL0) {
+                off[k] = 0;  // read from the caller's prior, no copy
+            } else {
+                off[k] = (uint32_t)*spill;
+                *spill += st.sz[k];
+            }
         }
     }
+    *scratch = (uint32_t)*lds;
+    *lds += align_up(12LL * (block / 64) + 8, 16);  // s_red doubles, s_flag ints, s_next[2]
 }
 
 }  // namespace
@@ -927,18 +982,19 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
     int64_t nidx = 0, nword = 0;
     for (int32_t s = 0; s < nrs; ++s) {
         const int32_t w = rdeg[s * kSlice];                         // widest row comes first
-        const int32_t wmin = rdeg[std::min(C, (s + 1) * kSlice) - 1];  // narrowest comes last
         const int32_t wpad = (w + kChunk - 1) / kChunk * kChunk;
-        rs[s] = make_int4((int32_t)nidx, (int32_t)nword, wpad, (std::min(C, (s + 1) * kSlice) - s * kSlice) == kSlice ? wmin : 0);
+        rs[s] = make_int4((int32_t)nidx, (int32_t)nword, wpad, 0);
         nidx += (int64_t)wpad * kSlice;
         nword += (int64_t)((wpad + kEdgesPerWord - 1) / kEdgesPerWord) * kSlice;
     }
-    const int64_t cs_bytes = align_up(8LL * C, 16);
+    // Canonical LDS layout (must match pel_graph::state and place_arrays).
+    const int64_t cs_bytes = align_up(16LL * (C + 1), 16);
     const int64_t sb_bytes = align_up(4LL * std::max<int64_t>(nword, 1), 16);
     const int64_t lsb = cs_bytes + sb_bytes;
-    if (nidx > INT32_MAX / 2 || lsb + 8LL * V > (1LL << 26))
+    if (nidx > INT32_MAX / 2 || lsb + 8LL * (V + 1) > (1LL << 26))
         return fail(PEL_EINVAL, "pel_graph_create: graph too large for 32-bit state addressing");
-    std::vector<uint32_t> row_ent(std::max<int64_t>(nidx, 1), (uint32_t)lsb);  // padding -> LS[0]
+    // padding -> the sentinel variable at position V
+    std::vector<uint32_t> row_ent(std::max<int64_t>(nidx, 1), (uint32_t)(lsb + 8LL * V));
     for (int32_t p = 0; p < C; ++p) {
         const int32_t s = p / kSlice, lane = p % kSlice, i = rorig[p];
         for (int32_t k = 0; k < rdeg[p]; ++k)
@@ -949,16 +1005,14 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
     std::vector<int4> vs(nvs);
     int64_t nve = 0;
     for (int32_t s = 0; s < nvs; ++s) {
-        const int32_t w = vdeg[s * kSlice];
-        const int32_t last = std::min(V, (s + 1) * kSlice) - 1;
-        const int32_t wmin = (last - s * kSlice + 1) == kSlice ? vdeg[last] : 0;
-        const int32_t wpad = (w + kVChunk - 1) / kVChunk * kVChunk;
-        vs[s] = make_int4((int32_t)nve, wpad, wmin, 0);
-        nve += (int64_t)wpad * kSlice;
+        const int32_t w = vdeg[s * kSlice];  // widest variable comes first
+        vs[s] = make_int4((int32_t)nve, w, 0, 0);
+        nve += (int64_t)w * kSlice;
     }
     if (nve > INT32_MAX / 2) return fail(PEL_EINVAL, "pel_graph_create: graph too large for 32-bit indexing");
-    const uint32_t pad_sb = (uint32_t)(cs_bytes << 5);  // word 0 of SB, bit 0
-    std::vector<uint2> var_ent(std::max<int64_t>(nve, 1), make_uint2(0u, pad_sb));
+    // padding -> the sentinel check at position C (all four messages +0), code word 0
+    const uint32_t pad_sb = (uint32_t)(cs_bytes << 5);
+    std::vector<uint2> var_ent(std::max<int64_t>(nve, 1), make_uint2(16u * (uint32_t)C, pad_sb));
     std::vector<int32_t> fill(V, 0);
     for (int32_t i = 0; i < C; ++i) {
         const int32_t p = rpos[i], s = p / kSlice, lane = p % kSlice;
@@ -966,7 +1020,7 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
             const int32_t v = vposv[chk_var[e]], k = e - chk_ptr[i];
             const int64_t word = (int64_t)rs[s].y + (k / kEdgesPerWord) * kSlice + lane;
             const uint32_t sbaddr = (uint32_t)((cs_bytes + 4 * word) << 5) | (uint32_t)(2 * (k % kEdgesPerWord));
-            var_ent[vs[v / kSlice].x + (int64_t)fill[v]++ * kSlice + (v % kSlice)] = make_uint2(8u * (uint32_t)p, sbaddr);
+            var_ent[vs[v / kSlice].x + (int64_t)fill[v]++ * kSlice + (v % kSlice)] = make_uint2(16u * (uint32_t)p, sbaddr);
         }
     }
     // One device allocation for every array.
@@ -1084,13 +1138,12 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32
     if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
         return fail(PEL_EINVAL, "pel_plan_decode: block must be 64, 128, 256, 512 or 1024");
     const StateSizes st = g->state();
-    const int64_t static_lds = 12LL * (block / 64) + 64;
-    const int64_t budget = (int64_t)max_lds - static_lds;
+    const int64_t budget = (int64_t)max_lds;
     int place = 0;
     for (int p : kPlacements) {
-        uint32_t o[kNumArrays];
+        uint32_t o[kNumArrays], sc = 0;
         int64_t l = 0, s = 0;
-        place_arrays(st, p, o, &l, &s);
+        place_arrays(st, p, block, o, &sc, &l, &s);
         if (l <= budget) place = p;
     }
     if (placement_override >= 0) {
@@ -1101,10 +1154,18 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32
             return fail(PEL_EINVAL, "pel_plan_decode: requested placement exceeds the LDS budget");
         place = placement_override;
     }
-    uint32_t offs[kNumArrays];
+    uint32_t offs[kNumArrays], scratch = 0;
     int64_t lds = 0, spill = 0;
-    place_arrays(st, place, offs, &lds, &spill);
+    place_arrays(st, place, block, offs, &scratch, &lds, &spill);
     KernelFn fn = pick_kernel(block, place);
+    if ((place & 7) == 7) {
+        // Fast placements address LDS absolutely: the layout must be the one the
+        // graph's entries were baked for, starting at LDS address 0.
+        hipFuncAttributes fa;
+        PEL_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(fn)));
+        if (fa.sharedSizeBytes != 0 || offs[kLS] != g->d.lsb || offs[kSB] != g->d.sbb || offs[kCS] != 0)
+            return fail(PEL_EDEVICE, "pel_plan_decode: LDS layout does not match the graph's baked addresses");
+    }
     if (lds > 65536) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds);
@@ -1148,9 +1209,11 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     bool known = false;
     for (int pl : kPlacements) known |= (pl == plan->placement);
     const StateSizes st = g->state();
-    uint32_t offs[kNumArrays];
+    if (plan->block != 64 && plan->block != 128 && plan->block != 256 && plan->block != 512 && plan->block != 1024)
+        return fail(PEL_EINVAL, "pel_decode: plan block must be 64, 128, 256, 512 or 1024");
+    uint32_t offs[kNumArrays], scratch = 0;
     int64_t lds = 0, spill = 0;
-    place_arrays(st, plan->placement, offs, &lds, &spill);
+    place_arrays(st, plan->placement, plan->block, offs, &scratch, &lds, &spill);
     if (plan->requeue_below < 0) return fail(PEL_EINVAL, "pel_decode: requeue_below must be >= 0");
     const WsLayout w = ws_layout(g, b->B, plan->requeue_below > 0);
     if (!known || lds != plan->lds_bytes || align_up(spill, 256) != plan->spill_stride || plan->grid <= 0 ||
@@ -1199,6 +1262,7 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     a.spill = ws + w.spill;
     a.spill_stride = plan->spill_stride;
     for (int k = 0; k < kNumArrays; ++k) a.off[k] = offs[k];
+    a.scratch_off = scratch;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int key_blocks = (int)((b->B + kOrderBlock / 64 - 1) / (kOrderBlock / 64));
     hipLaunchKernelGGL(syndrome_key_kernel, dim3(key_blocks), dim3(kOrderBlock), 0, s, b->syndromes, b->B, g->C,
