@@ -11,8 +11,10 @@
 //   * Per-shot state is kept in compressed min-sum form instead of per-edge
 //     messages (the reference's muT/nuT [E,B], decoder.py:1527-1529):
 //       variables  LS[v] = {lam_mixed_v, S_v}   (8 B)  S_v = sum of incoming nu
-//       checks     CS[r] = {P1, -P1, P2, -P2}   (16 B) outgoing magnitudes m1, m2 with
-//                                                      the check's sign applied, and negated
+//       checks     CS[c][r], c = 0..3 = {P1, -P1, P2, -P2}  (16 B) outgoing magnitudes
+//                                     m1, m2 with the check's sign applied, and negated;
+//                                     four planes, so lanes reading consecutive checks
+//                                     hit distinct LDS banks
 //       edge codes SB    2 bits per edge: sign of the variable->check message and
 //                         "this edge holds the check's minimum"
 //     Every message is a pure function of this state, bit for bit:
@@ -88,6 +90,7 @@ enum { kLS = 0, kCS = 1, kSB = 2, kL0 = 3, kNumArrays = 4 };
 struct GraphDev {
     int32_t C, V, E;
     uint32_t sbb, lsb;          // canonical LDS byte offsets of SB and LS
+    uint32_t cs_shift;          // log2 of the byte stride between the four CS planes
     // check side, by row position (rows sorted by decreasing degree)
     const int32_t *row_orig;    // [C]  original check index
     const int32_t *row_deg;     // [C]
@@ -99,7 +102,7 @@ struct GraphDev {
     const int32_t *vpos;        // [V]  caller's variable index -> position
     const int32_t *var_deg;     // [V]
     const int4 *vslice;         // [V/64] {first ELL element, max degree, 0, 0}
-    const uint2 *var_ent;       // ELL {16 * row position, ((sbb + 4 * word) << 5) | bit}, ascending edge id;
+    const uint2 *var_ent;       // ELL {4 * row position, ((sbb + 4 * word) << 5) | bit}, ascending edge id;
                                 // padding: the sentinel check at position C
 };
 
@@ -135,6 +138,7 @@ struct KArgs {
     int64_t spill_stride;
     uint32_t off[kNumArrays];  // byte offsets of the state arrays (LDS or spill slot)
     uint32_t scratch_off;      // LDS byte offset of the workgroup's reduction scratch
+    uint32_t rtab, vtab;       // LDS byte addresses of the row and variable slice descriptor tables
 };
 
 template <int PLACE>
@@ -159,6 +163,7 @@ struct ShotMem {
     uint8_t *ls, *cs, *sb;  // array bases (LDS or workspace slot)
     const float *l0g;   // the caller's prior array (caller's order) when kL0 is clear
     uint32_t lsb, sbb, l0b;  // canonical LDS addresses of LS, SB and L0
+    uint32_t cs_shift;       // CS plane stride = 1 << cs_shift bytes
 
     static constexpr bool kFast = IsFast<PLACE>::value;
 
@@ -188,10 +193,20 @@ struct ShotMem {
         return *reinterpret_cast<const float2 *>(ls + 8u * (uint32_t)v);
     }
     __device__ __forceinline__ void cs_set(int r, float4 x) const {
-        if (kFast) *lds_at<vec4f>(16u * (uint32_t)r) = vec4f{x.x, x.y, x.z, x.w};
-        else *reinterpret_cast<float4 *>(cs + 16u * (uint32_t)r) = x;
+        const uint32_t o = 4u * (uint32_t)r, st = 1u << cs_shift;
+        if (kFast) {
+            *lds_at<float>(o) = x.x;
+            *lds_at<float>(o + st) = x.y;
+            *lds_at<float>(o + 2u * st) = x.z;
+            *lds_at<float>(o + 3u * st) = x.w;
+        } else {
+            *reinterpret_cast<float *>(cs + o) = x.x;
+            *reinterpret_cast<float *>(cs + o + st) = x.y;
+            *reinterpret_cast<float *>(cs + o + 2u * st) = x.z;
+            *reinterpret_cast<float *>(cs + o + 3u * st) = x.w;
+        }
     }
-    __device__ __forceinline__ float cs_msg(uint32_t e) const {  // e = 16 * row position + 4 * code
+    __device__ __forceinline__ float cs_msg(uint32_t e) const {  // e = 4 * row position + (code << cs_shift)
         if (kFast) return *lds_at<float>(e);
         return *reinterpret_cast<const float *>(cs + e);
     }
@@ -294,6 +309,16 @@ __device__ __forceinline__ int block_or(int pred, int *s_flag) {
     return uniform_i32(r);
 }
 
+// Slice descriptors, read from the workgroup's LDS copy (one broadcast read).
+__device__ __forceinline__ int4 row_slice(const KArgs &a, int pos) {
+    const vec4f r = *lds_at<vec4f>(a.rtab + 16u * (uint32_t)(pos >> 6));
+    return uniform_i4(make_int4(__float_as_int(r.x), __float_as_int(r.y), __float_as_int(r.z), __float_as_int(r.w)));
+}
+__device__ __forceinline__ int2 var_slice(const KArgs &a, int v) {
+    const vec2f r = *lds_at<vec2f>(a.vtab + 8u * (uint32_t)(v >> 6));
+    return make_int2(uniform_i32(__float_as_int(r.x)), uniform_i32(__float_as_int(r.y)));
+}
+
 // Check pass for row position pos (kernels.py:1128-1215).  FIRST: messages were
 // just reset (mu_e = lam0_v, kernels.py:981-1000), so nothing is reconstructed
 // and no parity is tested.  Otherwise reconstructs mu_e = lam_mixed + (S - nu_e)
@@ -314,13 +339,13 @@ __device__ __forceinline__ int block_or(int pred, int *s_flag) {
 template <bool FIRST, int PLACE>
 __device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m, int pos, float alpha,
                                          const uint8_t *__restrict__ syn) {
-    const int4 rs = uniform_i4(a.g.rslice[pos >> 6]);  // {ent, word, width, 0}
+    const int4 rs = row_slice(a, pos);  // {ent, word, width, 0}
     const int lane = pos & 63;
     const int deg = a.g.row_deg[pos];
     const uint32_t s_i = syn[a.g.row_orig[pos]] & 1u;
     const uint32_t wbase = (uint32_t)(rs.y + lane);
     const uint32_t lane_off = 4u * (uint32_t)lane;
-    const uint32_t qrow = 16u * (uint32_t)pos;  // this row's messages CS[pos][0..3]
+    const uint32_t qrow = 4u * (uint32_t)pos;  // this row's messages CS[0..3][pos]
     float m1 = INFINITY, m2 = INFINITY;
     uint32_t argk = 0, argu = 0;
     uint64_t hpar = 0;  // lane mask: parity of hard decisions so far
@@ -337,7 +362,7 @@ __device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m
 #pragma unroll
         for (int u = 0; u < kChunk; ++u) e[u] = load_ell(chunk, lane_off, u);
         float x[kChunk], y[kChunk], nuo[kChunk];
-        const uint32_t w4 = FIRST ? 0u : ((wold >> half) << 2);  // slot u's code times 4 at bit 2u
+        const uint32_t wcur = FIRST ? 0u : (wold >> half);  // slot u's code at bit 2u
 #pragma unroll
         for (int u = 0; u < kChunk; ++u) {
             if (FIRST) {
@@ -353,7 +378,7 @@ __device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m
                 const float2 ls = m.ls_ent(e[u]);
                 x[u] = ls.x;
                 y[u] = ls.y;
-                nuo[u] = m.cs_msg(((w4 >> (2u * (uint32_t)u)) & 12u) | qrow);
+                nuo[u] = m.cs_msg((__builtin_amdgcn_ubfe(wcur, 2u * (uint32_t)u, 2u) << m.cs_shift) | qrow);
             }
         }
         uint32_t bits = 0;
@@ -408,7 +433,7 @@ __device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m
 template <int PLACE>
 __device__ __forceinline__ int parity_row(const KArgs &a, const ShotMem<PLACE> &m, int pos,
                                           const uint8_t *__restrict__ syn) {
-    const int4 rs = uniform_i4(a.g.rslice[pos >> 6]);
+    const int4 rs = row_slice(a, pos);
     bool hpar = (syn[a.g.row_orig[pos]] & 1u) != 0u;
     const uint32_t lane_off = 4u * (uint32_t)(pos & 63);
     for (int k0 = 0; k0 < rs.z; k0 += kChunk) {
@@ -426,9 +451,9 @@ __device__ __forceinline__ int parity_row(const KArgs &a, const ShotMem<PLACE> &
 }
 
 // W consecutive incoming messages of one variable, added to s in slot order
-// (ascending edge id).  Entry {16 * row position, ((sbb + 4 word) << 5) | bit}:
+// (ascending edge id).  Entry {4 * row position, ((sbb + 4 word) << 5) | bit}:
 // the 2-bit code is read at the bit offset held in the entry's low 5 bits
-// (v_bfe takes its offset modulo 32) and selects CS[row][code].  Padding slots
+// (v_bfe takes its offset modulo 32) and selects CS[code][row].  Padding slots
 // select the sentinel check's +0, which leaves s unchanged (s is never -0).
 template <int W, int PLACE>
 __device__ __forceinline__ void var_messages(const ShotMem<PLACE> &m, const uint2 *__restrict__ chunk,
@@ -440,7 +465,7 @@ __device__ __forceinline__ void var_messages(const ShotMem<PLACE> &m, const uint
 #pragma unroll
     for (int u = 0; u < W; ++u) {
         const uint32_t w = m.sb_ent(e[u].y);
-        nu[u] = m.cs_msg(e[u].x + 4u * __builtin_amdgcn_ubfe(w, e[u].y, 2u));
+        nu[u] = m.cs_msg(e[u].x + (__builtin_amdgcn_ubfe(w, e[u].y, 2u) << m.cs_shift));
     }
 #pragma unroll
     for (int u = 0; u < W; ++u) s += nu[u];
@@ -450,7 +475,7 @@ __device__ __forceinline__ void var_messages(const ShotMem<PLACE> &m, const uint
 // messages are added one at a time in ascending edge id.
 template <int PLACE>
 __device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> &m, int v, const GammaSrc &gs) {
-    const int4 vs = uniform_i4(a.g.vslice[v >> 6]);  // {ent, width, 0, 0}
+    const int2 vs = var_slice(a, v);  // {ent, width}
     const float g = gs(v);
     const float l0 = (PLACE & (1 << kL0)) ? m.l0_pos(v) : m.l0g[a.g.vperm[v]];
     const float2 ls = m.ls_pos(v);
@@ -692,6 +717,20 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
     m.lsb = a.g.lsb;
     m.sbb = a.g.sbb;
     m.l0b = a.off[kL0];
+    m.cs_shift = a.g.cs_shift;
+    // This workgroup's copy of the slice descriptor tables.
+    {
+        const int nrs = (a.g.C + kSlice - 1) / kSlice, nvs = (a.g.V + kSlice - 1) / kSlice;
+        for (int i = threadIdx.x; i < nrs; i += BLOCK) {
+            const int4 r = a.g.rslice[i];
+            *lds_at<vec4f>(a.rtab + 16u * (uint32_t)i) =
+                vec4f{__int_as_float(r.x), __int_as_float(r.y), __int_as_float(r.z), __int_as_float(r.w)};
+        }
+        for (int i = threadIdx.x; i < nvs; i += BLOCK) {
+            const int4 r = a.g.vslice[i];
+            *lds_at<vec2f>(a.vtab + 8u * (uint32_t)i) = vec2f{__int_as_float(r.x), __int_as_float(r.y)};
+        }
+    }
     // Sentinels that padding slots point at; no pass ever writes them.
     if (opaque_tid() == 0) {
         m.ls_set(a.g.V, make_float2(INFINITY, 0.f));
@@ -863,19 +902,22 @@ int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 struct StateSizes {
     int64_t sz[kNumArrays];  // LS, CS, SB, L0 bytes per shot
+    int64_t nrs, nvs;        // row and variable slices (their descriptors are copied to LDS)
 };
 
 }  // namespace
 
 struct pel_graph {
     int device;
-    int32_t C, V, E, sb_words, max_row_deg;
+    int32_t C, V, E, sb_words, max_row_deg, cs_shift;
     void *dev_mem;  // one allocation holding every array of d
     GraphDev d;
     StateSizes state() const {
         // each array has one sentinel entry past its last variable or check
-        return {{align_up(8LL * (V + 1), 16), align_up(16LL * (C + 1), 16), align_up(4LL * std::max(sb_words, 1), 16),
-                 align_up(4LL * (V + 1), 16)}};
+        return {{align_up(8LL * (V + 1), 16), 4LL << cs_shift, align_up(4LL * std::max(sb_words, 1), 16),
+                 align_up(4LL * (V + 1), 16)},
+                (C + kSlice - 1) / kSlice,
+                (V + kSlice - 1) / kSlice};
     }
 };
 
@@ -926,8 +968,15 @@ void place_arrays(const StateSizes &st, int place, int block, uint32_t off[kNumA
             }
         }
     }
+    // scratch: s_red doubles, s_flag ints, s_next[2]; then the slice descriptor
+    // tables: rows {ent, word, width, 0}, variables {ent, width}
     *scratch = (uint32_t)*lds;
-    *lds += align_up(12LL * (block / 64) + 8, 16);  // s_red doubles, s_flag ints, s_next[2]
+    *lds += align_up(12LL * (block / 64) + 8, 16) + 16LL * st.nrs + align_up(8LL * st.nvs, 16);
+}
+
+// LDS byte addresses of the slice descriptor tables after the scratch.
+uint32_t row_table_addr(uint32_t scratch, int block) {
+    return scratch + (uint32_t)align_up(12LL * (block / 64) + 8, 16);
 }
 
 }  // namespace
@@ -988,7 +1037,9 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
         nword += (int64_t)((wpad + kEdgesPerWord - 1) / kEdgesPerWord) * kSlice;
     }
     // Canonical LDS layout (must match pel_graph::state and place_arrays).
-    const int64_t cs_bytes = align_up(16LL * (C + 1), 16);
+    int32_t cs_shift = 4;
+    while ((1LL << cs_shift) < 4LL * (C + 1)) ++cs_shift;  // plane stride: a power of two >= 4 (C + 1)
+    const int64_t cs_bytes = 4LL << cs_shift;
     const int64_t sb_bytes = align_up(4LL * std::max<int64_t>(nword, 1), 16);
     const int64_t lsb = cs_bytes + sb_bytes;
     if (nidx > INT32_MAX / 2 || lsb + 8LL * (V + 1) > (1LL << 26))
@@ -1012,7 +1063,7 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
     if (nve > INT32_MAX / 2) return fail(PEL_EINVAL, "pel_graph_create: graph too large for 32-bit indexing");
     // padding -> the sentinel check at position C (all four messages +0), code word 0
     const uint32_t pad_sb = (uint32_t)(cs_bytes << 5);
-    std::vector<uint2> var_ent(std::max<int64_t>(nve, 1), make_uint2(16u * (uint32_t)C, pad_sb));
+    std::vector<uint2> var_ent(std::max<int64_t>(nve, 1), make_uint2(4u * (uint32_t)C, pad_sb));
     std::vector<int32_t> fill(V, 0);
     for (int32_t i = 0; i < C; ++i) {
         const int32_t p = rpos[i], s = p / kSlice, lane = p % kSlice;
@@ -1020,7 +1071,7 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
             const int32_t v = vposv[chk_var[e]], k = e - chk_ptr[i];
             const int64_t word = (int64_t)rs[s].y + (k / kEdgesPerWord) * kSlice + lane;
             const uint32_t sbaddr = (uint32_t)((cs_bytes + 4 * word) << 5) | (uint32_t)(2 * (k % kEdgesPerWord));
-            var_ent[vs[v / kSlice].x + (int64_t)fill[v]++ * kSlice + (v % kSlice)] = make_uint2(16u * (uint32_t)p, sbaddr);
+            var_ent[vs[v / kSlice].x + (int64_t)fill[v]++ * kSlice + (v % kSlice)] = make_uint2(4u * (uint32_t)p, sbaddr);
         }
     }
     // One device allocation for every array.
@@ -1064,6 +1115,8 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
     g->E = E;
     g->sb_words = (int32_t)nword;
     g->max_row_deg = maxdeg;
+    g->cs_shift = cs_shift;
+    g->d.cs_shift = (uint32_t)cs_shift;
     g->d.C = C;
     g->d.V = V;
     g->d.E = E;
@@ -1158,12 +1211,13 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32
     int64_t lds = 0, spill = 0;
     place_arrays(st, place, block, offs, &scratch, &lds, &spill);
     KernelFn fn = pick_kernel(block, place);
-    if ((place & 7) == 7) {
-        // Fast placements address LDS absolutely: the layout must be the one the
-        // graph's entries were baked for, starting at LDS address 0.
+    {
+        // LDS is addressed absolutely (the dynamic region must start at address 0),
+        // and fast placements use the layout the graph's entries were baked for.
         hipFuncAttributes fa;
         PEL_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(fn)));
-        if (fa.sharedSizeBytes != 0 || offs[kLS] != g->d.lsb || offs[kSB] != g->d.sbb || offs[kCS] != 0)
+        if (fa.sharedSizeBytes != 0 ||
+            ((place & 7) == 7 && (offs[kLS] != g->d.lsb || offs[kSB] != g->d.sbb || offs[kCS] != 0)))
             return fail(PEL_EDEVICE, "pel_plan_decode: LDS layout does not match the graph's baked addresses");
     }
     if (lds > 65536) {
@@ -1263,6 +1317,8 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     a.spill_stride = plan->spill_stride;
     for (int k = 0; k < kNumArrays; ++k) a.off[k] = offs[k];
     a.scratch_off = scratch;
+    a.rtab = row_table_addr(scratch, plan->block);
+    a.vtab = a.rtab + (uint32_t)(16LL * st.nrs);
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int key_blocks = (int)((b->B + kOrderBlock / 64 - 1) / (kOrderBlock / 64));
     hipLaunchKernelGGL(syndrome_key_kernel, dim3(key_blocks), dim3(kOrderBlock), 0, s, b->syndromes, b->B, g->C,
