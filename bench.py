@@ -46,6 +46,9 @@ def bytes_per_iteration(C: int, V: int, E: int) -> int:
 
 
 def cpu_baseline(dec, syn_np, lam, gtab, target_s: float = 10.0):
+    """The C oracle (one shot per OpenMP thread, all host cores) on the same
+    shots and gamma table, repeated over the batch until about target_s of CPU
+    work; plus a single-thread figure on a smaller prefix."""
     import numpy as np
 
     import oracle
@@ -55,26 +58,81 @@ def cpu_baseline(dec, syn_np, lam, gtab, target_s: float = 10.0):
         threads = len(os.sched_getaffinity(0))
     except AttributeError:
         pass
-    n = 64
     kw = dict(pre_iter=RELAY["pre_iter"], num_sets=RELAY["num_sets"],
               set_max_iter=RELAY["set_max_iter"], stop_nconv=RELAY["stop_nconv"],
-              gamma0=RELAY["gamma0"], nthreads=threads)
+              gamma0=RELAY["gamma0"])
     g = dec.graph
+    B = syn_np.shape[0]
+    done, reps, ref = 0, 0, None
     t0 = time.perf_counter()
-    oracle.decode(g.chk_ptr, g.edge_var, g.V, syn_np[:n], lam, gtab, **kw)
-    dt = time.perf_counter() - t0
-    n2 = int(min(syn_np.shape[0], max(n, n * target_s / max(dt, 1e-3))))
-    t0 = time.perf_counter()
-    ref = oracle.decode(g.chk_ptr, g.edge_var, g.V, syn_np[:n2], lam, gtab, **kw)
-    dt = time.perf_counter() - t0
+    while True:
+        out = oracle.decode(g.chk_ptr, g.edge_var, g.V, syn_np, lam, gtab, nthreads=threads, **kw)
+        ref = ref or out
+        done += B
+        reps += 1
+        dt = time.perf_counter() - t0
+        if dt >= target_s or reps >= 50:
+            break
+    n1 = min(B, 256)
+    t1 = time.perf_counter()
+    oracle.decode(g.chk_ptr, g.edge_var, g.V, syn_np[:n1], lam, gtab, nthreads=1, **kw)
+    dt1 = time.perf_counter() - t1
     return {
-        "value": n2 / dt,
+        "value": round(done / dt, 1),
         "unit": "shots/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"first {n2} of the {syn_np.shape[0]} shots, C oracle, OpenMP one shot per thread, "
-                  f"{dt:.1f} s",
-    }, ref, n2
+        "sample": f"all {B} shots x {reps} passes ({dt:.1f} s), C oracle, OpenMP one shot per thread; "
+                  f"single thread: {n1 / dt1:.1f} shots/s on the first {n1} shots",
+        "single_thread_value": round(n1 / dt1, 1),
+    }, ref, B
+
+
+def _pmc_rows(folder: str):
+    import csv
+    import glob
+
+    rows = []
+    for f in sorted(glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True)):
+        with open(f, newline="") as fh:
+            rows += list(csv.DictReader(fh))
+    return rows
+
+
+def traffic_record(fetch_dir: str, write_dir: str, cfetch_dir: str, cwrite_dir: str) -> int:
+    """HBM bytes per decode launch from FETCH_SIZE / WRITE_SIZE (KiB per
+    dispatch), each scaled by what the same counter reads for a device copy of
+    known size (the copy reads and writes copy_probe_bytes)."""
+    def kib(rows, kernel_sub, counter):
+        vals = [float(r["Counter_Value"]) for r in rows
+                if kernel_sub in r["Kernel_Name"] and r["Counter_Name"] == counter]
+        return vals
+
+    dec_f = kib(_pmc_rows(fetch_dir), "relay_decode_kernel", "FETCH_SIZE")
+    dec_w = kib(_pmc_rows(write_dir), "relay_decode_kernel", "WRITE_SIZE")
+    cp_f = kib(_pmc_rows(cfetch_dir), "copyBuffer", "FETCH_SIZE")
+    cp_w = kib(_pmc_rows(cwrite_dir), "copyBuffer", "WRITE_SIZE")
+    copy_bytes = 1024 << 20
+    f_scale = copy_bytes / (sum(cp_f) / len(cp_f) * 1024)
+    w_scale = copy_bytes / (sum(cp_w) / len(cp_w) * 1024)
+    fetch = sum(dec_f) / len(dec_f) * 1024 * f_scale
+    write = sum(dec_w) / len(dec_w) * 1024 * w_scale
+    rec = {
+        "kernel": "relay_decode_kernel",
+        "workload": "banded 936x8785, 4096 shots, one launch",
+        "fetch_kib_raw": sum(dec_f) / len(dec_f),
+        "write_kib_raw": sum(dec_w) / len(dec_w),
+        "copy_probe_bytes": copy_bytes,
+        "copy_fetch_kib_raw": sum(cp_f) / len(cp_f),
+        "copy_write_kib_raw": sum(cp_w) / len(cp_w),
+        "fetch_scale": round(f_scale, 4),
+        "write_scale": round(w_scale, 4),
+        "hbm_bytes_per_launch": int(fetch + write),
+    }
+    out = ROOT / "profiles" / "relay_decode_banded936x8785_traffic.json"
+    out.write_text(json.dumps(rec, indent=1) + "\n")
+    print(json.dumps(rec))
+    return 0
 
 
 def main() -> int:
@@ -88,7 +146,13 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (3 times), the known-size "
                          "reference that scales the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--traffic-from", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
+                    help="only turn four rocprofv3 --pmc csv output folders (FETCH_SIZE and WRITE_SIZE "
+                         "of a 1-step run of this benchmark, then of --copy-probe) into the HBM-traffic "
+                         "record read by the roofline object")
     args = ap.parse_args()
+    if args.traffic_from:
+        return traffic_record(*args.traffic_from)
     if args.copy_probe:
         import torch
 
