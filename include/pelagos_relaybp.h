@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define PEL_ABI_VERSION 2
+#define PEL_ABI_VERSION 3
 
 /* status codes */
 #define PEL_OK 0
@@ -33,6 +33,13 @@ extern "C" {
 #define PEL_ENOMEM (-3)
 
 typedef struct pel_graph pel_graph;
+
+/* Learned memory-strength proposal: a per-variable MLP 8 -> 32 -> 32 -> 1
+ * (relu, relu, then gamma = -0.24 + 0.9 sigmoid(z)).  Parameters are one fp32
+ * vector: W1[32][8], b1[32], W2[32][32], b2[32], w3[32], b3. */
+#define PEL_MLP_FEATURES 8
+#define PEL_MLP_HIDDEN 32
+#define PEL_MLP_PARAMS (32 * 8 + 32 + 32 * 32 + 32 + 32 + 1)
 
 /* Relay-BP-S schedule parameters (RelayBPDecoder.__init__, decoder.py:271-307;
  * per-call overrides of decode(), decoder.py:2317-2361). */
@@ -74,6 +81,17 @@ typedef struct {
                                     shot started and ended, and the workgroup that finished it */
     void *ev_begin;              /* hipEvent_t or NULL: recorded on the stream right before the */
     void *ev_end;                /* decode kernel and right after it (kernel-only timing) */
+    /* Learned memory strengths (plans made with PEL_PLAN_LEARNED; no counterpart in the reference,
+     * whose plug points for them are explicit_gammas / init_gamma, decoder.py:480-491, 2322). */
+    const float *mlp;            /* network parameters [PEL_MLP_PARAMS], or NULL: relay legs
+                                    0 .. learned_legs-1 then use per-variable strengths from the
+                                    network instead of gamma_legs rows 0 .. learned_legs-1 */
+    int32_t learned_legs;
+    float *feat_out;             /* out [B][learned_legs][V][8] or NULL: the network's features at
+                                    the start of each of those legs (also without parameters) */
+    uint8_t *hard_out;           /* out [B][learned_legs][V] or NULL: hard decision there; rows of
+                                    legs a shot never reached are left untouched */
+    float *gamma_out;            /* out [B][learned_legs][V] or NULL: the strengths used */
 } pel_batch;
 
 /* Launch plan for one (graph, B) pair: chosen by pel_plan_decode. */
@@ -89,7 +107,11 @@ typedef struct {
                                 re-queued behind all first legs (their posterior is parked in
                                 the workspace); 0 = never.  Lowering it to 0 after planning is
                                 allowed; raising it needs a plan made with it. */
+    int32_t learned;         /* planned with PEL_PLAN_LEARNED */
 } pel_plan;
+
+/* pel_plan_decode flags */
+#define PEL_PLAN_LEARNED 1   /* reserve LDS for learned memory strengths (fast placements only) */
 
 /* ABI version (PEL_ABI_VERSION). */
 int pel_abi_version(void);
@@ -125,7 +147,7 @@ int pel_graph_permute_vars(const pel_graph *g, const float *src, float *dst, int
  * launch-configuration resolution of RelayBPDecoder._resolve_configs /
  * _get_tuned_meta (decoder.py:905-1222). */
 int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override,
-                    int32_t placement_override, pel_plan *out);
+                    int32_t placement_override, int32_t flags, pel_plan *out);
 
 /* Decode a batch with one persistent launch: every shot runs the full
  * Relay-BP-S schedule (pre-iter leg, then relay legs until stop_nconv
@@ -137,6 +159,32 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override,
  * the kernels it launches (kernels.py:446-1350). */
 int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, const pel_plan *plan,
                void *stream);
+
+/* ---- learned memory strengths (no counterpart in the reference) ----------
+ * Training entry points for the MLP above.  x [n][8] features, y [n] targets
+ * in [0,1] (1: the variable's hard decision is correct), w [n] sample
+ * weights.  All arrays are device memory; work is ordered on `stream`. */
+
+/* PEL_MLP_PARAMS. */
+int pel_mlp_param_count(void);
+
+/* Last error of the training entry points on the calling thread. */
+const char *pel_mlp_last_error(void);
+
+/* Workspace bytes pel_mlp_grad needs. */
+int64_t pel_mlp_grad_workspace_bytes(void);
+
+/* grad = d/dparams of mean_i w_i (softplus(z_i) - y_i z_i) and *loss its value
+ * (loss may be NULL).  The sum runs in a fixed order (bit-reproducible). */
+int pel_mlp_grad(const float *params, const float *x, const float *y, const float *w, int64_t n, float *grad,
+                 float *loss, void *workspace, int64_t workspace_bytes, void *stream);
+
+/* One Adam step (step counts from 1) on params with moment buffers m1, m2. */
+int pel_mlp_adam(float *params, float *m1, float *m2, const float *grad, float lr, float beta1, float beta2,
+                 float eps, int32_t step, void *stream);
+
+/* z [n] = the network's logits for features x [n][8] (fp32). */
+int pel_mlp_forward(const float *params, const float *x, int64_t n, float *z, void *stream);
 
 #ifdef __cplusplus
 }

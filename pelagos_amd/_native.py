@@ -13,7 +13,9 @@ from typing import Optional
 
 LIB_PATH = Path(__file__).resolve().parent / "libpelagos_relaybp.so"
 
-PEL_ABI_VERSION = 2
+PEL_ABI_VERSION = 3
+PEL_PLAN_LEARNED = 1
+PEL_MLP_PARAMS = 32 * 8 + 32 + 32 * 32 + 32 + 32 + 1
 PEL_OK = 0
 PEL_EINVAL = -1
 PEL_EDEVICE = -2
@@ -60,6 +62,11 @@ class PelBatch(ctypes.Structure):
         ("shot_clock", _vp),
         ("ev_begin", _vp),
         ("ev_end", _vp),
+        ("mlp", _vp),
+        ("learned_legs", ctypes.c_int32),
+        ("feat_out", _vp),
+        ("hard_out", _vp),
+        ("gamma_out", _vp),
     ]
 
 
@@ -72,6 +79,7 @@ class PelPlan(ctypes.Structure):
         ("spill_stride", ctypes.c_int64),
         ("workspace_bytes", ctypes.c_int64),
         ("requeue_below", ctypes.c_int32),
+        ("learned", ctypes.c_int32),
     ]
 
 
@@ -86,10 +94,17 @@ SIGNATURES = {
                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
                                       ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "pel_graph_permute_vars": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, _vp]),
-    "pel_plan_decode": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+    "pel_plan_decode": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.POINTER(PelPlan)]),
     "pel_decode": (ctypes.c_int, [_vp, ctypes.POINTER(PelParams), ctypes.POINTER(PelBatch),
                                   ctypes.POINTER(PelPlan), _vp]),
+    "pel_mlp_param_count": (ctypes.c_int, []),
+    "pel_mlp_last_error": (ctypes.c_char_p, []),
+    "pel_mlp_grad_workspace_bytes": (ctypes.c_int64, []),
+    "pel_mlp_grad": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int64, _vp]),
+    "pel_mlp_adam": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_int32, _vp]),
+    "pel_mlp_forward": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, _vp, _vp]),
 }
 
 _LIB: Optional[ctypes.CDLL] = None
@@ -120,11 +135,12 @@ def lib() -> ctypes.CDLL:
     return L
 
 
-def check(rc: int, what: str) -> None:
+def check(rc: int, what: str, mlp: bool = False) -> None:
     """Map a status code to the reference's exception types."""
     if rc == PEL_OK:
         return
-    msg = lib().pel_last_error().decode(errors="replace")
+    err = lib().pel_mlp_last_error if mlp else lib().pel_last_error
+    msg = err().decode(errors="replace")
     if rc == PEL_EINVAL:
         raise ValueError(f"{what}: {msg}")
     if rc == PEL_ENOMEM:

@@ -13,7 +13,7 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
-SRC = PKG / "csrc" / "relaybp.hip"
+SRCS = [PKG / "csrc" / "relaybp.hip", PKG / "csrc" / "gamma_mlp.hip"]
 OUT = PKG / "libpelagos_relaybp.so"
 ARCH = os.environ.get("PELAGOS_OFFLOAD_ARCH", "gfx950")
 
@@ -40,7 +40,7 @@ def hipcc() -> str:
 
 def build(verbose: bool = False) -> Path:
     tmp = OUT.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{ROOT / 'include'}", str(SRC), "-o", str(tmp)]
+    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{ROOT / 'include'}", *map(str, SRCS), "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

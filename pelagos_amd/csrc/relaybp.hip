@@ -139,6 +139,15 @@ struct KArgs {
     uint32_t off[kNumArrays];  // byte offsets of the state arrays (LDS or spill slot)
     uint32_t scratch_off;      // LDS byte offset of the workgroup's reduction scratch
     uint32_t rtab, vtab;       // LDS byte addresses of the row and variable slice descriptor tables
+    // learned memory strengths (plans made with PEL_PLAN_LEARNED)
+    const float *mlp;          // network parameters, or null (then features are only recorded)
+    int32_t learned_legs;      // relay legs 0 .. learned_legs-1 get per-variable strengths
+    float *feat_out;           // [B][learned_legs][V][8] features at each such leg's start, or null
+    uint8_t *hard_out;         // [B][learned_legs][V] hard decision there, or null
+    float *gamma_out;          // [B][learned_legs][V] strengths used, or null
+    uint32_t flips_lds, unsat_lds, gamma_lds;  // LDS addresses: flip counts u8 [V+1], violated checks
+                                               // u8 [C+1], strengths bf16 [V]
+    uint16_t *rq_gamma;        // [B][V] relay-leg-0 strengths of re-queued shots (bf16, by position)
 };
 
 template <int PLACE>
@@ -230,7 +239,9 @@ struct GammaSrc {
     const float *row;
     const int32_t *perm;  // non-null: row is in the caller's order
     float konst;
+    uint32_t lds = 0;     // non-zero: bf16 strengths by position at this LDS address
     __device__ __forceinline__ float operator()(int v) const {
+        if (lds) return __uint_as_float((uint32_t)*lds_at<uint16_t>(lds + 2u * (uint32_t)v) << 16);
         if (!row) return konst;
         return perm ? row[perm[v]] : row[v];
     }
@@ -473,8 +484,9 @@ __device__ __forceinline__ void var_messages(const ShotMem<PLACE> &m, const uint
 
 // Variable pass for variable position v (kernels.py:1299-1350).  The incoming
 // messages are added one at a time in ascending edge id.
-template <int PLACE>
-__device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> &m, int v, const GammaSrc &gs) {
+template <int PLACE, bool LEARN = false>
+__device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> &m, int v, const GammaSrc &gs,
+                                           bool flips_on = false) {
     const int2 vs = var_slice(a, v);  // {ent, width}
     const float g = gs(v);
     const float l0 = (PLACE & (1 << kL0)) ? m.l0_pos(v) : m.l0g[a.g.vperm[v]];
@@ -491,6 +503,168 @@ __device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> 
     else if (rest == 2) var_messages<2, PLACE>(m, base + (int64_t)q * kSlice, lane_off, s);
     else if (rest == 1) var_messages<1, PLACE>(m, base + (int64_t)q * kSlice, lane_off, s);
     m.ls_set(v, make_float2(lm, s));
+    if (LEARN && flips_on && ((post_prev <= 0.f) != (lm + s <= 0.f))) {
+        // hard-decision flips during the leg (a feature of the learned strengths)
+        __attribute__((address_space(3))) uint8_t *c = lds_at<uint8_t>(a.flips_lds + (uint32_t)v);
+        *c = (uint8_t)min((int)*c + 1, 255);
+    }
+}
+
+// ---- learned memory strengths --------------------------------------------------------------
+// At the start of relay leg r < learned_legs every variable gets its own memory
+// strength from the network declared in pelagos_relaybp.h, evaluated on
+// features of the state the previous leg ended in.  No counterpart in the
+// reference (its plug points are explicit_gammas / init_gamma,
+// decoder.py:480-491, 2322).
+//
+// Network evaluation runs on the matrix cores (bf16 operands, fp32
+// accumulation), transposed so that each layer's output tile is already the
+// next layer's B operand: with v_mfma_f32_16x16x16_bf16, lane l holds B
+// column l % 16 (a variable) and rows 4 (l / 16) .. +3, and D rows
+// 4 (l / 16) + i of column l % 16, so H1^T = W1 X^T, H2^T = W2 H1^T need no
+// data movement between layers.
+
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {  // round to nearest even (finite inputs)
+    const uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+constexpr int kMW1 = 0, kMB1 = 32 * 8, kMW2 = kMB1 + 32, kMB2 = kMW2 + 32 * 32, kMW3 = kMB2 + 32, kMB3 = kMW3 + 32;
+static_assert(kMB3 + 1 == PEL_MLP_PARAMS, "parameter layout");
+
+// This lane's slices of the network.
+struct MlpRegs {
+    bf16x4_t w1[2];     // layer-1 A operands (hidden block mb, features 4 (l/16) .. +3)
+    bf16x4_t w2[2][2];  // layer-2 A operands [hidden block][input block]
+    float b1[2][4], b2[2][4], w3[2][4], b3;
+};
+
+__device__ void load_mlp(const float *__restrict__ p, MlpRegs &r) {
+    const int l = threadIdx.x & 63, row = l & 15, kq = 4 * (l >> 4);
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = kq + j;
+            r.w1[mb][j] = (short)(k < 8 ? f32_to_bf16(p[kMW1 + (16 * mb + row) * 8 + k]) : 0);
+#pragma unroll
+            for (int sb = 0; sb < 2; ++sb) r.w2[mb][sb][j] = (short)f32_to_bf16(p[kMW2 + (16 * mb + row) * 32 + 16 * sb + k]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int h = 16 * mb + kq + i;
+            r.b1[mb][i] = p[kMB1 + h];
+            r.b2[mb][i] = p[kMB2 + h];
+            r.w3[mb][i] = p[kMW3 + h];
+        }
+    }
+    r.b3 = p[kMB3];
+}
+
+// Logit z of the 16 variables whose features this wave holds (lane l: variable
+// l % 16, features 4 (l/16) .. +3; lanes 32-63 hold the zero padding of K).
+// Every lane returns z of its variable.  All 64 lanes must be active.
+__device__ __forceinline__ float mlp_logit(const MlpRegs &r, bf16x4_t x) {
+    const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
+    bf16x4_t h1[2];
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+        const f32x4_t d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w1[mb], x, zero, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h1[mb][i] = (short)f32_to_bf16(fmaxf(d[i] + r.b1[mb][i], 0.f));
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+        f32x4_t d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w2[mb][0], h1[0], zero, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w2[mb][1], h1[1], d, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) part += r.w3[mb][i] * fmaxf(d[i] + r.b2[mb][i], 0.f);
+    }
+    part += __shfl_xor(part, 16, 64);
+    part += __shfl_xor(part, 32, 64);
+    return part + r.b3;
+}
+
+__device__ __forceinline__ float clamp4(float x) { return fminf(fmaxf(x, -4.f), 4.f); }
+
+// Features of every variable at the start of relay leg r (the state the
+// previous leg ended in), scaled to O(1):
+//   0 prior LLR / 8            1 posterior / 16 (clamped to +-4)
+//   2 hard-decision flips during the previous leg / 8
+//   3 mean incoming check message / 8      4 smallest incoming message / 8
+//   5 column weight / 8        6 fraction of the variable's checks violated
+//   7 r / 4
+// Each wave takes 16 variables at a time: lanes 0-15 compute features 0-3,
+// lanes 16-31 features 4-7.  Then, with parameters, the network gives every
+// variable its strength for leg r (bf16 in LDS); the flip counts are cleared
+// for the next leg.  Optionally records features, hard decisions and the
+// strengths used in the caller's variable order.
+template <int BLOCK, int PLACE>
+__device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int r, const uint8_t *__restrict__ syn) {
+    const int C = a.g.C, V = a.g.V, L = a.learned_legs;
+    MlpRegs mr;
+    if (a.mlp) load_mlp(a.mlp, mr);
+    for (int i = threadIdx.x; i <= C; i += BLOCK)
+        *lds_at<uint8_t>(a.unsat_lds + (uint32_t)i) = (uint8_t)(i < C ? parity_row<PLACE>(a, m, i, syn) : 0);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, q = lane >> 4, wave = threadIdx.x >> 6;
+    for (int g0 = 16 * wave; g0 < V; g0 += 16 * (BLOCK / 64)) {
+        const int v = g0 + (lane & 15);
+        const bool valid = v < V;
+        const int vc = valid ? v : V - 1;
+        float f[4] = {0.f, 0.f, 0.f, 0.f};
+        bool hard = false;
+        if (q == 0) {
+            const float2 ls = m.ls_pos(vc);
+            const float post = ls.x + ls.y;
+            const int deg = a.g.var_deg[vc];
+            const float l0 = (PLACE & (1 << kL0)) ? m.l0_pos(vc) : m.l0g[a.g.vperm[vc]];
+            hard = post <= 0.f;
+            f[0] = l0 * 0.125f;
+            f[1] = clamp4(post * 0.0625f);
+            f[2] = (float)*lds_at<uint8_t>(a.flips_lds + (uint32_t)vc) * 0.125f;
+            f[3] = deg > 0 ? clamp4(ls.y / (float)deg * 0.125f) : 0.f;
+        } else if (q == 1) {
+            const int2 vs = var_slice(a, vc);
+            const int deg = a.g.var_deg[vc];
+            float mn = INFINITY;
+            int uns = 0;
+            for (int k = 0; k < deg; ++k) {
+                const uint2 e = a.g.var_ent[vs.x + k * kSlice + (vc & 63)];
+                const uint32_t w = m.sb_ent(e.y);
+                mn = fminf(mn, m.cs_msg(e.x + (__builtin_amdgcn_ubfe(w, e.y, 2u) << m.cs_shift)));
+                uns += *lds_at<uint8_t>(a.unsat_lds + (e.x >> 2));
+            }
+            f[0] = deg > 0 ? clamp4(mn * 0.125f) : 0.f;
+            f[1] = (float)deg * 0.125f;
+            f[2] = deg > 0 ? (float)uns / (float)deg : 0.f;
+            f[3] = (float)r * 0.25f;
+        }
+        if (valid && q < 2) {
+            const size_t row = ((size_t)b * L + r) * V + a.g.vperm[v];
+            if (a.feat_out) *reinterpret_cast<float4 *>(a.feat_out + row * 8 + 4 * q) = make_float4(f[0], f[1], f[2], f[3]);
+            if (a.hard_out && q == 0) a.hard_out[row] = hard ? 1 : 0;
+        }
+        if (a.mlp) {
+            bf16x4_t x;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) x[i] = (short)f32_to_bf16(f[i]);
+            const float z = mlp_logit(mr, x);
+            const float gam = fminf(fmaxf(-0.24f + 0.9f / (1.f + expf(-z)), -0.2395f), 0.6562f);
+            const uint16_t gb = f32_to_bf16(gam);
+            if (valid && q == 0) {
+                *lds_at<uint16_t>(a.gamma_lds + 2u * (uint32_t)v) = gb;
+                if (a.gamma_out) a.gamma_out[((size_t)b * L + r) * V + a.g.vperm[v]] = bf16_to_f32(gb);
+            }
+        }
+        if (valid && q == 0) *lds_at<uint8_t>(a.flips_lds + (uint32_t)v) = 0;
+    }
+    __syncthreads();
 }
 
 // check_and_select_kernel (kernels.py:657-741) for a shot whose hard decision
@@ -518,10 +692,10 @@ __device__ void select_solution(const KArgs &a, const ShotMem<PLACE> &m, const f
 
 // One leg of up to T iterations.  The shot leaves the leg at its first valid
 // solution (freeze_finished_lanes_kernel with sought = found + 1).
-template <int BLOCK, int PLACE>
+template <int BLOCK, int PLACE, bool LEARN = false>
 __device__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m, int T, const GammaSrc &gs,
                         const float *__restrict__ wj, const uint8_t *__restrict__ syn, uint8_t *err,
-                        ShotState &st, double *s_red, int *s_flag) {
+                        ShotState &st, double *s_red, int *s_flag, bool flips_on = false) {
     const int C = a.g.C, V = a.g.V;
     for (int t = 0; t < T; ++t) {
         const float alpha = a.use_alpha ? a.alpha_t[t] : 1.0f;
@@ -536,7 +710,7 @@ __device__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m, int T, const Ga
             select_solution<BLOCK, PLACE>(a, m, wj, err, st, s_red);
             return true;
         }
-        for (int v = threadIdx.x; v < V; v += BLOCK) var_update<PLACE>(a, m, v, gs);
+        for (int v = threadIdx.x; v < V; v += BLOCK) var_update<PLACE, LEARN>(a, m, v, gs, flips_on);
         __syncthreads();
         st.iters += 1;
         st.any_iter = 1;
@@ -582,7 +756,7 @@ __device__ __forceinline__ ShotIO shot_io(const KArgs &a, int b) {
 // (decoder.py:3470-3598).  The variable state starts as lam_mixed = post
 // (prior or init_posterior), S = 0, so that lam_mixed + S == post exactly.
 // Returns true when relay legs remain to run.
-template <int BLOCK, int PLACE>
+template <int BLOCK, int PLACE, bool LEARN>
 __device__ bool first_leg(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &st, double *s_red,
                           int *s_flag) {
     const int V = a.g.V;
@@ -593,6 +767,7 @@ __device__ bool first_leg(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &s
         const float p0 = a.init_post ? a.init_post[(size_t)b * V + j] : lj;
         m.ls_set(v, make_float2(p0, 0.f));
         if (PLACE & (1 << kL0)) m.l0_set(v, lj);
+        if (LEARN) *lds_at<uint8_t>(a.flips_lds + (uint32_t)v) = 0;
         io.err[j] = 0;
     }
     m.l0g = io.l0;
@@ -604,7 +779,9 @@ __device__ bool first_leg(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &s
         g0.row = a.init_gamma + (size_t)b * V;
         g0.perm = a.g.vperm;
     }
-    if (st.found < a.stop_nconv) run_leg<BLOCK, PLACE>(a, m, a.pre_iter, g0, io.wj, io.syn, io.err, st, s_red, s_flag);
+    if (st.found < a.stop_nconv)
+        run_leg<BLOCK, PLACE, LEARN>(a, m, a.pre_iter, g0, io.wj, io.syn, io.err, st, s_red, s_flag,
+                                     LEARN && a.learned_legs > 0);
     // first_leg_hard_dec (decoder.py:3597-3598): kept in err while no solution exists.
     if (!a.last_leg && st.found == 0) write_hard<BLOCK, PLACE>(a, m, io.err, st.any_iter);
     return !a.skip_relay_legs && a.num_sets > 0 && st.found < a.stop_nconv;
@@ -612,9 +789,9 @@ __device__ bool first_leg(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &s
 
 // Stage 2: relay legs until stop_nconv solutions (decoder.py:3655-3835), then
 // the unconverged correction and the optional last posterior.
-template <int BLOCK, int PLACE>
+template <int BLOCK, int PLACE, bool LEARN>
 __device__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &st, bool run, double *s_red,
-                           int *s_flag) {
+                           int *s_flag, bool leg0_ready = false) {
     const int V = a.g.V;
     const ShotIO io = shot_io(a, b);
     if (run) {
@@ -627,7 +804,14 @@ __device__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &
                 gl.row = a.gamma_legs + (size_t)leg * V;
                 gl.perm = a.gamma_by_position ? nullptr : a.g.vperm;
             }
-            run_leg<BLOCK, PLACE>(a, m, a.set_max_iter, gl, io.wj, io.syn, io.err, st, s_red, s_flag);
+            if (LEARN && leg < a.learned_legs) {
+                // a re-queued shot had leg 0's features and strengths computed before it
+                // was parked (the parked state is the posterior only)
+                if (!(leg == 0 && leg0_ready)) leg_features<BLOCK, PLACE>(a, b, m, leg, io.syn);
+                if (a.mlp) gl.lds = a.gamma_lds;
+            }
+            run_leg<BLOCK, PLACE, LEARN>(a, m, a.set_max_iter, gl, io.wj, io.syn, io.err, st, s_red, s_flag,
+                                         LEARN && leg + 1 < a.learned_legs);
             if (leg == 0 && a.pre_iter == 0 && !a.last_leg && st.found == 0)
                 write_hard<BLOCK, PLACE>(a, m, io.err, st.any_iter);  // decoder.py:3809-3810
         }
@@ -649,13 +833,14 @@ __device__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &
 // scalars to rq_state, and the shot is published in the next rq_slot.  Every
 // thread's stores are made visible device-wide (the resuming workgroup may run
 // on another XCD) before the leader publishes.
-template <int BLOCK, int PLACE>
+template <int BLOCK, int PLACE, bool LEARN>
 __device__ void requeue_shot(const KArgs &a, int b, const ShotMem<PLACE> &m, const ShotState &st) {
     const int V = a.g.V;
     float *post = a.rq_post + (size_t)b * V;
     for (int v = threadIdx.x; v < V; v += BLOCK) {
         const float2 ls = m.ls_pos(v);
         post[v] = ls.x + ls.y;
+        if (LEARN && a.mlp) a.rq_gamma[(size_t)b * V + v] = *lds_at<uint16_t>(a.gamma_lds + 2u * (uint32_t)v);
     }
     __threadfence();
     __syncthreads();
@@ -668,7 +853,7 @@ __device__ void requeue_shot(const KArgs &a, int b, const ShotMem<PLACE> &m, con
 }
 
 // Resume a re-queued shot: posterior back as lam_mixed = post, S = 0.
-template <int BLOCK, int PLACE>
+template <int BLOCK, int PLACE, bool LEARN>
 __device__ void resume_shot(const KArgs &a, int b, ShotMem<PLACE> &m) {
     const int V = a.g.V;
     const ShotIO io = shot_io(a, b);
@@ -676,6 +861,10 @@ __device__ void resume_shot(const KArgs &a, int b, ShotMem<PLACE> &m) {
     for (int v = threadIdx.x; v < V; v += BLOCK) {
         m.ls_set(v, make_float2(post[v], 0.f));
         if (PLACE & (1 << kL0)) m.l0_set(v, io.l0[a.g.vperm[v]]);
+        if (LEARN) {
+            *lds_at<uint8_t>(a.flips_lds + (uint32_t)v) = 0;  // leg 0's flips start from zero
+            if (a.mlp) *lds_at<uint16_t>(a.gamma_lds + 2u * (uint32_t)v) = a.rq_gamma[(size_t)b * V + v];
+        }
     }
     m.l0g = io.l0;
     __syncthreads();
@@ -700,7 +889,7 @@ __device__ int wait_requeued(const KArgs &a, int t2) {
     }
 }
 
-template <int BLOCK, int PLACE>
+template <int BLOCK, int PLACE, bool LEARN>
 __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
     // All LDS is dynamic (state arrays, then the reduction scratch), so the
     // baked state addresses are absolute LDS addresses.
@@ -772,10 +961,11 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
         if (b >= a.B) break;
         bool more = true;
         if (stage == 1) {
-            more = first_leg<BLOCK, PLACE>(a, b, m, st, s_red, s_flag);
+            more = first_leg<BLOCK, PLACE, LEARN>(a, b, m, st, s_red, s_flag);
             if (a.requeue_below > 0) {
                 if (more && st.found > 0 && st.iters < a.requeue_below) {
-                    requeue_shot<BLOCK, PLACE>(a, b, m, st);
+                    if (LEARN && a.learned_legs > 0) leg_features<BLOCK, PLACE>(a, b, m, 0, a.syn + (size_t)b * a.g.C);
+                    requeue_shot<BLOCK, PLACE, LEARN>(a, b, m, st);
                     prev = -1;
                     continue;
                 }
@@ -783,11 +973,11 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
                     __hip_atomic_fetch_add(&a.ctrl[kCtrlLeft], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             }
         } else {
-            resume_shot<BLOCK, PLACE>(a, b, m);
+            resume_shot<BLOCK, PLACE, LEARN>(a, b, m);
             const int4 rs = uniform_i4(a.rq_state[b]);
             st = ShotState{__int_as_float(rs.x), rs.y, rs.z, rs.w};
         }
-        relay_legs<BLOCK, PLACE>(a, b, m, st, more, s_red, s_flag);
+        relay_legs<BLOCK, PLACE, LEARN>(a, b, m, st, more, s_red, s_flag, stage == 2);
         prev = b;
     }
 }
@@ -878,23 +1068,25 @@ typedef void (*KernelFn)(KArgs);
 constexpr int kPlacements[] = {0, 1, 3, 7, 15};
 
 template <int BLOCK>
-KernelFn pick_place(int place) {
+KernelFn pick_place(int place, bool learn) {
+    if (learn) return place == 15 ? relay_decode_kernel<BLOCK, 15, true> : relay_decode_kernel<BLOCK, 7, true>;
     switch (place) {
-        case 15: return relay_decode_kernel<BLOCK, 15>;
-        case 7: return relay_decode_kernel<BLOCK, 7>;
-        case 3: return relay_decode_kernel<BLOCK, 3>;
-        case 1: return relay_decode_kernel<BLOCK, 1>;
-        default: return relay_decode_kernel<BLOCK, 0>;
+        case 15: return relay_decode_kernel<BLOCK, 15, false>;
+        case 7: return relay_decode_kernel<BLOCK, 7, false>;
+        case 3: return relay_decode_kernel<BLOCK, 3, false>;
+        case 1: return relay_decode_kernel<BLOCK, 1, false>;
+        default: return relay_decode_kernel<BLOCK, 0, false>;
     }
 }
 
-KernelFn pick_kernel(int block, int place) {
+// learn: the kernel with learned memory strengths (fast placements only).
+KernelFn pick_kernel(int block, int place, bool learn) {
     switch (block) {
-        case 64: return pick_place<64>(place);
-        case 128: return pick_place<128>(place);
-        case 256: return pick_place<256>(place);
-        case 512: return pick_place<512>(place);
-        default: return pick_place<1024>(place);
+        case 64: return pick_place<64>(place, learn);
+        case 128: return pick_place<128>(place, learn);
+        case 256: return pick_place<256>(place, learn);
+        case 512: return pick_place<512>(place, learn);
+        default: return pick_place<1024>(place, learn);
     }
 }
 
@@ -903,6 +1095,12 @@ int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 struct StateSizes {
     int64_t sz[kNumArrays];  // LS, CS, SB, L0 bytes per shot
     int64_t nrs, nvs;        // row and variable slices (their descriptors are copied to LDS)
+    int64_t C, V;
+};
+
+// LDS addresses of the learned-strength arrays (after the slice tables).
+struct LearnLds {
+    uint32_t flips, unsat, gamma;
 };
 
 }  // namespace
@@ -917,19 +1115,22 @@ struct pel_graph {
         return {{align_up(8LL * (V + 1), 16), 4LL << cs_shift, align_up(4LL * std::max(sb_words, 1), 16),
                  align_up(4LL * (V + 1), 16)},
                 (C + kSlice - 1) / kSlice,
-                (V + kSlice - 1) / kSlice};
+                (V + kSlice - 1) / kSlice,
+                C,
+                V};
     }
 };
 
 namespace {
 
 // Workspace layout: [0,256) queue counters | order [B] | key [B] | rq_slot [B]
-// | rq_state [B] | rq_post [B,V] (only when shots are re-queued) | spill slots
+// | rq_state [B] | rq_post [B,V] (only when shots are re-queued) | rq_gamma
+// [B,V] bf16 (re-queue with learned strengths) | spill slots
 struct WsLayout {
-    int64_t order, key, rq_slot, rq_state, rq_post, spill, total_fixed;
+    int64_t order, key, rq_slot, rq_state, rq_post, rq_gamma, spill, total_fixed;
 };
 
-WsLayout ws_layout(const pel_graph *g, int64_t B, bool requeue) {
+WsLayout ws_layout(const pel_graph *g, int64_t B, bool requeue, bool learn) {
     const int64_t nb = std::max<int64_t>(B, 1);
     WsLayout w;
     w.order = 256;
@@ -937,15 +1138,16 @@ WsLayout ws_layout(const pel_graph *g, int64_t B, bool requeue) {
     w.rq_slot = align_up(w.key + 4 * nb, 256);
     w.rq_state = align_up(w.rq_slot + 4 * nb, 256);
     w.rq_post = align_up(w.rq_state + 16 * nb, 256);
-    w.spill = align_up(w.rq_post + (requeue ? 4 * nb * g->V : 0), 256);
+    w.rq_gamma = align_up(w.rq_post + (requeue ? 4 * nb * g->V : 0), 256);
+    w.spill = align_up(w.rq_gamma + (requeue && learn ? 2 * nb * g->V : 0), 256);
     w.total_fixed = w.spill;
     return w;
 }
 
 // Byte offsets of the state arrays for a placement and of the workgroup's
 // reduction scratch (always in LDS, after the arrays); LDS and spill totals.
-void place_arrays(const StateSizes &st, int place, int block, uint32_t off[kNumArrays], uint32_t *scratch,
-                  int64_t *lds, int64_t *spill) {
+void place_arrays(const StateSizes &st, int place, int block, bool learn, uint32_t off[kNumArrays],
+                  uint32_t *scratch, int64_t *lds, int64_t *spill, LearnLds *ll = nullptr) {
     *lds = 0;
     *spill = 0;
     if ((place & 7) == 7) {  // canonical LDS layout: CS | SB | LS | L0
@@ -972,6 +1174,16 @@ void place_arrays(const StateSizes &st, int place, int block, uint32_t off[kNumA
     // tables: rows {ent, word, width, 0}, variables {ent, width}
     *scratch = (uint32_t)*lds;
     *lds += align_up(12LL * (block / 64) + 8, 16) + 16LL * st.nrs + align_up(8LL * st.nvs, 16);
+    LearnLds l{0, 0, 0};
+    if (learn) {  // flip counts u8 [V+1], violated checks u8 [C+1], strengths bf16 [V]
+        l.flips = (uint32_t)*lds;
+        *lds += align_up(st.V + 1, 16);
+        l.unsat = (uint32_t)*lds;
+        *lds += align_up(st.C + 1, 16);
+        l.gamma = (uint32_t)*lds;
+        *lds += align_up(2 * st.V, 16);
+    }
+    if (ll) *ll = l;
 }
 
 // LDS byte addresses of the slice descriptor tables after the scratch.
@@ -1173,7 +1385,7 @@ int pel_graph_permute_vars(const pel_graph *g, const float *src, float *dst, int
 }
 
 int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32_t placement_override,
-                    pel_plan *out) {
+                    int32_t flags, pel_plan *out) {
     if (!g || !out) return fail(PEL_EINVAL, "pel_plan_decode: NULL argument");
     if (B < 0) return fail(PEL_EINVAL, "pel_plan_decode: negative batch");
     int dev = 0;
@@ -1190,27 +1402,32 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32
     }
     if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
         return fail(PEL_EINVAL, "pel_plan_decode: block must be 64, 128, 256, 512 or 1024");
+    if (flags & ~PEL_PLAN_LEARNED) return fail(PEL_EINVAL, "pel_plan_decode: unknown flags");
+    const bool learn = (flags & PEL_PLAN_LEARNED) != 0;
     const StateSizes st = g->state();
     const int64_t budget = (int64_t)max_lds;
-    int place = 0;
+    int place = -1;
     for (int p : kPlacements) {
+        if (learn && (p & 7) != 7) continue;  // learned strengths run in the fast placements
         uint32_t o[kNumArrays], sc = 0;
         int64_t l = 0, s = 0;
-        place_arrays(st, p, block, o, &sc, &l, &s);
+        place_arrays(st, p, block, learn, o, &sc, &l, &s);
         if (l <= budget) place = p;
     }
+    if (place < 0)
+        return fail(PEL_EINVAL, "pel_plan_decode: learned strengths need the graph's state in LDS, which does not fit");
     if (placement_override >= 0) {
         bool known = false;
         for (int p : kPlacements) known |= (p == placement_override);
         if (!known) return fail(PEL_EINVAL, "pel_plan_decode: placement must be 0, 1, 3, 7 or 15");
-        if (placement_override > place)
+        if (placement_override > place || (learn && (placement_override & 7) != 7))
             return fail(PEL_EINVAL, "pel_plan_decode: requested placement exceeds the LDS budget");
         place = placement_override;
     }
     uint32_t offs[kNumArrays], scratch = 0;
     int64_t lds = 0, spill = 0;
-    place_arrays(st, place, block, offs, &scratch, &lds, &spill);
-    KernelFn fn = pick_kernel(block, place);
+    place_arrays(st, place, block, learn, offs, &scratch, &lds, &spill);
+    KernelFn fn = pick_kernel(block, place, learn);
     {
         // LDS is addressed absolutely (the dynamic region must start at address 0),
         // and fast placements use the layout the graph's entries were baked for.
@@ -1232,7 +1449,8 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32
     int64_t grid = (int64_t)per_cu * n_cu;
     grid = std::min<int64_t>(grid, std::max<int32_t>(B, 1));
     out->requeue_below = kDefaultRequeueBelow;
-    const WsLayout w = ws_layout(g, B, out->requeue_below > 0);
+    out->learned = learn ? 1 : 0;
+    const WsLayout w = ws_layout(g, B, out->requeue_below > 0, learn);
     out->block = block;
     out->grid = (int32_t)grid;
     out->placement = place;
@@ -1257,6 +1475,12 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     if (b->gamma_per_row && b->gamma_by_position)
         return fail(PEL_EINVAL, "pel_decode: per-shot gamma rows are taken in the caller's order");
     if (p->use_alpha && !p->alpha_t) return fail(PEL_EINVAL, "pel_decode: alpha_t is NULL with use_alpha");
+    if (b->learned_legs < 0) return fail(PEL_EINVAL, "pel_decode: learned_legs must be >= 0");
+    if ((b->learned_legs > 0 || b->mlp || b->feat_out || b->hard_out || b->gamma_out) && !plan->learned)
+        return fail(PEL_EINVAL, "pel_decode: learned strengths need a plan made with PEL_PLAN_LEARNED");
+    if ((b->mlp || b->feat_out || b->hard_out || b->gamma_out) && b->learned_legs == 0)
+        return fail(PEL_EINVAL, "pel_decode: learned strengths or records with learned_legs == 0");
+    if (b->gamma_out && !b->mlp) return fail(PEL_EINVAL, "pel_decode: gamma_out needs network parameters");
     int dev = 0;
     PEL_HIP(hipGetDevice(&dev));
     if (dev != g->device) return fail(PEL_EINVAL, "pel_decode: current device differs from the graph's");
@@ -1267,10 +1491,12 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
         return fail(PEL_EINVAL, "pel_decode: plan block must be 64, 128, 256, 512 or 1024");
     uint32_t offs[kNumArrays], scratch = 0;
     int64_t lds = 0, spill = 0;
-    place_arrays(st, plan->placement, plan->block, offs, &scratch, &lds, &spill);
+    const bool learn = plan->learned != 0;
+    LearnLds ll;
+    place_arrays(st, plan->placement, plan->block, learn, offs, &scratch, &lds, &spill, &ll);
     if (plan->requeue_below < 0) return fail(PEL_EINVAL, "pel_decode: requeue_below must be >= 0");
-    const WsLayout w = ws_layout(g, b->B, plan->requeue_below > 0);
-    if (!known || lds != plan->lds_bytes || align_up(spill, 256) != plan->spill_stride || plan->grid <= 0 ||
+    const WsLayout w = ws_layout(g, b->B, plan->requeue_below > 0, learn);
+    if (!known || (learn && (plan->placement & 7) != 7) || lds != plan->lds_bytes || align_up(spill, 256) != plan->spill_stride || plan->grid <= 0 ||
         plan->workspace_bytes < w.total_fixed + plan->spill_stride * plan->grid)
         return fail(PEL_EINVAL, "pel_decode: plan does not match this graph and batch size");
     KArgs a;
@@ -1313,6 +1539,15 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     a.rq_slot = reinterpret_cast<int32_t *>(ws + w.rq_slot);
     a.rq_state = reinterpret_cast<int4 *>(ws + w.rq_state);
     a.rq_post = plan->requeue_below > 0 ? reinterpret_cast<float *>(ws + w.rq_post) : nullptr;
+    a.rq_gamma = plan->requeue_below > 0 && learn ? reinterpret_cast<uint16_t *>(ws + w.rq_gamma) : nullptr;
+    a.mlp = b->mlp;
+    a.learned_legs = learn ? b->learned_legs : 0;
+    a.feat_out = b->feat_out;
+    a.hard_out = b->hard_out;
+    a.gamma_out = b->gamma_out;
+    a.flips_lds = ll.flips;
+    a.unsat_lds = ll.unsat;
+    a.gamma_lds = ll.gamma;
     a.spill = ws + w.spill;
     a.spill_stride = plan->spill_stride;
     for (int k = 0; k < kNumArrays; ++k) a.off[k] = offs[k];
@@ -1327,7 +1562,7 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     hipLaunchKernelGGL(order_shots_kernel, dim3(1), dim3(kOrderBlock), 0, s, key, b->B, order, a.ctrl);
     PEL_HIP(hipGetLastError());
     if (b->ev_begin) PEL_HIP(hipEventRecord(static_cast<hipEvent_t>(b->ev_begin), s));
-    KernelFn fn = pick_kernel(plan->block, plan->placement);
+    KernelFn fn = pick_kernel(plan->block, plan->placement, learn);
     hipLaunchKernelGGL(fn, dim3(plan->grid), dim3(plan->block), (size_t)plan->lds_bytes, s, a);
     PEL_HIP(hipGetLastError());
     if (b->ev_end) PEL_HIP(hipEventRecord(static_cast<hipEvent_t>(b->ev_end), s));

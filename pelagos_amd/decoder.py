@@ -85,10 +85,11 @@ class _GraphHandle:
             )
         self.h = h
 
-    def plan(self, B: int, block: int = 0, placement: int = -1) -> _native.PelPlan:
+    def plan(self, B: int, block: int = 0, placement: int = -1, learned: bool = False) -> _native.PelPlan:
         p = _native.PelPlan()
+        flags = _native.PEL_PLAN_LEARNED if learned else 0
         with torch.cuda.device(self.device):
-            _native.check(_native.lib().pel_plan_decode(self.h, int(B), int(block), int(placement),
+            _native.check(_native.lib().pel_plan_decode(self.h, int(B), int(block), int(placement), flags,
                                                         ctypes.byref(p)),
                           "pel_plan_decode")
         return p
@@ -116,6 +117,10 @@ class RelayBPDecoder:
     ``data_type`` other than float32, layered scheduling, more than one GPU.
     ``block_size`` and ``state_placement`` (additions) force the workgroup size
     and the LDS placement bits of the decode kernel (see ``pel_plan_decode``).
+    ``learned_gamma`` (addition: a :class:`pelagos_amd.learned.GammaMLP` or its
+    parameter vector) with ``learned_legs = L > 0`` gives relay legs 0..L-1
+    per-variable memory strengths from the network, evaluated inside the decode
+    kernel; the remaining legs use the uniform draws as before.
     """
 
     def __init__(
@@ -150,6 +155,8 @@ class RelayBPDecoder:
         unconverged_correction: str = "first_leg",
         block_size: Optional[int] = None,
         state_placement: Optional[int] = None,
+        learned_gamma=None,
+        learned_legs: int = 0,
     ):
         # Validation in the reference's order and wording (decoder.py:362-512).
         if unconverged_correction not in UNCONVERGED_CORRECTIONS:
@@ -292,6 +299,20 @@ class RelayBPDecoder:
         # Re-queue threshold of the launch plan (None: the library's default; 0:
         # every shot runs to completion in one go).
         self.requeue_below: Optional[int] = None
+        self.learned_legs = int(learned_legs)
+        if self.learned_legs < 0:
+            raise ValueError("learned_legs must be >= 0")
+        self.learned_params: Optional[torch.Tensor] = None
+        if learned_gamma is not None:
+            prm = getattr(learned_gamma, "params", learned_gamma)
+            prm = torch.as_tensor(prm, dtype=torch.float32).reshape(-1)
+            if prm.numel() != _native.PEL_MLP_PARAMS:
+                raise ValueError(f"learned_gamma must hold {_native.PEL_MLP_PARAMS} parameters")
+            self.learned_params = prm.to(self.device).contiguous()
+        # When True, decode() with learned strengths keeps the strengths it used
+        # in self.learned_gammas ([B, learned_legs, V], caller's variable order).
+        self.record_learned_gammas = False
+        self.learned_gammas: Optional[torch.Tensor] = None
 
     # ------------------------------------------------------------------ helpers
     @staticmethod
@@ -387,17 +408,17 @@ class RelayBPDecoder:
             self._alpha_cache[n] = t
         return t
 
-    def plan(self, B: int) -> _native.PelPlan:
-        p = self._plans.get(B)
+    def plan(self, B: int, learned: bool = False) -> _native.PelPlan:
+        p = self._plans.get((B, learned))
         if p is None:
-            p = self._handle.plan(B, self._block_size, self._placement)
+            p = self._handle.plan(B, self._block_size, self._placement, learned)
             if self.requeue_below is not None:
                 if self.requeue_below < 0:
                     raise ValueError("requeue_below must be >= 0")
                 if self.requeue_below > 0 and p.requeue_below == 0:
                     raise ValueError("this plan has no workspace for re-queued shots")
                 p.requeue_below = int(self.requeue_below)
-            self._plans[B] = p
+            self._plans[(B, learned)] = p
         return p
 
     def _shot_clock_for(self, B: int) -> Optional[torch.Tensor]:
@@ -438,6 +459,7 @@ class RelayBPDecoder:
         pre_iter_override: Optional[int] = None,
         return_last_posterior: bool = False,
         prior_llr: Optional[torch.Tensor] = None,
+        _records: Optional[dict] = None,
     ) -> Dict[str, torch.Tensor]:
         """Decode ``syndromes [B, C]`` (uint8); returns the reference's result dict.
 
@@ -490,14 +512,33 @@ class RelayBPDecoder:
             iters = torch.empty(B, dtype=torch.int32, device=dev)
             found = torch.empty(B, dtype=torch.int32, device=dev)
             last_post = torch.empty(B, V, dtype=self.dtype, device=dev) if return_last_posterior else None
+            learn = _records
+            if learn is None and self.learned_params is not None and self.learned_legs > 0:
+                learn = dict(mlp=self.learned_params, legs=self.learned_legs)
+                if self.record_learned_gammas:
+                    self.learned_gammas = torch.zeros(B, self.learned_legs, V, dtype=self.dtype, device=dev)
+                    learn["gamma_out"] = self.learned_gammas
             if B > 0:
                 self._launch(syn, prior, wj, prior_llr is not None, gtab, ig, ip, errors, weights,
-                             iters, found, last_post, pre_iter, skip_relay_legs, B)
+                             iters, found, last_post, pre_iter, skip_relay_legs, B, learn)
             return self._assemble(errors, weights, iters, found, last_post)
 
+    def collect_traces(self, syndromes: torch.Tensor, legs: int) -> Dict[str, torch.Tensor]:
+        """Decode as ``decode`` does (uniform-draw legs, no learned strengths)
+        and record, at the start of relay legs 0..legs-1, the learned network's
+        features of every variable (``features`` [B, legs, V, 8]) and the hard
+        decision (``hard`` [B, legs, V]; 255 where the shot stopped earlier)."""
+        B = int(syndromes.shape[0])
+        feats = torch.zeros(B, legs, self.V, 8, dtype=torch.float32, device=self.device)
+        hard = torch.full((B, legs, self.V), 255, dtype=torch.uint8, device=self.device)
+        out = self.decode(syndromes, _records=dict(mlp=None, legs=int(legs), feat_out=feats, hard_out=hard))
+        out["features"] = feats
+        out["hard"] = hard
+        return out
+
     def _launch(self, syn, prior, wj, prior_batched, gtab, ig, ip, errors, weights, iters, found,
-                last_post, pre_iter, skip_relay_legs, B) -> None:
-        plan = self.plan(B)
+                last_post, pre_iter, skip_relay_legs, B, learn=None) -> None:
+        plan = self.plan(B, learn is not None)
         ws = self._workspace_for(plan.workspace_bytes)
         per_row = gtab.dim() == 3
         if not per_row:
@@ -535,6 +576,12 @@ class RelayBPDecoder:
             workspace=ws.data_ptr(),
             shot_clock=_ptr(self._shot_clock_for(B)),
         )
+        if learn is not None:
+            batch.mlp = _ptr(learn.get("mlp"))
+            batch.learned_legs = int(learn["legs"])
+            batch.feat_out = _ptr(learn.get("feat_out"))
+            batch.hard_out = _ptr(learn.get("hard_out"))
+            batch.gamma_out = _ptr(learn.get("gamma_out"))
         cur = torch.cuda.current_stream(self.device)
         ev = self.kernel_events
         if ev is not None:

@@ -25,7 +25,7 @@ ROOT = Path(__file__).resolve().parent.parent
 
 def _header_symbols():
     text = (ROOT / "include" / "pelagos_relaybp.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(pel_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char \*)\s*(pel_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -50,7 +50,7 @@ def test_library_rejects_bad_graphs_without_a_gpu():
     rc = L.pel_graph_create(bad_ptr.ctypes.data, var.ctypes.data, 1, 6, 2, ctypes.byref(h))
     assert rc == _native.PEL_EINVAL
     assert L.pel_graph_destroy(None) == 0
-    assert L.pel_plan_decode(None, 4, 0, -1, None) == _native.PEL_EINVAL
+    assert L.pel_plan_decode(None, 4, 0, -1, 0, None) == _native.PEL_EINVAL
     assert L.pel_decode(None, None, None, None, None) == _native.PEL_EINVAL
 
 

@@ -1,0 +1,193 @@
+"""Learned memory strengths: the training kernels against the CPU restatement
+(oracle/mlp_oracle.py, itself pinned by finite differences), the matrix-core
+inference inside the decode kernel against the same network in bf16, and the
+learned decode against the CPU oracle fed the strengths it used."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from oracle import mlp_oracle as mo
+from pelagos_amd import codes, learned
+
+P = mo.P
+
+
+def _problem(n=3000, seed=0):
+    rng = np.random.default_rng(seed)
+    p = learned.init_params(seed)
+    p[:] += rng.normal(0, 0.05, P).astype(np.float32)  # non-trivial biases
+    x = rng.normal(0, 1, (n, 8)).astype(np.float32)
+    y = (rng.random(n) < 0.7).astype(np.float32)
+    w = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    return p, x, y, w
+
+
+# --- CPU: pins of the restatement ------------------------------------------------------------
+
+
+def test_parameter_count():
+    assert P == 32 * 8 + 32 + 32 * 32 + 32 + 32 + 1 == learned.N_PARAMS
+
+
+def test_oracle_gradient_matches_finite_differences():
+    p, x, y, w = _problem(200, seed=1)
+    g, _ = mo.grad(p.astype(np.float64), x, y, w)
+    rng = np.random.default_rng(2)
+    for k in rng.choice(P, 60, replace=False).tolist() + [P - 1, 0, 300]:
+        e = np.zeros(P)
+        e[k] = 1e-5
+        num = (mo.loss(p + e, x, y, w) - mo.loss(p - e, x, y, w)) / 2e-5
+        assert abs(num - g[k]) <= 1e-6 + 1e-4 * abs(num), (k, num, g[k])
+
+
+def test_oracle_loss_closed_form_at_zero_network():
+    """All weights zero: z = b3 everywhere, loss = mean w (softplus(b3) - y b3)."""
+    p = np.zeros(P)
+    p[-1] = 0.3
+    _, x, y, w = _problem(50, seed=3)
+    y, w = y.astype(np.float64), w.astype(np.float64)
+    exp = np.mean(w * (np.log1p(np.exp(0.3)) - y * 0.3))
+    assert abs(mo.loss(p, x, y, w) - exp) < 1e-12
+    g, _ = mo.grad(p, x, y, w)
+    assert abs(g[-1] - np.mean(w * (1 / (1 + np.exp(-0.3)) - y))) < 1e-12
+    assert not g[:-1].any()
+
+
+# --- GPU: training kernels ------------------------------------------------------------------
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_matches_oracle_and_is_reproducible():
+    p, x, y, w = _problem(5000, seed=4)
+    net = learned.GammaMLP(p)
+    X, Y, W = (torch.from_numpy(a).cuda() for a in (x, y, w))
+    print("[gpu] mlp grad n=5000", flush=True)
+    g1, l1 = net.grad(X, Y, W)
+    g2, l2 = net.grad(X, Y, W)
+    torch.cuda.synchronize()
+    assert torch.equal(g1, g2) and torch.equal(l1, l2)
+    ge, le = mo.grad(p, x, y, w)
+    np.testing.assert_allclose(g1.cpu().numpy(), ge, rtol=2e-4, atol=2e-6)
+    assert abs(float(l1[0]) - le) <= 1e-5 * abs(le) + 1e-6
+
+
+@pytest.mark.gpu
+def test_gpu_forward_and_adam_match_the_formulas():
+    p, x, y, w = _problem(4000, seed=5)
+    net = learned.GammaMLP(p)
+    z = net.logits(torch.from_numpy(x).cuda()).cpu().numpy()
+    ze, _ = mo.forward(p, x)
+    np.testing.assert_allclose(z, ze, rtol=1e-4, atol=1e-5)
+    g = torch.from_numpy(np.random.default_rng(6).normal(0, 1, P).astype(np.float32)).cuda()
+    net.adam_step(g, lr=0.01)
+    torch.cuda.synchronize()
+    gn = g.cpu().numpy().astype(np.float64)
+    m1, m2 = 0.1 * gn, 0.001 * gn * gn
+    exp = p - 0.01 * (m1 / 0.1) / (np.sqrt(m2 / 0.001) + 1e-8)
+    np.testing.assert_allclose(net.params.cpu().numpy(), exp, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_gpu_training_reduces_the_loss():
+    rng = np.random.default_rng(7)
+    x = rng.normal(0, 1, (20000, 8)).astype(np.float32)
+    y = ((x[:, 0] - 0.5 * x[:, 3] + 0.2 * rng.normal(size=20000)) > 0).astype(np.float32)
+    net = learned.GammaMLP(seed=1)
+    X, Y, W = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda(), torch.ones(20000, device="cuda")
+    print("[gpu] mlp fit", flush=True)
+    losses = net.fit(X, Y, W, epochs=6, batch=2048, lr=5e-3)
+    assert losses[-1] < 0.5 * losses[0], losses
+
+
+# --- GPU: the learned decode ------------------------------------------------------------------
+
+
+def _banded(shots, seed):
+    H, p = codes.banded_circuit_code(n_blocks=4, V=2700, seed=2)
+    err = codes.sample_errors(p, shots, seed=seed, scale=2.0)
+    return H, p, err, codes.syndromes_of(H, err)
+
+
+def _decoder(H, p, **kw):
+    from pelagos_amd import RelayBPDecoder
+
+    return RelayBPDecoder(H, p, pre_iter=30, num_sets=12, set_max_iter=20, gamma0=0.65,
+                          gamma_dist_interval=(-0.24, 0.66), stop_nconv=4, data_type="float32", algo="relay",
+                          seed=3, device=[0], **kw)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("requeue", [0, None])
+def test_gpu_learned_decode_matches_oracle_given_its_strengths(requeue):
+    """The learned decode equals the CPU oracle run with the strengths the
+    network gave (per shot, per learned leg) and the uniform rows after them."""
+    H, p, err, syn = _banded(96, seed=11)
+    L = 3
+    dec = _decoder(H, p, learned_gamma=learned.GammaMLP(seed=2), learned_legs=L)
+    dec.requeue_below = requeue
+    dec.record_learned_gammas = True
+    print(f"[gpu] learned decode requeue={requeue}", flush=True)
+    out = dec.decode(torch.from_numpy(syn).cuda())
+    torch.cuda.synchronize()
+    gl = dec.learned_gammas.cpu().numpy()                      # [B, L, V]
+    B, V = syn.shape[0], H.shape[1]
+    rows = dec.gamma_table(B).cpu().numpy()                    # [R, V]
+    table = np.broadcast_to(rows[:, None, :], (rows.shape[0], B, V)).copy()
+    table[:L] = np.transpose(gl, (1, 0, 2))
+    lam = dec.log_prior_ratios_per_gpu[0].reshape(-1).cpu().numpy()
+    ref = oracle.decode(dec.graph.chk_ptr, dec.graph.edge_var, V, syn, lam, table, pre_iter=30, num_sets=12,
+                        set_max_iter=20, stop_nconv=4, gamma0=0.65, gamma_per_row=True)
+    for k in ("errors", "iterations", "found_count", "weights"):
+        np.testing.assert_array_equal(out[k].cpu().numpy(), ref[k], err_msg=k)
+    # every learned strength lies strictly inside (-0.24, 0.66)
+    used = gl[gl != 0]
+    assert used.size and used.min() > -0.24 and used.max() < 0.66
+
+
+@pytest.mark.gpu
+def test_gpu_inference_matches_the_network_in_bf16():
+    """Strengths of relay leg 0 (features from the pre-iter leg, the same in
+    the plain and the learned decode) against the restatement with bf16
+    operands."""
+    H, p, err, syn = _banded(64, seed=12)
+    net = learned.GammaMLP(seed=4)
+    plain = _decoder(H, p)
+    tr = plain.collect_traces(torch.from_numpy(syn).cuda(), legs=1)
+    dec = _decoder(H, p, learned_gamma=net, learned_legs=1)
+    dec.record_learned_gammas = True
+    print("[gpu] learned inference", flush=True)
+    dec.decode(torch.from_numpy(syn).cuda())
+    torch.cuda.synchronize()
+    hard = tr["hard"].cpu().numpy()[:, 0]
+    reached = hard[:, 0] != 255
+    feats = tr["features"].cpu().numpy()[:, 0][reached].reshape(-1, 8)
+    got = dec.learned_gammas.cpu().numpy()[:, 0][reached].reshape(-1)
+    exp = mo.gamma_bf16(net.params.cpu().numpy(), feats)
+    assert reached.sum() > 10
+    np.testing.assert_allclose(got, exp, atol=4e-3)
+
+
+@pytest.mark.gpu
+def test_gpu_traces_record_the_state_at_each_leg():
+    H, p, err, syn = _banded(48, seed=13)
+    dec = _decoder(H, p)
+    out = dec.collect_traces(torch.from_numpy(syn).cuda(), legs=2)
+    torch.cuda.synchronize()
+    f = out["features"].cpu().numpy()
+    hard = out["hard"].cpu().numpy()
+    V = H.shape[1]
+    deg = np.diff(H.tocsc().indptr)
+    reached = hard[:, 0, 0] != 255
+    assert reached.any()
+    np.testing.assert_allclose(f[reached, 0, :, 5], np.broadcast_to(deg / 8.0, (reached.sum(), V)))
+    np.testing.assert_allclose(f[reached, 0, :, 7], 0.0)
+    # hard decision = sign of the recorded posterior feature
+    post = f[reached, 0, :, 1]
+    np.testing.assert_array_equal(hard[reached, 0] == 1, post <= 0)
+    # a shot that needed no relay leg records nothing
+    done = out["found_count"].cpu().numpy() >= 4
+    assert ((hard[:, 0, 0] == 255) <= done).all()
