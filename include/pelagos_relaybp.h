@@ -84,14 +84,16 @@ typedef struct {
     /* Learned memory strengths (plans made with PEL_PLAN_LEARNED; no counterpart in the reference,
      * whose plug points for them are explicit_gammas / init_gamma, decoder.py:480-491, 2322). */
     const float *mlp;            /* network parameters [PEL_MLP_PARAMS], or NULL: relay legs
-                                    0 .. learned_legs-1 then use per-variable strengths from the
-                                    network instead of gamma_legs rows 0 .. learned_legs-1 */
+                                    0 .. learned_legs-1 then shift their gamma_legs rows per
+                                    variable by the network's output (see learned_lo) */
     int32_t learned_legs;
     float *feat_out;             /* out [B][learned_legs][V][8] or NULL: the network's features at
                                     the start of each of those legs (also without parameters) */
     uint8_t *hard_out;           /* out [B][learned_legs][V] or NULL: hard decision there; rows of
                                     legs a shot never reached are left untouched */
     float *gamma_out;            /* out [B][learned_legs][V] or NULL: the strengths used */
+    float learned_lo, learned_hi; /* strength interval: a learned leg uses clamp(u + (hi - lo)
+                                    (sigmoid(z) - 1/2), lo, hi) with u that leg's gamma_legs row */
 } pel_batch;
 
 /* Launch plan for one (graph, B) pair: chosen by pel_plan_decode. */

@@ -67,6 +67,8 @@ class PelBatch(ctypes.Structure):
         ("feat_out", _vp),
         ("hard_out", _vp),
         ("gamma_out", _vp),
+        ("learned_lo", ctypes.c_float),
+        ("learned_hi", ctypes.c_float),
     ]
 
 

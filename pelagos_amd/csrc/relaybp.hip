@@ -146,7 +146,8 @@ struct KArgs {
     uint8_t *hard_out;         // [B][learned_legs][V] hard decision there, or null
     float *gamma_out;          // [B][learned_legs][V] strengths used, or null
     uint32_t flips_lds, unsat_lds, gamma_lds;  // LDS addresses: flip counts u8 [V+1], violated checks
-                                               // u8 [C+1], strengths bf16 [V]
+                                               // u8 [C+1], learned shifts bf16 [V]
+    float learned_lo, learned_hi;              // strength interval of the relay legs
     uint16_t *rq_gamma;        // [B][V] relay-leg-0 strengths of re-queued shots (bf16, by position)
 };
 
@@ -239,11 +240,17 @@ struct GammaSrc {
     const float *row;
     const int32_t *perm;  // non-null: row is in the caller's order
     float konst;
-    uint32_t lds = 0;     // non-zero: bf16 strengths by position at this LDS address
-    __device__ __forceinline__ float operator()(int v) const {
-        if (lds) return __uint_as_float((uint32_t)*lds_at<uint16_t>(lds + 2u * (uint32_t)v) << 16);
+    uint32_t lds = 0;     // non-zero: learned per-variable shifts (bf16, by position) at this LDS
+    float lo = 0.f, hi = 0.f;  // address, added to the row's strength and clamped to [lo, hi]
+    __device__ __forceinline__ float base(int v) const {
         if (!row) return konst;
         return perm ? row[perm[v]] : row[v];
+    }
+    __device__ __forceinline__ float operator()(int v) const {
+        const float u = base(v);
+        if (!lds) return u;
+        const float d = __uint_as_float((uint32_t)*lds_at<uint16_t>(lds + 2u * (uint32_t)v) << 16);
+        return fminf(fmaxf(u + d, lo), hi);
     }
 };
 
@@ -602,10 +609,14 @@ __device__ __forceinline__ float clamp4(float x) { return fminf(fmaxf(x, -4.f), 
 // Each wave takes 16 variables at a time: lanes 0-15 compute features 0-3,
 // lanes 16-31 features 4-7.  Then, with parameters, the network gives every
 // variable its strength for leg r (bf16 in LDS); the flip counts are cleared
-// for the next leg.  Optionally records features, hard decisions and the
-// strengths used in the caller's variable order.
+// for the next leg.  The network does not replace the leg's random draw u (the
+// disorder that lets successive legs explore differently): it shifts it,
+// gamma = clamp(u + (hi - lo) (sigmoid(z) - 1/2), lo, hi), so a network that
+// outputs z = 0 reproduces the uniform legs exactly.  Optionally records
+// features, hard decisions and the strengths used in the caller's order.
 template <int BLOCK, int PLACE>
-__device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int r, const uint8_t *__restrict__ syn) {
+__device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int r, const uint8_t *__restrict__ syn,
+                             GammaSrc gs) {
     const int C = a.g.C, V = a.g.V, L = a.learned_legs;
     MlpRegs mr;
     if (a.mlp) load_mlp(a.mlp, mr);
@@ -655,11 +666,17 @@ __device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int
 #pragma unroll
             for (int i = 0; i < 4; ++i) x[i] = (short)f32_to_bf16(f[i]);
             const float z = mlp_logit(mr, x);
-            const float gam = fminf(fmaxf(-0.24f + 0.9f / (1.f + expf(-z)), -0.2395f), 0.6562f);
-            const uint16_t gb = f32_to_bf16(gam);
+            // shift of the leg's own draw: (hi - lo) (sigmoid(z) - 1/2); zero at z = 0
+            const float d = (a.learned_hi - a.learned_lo) * (1.f / (1.f + expf(-z)) - 0.5f);
+            const uint16_t db = f32_to_bf16(d);
             if (valid && q == 0) {
-                *lds_at<uint16_t>(a.gamma_lds + 2u * (uint32_t)v) = gb;
-                if (a.gamma_out) a.gamma_out[((size_t)b * L + r) * V + a.g.vperm[v]] = bf16_to_f32(gb);
+                *lds_at<uint16_t>(a.gamma_lds + 2u * (uint32_t)v) = db;
+                if (a.gamma_out) {
+                    gs.lds = a.gamma_lds;
+                    gs.lo = a.learned_lo;
+                    gs.hi = a.learned_hi;
+                    a.gamma_out[((size_t)b * L + r) * V + a.g.vperm[v]] = gs(v);
+                }
             }
         }
         if (valid && q == 0) *lds_at<uint8_t>(a.flips_lds + (uint32_t)v) = 0;
@@ -787,6 +804,19 @@ __device__ bool first_leg(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &s
     return !a.skip_relay_legs && a.num_sets > 0 && st.found < a.stop_nconv;
 }
 
+// The memory strengths of relay leg `leg` for shot b: row leg of the table.
+__device__ __forceinline__ GammaSrc leg_gamma(const KArgs &a, int b, int leg) {
+    GammaSrc gl{nullptr, nullptr, 0.f};
+    if (a.gamma_per_row) {
+        gl.row = a.gamma_legs + ((size_t)leg * a.B + b) * a.g.V;
+        gl.perm = a.g.vperm;
+    } else {
+        gl.row = a.gamma_legs + (size_t)leg * a.g.V;
+        gl.perm = a.gamma_by_position ? nullptr : a.g.vperm;
+    }
+    return gl;
+}
+
 // Stage 2: relay legs until stop_nconv solutions (decoder.py:3655-3835), then
 // the unconverged correction and the optional last posterior.
 template <int BLOCK, int PLACE, bool LEARN>
@@ -796,19 +826,16 @@ __device__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &
     const ShotIO io = shot_io(a, b);
     if (run) {
         for (int leg = 0; leg < a.num_sets && st.found < a.stop_nconv; ++leg) {
-            GammaSrc gl{nullptr, nullptr, 0.f};
-            if (a.gamma_per_row) {
-                gl.row = a.gamma_legs + ((size_t)leg * a.B + b) * V;
-                gl.perm = a.g.vperm;
-            } else {
-                gl.row = a.gamma_legs + (size_t)leg * V;
-                gl.perm = a.gamma_by_position ? nullptr : a.g.vperm;
-            }
+            GammaSrc gl = leg_gamma(a, b, leg);
             if (LEARN && leg < a.learned_legs) {
                 // a re-queued shot had leg 0's features and strengths computed before it
                 // was parked (the parked state is the posterior only)
-                if (!(leg == 0 && leg0_ready)) leg_features<BLOCK, PLACE>(a, b, m, leg, io.syn);
-                if (a.mlp) gl.lds = a.gamma_lds;
+                if (!(leg == 0 && leg0_ready)) leg_features<BLOCK, PLACE>(a, b, m, leg, io.syn, gl);
+                if (a.mlp) {
+                    gl.lds = a.gamma_lds;
+                    gl.lo = a.learned_lo;
+                    gl.hi = a.learned_hi;
+                }
             }
             run_leg<BLOCK, PLACE, LEARN>(a, m, a.set_max_iter, gl, io.wj, io.syn, io.err, st, s_red, s_flag,
                                          LEARN && leg + 1 < a.learned_legs);
@@ -964,7 +991,8 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
             more = first_leg<BLOCK, PLACE, LEARN>(a, b, m, st, s_red, s_flag);
             if (a.requeue_below > 0) {
                 if (more && st.found > 0 && st.iters < a.requeue_below) {
-                    if (LEARN && a.learned_legs > 0) leg_features<BLOCK, PLACE>(a, b, m, 0, a.syn + (size_t)b * a.g.C);
+                    if (LEARN && a.learned_legs > 0)
+                        leg_features<BLOCK, PLACE>(a, b, m, 0, a.syn + (size_t)b * a.g.C, leg_gamma(a, b, 0));
                     requeue_shot<BLOCK, PLACE, LEARN>(a, b, m, st);
                     prev = -1;
                     continue;
@@ -1481,6 +1509,10 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     if ((b->mlp || b->feat_out || b->hard_out || b->gamma_out) && b->learned_legs == 0)
         return fail(PEL_EINVAL, "pel_decode: learned strengths or records with learned_legs == 0");
     if (b->gamma_out && !b->mlp) return fail(PEL_EINVAL, "pel_decode: gamma_out needs network parameters");
+    if (b->mlp && !(b->learned_lo <= b->learned_hi))
+        return fail(PEL_EINVAL, "pel_decode: learned_lo must not exceed learned_hi");
+    if (b->mlp && (p->skip_relay_legs || p->num_sets < b->learned_legs))
+        return fail(PEL_EINVAL, "pel_decode: learned legs need that many relay legs");
     int dev = 0;
     PEL_HIP(hipGetDevice(&dev));
     if (dev != g->device) return fail(PEL_EINVAL, "pel_decode: current device differs from the graph's");
@@ -1548,6 +1580,8 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     a.flips_lds = ll.flips;
     a.unsat_lds = ll.unsat;
     a.gamma_lds = ll.gamma;
+    a.learned_lo = b->learned_lo;
+    a.learned_hi = b->learned_hi;
     a.spill = ws + w.spill;
     a.spill_stride = plan->spill_stride;
     for (int k = 0; k < kNumArrays; ++k) a.off[k] = offs[k];

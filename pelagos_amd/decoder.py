@@ -523,6 +523,40 @@ class RelayBPDecoder:
                              iters, found, last_post, pre_iter, skip_relay_legs, B, learn)
             return self._assemble(errors, weights, iters, found, last_post)
 
+    # ------------------------------------------------------------ serving path
+    def prepare(self, B: int) -> int:
+        """Build everything a decode of B shots needs (launch plan, gamma rows in
+        position order, alpha table) so that decode_async makes no allocation
+        and no host synchronisation.  Returns the workspace bytes per launch."""
+        learned = self.learned_params is not None and self.learned_legs > 0
+        plan = self.plan(B, learned)
+        gtab = self.gamma_table(B)
+        if gtab.dim() == 2:
+            self._gamma_by_position(gtab)
+        self._alpha_table(max(self.pre_iter, self.set_max_iter))
+        torch.cuda.synchronize(self.device)
+        return int(plan.workspace_bytes)
+
+    def decode_async(self, syndromes: torch.Tensor, errors: torch.Tensor, weights: torch.Tensor,
+                     iterations: torch.Tensor, found_count: torch.Tensor, workspace: torch.Tensor,
+                     stream: torch.cuda.Stream) -> None:
+        """Decode device-resident ``syndromes [B, C]`` into caller-owned buffers
+        (errors [B, V] uint8, weights [B] float32, iterations / found_count [B]
+        int32) with a caller-owned workspace, ordered on ``stream``, without
+        allocating or waiting: the launch a serving loop or a captured graph
+        makes.  Call prepare(B) once first (broadcast gamma draws only)."""
+        B = int(syndromes.shape[0])
+        if B == 0:
+            return
+        if self.gamma_draw == "per_row" and self._explicit_gammas is None:
+            raise ValueError("decode_async needs broadcast gamma rows (gamma_draw='broadcast')")
+        learn = None
+        if self.learned_params is not None and self.learned_legs > 0:
+            learn = dict(mlp=self.learned_params, legs=self.learned_legs)
+        self._launch(syndromes, self.log_prior_ratios_per_gpu[0].reshape(self.V), self.wj, False,
+                     self.gamma_table(B), None, None, errors, weights, iterations, found_count, None,
+                     self.pre_iter, False, B, learn, workspace=workspace, stream=stream)
+
     def collect_traces(self, syndromes: torch.Tensor, legs: int) -> Dict[str, torch.Tensor]:
         """Decode as ``decode`` does (uniform-draw legs, no learned strengths)
         and record, at the start of relay legs 0..legs-1, the learned network's
@@ -537,9 +571,14 @@ class RelayBPDecoder:
         return out
 
     def _launch(self, syn, prior, wj, prior_batched, gtab, ig, ip, errors, weights, iters, found,
-                last_post, pre_iter, skip_relay_legs, B, learn=None) -> None:
+                last_post, pre_iter, skip_relay_legs, B, learn=None, workspace=None, stream=None) -> None:
         plan = self.plan(B, learn is not None)
-        ws = self._workspace_for(plan.workspace_bytes)
+        if workspace is not None:
+            if workspace.numel() < plan.workspace_bytes:
+                raise ValueError(f"workspace holds {workspace.numel()} bytes, the plan needs {plan.workspace_bytes}")
+            ws = workspace
+        else:
+            ws = self._workspace_for(plan.workspace_bytes)
         per_row = gtab.dim() == 3
         if not per_row:
             gtab = self._gamma_by_position(gtab)
@@ -582,7 +621,9 @@ class RelayBPDecoder:
             batch.feat_out = _ptr(learn.get("feat_out"))
             batch.hard_out = _ptr(learn.get("hard_out"))
             batch.gamma_out = _ptr(learn.get("gamma_out"))
-        cur = torch.cuda.current_stream(self.device)
+            batch.learned_lo = float(self.gamma_dist_interval[0])
+            batch.learned_hi = float(self.gamma_dist_interval[1])
+        cur = stream if stream is not None else torch.cuda.current_stream(self.device)
         ev = self.kernel_events
         if ev is not None:
             # The library records the pair right around the decode kernel on this

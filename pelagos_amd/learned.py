@@ -3,10 +3,13 @@
 A small network (include/pelagos_relaybp.h: 8 -> 32 -> 32 -> 1) proposes the
 memory strength of every variable at the start of each of the first
 ``learned_legs`` relay legs, from features of the state the previous leg ended
-in.  It is trained on traces of the plain decoder (uniform-draw legs): the
-target is whether a variable's hard decision is already correct, so
-confident-correct variables keep strong memory and suspect ones get
-disordered memory.  Training runs on the GPU through the library's own
+in.  The network does not replace a leg's random draw, whose disorder is what
+lets successive legs explore differently (a fixed strength per variable
+repeats a failed leg forever); it shifts each variable's draw by
+(hi - lo) (sigmoid(z) - 1/2).  It is trained on traces of the plain decoder
+(uniform-draw legs) to predict, from the features, the mean draw of the legs
+that converged, so it moves each variable's strengths towards the values that
+let legs converge.  Training runs on the GPU through the library's own
 forward/backward and Adam kernels (``pel_mlp_grad``, ``pel_mlp_adam``); at
 decode time the network runs inside the decode kernel on the matrix cores.
 
@@ -131,44 +134,58 @@ class GammaMLP:
             return cls(f["params"], device=device)
 
 
-def training_set(dec, errors: np.ndarray, syndromes: np.ndarray, legs: int, max_samples: int, seed: int,
-                 batch: int = 256) -> Dict[str, torch.Tensor]:
-    """Traces of the plain decoder ``dec`` on the given shots: features at the
-    start of relay legs 0..legs-1 with target 1 where the hard decision equals
-    the true error bit.  Keeps every incorrect variable and a uniform sample
-    of the correct ones, up to ``max_samples`` in total; the correct ones are
-    re-weighted to stand for all of them."""
+def _syndrome_ok(H, hard: np.ndarray, syn: np.ndarray) -> np.ndarray:
+    """[n, k] bool: row hard[i, j] satisfies H h = syn[i]."""
+    n, k, V = hard.shape
+    s = (H @ hard.reshape(n * k, V).T.astype(np.int32)) % 2            # [C, n k]
+    return (s.T.reshape(n, k, -1) == syn[:, None, :]).all(2)
+
+
+def training_set(dec, syndromes: np.ndarray, legs: int, max_samples: int, seed: int, batch: int = 256,
+                 hard_weight: float = 4.0) -> Dict[str, torch.Tensor]:
+    """Traces of the plain decoder ``dec`` (uniform-draw legs) on the given
+    shots.  For every relay leg r < legs that a shot started: the features of
+    every variable at the leg's start, and as target the variable's draw u
+    of that leg mapped to [0, 1], (u - lo) / (hi - lo).  Only legs that
+    converged count (the next leg starts from a valid solution, or the shot
+    stopped after it), so the fitted sigmoid is the mean draw of successful
+    legs given the features; legs that started from a state violating some
+    check weigh ``hard_weight``.  A uniform sample of at most ``max_samples``
+    variables is kept."""
     dev = dec.device
-    xs, ys = [], []
-    n_correct_total = 0
+    H = dec.H_csr.tocsr().astype(np.int32)
+    lo, hi = (float(x) for x in dec.gamma_dist_interval)
+    rows = dec.gamma_table(1).cpu().numpy()                              # [R, V] caller order
+    xs, ys, ws = [], [], []
+    n_legs = n_ok = n_hard = 0
     for i in range(0, syndromes.shape[0], batch):
-        syn = torch.from_numpy(syndromes[i:i + batch]).to(dev)
-        out = dec.collect_traces(syn, legs)
-        hard = out["hard"]                                     # [b, legs, V]
-        truth = torch.from_numpy(errors[i:i + batch]).to(dev)[:, None, :].expand_as(hard)
-        ok = hard != 255
-        correct = (hard == truth) & ok
-        feats = out["features"][ok]
-        ys.append(correct[ok].to(torch.float32))
-        xs.append(feats)
-        n_correct_total += int(correct.sum())
-    x = torch.cat(xs)
-    y = torch.cat(ys)
-    wrong = torch.nonzero(y == 0).flatten()
-    right = torch.nonzero(y == 1).flatten()
-    keep_right = max(0, min(right.numel(), max_samples - wrong.numel()))
+        syn = syndromes[i:i + batch]
+        out = dec.collect_traces(torch.from_numpy(syn).to(dev), legs + 1)
+        hard = out["hard"].cpu().numpy()                                 # [b, legs+1, V]
+        reached = hard[:, :, 0] != 255
+        valid = np.zeros_like(reached)
+        valid[reached] = _syndrome_ok(H, np.where(hard == 255, 0, hard), syn)[reached]
+        for r in range(legs):
+            started = reached[:, r]
+            ok = started & (valid[:, r + 1] | ~reached[:, r + 1])
+            from_bad = started & ~valid[:, r]
+            n_legs += int(started.sum())
+            n_ok += int(ok.sum())
+            n_hard += int((ok & from_bad).sum())
+            idx = np.nonzero(ok)[0]
+            if idx.size == 0:
+                continue
+            f = out["features"][torch.from_numpy(idx).to(dev), r]          # [k, V, 8]
+            y = torch.from_numpy(((rows[r] - lo) / (hi - lo)).astype(np.float32)).to(dev)
+            w = torch.from_numpy(np.where(from_bad[idx], hard_weight, 1.0).astype(np.float32)).to(dev)
+            xs.append(f.reshape(-1, N_FEATURES))
+            ys.append(y[None, :].expand(idx.size, -1).reshape(-1))
+            ws.append(w[:, None].expand(-1, dec.V).reshape(-1))
+    x, y, w = torch.cat(xs), torch.cat(ys), torch.cat(ws)
     gen = torch.Generator(device="cpu").manual_seed(seed)
-    pick = right[torch.randperm(right.numel(), generator=gen)[:keep_right].to(dev)]
-    idx = torch.cat([wrong, pick])
-    w = torch.ones(idx.numel(), dtype=torch.float32, device=dev)
-    # correct samples stand for all correct variables; incorrect ones are up-weighted
-    # so that the two classes carry comparable total weight (capped)
-    scale_right = right.numel() / max(keep_right, 1)
-    w[wrong.numel():] = 1.0
-    ratio = min(50.0, (right.numel() / max(wrong.numel(), 1)) / scale_right)
-    w[:wrong.numel()] = ratio
-    return dict(x=x[idx].contiguous(), y=y[idx].contiguous(), w=w, n_wrong=int(wrong.numel()),
-                n_right=int(right.numel()))
+    keep = torch.randperm(x.shape[0], generator=gen)[:max_samples].to(dev)
+    return dict(x=x[keep].contiguous(), y=y[keep].contiguous(), w=w[keep].contiguous(), legs_started=n_legs,
+                legs_converged=n_ok, converged_from_violated=n_hard, candidates=int(x.shape[0]))
 
 
 def default_weights_path() -> Path:

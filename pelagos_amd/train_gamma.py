@@ -1,0 +1,148 @@
+"""Train the learned memory strengths on the 936 x 8785 circuit-shaped code and
+compare the learned form with uniform-draw legs.
+
+    python -m pelagos_amd.train_gamma [--out PATH] [--train-shots 512] [--legs 4]
+        [--samples 2000000] [--epochs 6] [--eval-shots 65536] [--learned-legs 1 2 4]
+
+Training traces come from the plain decoder (uniform-draw legs) on shots whose
+errors are drawn at the workload's priors scaled to a mean of 0.003 (seed 1):
+features at the start of relay legs 0..legs-1 of every variable, target = the
+variable's draw in that leg, kept for legs that converged
+(pelagos_amd.learned.training_set).  The evaluation decodes fresh
+shots (seed 2) drawn at the workload's own priors with both forms and prints,
+per form, decoded shots/s (HIP-event kernel time and wall time per
+4096-shot batch), mean iterations, and the frame-error and
+synthetic-observable error rates with Wilson 95% intervals.  One JSON line per
+result.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import RelayBPDecoder, codes, learned
+
+RELAY = dict(pre_iter=80, num_sets=100, set_max_iter=60, gamma0=0.65, gamma_dist_interval=(-0.24, 0.66),
+             stop_nconv=5)
+
+
+def wilson(k: int, n: int, z: float = 1.959964):
+    if n == 0:
+        return (0.0, 0.0, 1.0)
+    p = k / n
+    den = 1 + z * z / n
+    c = (p + z * z / (2 * n)) / den
+    h = z * math.sqrt(p * (1 - p) / n + z * z / (4 * n * n)) / den
+    return (p, max(0.0, c - h), min(1.0, c + h))
+
+
+def evaluate(dec: RelayBPDecoder, H, err: np.ndarray, syn: np.ndarray, batch: int = 4096) -> dict:
+    """Decode all shots in batches; time each batch's decode kernel (HIP events)
+    and wall clock; count frame errors and synthetic-observable failures."""
+    O = codes.synthetic_observables(H.shape[1], seed=0).astype(np.int64)
+    dev = dec.device
+    kern_ms, wall_s, iters, fe, le = 0.0, 0.0, 0, 0, 0
+    dec.decode(torch.from_numpy(syn[:batch]).to(dev))  # warm-up (plans, gamma tables)
+    torch.cuda.synchronize()
+    for i in range(0, syn.shape[0], batch):
+        s = torch.from_numpy(syn[i:i + batch]).to(dev)
+        ev = []
+        dec.kernel_events = ev
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = dec.decode(s)
+        torch.cuda.synchronize()
+        wall_s += time.perf_counter() - t0
+        dec.kernel_events = None
+        kern_ms += sum(a.elapsed_time(b) for a, b in ev)
+        e = out["errors"].cpu().numpy()
+        conv = out["found_count"].cpu().numpy() >= dec.stop_nconv
+        iters += int(out["iterations"].cpu().numpy().astype(np.int64).sum())
+        truth = err[i:i + batch]
+        fe += int(((~conv) | (e != truth).any(1)).sum())
+        resid = (e ^ truth).astype(np.int64)
+        le += int(((~conv) | ((O @ resid.T) % 2).any(0)).sum())
+    n = syn.shape[0]
+    return dict(shots=n, shots_per_s_wall=round(n / wall_s, 1), shots_per_s_kernel=round(n / (kern_ms * 1e-3), 1),
+                mean_iterations=round(iters / n, 3), frame_error=[round(x, 6) for x in wilson(fe, n)],
+                synthetic_observable_ler=[round(x, 6) for x in wilson(le, n)], frame_errors=fe, logical_errors=le)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=str(learned.default_weights_path()))
+    ap.add_argument("--train-shots", type=int, default=512)
+    ap.add_argument("--legs", type=int, default=4)
+    ap.add_argument("--samples", type=int, default=2_000_000)
+    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--eval-shots", type=int, default=65536)
+    ap.add_argument("--learned-legs", type=int, nargs="+", default=[1, 2, 4])
+    ap.add_argument("--skip-train", action="store_true", help="evaluate the weights already at --out")
+    ap.add_argument("--hard-weight", type=float, default=4.0,
+                    help="weight of legs that started from a state violating some check")
+    ap.add_argument("--const-gamma", type=float, nargs="*", default=[],
+                    help="also evaluate legs with this constant strength for every variable")
+    args = ap.parse_args()
+    log = lambda msg: print(msg, file=sys.stderr, flush=True)  # noqa: E731
+
+    H, p = codes.banded_circuit_code(seed=0)
+    out_path = Path(args.out)
+    if not args.skip_train:
+        scale = 0.003 / float(p.mean())
+        err = codes.sample_errors(p, args.train_shots, seed=1, scale=scale)
+        syn = codes.syndromes_of(H, err)
+        dec = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=1, device=[0], **RELAY)
+        t0 = time.perf_counter()
+        ts = learned.training_set(dec, syn, args.legs, args.samples, seed=1, hard_weight=args.hard_weight)
+        log(f"[train] traces: {ts['x'].shape[0]} samples of {ts['candidates']} from {ts['legs_converged']} "
+            f"converged legs of {ts['legs_started']} ({ts['converged_from_violated']} from a violated state) "
+            f"in {time.perf_counter() - t0:.1f} s")
+        net = learned.GammaMLP(seed=1)
+        t0 = time.perf_counter()
+        losses = net.fit(ts["x"], ts["y"], ts["w"], epochs=args.epochs, lr=args.lr, log=log)
+        torch.cuda.synchronize()
+        train_s = time.perf_counter() - t0
+        out_path.parent.mkdir(parents=True, exist_ok=True)
+        net.save(out_path)
+        print(json.dumps({"train": {"samples": int(ts["x"].shape[0]), "candidates": ts["candidates"],
+                                    "legs_started": ts["legs_started"], "legs_converged": ts["legs_converged"],
+                                    "converged_from_violated": ts["converged_from_violated"], "epochs": args.epochs,
+                                    "losses": [round(x, 5) for x in losses], "seconds": round(train_s, 2),
+                                    "prior_scale": round(scale, 4), "hard_weight": args.hard_weight,
+                                    "weights": str(out_path)}}), flush=True)
+    net = learned.GammaMLP.load(out_path)
+
+    err = codes.sample_errors(p, args.eval_shots, seed=2)
+    syn = codes.syndromes_of(H, err)
+    plain = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=2, device=[0], **RELAY)
+    res = {"plain": evaluate(plain, H, err, syn)}
+    print(json.dumps({"eval": "plain", **res["plain"]}), flush=True)
+    for cg in args.const_gamma:
+        eg = np.full((1, H.shape[1]), cg, np.float32)
+        dec = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=2, device=[0], explicit_gammas=eg,
+                             **RELAY)
+        r = evaluate(dec, H, err, syn)
+        r["speedup_kernel"] = round(r["shots_per_s_kernel"] / res["plain"]["shots_per_s_kernel"], 3)
+        print(json.dumps({"eval": f"constant_gamma={cg}", **r}), flush=True)
+    for L in args.learned_legs:
+        dec = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=2, device=[0], learned_gamma=net,
+                             learned_legs=L, **RELAY)
+        r = evaluate(dec, H, err, syn)
+        r["speedup_kernel"] = round(r["shots_per_s_kernel"] / res["plain"]["shots_per_s_kernel"], 3)
+        lo, hi = res["plain"]["synthetic_observable_ler"][1:]
+        r["ler_within_plain_ci"] = bool(lo <= r["synthetic_observable_ler"][0] <= hi)
+        print(json.dumps({"eval": f"learned_legs={L}", **r}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,69 @@
+"""Several codes in one captured graph launch, and the launch-shape autotuner.
+
+The graph is replayed several times with fresh syndromes and ordinary eager
+decodes in between, and every replay is checked against the eager decode of
+the same syndromes (bit for bit)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from pelagos_amd import codes
+
+pytestmark = pytest.mark.gpu
+
+RELAY = dict(pre_iter=40, num_sets=20, set_max_iter=30, gamma0=0.65, gamma_dist_interval=(-0.24, 0.66),
+             stop_nconv=3, data_type="float32", algo="relay", seed=9, device=[0])
+
+
+def _codes():
+    Hs, ps = codes.banded_circuit_code(n_blocks=4, V=2700, seed=2)
+    Hm, pm = codes.banded_circuit_code(n_blocks=6, V=4000, seed=5)
+    return {"bb72": (codes.bb_72_12_6(), np.full(72, 0.03)), "banded4": (Hs, ps), "banded6": (Hm, pm)}
+
+
+def test_graph_replays_match_eager_decodes():
+    from pelagos_amd import RelayBPDecoder
+    from pelagos_amd.batching import MultiCodeBatch
+
+    cs = _codes()
+    decs = {k: RelayBPDecoder(H, p, **RELAY) for k, (H, p) in cs.items()}
+    shots = {"bb72": 512, "banded4": 128, "banded6": 64}
+    mb = MultiCodeBatch(decs, shots)
+    print("[gpu] capture", flush=True)
+    mb.capture()
+    rng = np.random.default_rng(0)
+    for rep in range(3):
+        syn = {}
+        for k, (H, p) in cs.items():
+            err = codes.sample_errors(p, shots[k], seed=int(rng.integers(1 << 30)), scale=1.5)
+            syn[k] = torch.from_numpy(codes.syndromes_of(H, err)).cuda()
+            mb.syndromes[k].copy_(syn[k])
+        print(f"[gpu] replay {rep}", flush=True)
+        mb.replay()
+        torch.cuda.synchronize()
+        for k, dec in decs.items():
+            ref = dec.decode(syn[k])          # eager work between replays
+            torch.cuda.synchronize()
+            for key in ("errors", "weights", "iterations", "found_count"):
+                assert torch.equal(mb.out[k][key], ref[key]), (rep, k, key)
+
+
+def test_autotune_picks_a_shape_and_caches_it(tmp_path):
+    from pelagos_amd import RelayBPDecoder
+    from pelagos_amd.batching import autotune
+
+    H, p = codes.banded_circuit_code(n_blocks=4, V=2700, seed=2)
+    dec = RelayBPDecoder(H, p, **RELAY)
+    syn = torch.from_numpy(codes.syndromes_of(H, codes.sample_errors(p, 256, seed=1, scale=1.5))).cuda()
+    base = dec.decode(syn)
+    cache = tmp_path / "tune.json"
+    print("[gpu] autotune", flush=True)
+    blk, rq, ms = autotune(dec, syn, blocks=(256, 1024), requeue=(0, 16), reps=1, cache=cache)
+    assert blk in (256, 1024) and rq in (0, 16) and ms > 0 and cache.exists()
+    out = dec.decode(syn)
+    for key in ("errors", "weights", "iterations", "found_count"):
+        assert torch.equal(out[key], base[key]), key
+    assert autotune(dec, syn, blocks=(256, 1024), requeue=(0, 16), reps=1, cache=cache) == (blk, rq, ms)
