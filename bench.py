@@ -1,12 +1,19 @@
-"""Decoded shots/s of plain Relay-BP-S on one MI355X (936 x 8785 banded circuit-shaped code).
+"""Decoded shots/s of Relay-BP-S on one MI355X (936 x 8785 banded circuit-shaped code),
+plain and with learned strengths, plus serving latency.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--shots 4096] [--no-cpu]
+                    [--no-learned] [--no-serving] [--no-batched]
 
 A step is one ``RelayBPDecoder.decode`` of the whole 4096-shot batch, with the
-syndromes already resident in HBM.  Prints ONE JSON line (rank 0) with the
-throughput, the decode kernel's roofline figures (algorithmic edge-message
-bytes over the kernel time measured with HIP events on the decode stream) and
-the CPU oracle timed on a bounded sample of the same shots on this host.
+syndromes already resident in HBM.  Prints ONE JSON line (rank 0) with:
+the throughput of the plain form (uniform-draw legs) and, when trained weights
+are present, of the learned form on the same shots with both error rates;
+``value`` is the faster form whose synthetic-observable error rate lies in
+the plain form's 95% interval.  Also the decode kernel's roofline figures
+(algorithmic edge-message bytes over the kernel time measured with HIP events
+on the decode stream), one graph launch over three codes, the
+continuous-batching server's p50/p99 latency at five Poisson arrival rates,
+and the CPU oracle timed on the same shots on this host.
 """
 
 from __future__ import annotations
@@ -135,6 +142,146 @@ def traffic_record(fetch_dir: str, write_dir: str, cfetch_dir: str, cwrite_dir: 
     return 0
 
 
+def learned_form(H, p, err, syn, syn_np, args, plain_rate, plain_ler) -> dict:
+    """The learned-strength form on the same shots (same steps/warmup)."""
+    import numpy as np
+    import torch
+
+    from pelagos_amd import RelayBPDecoder, codes, learned
+
+    wpath = learned.default_weights_path()
+    if not wpath.exists():
+        return {"skipped": f"no trained weights at {wpath.relative_to(ROOT)}"}
+    with np.load(wpath, allow_pickle=False) as f:
+        legs = int(f["learned_legs"]) if "learned_legs" in f.files else 1
+    net = learned.GammaMLP.load(wpath)
+    dec = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0], learned_gamma=net,
+                         learned_legs=legs, **RELAY)
+    for _ in range(args.warmup):
+        out = dec.decode(syn)
+    ev = []
+    dec.kernel_events = ev
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = dec.decode(syn)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    dec.kernel_events = None
+    kms = sum(a.elapsed_time(b) for a, b in ev) / max(len(ev), 1)
+    errors = out["errors"].cpu().numpy()
+    conv = out["found_count"].cpu().numpy() >= RELAY["stop_nconv"]
+    O = codes.synthetic_observables(H.shape[1], seed=0).astype(np.int64)
+    logical = (~conv) | ((O @ (errors ^ err).T.astype(np.int64)) % 2).any(0)
+    le = wilson(int(logical.sum()), errors.shape[0])
+    rate = args.shots * args.steps / elapsed
+    return {"shots_per_s": round(rate, 1), "kernel_ms": round(kms, 4), "learned_legs": legs,
+            "speedup_vs_plain": round(rate / plain_rate, 3),
+            "mean_iterations": round(float(out["iterations"].float().mean()), 3),
+            "synthetic_observable_ler": [round(x, 5) for x in le],
+            "ler_within_plain_ci": bool(plain_ler[1] <= le[0] <= plain_ler[2]),
+            "weights": str(wpath.relative_to(ROOT))}
+
+
+THREE_CODES = {
+    # name: (shots, description)
+    "bb72": (8192, "[[72,12,6]] BB code, uniform priors"),
+    "banded936": (4096, "936x8785 banded"),
+    "banded1872": (1024, "1872x17570 banded"),
+}
+
+
+def _three_codes(prior_scale: float):
+    """The three codes with priors scaled to mean prior_scale."""
+    import numpy as np
+
+    from pelagos_amd import codes
+
+    out = {}
+    out["bb72"] = (codes.bb_72_12_6(), np.full(72, prior_scale))
+    H, p = codes.banded_circuit_code(**WORKLOAD)
+    out["banded936"] = (H, np.minimum(p * (prior_scale / float(p.mean())), 0.45).astype(np.float32))
+    H2, p2 = codes.banded_circuit_code(n_blocks=26, V=17570, seed=0)
+    out["banded1872"] = (H2, np.minimum(p2 * (prior_scale / float(p2.mean())), 0.45).astype(np.float32))
+    return out
+
+
+def batched_graph(scales=(0.001, 0.003, 0.01)) -> dict:
+    """One captured graph launch decoding the three codes' batches together, at
+    each prior scale: shots/s over all three codes (HIP events around the
+    replay)."""
+    import torch
+
+    from pelagos_amd import RelayBPDecoder, codes
+    from pelagos_amd.batching import MultiCodeBatch
+
+    res = []
+    for sc in scales:
+        cs = _three_codes(sc)
+        decs = {k: RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0], **RELAY)
+                for k, (H, p) in cs.items()}
+        shots = {k: THREE_CODES[k][0] for k in cs}
+        mb = MultiCodeBatch(decs, shots)
+        for k, (H, p) in cs.items():
+            e = codes.sample_errors(p, shots[k], seed=3)
+            mb.syndromes[k].copy_(torch.from_numpy(codes.syndromes_of(H, e)))
+        mb.capture()
+        mb.replay()  # warm-up
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 3
+        e0.record()
+        for _ in range(reps):
+            mb.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        conv = {k: float((mb.out[k]["found_count"] >= RELAY["stop_nconv"]).float().mean()) for k in cs}
+        total = sum(shots.values())
+        res.append({"prior_scale": sc, "ms_per_launch": round(ms, 3), "shots": total,
+                    "shots_per_s": round(total / (ms * 1e-3), 1),
+                    "converged_frac": {k: round(v, 5) for k, v in conv.items()}})
+        del mb, decs
+        torch.cuda.empty_cache()
+    return {"codes": {k: {"shots": v[0], "code": v[1]} for k, v in THREE_CODES.items()}, "launches": res}
+
+
+def serving_runs(seconds: float, rates=(2_000, 10_000, 50_000, 100_000, 200_000)) -> dict:
+    """The continuous-batching server on the three codes (prior scale 0.003,
+    requests of 1-256 shots, code chosen uniformly, Poisson arrivals)."""
+    import torch
+
+    from pelagos_amd import RelayBPDecoder, codes, learned
+    from pelagos_amd.serving import DecodeServer, poisson_trace, run_trace
+
+    cs = _three_codes(0.003)
+    net = None
+    wpath = learned.default_weights_path()
+    decs = {}
+    for k, (H, p) in cs.items():
+        kw = {}
+        if k == "banded936" and wpath.exists():
+            net = net or learned.GammaMLP.load(wpath)
+            kw = dict(learned_gamma=net, learned_legs=1)
+        decs[k] = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0], **RELAY, **kw)
+    pools = {}
+    for k, (H, p) in cs.items():
+        e = codes.sample_errors(p, 4096, seed=4)
+        pools[k] = torch.from_numpy(codes.syndromes_of(H, e)).cuda()
+    server = DecodeServer(decs, max_batch=2048, slots=4, hbm_budget=64 << 30)
+    out = []
+    for i, rate in enumerate(rates):
+        trace = poisson_trace(rate, seconds, list(cs), seed=10 + i)
+        st = run_trace(server, trace, pools)
+        st["rate"] = rate
+        out.append(st)
+        print(f"[bench] serving {rate}/s: p50 {st['p50_ms']} ms p99 {st['p99_ms']} ms "
+              f"{st['shots_per_s']} shots/s", file=sys.stderr, flush=True)
+    return {"codes": list(cs), "seconds_per_rate": seconds, "request_shots": "1-256 uniform",
+            "learned_form_on": "banded936" if net is not None else None, "hbm_budget_gb": 64, "slots": 4,
+            "rates": out}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +289,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--shots", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-strength form")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-latency runs")
+    ap.add_argument("--no-batched", action="store_true", help="skip the three-code graph launch")
+    ap.add_argument("--serving-seconds", type=float, default=1.5,
+                    help="length of each serving run (one per arrival rate)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (3 times), the known-size "
@@ -231,8 +383,9 @@ def main() -> int:
             traffic = None
 
     result = {
-        "metric": "decoded shots/s, plain Relay-BP-S (uniform-gamma legs), 1 GPU",
+        "metric": "decoded shots/s at matched LER (learned vs plain gamma) + p99 latency, 1 GPU",
         "value": round(shots_per_s, 1),
+        "value_form": "plain",
         "unit": "shots/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -272,6 +425,19 @@ def main() -> int:
             "synthetic_observable_ler": [round(x, 5) for x in le],
         },
     }
+    result["plain"] = {"shots_per_s": round(shots_per_s, 1), "kernel_ms": round(kern_avg_ms, 4),
+                       "synthetic_observable_ler": [round(x, 5) for x in le]}
+    if not args.no_learned:
+        result["learned"] = learned_form(H, p, err, syn, syn_np, args, shots_per_s, le)
+        lr = result["learned"]
+        if lr.get("ler_within_plain_ci") and lr.get("shots_per_s", 0) > shots_per_s:
+            result["value"] = lr["shots_per_s"]
+            result["value_form"] = "learned"
+    if not args.no_batched:
+        result["batched"] = batched_graph()
+    if not args.no_serving:
+        result["serving"] = serving_runs(args.serving_seconds)
+        result["p99_latency_ms"] = {str(r["rate"]): r["p99_ms"] for r in result["serving"]["rates"]}
     if not args.no_cpu:
         lam = utils.compute_log_prior_ratios(p)
         gtab = dec.gamma_table(args.shots).cpu().numpy()

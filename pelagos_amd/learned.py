@@ -125,8 +125,9 @@ class GammaMLP:
         return losses
 
     # -- persistence ---------------------------------------------------------------------------
-    def save(self, path) -> None:
-        np.savez(path, params=self.params.cpu().numpy())
+    def save(self, path, learned_legs: Optional[int] = None) -> None:
+        extra = {} if learned_legs is None else {"learned_legs": np.int32(learned_legs)}
+        np.savez(path, params=self.params.cpu().numpy(), **extra)
 
     @classmethod
     def load(cls, path, device="cuda:0") -> "GammaMLP":
@@ -142,7 +143,7 @@ def _syndrome_ok(H, hard: np.ndarray, syn: np.ndarray) -> np.ndarray:
 
 
 def training_set(dec, syndromes: np.ndarray, legs: int, max_samples: int, seed: int, batch: int = 256,
-                 hard_weight: float = 4.0) -> Dict[str, torch.Tensor]:
+                 hard_weight: float = 4.0, easy_weight: float = 1.0) -> Dict[str, torch.Tensor]:
     """Traces of the plain decoder ``dec`` (uniform-draw legs) on the given
     shots.  For every relay leg r < legs that a shot started: the features of
     every variable at the leg's start, and as target the variable's draw u
@@ -150,8 +151,8 @@ def training_set(dec, syndromes: np.ndarray, legs: int, max_samples: int, seed: 
     converged count (the next leg starts from a valid solution, or the shot
     stopped after it), so the fitted sigmoid is the mean draw of successful
     legs given the features; legs that started from a state violating some
-    check weigh ``hard_weight``.  A uniform sample of at most ``max_samples``
-    variables is kept."""
+    check weigh ``hard_weight``, the others ``easy_weight`` (0 drops them).  A
+    uniform sample of at most ``max_samples`` variables is kept."""
     dev = dec.device
     H = dec.H_csr.tocsr().astype(np.int32)
     lo, hi = (float(x) for x in dec.gamma_dist_interval)
@@ -169,6 +170,8 @@ def training_set(dec, syndromes: np.ndarray, legs: int, max_samples: int, seed: 
             started = reached[:, r]
             ok = started & (valid[:, r + 1] | ~reached[:, r + 1])
             from_bad = started & ~valid[:, r]
+            if easy_weight <= 0:
+                ok &= from_bad
             n_legs += int(started.sum())
             n_ok += int(ok.sum())
             n_hard += int((ok & from_bad).sum())
@@ -177,7 +180,7 @@ def training_set(dec, syndromes: np.ndarray, legs: int, max_samples: int, seed: 
                 continue
             f = out["features"][torch.from_numpy(idx).to(dev), r]          # [k, V, 8]
             y = torch.from_numpy(((rows[r] - lo) / (hi - lo)).astype(np.float32)).to(dev)
-            w = torch.from_numpy(np.where(from_bad[idx], hard_weight, 1.0).astype(np.float32)).to(dev)
+            w = torch.from_numpy(np.where(from_bad[idx], hard_weight, easy_weight).astype(np.float32)).to(dev)
             xs.append(f.reshape(-1, N_FEATURES))
             ys.append(y[None, :].expand(idx.size, -1).reshape(-1))
             ws.append(w[:, None].expand(-1, dec.V).reshape(-1))

@@ -20,7 +20,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import sys
 import time
 from pathlib import Path
@@ -29,19 +28,10 @@ import numpy as np
 import torch
 
 from . import RelayBPDecoder, codes, learned
+from .observables import wilson
 
 RELAY = dict(pre_iter=80, num_sets=100, set_max_iter=60, gamma0=0.65, gamma_dist_interval=(-0.24, 0.66),
              stop_nconv=5)
-
-
-def wilson(k: int, n: int, z: float = 1.959964):
-    if n == 0:
-        return (0.0, 0.0, 1.0)
-    p = k / n
-    den = 1 + z * z / n
-    c = (p + z * z / (2 * n)) / den
-    h = z * math.sqrt(p * (1 - p) / n + z * z / (4 * n * n)) / den
-    return (p, max(0.0, c - h), min(1.0, c + h))
 
 
 def evaluate(dec: RelayBPDecoder, H, err: np.ndarray, syn: np.ndarray, batch: int = 4096) -> dict:
@@ -89,6 +79,11 @@ def main() -> int:
     ap.add_argument("--skip-train", action="store_true", help="evaluate the weights already at --out")
     ap.add_argument("--hard-weight", type=float, default=4.0,
                     help="weight of legs that started from a state violating some check")
+    ap.add_argument("--easy-weight", type=float, default=1.0,
+                    help="weight of legs that started from a valid solution (0 drops them)")
+    ap.add_argument("--prior-mean", type=float, default=0.003,
+                    help="mean prior of the training shots' error draws")
+    ap.add_argument("--save-legs", type=int, default=1, help="learned legs recorded with the weights")
     ap.add_argument("--const-gamma", type=float, nargs="*", default=[],
                     help="also evaluate legs with this constant strength for every variable")
     args = ap.parse_args()
@@ -97,12 +92,13 @@ def main() -> int:
     H, p = codes.banded_circuit_code(seed=0)
     out_path = Path(args.out)
     if not args.skip_train:
-        scale = 0.003 / float(p.mean())
+        scale = args.prior_mean / float(p.mean())
         err = codes.sample_errors(p, args.train_shots, seed=1, scale=scale)
         syn = codes.syndromes_of(H, err)
         dec = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=1, device=[0], **RELAY)
         t0 = time.perf_counter()
-        ts = learned.training_set(dec, syn, args.legs, args.samples, seed=1, hard_weight=args.hard_weight)
+        ts = learned.training_set(dec, syn, args.legs, args.samples, seed=1, hard_weight=args.hard_weight,
+                                  easy_weight=args.easy_weight)
         log(f"[train] traces: {ts['x'].shape[0]} samples of {ts['candidates']} from {ts['legs_converged']} "
             f"converged legs of {ts['legs_started']} ({ts['converged_from_violated']} from a violated state) "
             f"in {time.perf_counter() - t0:.1f} s")
@@ -112,12 +108,13 @@ def main() -> int:
         torch.cuda.synchronize()
         train_s = time.perf_counter() - t0
         out_path.parent.mkdir(parents=True, exist_ok=True)
-        net.save(out_path)
+        net.save(out_path, learned_legs=args.save_legs)
         print(json.dumps({"train": {"samples": int(ts["x"].shape[0]), "candidates": ts["candidates"],
                                     "legs_started": ts["legs_started"], "legs_converged": ts["legs_converged"],
                                     "converged_from_violated": ts["converged_from_violated"], "epochs": args.epochs,
                                     "losses": [round(x, 5) for x in losses], "seconds": round(train_s, 2),
                                     "prior_scale": round(scale, 4), "hard_weight": args.hard_weight,
+                                    "easy_weight": args.easy_weight,
                                     "weights": str(out_path)}}), flush=True)
     net = learned.GammaMLP.load(out_path)
 
