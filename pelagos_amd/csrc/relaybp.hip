@@ -31,7 +31,7 @@
 //     workgroup barriers.  No cross-lane reductions are needed.
 //   * Rows and variables are renumbered by decreasing degree and stored as
 //     64-wide ELL slices (one slice per wave), so the lanes of a wave do equal
-//     work and element k of every row of a slice is one coalesced 256-byte
+//     work (row slices are then reordered to balance the SIMDs) and element k of every row of a slice is one coalesced 256-byte
 //     read.  The index entries are the LDS byte addresses of the state they
 //     select, so a gather needs no address arithmetic.  Per-variable inputs
 //     and outputs (priors, gammas, error bits) stay in the caller's order and
@@ -91,7 +91,7 @@ struct GraphDev {
     int32_t C, V, E;
     uint32_t sbb, lsb;          // canonical LDS byte offsets of SB and LS
     uint32_t cs_shift;          // log2 of the byte stride between the four CS planes
-    // check side, by row position (rows sorted by decreasing degree)
+    // check side, by row position (slices of rows sorted by decreasing degree)
     const int32_t *row_orig;    // [C]  original check index
     const int32_t *row_deg;     // [C]
     const int4 *rslice;         // [C/64] {first ELL element, first code word, width (whole chunks), 0}
@@ -503,12 +503,27 @@ __device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> 
     const uint32_t lane_off = 8u * (uint32_t)(v & 63);
     const uint2 *__restrict__ base = a.g.var_ent + vs.x;
     float s = 0.f;
-    int q = 0;
-    for (; q + kVChunk <= vs.y; q += kVChunk) var_messages<kVChunk, PLACE>(m, base + (int64_t)q * kSlice, lane_off, s);
-    const int rest = vs.y - q;
-    if (rest == 3) var_messages<3, PLACE>(m, base + (int64_t)q * kSlice, lane_off, s);
-    else if (rest == 2) var_messages<2, PLACE>(m, base + (int64_t)q * kSlice, lane_off, s);
-    else if (rest == 1) var_messages<1, PLACE>(m, base + (int64_t)q * kSlice, lane_off, s);
+    // The slice's width (uniform): up to 8 messages are gathered in one round
+    // (index loads, then code words, then messages), wider slices in chunks.
+    switch (vs.y) {
+        case 1: var_messages<1, PLACE>(m, base, lane_off, s); break;
+        case 2: var_messages<2, PLACE>(m, base, lane_off, s); break;
+        case 3: var_messages<3, PLACE>(m, base, lane_off, s); break;
+        case 4: var_messages<4, PLACE>(m, base, lane_off, s); break;
+        case 5: var_messages<5, PLACE>(m, base, lane_off, s); break;
+        case 6: var_messages<6, PLACE>(m, base, lane_off, s); break;
+        case 7: var_messages<7, PLACE>(m, base, lane_off, s); break;
+        case 8: var_messages<8, PLACE>(m, base, lane_off, s); break;
+        default: {
+            int q = 0;
+            for (; q + kVChunk <= vs.y; q += kVChunk)
+                var_messages<kVChunk, PLACE>(m, base + (int64_t)q * kSlice, lane_off, s);
+            const int rest = vs.y - q;
+            if (rest == 3) var_messages<3, PLACE>(m, base + (int64_t)q * kSlice, lane_off, s);
+            else if (rest == 2) var_messages<2, PLACE>(m, base + (int64_t)q * kSlice, lane_off, s);
+            else if (rest == 1) var_messages<1, PLACE>(m, base + (int64_t)q * kSlice, lane_off, s);
+        }
+    }
     m.ls_set(v, make_float2(lm, s));
     if (LEARN && flips_on && ((post_prev <= 0.f) != (lm + s <= 0.f))) {
         // hard-decision flips during the leg (a feature of the learned strengths)
@@ -606,14 +621,22 @@ __device__ __forceinline__ float clamp4(float x) { return fminf(fmaxf(x, -4.f), 
 //   3 mean incoming check message / 8      4 smallest incoming message / 8
 //   5 column weight / 8        6 fraction of the variable's checks violated
 //   7 r / 4
-// Each wave takes 16 variables at a time: lanes 0-15 compute features 0-3,
-// lanes 16-31 features 4-7.  Then, with parameters, the network gives every
-// variable its strength for leg r (bf16 in LDS); the flip counts are cleared
-// for the next leg.  The network does not replace the leg's random draw u (the
-// disorder that lets successive legs explore differently): it shifts it,
-// gamma = clamp(u + (hi - lo) (sigmoid(z) - 1/2), lo, hi), so a network that
-// outputs z = 0 reproduces the uniform legs exactly.  Optionally records
-// features, hard decisions and the strengths used in the caller's order.
+// Every lane computes the eight features of one variable (a wave takes a
+// 64-variable slice).  Then, with parameters, the network gives every variable
+// its strength for leg r (bf16 in LDS): the slice is evaluated as four
+// 16-variable MFMA groups, whose B operands (lane l: variable l % 16, features
+// 4 (l / 16) .. +3, zero for l >= 32) are gathered from the owning lanes with
+// ds_bpermute; lane l's own logit comes out of group l / 16.  The flip counts
+// are cleared for the next leg.  The network does not replace the leg's random
+// draw u (the disorder that lets successive legs explore differently): it
+// shifts it, gamma = clamp(u + (hi - lo) (sigmoid(z) - 1/2), lo, hi), so a
+// network that outputs z = 0 reproduces the uniform legs exactly.  Optionally
+// records features, hard decisions and the strengths used in the caller's
+// order.
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
+    return (uint32_t)f32_to_bf16(a) | ((uint32_t)f32_to_bf16(b) << 16);
+}
+
 template <int BLOCK, int PLACE>
 __device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int r, const uint8_t *__restrict__ syn,
                              GammaSrc gs) {
@@ -623,54 +646,86 @@ __device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int
     for (int i = threadIdx.x; i <= C; i += BLOCK)
         *lds_at<uint8_t>(a.unsat_lds + (uint32_t)i) = (uint8_t)(i < C ? parity_row<PLACE>(a, m, i, syn) : 0);
     __syncthreads();
-    const int lane = threadIdx.x & 63, q = lane >> 4, wave = threadIdx.x >> 6;
-    for (int g0 = 16 * wave; g0 < V; g0 += 16 * (BLOCK / 64)) {
-        const int v = g0 + (lane & 15);
+    const int lane = threadIdx.x & 63, q = lane >> 4;
+    // v0 is uniform per wave, so every lane of a wave runs the same rounds (the
+    // MFMAs need all 64 lanes); lanes past V compute a copy of variable V - 1
+    // (in the same, last slice) and store nothing.
+    for (int v0 = threadIdx.x & ~63; v0 < V; v0 += BLOCK) {
+        const int v = v0 + lane;
         const bool valid = v < V;
         const int vc = valid ? v : V - 1;
-        float f[4] = {0.f, 0.f, 0.f, 0.f};
-        bool hard = false;
-        if (q == 0) {
-            const float2 ls = m.ls_pos(vc);
-            const float post = ls.x + ls.y;
-            const int deg = a.g.var_deg[vc];
-            const float l0 = (PLACE & (1 << kL0)) ? m.l0_pos(vc) : m.l0g[a.g.vperm[vc]];
-            hard = post <= 0.f;
-            f[0] = l0 * 0.125f;
-            f[1] = clamp4(post * 0.0625f);
-            f[2] = (float)*lds_at<uint8_t>(a.flips_lds + (uint32_t)vc) * 0.125f;
-            f[3] = deg > 0 ? clamp4(ls.y / (float)deg * 0.125f) : 0.f;
-        } else if (q == 1) {
-            const int2 vs = var_slice(a, vc);
-            const int deg = a.g.var_deg[vc];
-            float mn = INFINITY;
-            int uns = 0;
-            for (int k = 0; k < deg; ++k) {
-                const uint2 e = a.g.var_ent[vs.x + k * kSlice + (vc & 63)];
-                const uint32_t w = m.sb_ent(e.y);
-                mn = fminf(mn, m.cs_msg(e.x + (__builtin_amdgcn_ubfe(w, e.y, 2u) << m.cs_shift)));
-                uns += *lds_at<uint8_t>(a.unsat_lds + (e.x >> 2));
+        const float2 ls = m.ls_pos(vc);
+        const float post = ls.x + ls.y;
+        const int deg = a.g.var_deg[vc];
+        const float l0 = (PLACE & (1 << kL0)) ? m.l0_pos(vc) : m.l0g[a.g.vperm[vc]];
+        const int2 vs = var_slice(a, vc);
+        float mn = INFINITY;
+        int uns = 0;
+        // the slice's columns in rounds of four (loads, then code words, then
+        // messages); slots past the slice width read the sentinel check, slots
+        // past this variable's degree are not counted
+        const uint2 pad = make_uint2(4u * (uint32_t)C, m.sbb << 5);
+        for (int k0 = 0; k0 < vs.y; k0 += 4) {
+            uint2 e[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                e[u] = k0 + u < vs.y ? a.g.var_ent[vs.x + (k0 + u) * kSlice + (vc & 63)] : pad;
+            float msg[4];
+            int un[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t w = m.sb_ent(e[u].y);
+                msg[u] = m.cs_msg(e[u].x + (__builtin_amdgcn_ubfe(w, e[u].y, 2u) << m.cs_shift));
+                un[u] = *lds_at<uint8_t>(a.unsat_lds + (e[u].x >> 2));
             }
-            f[0] = deg > 0 ? clamp4(mn * 0.125f) : 0.f;
-            f[1] = (float)deg * 0.125f;
-            f[2] = deg > 0 ? (float)uns / (float)deg : 0.f;
-            f[3] = (float)r * 0.25f;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (k0 + u < deg) {
+                    mn = fminf(mn, msg[u]);
+                    uns += un[u];
+                }
+            }
         }
-        if (valid && q < 2) {
+        float f[8];
+        f[0] = l0 * 0.125f;
+        f[1] = clamp4(post * 0.0625f);
+        f[2] = (float)*lds_at<uint8_t>(a.flips_lds + (uint32_t)vc) * 0.125f;
+        f[3] = deg > 0 ? clamp4(ls.y / (float)deg * 0.125f) : 0.f;
+        f[4] = deg > 0 ? clamp4(mn * 0.125f) : 0.f;
+        f[5] = (float)deg * 0.125f;
+        f[6] = deg > 0 ? (float)uns / (float)deg : 0.f;
+        f[7] = (float)r * 0.25f;
+        if (valid) {
             const size_t row = ((size_t)b * L + r) * V + a.g.vperm[v];
-            if (a.feat_out) *reinterpret_cast<float4 *>(a.feat_out + row * 8 + 4 * q) = make_float4(f[0], f[1], f[2], f[3]);
-            if (a.hard_out && q == 0) a.hard_out[row] = hard ? 1 : 0;
+            if (a.feat_out) {
+                *reinterpret_cast<float4 *>(a.feat_out + row * 8) = make_float4(f[0], f[1], f[2], f[3]);
+                *reinterpret_cast<float4 *>(a.feat_out + row * 8 + 4) = make_float4(f[4], f[5], f[6], f[7]);
+            }
+            if (a.hard_out) a.hard_out[row] = post <= 0.f ? 1 : 0;
         }
         if (a.mlp) {
-            bf16x4_t x;
+            const uint32_t p01 = pack_bf16x2(f[0], f[1]), p23 = pack_bf16x2(f[2], f[3]);
+            const uint32_t p45 = pack_bf16x2(f[4], f[5]), p67 = pack_bf16x2(f[6], f[7]);
+            float z = 0.f;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) x[i] = (short)f32_to_bf16(f[i]);
-            const float z = mlp_logit(mr, x);
+            for (int grp = 0; grp < 4; ++grp) {
+                const int src = 16 * grp + (lane & 15);
+                const uint32_t a01 = (uint32_t)__shfl((int)p01, src, 64), a23 = (uint32_t)__shfl((int)p23, src, 64);
+                const uint32_t b01 = (uint32_t)__shfl((int)p45, src, 64), b23 = (uint32_t)__shfl((int)p67, src, 64);
+                const uint32_t x01 = q == 0 ? a01 : (q == 1 ? b01 : 0u);
+                const uint32_t x23 = q == 0 ? a23 : (q == 1 ? b23 : 0u);
+                bf16x4_t x;
+                x[0] = (short)(x01 & 0xFFFFu);
+                x[1] = (short)(x01 >> 16);
+                x[2] = (short)(x23 & 0xFFFFu);
+                x[3] = (short)(x23 >> 16);
+                const float zg = mlp_logit(mr, x);
+                z = q == grp ? zg : z;
+            }
             // shift of the leg's own draw: (hi - lo) (sigmoid(z) - 1/2); zero at z = 0
             const float d = (a.learned_hi - a.learned_lo) * (1.f / (1.f + expf(-z)) - 0.5f);
-            const uint16_t db = f32_to_bf16(d);
-            if (valid && q == 0) {
-                *lds_at<uint16_t>(a.gamma_lds + 2u * (uint32_t)v) = db;
+            if (valid) {
+                *lds_at<uint16_t>(a.gamma_lds + 2u * (uint32_t)v) = f32_to_bf16(d);
                 if (a.gamma_out) {
                     gs.lds = a.gamma_lds;
                     gs.lo = a.learned_lo;
@@ -679,7 +734,7 @@ __device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int
                 }
             }
         }
-        if (valid && q == 0) *lds_at<uint8_t>(a.flips_lds + (uint32_t)v) = 0;
+        if (valid) *lds_at<uint8_t>(a.flips_lds + (uint32_t)v) = 0;
     }
     __syncthreads();
 }
@@ -692,9 +747,13 @@ __device__ void select_solution(const KArgs &a, const ShotMem<PLACE> &m, const f
                                 uint8_t *err, ShotState &st, double *s_red) {
     const int V = a.g.V;
     double acc = 0.0;
-    for (int j = threadIdx.x; j < V; j += BLOCK) {
-        const float2 ls = m.ls_pos(a.g.vpos[j]);
-        if (ls.x + ls.y <= 0.f) acc += (double)wj[j];
+    // position order: contiguous state reads; wj is gathered only for the few
+    // bits that are set.  The double sum of n fp32 weights is exact, whatever
+    // the order, while the set weights span less than 2^(29 - log2 n) in
+    // magnitude (2^15 at n = V = 8785).
+    for (int v = threadIdx.x; v < V; v += BLOCK) {
+        const float2 ls = m.ls_pos(v);
+        if (ls.x + ls.y <= 0.f) acc += (double)wj[a.g.vperm[v]];
     }
     const float w = (float)block_sum<BLOCK>(acc, s_red);
     if (st.best_w >= 1e20f || w < st.best_w) {
@@ -857,9 +916,12 @@ __device__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &
 // Re-queue a shot between its first leg and its relay legs.  At a leg boundary
 // the messages are about to be reset (kernels.py:935-975), so the posterior and
 // the leg bookkeeping are the whole state: the posterior goes to rq_post, the
-// scalars to rq_state, and the shot is published in the next rq_slot.  Every
-// thread's stores are made visible device-wide (the resuming workgroup may run
-// on another XCD) before the leader publishes.
+// scalars to rq_state, and the shot is published in the next rq_slot.  The
+// resuming workgroup may run on another XCD, so the stores must be visible
+// device-wide before the leader publishes: the barrier orders every thread's
+// stores before the leader's agent-scope release fence, which is cumulative,
+// so one fence (one L2 write-back) covers the whole workgroup.  A fence in
+// every thread would write back and invalidate the XCD's L2 once per wave.
 template <int BLOCK, int PLACE, bool LEARN>
 __device__ void requeue_shot(const KArgs &a, int b, const ShotMem<PLACE> &m, const ShotState &st) {
     const int V = a.g.V;
@@ -869,9 +931,9 @@ __device__ void requeue_shot(const KArgs &a, int b, const ShotMem<PLACE> &m, con
         post[v] = ls.x + ls.y;
         if (LEARN && a.mlp) a.rq_gamma[(size_t)b * V + v] = *lds_at<uint16_t>(a.gamma_lds + 2u * (uint32_t)v);
     }
-    __threadfence();
     __syncthreads();
     if (opaque_tid() == 0) {
+        __threadfence();
         a.rq_state[b] = make_int4(__float_as_int(st.best_w), st.found, st.iters, st.any_iter);
         const int slot = atomicAdd(&a.ctrl[kCtrlPushed], 1);
         __hip_atomic_store(&a.rq_slot[slot], b, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -1219,6 +1281,65 @@ uint32_t row_table_addr(uint32_t scratch, int block) {
     return scratch + (uint32_t)align_up(12LL * (block / 64) + 8, 16);
 }
 
+// Order of the row slices across the CU's SIMDs.  The check pass gives slice s
+// to wave s mod (waves per workgroup), and a workgroup's waves sit on the four
+// SIMDs in turn, so slice s runs on SIMD s % 4 for every block size that is a
+// multiple of 256.  With rows sorted by degree the widest slices would meet on
+// SIMD 0.  The full slices (rorig holds the rows by decreasing degree) are
+// reassigned: widest first to the least-loaded SIMD with a free position, then
+// pairwise swaps while one lowers the larger of the two SIMDs' chunk totals.
+// Rows keep their degree order within a slice; a partial last slice stays last.
+void balance_row_slices(const int32_t *chk_ptr, int32_t C, std::vector<int32_t> &rorig) {
+    const int32_t nfull = C / kSlice;
+    if (nfull < 8) return;
+    auto chunks_at = [&](int32_t p) {
+        const int32_t d = chk_ptr[rorig[p] + 1] - chk_ptr[rorig[p]];
+        return (d + kChunk - 1) / kChunk;
+    };
+    std::vector<int32_t> chunks(nfull);
+    for (int32_t s = 0; s < nfull; ++s) chunks[s] = chunks_at(s * kSlice);
+    int64_t load[4] = {0, 0, 0, 0};
+    if (C % kSlice) load[nfull % 4] += chunks_at(nfull * kSlice);
+    std::vector<int32_t> cls[4];
+    size_t cap[4];
+    for (int r = 0; r < 4; ++r) cap[r] = r < nfull ? (size_t)((nfull - 1 - r) / 4 + 1) : 0;
+    for (int32_t s = 0; s < nfull; ++s) {
+        int best = -1;
+        for (int r = 0; r < 4; ++r)
+            if (cls[r].size() < cap[r] && (best < 0 || load[r] < load[best])) best = r;
+        cls[best].push_back(s);
+        load[best] += chunks[s];
+    }
+    auto improve = [&]() {
+        for (int x = 0; x < 4; ++x)
+            for (int y = 0; y < 4; ++y) {
+                if (x == y) continue;
+                for (size_t i = 0; i < cls[x].size(); ++i)
+                    for (size_t j = 0; j < cls[y].size(); ++j) {
+                        const int32_t a = cls[x][i], b = cls[y][j];
+                        const int64_t nx = load[x] - chunks[a] + chunks[b], ny = load[y] - chunks[b] + chunks[a];
+                        if (std::max(nx, ny) < std::max(load[x], load[y])) {
+                            cls[x][i] = b;
+                            cls[y][j] = a;
+                            load[x] = nx;
+                            load[y] = ny;
+                            return true;
+                        }
+                    }
+            }
+        return false;
+    };
+    for (int guard = 0; guard < 10000 && improve(); ++guard) {
+    }
+    std::vector<int32_t> out(rorig);
+    for (int32_t p = 0; p < nfull; ++p) {
+        const int32_t src = cls[p % 4][(size_t)(p / 4)];
+        std::copy(rorig.begin() + (int64_t)src * kSlice, rorig.begin() + (int64_t)(src + 1) * kSlice,
+                  out.begin() + (int64_t)p * kSlice);
+    }
+    rorig.swap(out);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1251,6 +1372,7 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
     std::stable_sort(rorig.begin(), rorig.end(), [&](int32_t x, int32_t y) {
         return (chk_ptr[x + 1] - chk_ptr[x]) > (chk_ptr[y + 1] - chk_ptr[y]);
     });
+    balance_row_slices(chk_ptr, C, rorig);
     for (int32_t p = 0; p < C; ++p) {
         rpos[rorig[p]] = p;
         rdeg[p] = chk_ptr[rorig[p] + 1] - chk_ptr[rorig[p]];

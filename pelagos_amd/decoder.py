@@ -354,7 +354,7 @@ class RelayBPDecoder:
         """
         R, V = self.num_sets, self.V
         per_row = self._explicit_gammas is None and self.gamma_draw == "per_row"
-        key = ("per_row", B) if per_row else ("broadcast",)
+        key = ("per_row", B, R) if per_row else ("broadcast", R)
         cached = self._gamma_cache.get(key)
         if cached is not None:
             return cached
@@ -546,6 +546,18 @@ class RelayBPDecoder:
         allocating or waiting: the launch a serving loop or a captured graph
         makes.  Call prepare(B) once first (broadcast gamma draws only)."""
         B = int(syndromes.shape[0])
+        # the buffers are handed to the kernel as raw pointers: check what it assumes
+        for name, t, shape, dtype in (("syndromes", syndromes, (B, self.C), torch.uint8),
+                                      ("errors", errors, (B, self.V), torch.uint8),
+                                      ("weights", weights, (B,), self.dtype),
+                                      ("iterations", iterations, (B,), torch.int32),
+                                      ("found_count", found_count, (B,), torch.int32)):
+            if tuple(t.shape) != shape or t.dtype != dtype or not t.is_contiguous() or t.device != self.device:
+                raise ValueError(f"decode_async: {name} must be a contiguous {dtype} tensor of shape {shape} on "
+                                 f"{self.device}, got {t.dtype} {tuple(t.shape)} on {t.device}"
+                                 f"{'' if t.is_contiguous() else ' (not contiguous)'}")
+        if workspace.dtype != torch.uint8 or not workspace.is_contiguous() or workspace.device != self.device:
+            raise ValueError("decode_async: workspace must be a contiguous uint8 tensor on the decoder's device")
         if B == 0:
             return
         if self.gamma_draw == "per_row" and self._explicit_gammas is None:
@@ -580,6 +592,11 @@ class RelayBPDecoder:
         else:
             ws = self._workspace_for(plan.workspace_bytes)
         per_row = gtab.dim() == 3
+        # the kernel reads rows 0 .. num_sets-1 (and, per row, B shots) of the table
+        if (self.num_sets > 0 and gtab.shape[0] < self.num_sets) or (per_row and gtab.shape[1] != B) \
+                or gtab.shape[-1] != self.V:
+            raise ValueError(f"gamma table of shape {tuple(gtab.shape)} does not cover num_sets={self.num_sets}, "
+                             f"B={B}, V={self.V}")
         if not per_row:
             gtab = self._gamma_by_position(gtab)
         params = _native.PelParams(

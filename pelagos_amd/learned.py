@@ -142,16 +142,35 @@ def _syndrome_ok(H, hard: np.ndarray, syn: np.ndarray) -> np.ndarray:
     return (s.T.reshape(n, k, -1) == syn[:, None, :]).all(2)
 
 
+def leg_iterations(dec, syndromes: torch.Tensor, legs: int) -> np.ndarray:
+    """[b, legs] BP iterations of relay legs 0 .. legs-1 of every shot under
+    ``dec`` (0 for a leg the shot never started): the differences of the
+    iteration counts of decodes stopped after r and r + 1 relay legs, whose
+    legs are the full decode's own (each leg's draw depends on the leg only)."""
+    saved = dec.num_sets
+    tot = []
+    try:
+        for r in range(legs + 1):
+            dec.num_sets = min(r, saved)
+            tot.append(dec.decode(syndromes)["iterations"].cpu().numpy().astype(np.int64))
+    finally:
+        dec.num_sets = saved
+    return np.diff(np.stack(tot, 1), axis=1)
+
+
 def training_set(dec, syndromes: np.ndarray, legs: int, max_samples: int, seed: int, batch: int = 256,
-                 hard_weight: float = 4.0, easy_weight: float = 1.0) -> Dict[str, torch.Tensor]:
+                 hard_weight: float = 4.0, easy_weight: float = 1.0,
+                 fast_iters: Optional[int] = None) -> Dict[str, torch.Tensor]:
     """Traces of the plain decoder ``dec`` (uniform-draw legs) on the given
     shots.  For every relay leg r < legs that a shot started: the features of
     every variable at the leg's start, and as target the variable's draw u
-    of that leg mapped to [0, 1], (u - lo) / (hi - lo).  Only legs that
-    converged count (the next leg starts from a valid solution, or the shot
-    stopped after it), so the fitted sigmoid is the mean draw of successful
-    legs given the features; legs that started from a state violating some
-    check weigh ``hard_weight``, the others ``easy_weight`` (0 drops them).  A
+    of that leg mapped to [0, 1], (u - lo) / (hi - lo).  Only successful legs
+    count, so the fitted sigmoid is the mean draw of successful legs given
+    the features.  A leg that started from a state violating some check
+    (weight ``hard_weight``) succeeds when it converged (the next leg starts
+    from a valid solution, or the shot stopped after it); a leg that started
+    from a valid solution (weight ``easy_weight``, 0 drops them) succeeds when
+    it converged within ``fast_iters`` iterations (any number when None).  A
     uniform sample of at most ``max_samples`` variables is kept."""
     dev = dec.device
     H = dec.H_csr.tocsr().astype(np.int32)
@@ -161,15 +180,19 @@ def training_set(dec, syndromes: np.ndarray, legs: int, max_samples: int, seed: 
     n_legs = n_ok = n_hard = 0
     for i in range(0, syndromes.shape[0], batch):
         syn = syndromes[i:i + batch]
-        out = dec.collect_traces(torch.from_numpy(syn).to(dev), legs + 1)
+        syn_dev = torch.from_numpy(syn).to(dev)
+        out = dec.collect_traces(syn_dev, legs + 1)
         hard = out["hard"].cpu().numpy()                                 # [b, legs+1, V]
         reached = hard[:, :, 0] != 255
         valid = np.zeros_like(reached)
         valid[reached] = _syndrome_ok(H, np.where(hard == 255, 0, hard), syn)[reached]
+        its = leg_iterations(dec, syn_dev, legs) if fast_iters is not None else None
         for r in range(legs):
             started = reached[:, r]
             ok = started & (valid[:, r + 1] | ~reached[:, r + 1])
             from_bad = started & ~valid[:, r]
+            if its is not None:
+                ok &= from_bad | (its[:, r] <= fast_iters)
             if easy_weight <= 0:
                 ok &= from_bad
             n_legs += int(started.sum())

@@ -83,6 +83,9 @@ def main() -> int:
                     help="weight of legs that started from a valid solution (0 drops them)")
     ap.add_argument("--prior-mean", type=float, default=0.003,
                     help="mean prior of the training shots' error draws")
+    ap.add_argument("--fast-iters", type=int, default=None,
+                    help="a leg that starts from a valid solution counts as successful only when it "
+                         "converges within this many iterations (default: any)")
     ap.add_argument("--save-legs", type=int, default=1, help="learned legs recorded with the weights")
     ap.add_argument("--const-gamma", type=float, nargs="*", default=[],
                     help="also evaluate legs with this constant strength for every variable")
@@ -98,7 +101,7 @@ def main() -> int:
         dec = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=1, device=[0], **RELAY)
         t0 = time.perf_counter()
         ts = learned.training_set(dec, syn, args.legs, args.samples, seed=1, hard_weight=args.hard_weight,
-                                  easy_weight=args.easy_weight)
+                                  easy_weight=args.easy_weight, fast_iters=args.fast_iters)
         log(f"[train] traces: {ts['x'].shape[0]} samples of {ts['candidates']} from {ts['legs_converged']} "
             f"converged legs of {ts['legs_started']} ({ts['converged_from_violated']} from a violated state) "
             f"in {time.perf_counter() - t0:.1f} s")
@@ -114,7 +117,7 @@ def main() -> int:
                                     "converged_from_violated": ts["converged_from_violated"], "epochs": args.epochs,
                                     "losses": [round(x, 5) for x in losses], "seconds": round(train_s, 2),
                                     "prior_scale": round(scale, 4), "hard_weight": args.hard_weight,
-                                    "easy_weight": args.easy_weight,
+                                    "easy_weight": args.easy_weight, "fast_iters": args.fast_iters,
                                     "weights": str(out_path)}}), flush=True)
     net = learned.GammaMLP.load(out_path)
 

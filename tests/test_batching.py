@@ -67,3 +67,33 @@ def test_autotune_picks_a_shape_and_caches_it(tmp_path):
     for key in ("errors", "weights", "iterations", "found_count"):
         assert torch.equal(out[key], base[key]), key
     assert autotune(dec, syn, blocks=(256, 1024), requeue=(0, 16), reps=1, cache=cache) == (blk, rq, ms)
+
+
+def test_decode_async_matches_decode_and_checks_its_buffers():
+    """decode_async on side streams with caller-owned buffers gives decode()'s
+    results; buffers the kernel cannot take as raw pointers are refused."""
+    from pelagos_amd import RelayBPDecoder
+
+    H, p = _codes()["banded4"]
+    dec = RelayBPDecoder(H, p, **RELAY)
+    err = codes.sample_errors(p, 96, seed=4, scale=1.5)
+    raw = torch.from_numpy(codes.syndromes_of(H, err)).cuda()   # transposed layout: not contiguous
+    syn = raw.contiguous()
+    ref = dec.decode(syn)
+    B, V = syn.shape[0], dec.V
+    ws = torch.empty(dec.prepare(B), dtype=torch.uint8, device="cuda")
+    out = dict(errors=torch.empty(B, V, dtype=torch.uint8, device="cuda"),
+               weights=torch.empty(B, dtype=torch.float32, device="cuda"),
+               iterations=torch.empty(B, dtype=torch.int32, device="cuda"),
+               found_count=torch.empty(B, dtype=torch.int32, device="cuda"))
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    dec.decode_async(syn, out["errors"], out["weights"], out["iterations"], out["found_count"], ws, s)
+    torch.cuda.synchronize()
+    for k, v in out.items():
+        assert torch.equal(v, ref[k]), k
+    if not raw.is_contiguous():
+        with pytest.raises(ValueError, match="contiguous"):
+            dec.decode_async(raw, out["errors"], out["weights"], out["iterations"], out["found_count"], ws, s)
+    with pytest.raises(ValueError, match="iterations"):
+        dec.decode_async(syn, out["errors"], out["weights"], out["iterations"].long(), out["found_count"], ws, s)

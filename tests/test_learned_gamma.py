@@ -208,3 +208,26 @@ def test_gpu_traces_record_the_state_at_each_leg():
     # a shot that needed no relay leg records nothing
     done = out["found_count"].cpu().numpy() >= 4
     assert ((hard[:, 0, 0] == 255) <= done).all()
+
+
+@pytest.mark.gpu
+def test_gpu_leg_iterations_match_truncated_decoders():
+    """Per-leg iteration counts (decodes stopped after r relay legs, on a fresh
+    decoder that has drawn no gamma table yet) add up to the full decode's, and
+    match decoders built with num_sets = r: every stopped decode draws a table
+    of its own length."""
+    H, p, err, syn = _banded(64, seed=17)
+    s = torch.from_numpy(syn).cuda()
+    dec = _decoder(H, p)
+    its = learned.leg_iterations(dec, s, 3)
+    full = dec.decode(s)["iterations"].cpu().numpy()
+    first = _decoder(H, p).decode(s, skip_relay_legs=True)["iterations"].cpu().numpy()
+    from pelagos_amd import RelayBPDecoder
+
+    for r in (1, 3):
+        dr = RelayBPDecoder(H, p, pre_iter=30, num_sets=r, set_max_iter=20, gamma0=0.65,
+                            gamma_dist_interval=(-0.24, 0.66), stop_nconv=4, data_type="float32", algo="relay",
+                            seed=3, device=[0])
+        np.testing.assert_array_equal(first + its[:, :r].sum(1), dr.decode(s)["iterations"].cpu().numpy())
+    assert (its >= 0).all() and (first + its.sum(1) <= full).all()
+    assert dec.num_sets == 12
