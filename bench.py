@@ -1,17 +1,22 @@
 """Decoded shots/s of Relay-BP-S on one MI355X (936 x 8785 banded circuit-shaped code),
 plain and with learned strengths, plus serving latency.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--shots 4096] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--shots 4096] [--depth 2] [--no-cpu]
                     [--no-learned] [--no-serving] [--no-batched]
 
-A step is one ``RelayBPDecoder.decode`` of the whole 4096-shot batch, with the
-syndromes already resident in HBM.  Prints ONE JSON line (rank 0) with:
-the throughput of the plain form (uniform-draw legs) and, when trained weights
-are present, of the learned form on the same shots with both error rates;
-``value`` is the faster form whose synthetic-observable error rate lies in
-the plain form's 95% interval.  Also the decode kernel's roofline figures
-(algorithmic edge-message bytes over the kernel time measured with HIP events
-on the decode stream), one graph launch over three codes, the
+A step decodes the whole 4096-shot batch (syndromes already resident in HBM)
+into its own output buffers.  The timed steps run with ``--depth`` launches in
+flight: step i is issued on stream i % depth (each stream with its own
+workspace and outputs), so a launch's last few slow shots overlap the next
+launch instead of leaving the GPU idle.  Every step's results are checked
+against a plain ``decode()`` of the batch.  Prints ONE JSON line (rank 0)
+with: the throughput of the plain form (uniform-draw legs) and, when trained
+weights are present, of the learned form on the same shots with both error
+rates; ``value`` is the faster form whose synthetic-observable error rate
+lies in the plain form's 95% interval.  Also the decode kernel's roofline
+figures (algorithmic edge-message bytes over the per-launch kernel time
+measured with HIP events on each launch's stream, the aggregate rate, and the
+same for launches one at a time), one graph launch over three codes, the
 continuous-batching server's p50/p99 latency at five Poisson arrival rates,
 and the CPU oracle timed on the same shots on this host.
 """
@@ -142,6 +147,56 @@ def traffic_record(fetch_dir: str, write_dir: str, cfetch_dir: str, cwrite_dir: 
     return 0
 
 
+def measure(dec, syn, steps: int, warmup: int, depth: int) -> dict:
+    """Time ``steps`` decodes of ``syn`` with ``depth`` launches in flight
+    (step i on stream i % depth, each stream with its own workspace and output
+    buffers), after ``warmup`` untimed ones, bracketed by device
+    synchronisation.  Returns the rate, the per-launch kernel times (HIP events
+    recorded by the library around each decode kernel on its stream), and the
+    outputs of the last step."""
+    import torch
+
+    B, V = int(syn.shape[0]), dec.V
+    dev = syn.device
+    wsb = dec.prepare(B)
+    streams = [torch.cuda.Stream(device=dev) for _ in range(depth)]
+    slots = [dict(ws=torch.empty(wsb, dtype=torch.uint8, device=dev),
+                  errors=torch.empty(B, V, dtype=torch.uint8, device=dev),
+                  weights=torch.empty(B, dtype=torch.float32, device=dev),
+                  iterations=torch.empty(B, dtype=torch.int32, device=dev),
+                  found_count=torch.empty(B, dtype=torch.int32, device=dev)) for _ in range(depth)]
+
+    def issue(n, events):
+        cur = torch.cuda.current_stream(dev)
+        for st in streams:
+            st.wait_stream(cur)
+        dec.kernel_events = events
+        for i in range(n):
+            o = slots[i % depth]
+            dec.decode_async(syn, o["errors"], o["weights"], o["iterations"], o["found_count"], o["ws"],
+                             streams[i % depth])
+        dec.kernel_events = None
+        for st in streams:
+            cur.wait_stream(st)
+
+    issue(max(warmup, depth), None)
+    torch.cuda.synchronize(dev)
+    ev = []
+    t0 = time.perf_counter()
+    issue(steps, ev)
+    torch.cuda.synchronize(dev)
+    elapsed = time.perf_counter() - t0
+    kern = [a.elapsed_time(b) for a, b in ev]
+    return {"rate": B * steps / elapsed, "elapsed": elapsed, "kernel_ms": kern,
+            "out": slots[(steps - 1) % depth], "slots": slots[:min(depth, steps)]}
+
+
+def same_outputs(slots, ref) -> bool:
+    import torch
+
+    return all(torch.equal(o[k], ref[k]) for o in slots for k in ("errors", "weights", "iterations", "found_count"))
+
+
 def learned_form(H, p, err, syn, syn_np, args, plain_rate, plain_ler) -> dict:
     """The learned-strength form on the same shots (same steps/warmup)."""
     import numpy as np
@@ -157,29 +212,24 @@ def learned_form(H, p, err, syn, syn_np, args, plain_rate, plain_ler) -> dict:
     net = learned.GammaMLP.load(wpath)
     dec = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0], learned_gamma=net,
                          learned_legs=legs, **RELAY)
-    for _ in range(args.warmup):
-        out = dec.decode(syn)
-    ev = []
-    dec.kernel_events = ev
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out = dec.decode(syn)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    dec.kernel_events = None
-    kms = sum(a.elapsed_time(b) for a, b in ev) / max(len(ev), 1)
+    ref = dec.decode(syn)
+    m = measure(dec, syn, args.steps, args.warmup, args.depth)
+    seq = measure(dec, syn, max(3, args.steps // 4), 1, 1)
+    out = m["out"]
     errors = out["errors"].cpu().numpy()
     conv = out["found_count"].cpu().numpy() >= RELAY["stop_nconv"]
     O = codes.synthetic_observables(H.shape[1], seed=0).astype(np.int64)
     logical = (~conv) | ((O @ (errors ^ err).T.astype(np.int64)) % 2).any(0)
     le = wilson(int(logical.sum()), errors.shape[0])
-    rate = args.shots * args.steps / elapsed
-    return {"shots_per_s": round(rate, 1), "kernel_ms": round(kms, 4), "learned_legs": legs,
-            "speedup_vs_plain": round(rate / plain_rate, 3),
+    rate = m["rate"]
+    return {"shots_per_s": round(rate, 1), "kernel_ms": round(sum(m["kernel_ms"]) / len(m["kernel_ms"]), 4),
+            "learned_legs": legs, "speedup_vs_plain": round(rate / plain_rate, 3),
+            "sequential_shots_per_s": round(seq["rate"], 1),
+            "sequential_kernel_ms": round(sum(seq["kernel_ms"]) / len(seq["kernel_ms"]), 4),
             "mean_iterations": round(float(out["iterations"].float().mean()), 3),
             "synthetic_observable_ler": [round(x, 5) for x in le],
             "ler_within_plain_ci": bool(plain_ler[1] <= le[0] <= plain_ler[2]),
+            "matches_decode": same_outputs(m["slots"] + seq["slots"], ref),
             "weights": str(wpath.relative_to(ROOT))}
 
 
@@ -288,6 +338,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--shots", type=int, default=4096)
+    ap.add_argument("--depth", type=int, default=2, help="decode launches in flight in the timed steps")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-strength form")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-latency runs")
@@ -333,28 +384,20 @@ def main() -> int:
     dec = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0],
                          block_size=args.block or None, **RELAY)
     C, V, E = dec.C, dec.V, dec.E
-    syn = torch.from_numpy(syn_np).to(dev)
+    syn = torch.from_numpy(syn_np).to(dev).contiguous()
     plan = dec.plan(args.shots)
     print(f"[bench] banded {C}x{V} E={E} shots={args.shots} block={plan.block} grid={plan.grid} "
-          f"placement={plan.placement} lds={plan.lds_bytes}", file=sys.stderr, flush=True)
+          f"placement={plan.placement} lds={plan.lds_bytes} depth={args.depth}", file=sys.stderr, flush=True)
 
-    for _ in range(args.warmup):
-        out = dec.decode(syn)
-    torch.cuda.synchronize()
-
-    stream = torch.cuda.current_stream(dev)
-    ev = []
-    dec.kernel_events = ev
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out = dec.decode(syn)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    dec.kernel_events = None
-    kern_ms = [a.elapsed_time(b) for a, b in ev]
+    ref = dec.decode(syn)
+    m = measure(dec, syn, args.steps, args.warmup, args.depth)   # the timed region
+    seq = measure(dec, syn, max(3, args.steps // 4), 1, 1)        # one launch at a time
+    elapsed = m["elapsed"]
+    kern_ms = m["kernel_ms"]
     kern_avg_ms = sum(kern_ms) / max(len(kern_ms), 1)
-    del stream
+    seq_ms = sum(seq["kernel_ms"]) / max(len(seq["kernel_ms"]), 1)
+    out = m["out"]
+    matches = same_outputs(m["slots"] + seq["slots"], ref)
 
     iters = out["iterations"].cpu().numpy().astype(np.int64)
     found = out["found_count"].cpu().numpy()
@@ -362,7 +405,9 @@ def main() -> int:
     total_iters = int(iters.sum())
     b_iter = bytes_per_iteration(C, V, E)
     achieved_gbs = total_iters * b_iter / (kern_avg_ms * 1e-3) / 1e9
-    shots_per_s = args.shots * args.steps / elapsed
+    aggregate_gbs = total_iters * b_iter * args.steps / elapsed / 1e9
+    isolated_gbs = total_iters * b_iter / (seq_ms * 1e-3) / 1e9
+    shots_per_s = m["rate"]
 
     # Quality: frame errors and a seeded synthetic-observable logical error rate.
     valid = (codes.syndromes_of(H, errors) == syn_np).all(1)
@@ -414,8 +459,14 @@ def main() -> int:
             "traffic": traffic,
             "kernel": "relay_decode_kernel",
             "kernel_ms": round(kern_avg_ms, 4),
+            "launches_in_flight": args.depth,
             "bytes_per_iteration": b_iter,
             "iterations_per_launch": total_iters,
+            "aggregate_achieved": round(aggregate_gbs, 1),
+            "aggregate_frac": round(aggregate_gbs / HBM_PEAK_GBS, 4),
+            "isolated_kernel_ms": round(seq_ms, 4),
+            "isolated_achieved": round(isolated_gbs, 1),
+            "isolated_frac": round(isolated_gbs / HBM_PEAK_GBS, 4),
         },
         "quality": {
             "mean_iterations": round(float(iters.mean()), 3),
@@ -426,6 +477,8 @@ def main() -> int:
         },
     }
     result["plain"] = {"shots_per_s": round(shots_per_s, 1), "kernel_ms": round(kern_avg_ms, 4),
+                       "sequential_shots_per_s": round(seq["rate"], 1), "sequential_kernel_ms": round(seq_ms, 4),
+                       "matches_decode": matches,
                        "synthetic_observable_ler": [round(x, 5) for x in le]}
     if not args.no_learned:
         result["learned"] = learned_form(H, p, err, syn, syn_np, args, shots_per_s, le)
