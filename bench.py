@@ -191,6 +191,30 @@ def measure(dec, syn, steps: int, warmup: int, depth: int) -> dict:
             "out": slots[(steps - 1) % depth], "slots": slots[:min(depth, steps)]}
 
 
+def ler_eval(dec, H, p, batches: int = 16, shots: int = 4096) -> dict:
+    """Synthetic-observable logical error rate of ``dec`` on ``batches`` fresh
+    batches of ``shots`` shots (errors drawn from the priors, seeds 101..), a
+    shot failing when it did not converge or its decoding differs from the
+    true error by an observable; Wilson 95% interval."""
+    import numpy as np
+    import torch
+
+    from pelagos_amd import codes
+
+    O = codes.synthetic_observables(H.shape[1], seed=0).astype(np.int64)
+    fails = n = 0
+    for k in range(batches):
+        err = codes.sample_errors(p, shots, seed=101 + k)
+        syn = torch.from_numpy(codes.syndromes_of(H, err)).to(dec.device).contiguous()
+        out = dec.decode(syn)
+        e = out["errors"].cpu().numpy()
+        conv = out["found_count"].cpu().numpy() >= dec.stop_nconv
+        fails += int(((~conv) | ((O @ (e ^ err).T.astype(np.int64)) % 2).any(0)).sum())
+        n += shots
+    lo_hi = wilson(fails, n)
+    return {"shots": n, "failures": fails, "ler": [round(x, 6) for x in lo_hi]}
+
+
 def same_outputs(slots, ref) -> bool:
     import torch
 
@@ -222,13 +246,15 @@ def learned_form(H, p, err, syn, syn_np, args, plain_rate, plain_ler) -> dict:
     logical = (~conv) | ((O @ (errors ^ err).T.astype(np.int64)) % 2).any(0)
     le = wilson(int(logical.sum()), errors.shape[0])
     rate = m["rate"]
+    big = ler_eval(dec, H, p)
     return {"shots_per_s": round(rate, 1), "kernel_ms": round(sum(m["kernel_ms"]) / len(m["kernel_ms"]), 4),
+            "ler_65536": big,
+            "ler_within_plain_ci": bool(plain_ler[1] <= big["ler"][0] <= plain_ler[2]),
             "learned_legs": legs, "speedup_vs_plain": round(rate / plain_rate, 3),
             "sequential_shots_per_s": round(seq["rate"], 1),
             "sequential_kernel_ms": round(sum(seq["kernel_ms"]) / len(seq["kernel_ms"]), 4),
             "mean_iterations": round(float(out["iterations"].float().mean()), 3),
             "synthetic_observable_ler": [round(x, 5) for x in le],
-            "ler_within_plain_ci": bool(plain_ler[1] <= le[0] <= plain_ler[2]),
             "matches_decode": same_outputs(m["slots"] + seq["slots"], ref),
             "weights": str(wpath.relative_to(ROOT))}
 
@@ -481,7 +507,9 @@ def main() -> int:
                        "matches_decode": matches,
                        "synthetic_observable_ler": [round(x, 5) for x in le]}
     if not args.no_learned:
-        result["learned"] = learned_form(H, p, err, syn, syn_np, args, shots_per_s, le)
+        big = ler_eval(dec, H, p)
+        result["plain"]["ler_65536"] = big
+        result["learned"] = learned_form(H, p, err, syn, syn_np, args, shots_per_s, big["ler"])
         lr = result["learned"]
         if lr.get("ler_within_plain_ci") and lr.get("shots_per_s", 0) > shots_per_s:
             result["value"] = lr["shots_per_s"]

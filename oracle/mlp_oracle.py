@@ -89,15 +89,18 @@ def bf16(a: np.ndarray) -> np.ndarray:
     return r.astype(np.uint32).view(np.float32)
 
 
-def shift_bf16(p: np.ndarray, x: np.ndarray, lo: float = -0.24, hi: float = 0.66) -> np.ndarray:
+def shift_q8(p: np.ndarray, x: np.ndarray, lo: float = -0.24, hi: float = 0.66) -> np.ndarray:
     """Per-variable shift of a leg's draw as the decode kernel computes it:
     bf16 features, weights and first-layer activations, exact accumulation,
-    (hi - lo) (sigmoid(z) - 1/2) in fp32, rounded to bf16.  The strength of
-    the variable is then clamp(u + shift, lo, hi) with u the leg's draw."""
+    then 254 (sigmoid(z) - 1/2) in fp32 rounded to the nearest integer (ties
+    to even) and scaled by (hi - lo) / 254.  The strength of the variable is
+    clamp(u + shift, lo, hi) with u the leg's draw."""
     q = unpack(np.asarray(p, np.float32))
     xb = bf16(x).astype(np.float64)
     h1 = bf16(np.maximum(xb @ bf16(q["W1"]).astype(np.float64).T + q["b1"], 0.0).astype(np.float32))
     h2 = np.maximum(h1.astype(np.float64) @ bf16(q["W2"]).astype(np.float64).T + q["b2"], 0.0)
-    z = h2 @ q["w3"] + q["b3"][0]
-    d = np.float32(hi - lo) * (1.0 / (1.0 + np.exp(-z)) - 0.5)
-    return bf16(d.astype(np.float32))
+    z = (h2 @ q["w3"] + q["b3"][0]).astype(np.float32)
+    sig = np.float32(1.0) / (np.float32(1.0) + np.exp(-z))
+    steps = np.rint(np.float32(254.0) * (sig - np.float32(0.5))).astype(np.float32)
+    scale = (np.float32(hi) - np.float32(lo)) * np.float32(1.0 / 254.0)
+    return (steps * scale).astype(np.float32)

@@ -148,7 +148,8 @@ struct KArgs {
     uint32_t flips_lds, unsat_lds, gamma_lds;  // LDS addresses: flip counts u8 [V+1], violated checks
                                                // u8 [C+1], learned shifts bf16 [V]
     float learned_lo, learned_hi;              // strength interval of the relay legs
-    uint16_t *rq_gamma;        // [B][V] relay-leg-0 strengths of re-queued shots (bf16, by position)
+    int8_t *rq_gamma;          // [B][V] relay-leg-0 strength shifts of re-queued shots (by position)
+    const uint32_t *mlp_frag;  // [64][kMlpStride] the network as each lane's MFMA operands
 };
 
 template <int PLACE>
@@ -240,8 +241,9 @@ struct GammaSrc {
     const float *row;
     const int32_t *perm;  // non-null: row is in the caller's order
     float konst;
-    uint32_t lds = 0;     // non-zero: learned per-variable shifts (bf16, by position) at this LDS
-    float lo = 0.f, hi = 0.f;  // address, added to the row's strength and clamped to [lo, hi]
+    uint32_t lds = 0;     // non-zero: learned per-variable shifts (int8 steps of (hi - lo) / 254, by
+    float lo = 0.f, hi = 0.f;  // position) at this LDS address, added to the row's strength and
+                               // clamped to [lo, hi]
     __device__ __forceinline__ float base(int v) const {
         if (!row) return konst;
         return perm ? row[perm[v]] : row[v];
@@ -249,7 +251,7 @@ struct GammaSrc {
     __device__ __forceinline__ float operator()(int v) const {
         const float u = base(v);
         if (!lds) return u;
-        const float d = __uint_as_float((uint32_t)*lds_at<uint16_t>(lds + 2u * (uint32_t)v) << 16);
+        const float d = (float)*lds_at<int8_t>(lds + (uint32_t)v) * ((hi - lo) * (1.f / 254.f));
         return fminf(fmaxf(u + d, lo), hi);
     }
 };
@@ -259,6 +261,8 @@ struct ShotState {
     int32_t found;
     int32_t iters;
     int32_t any_iter;
+    int32_t last_ok;  // the last leg ended at a valid solution (its final state satisfies every check)
+    int32_t shifted;  // the network gave this shot strength shifts for its learned legs
 };
 
 // Values every thread of the workgroup holds identically (LDS broadcasts,
@@ -491,9 +495,8 @@ __device__ __forceinline__ void var_messages(const ShotMem<PLACE> &m, const uint
 
 // Variable pass for variable position v (kernels.py:1299-1350).  The incoming
 // messages are added one at a time in ascending edge id.
-template <int PLACE, bool LEARN = false>
-__device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> &m, int v, const GammaSrc &gs,
-                                           bool flips_on = false) {
+template <int PLACE, bool FLIPS = false>
+__device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> &m, int v, const GammaSrc &gs) {
     const int2 vs = var_slice(a, v);  // {ent, width}
     const float g = gs(v);
     const float l0 = (PLACE & (1 << kL0)) ? m.l0_pos(v) : m.l0g[a.g.vperm[v]];
@@ -525,7 +528,7 @@ __device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> 
         }
     }
     m.ls_set(v, make_float2(lm, s));
-    if (LEARN && flips_on && ((post_prev <= 0.f) != (lm + s <= 0.f))) {
+    if (FLIPS && ((post_prev <= 0.f) != (lm + s <= 0.f))) {
         // hard-decision flips during the leg (a feature of the learned strengths)
         __attribute__((address_space(3))) uint8_t *c = lds_at<uint8_t>(a.flips_lds + (uint32_t)v);
         *c = (uint8_t)min((int)*c + 1, 255);
@@ -549,11 +552,6 @@ __device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> 
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint16_t f32_to_bf16(float f) {  // round to nearest even (finite inputs)
-    const uint32_t u = __float_as_uint(f);
-    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-}
-__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 constexpr int kMW1 = 0, kMB1 = 32 * 8, kMW2 = kMB1 + 32, kMB2 = kMW2 + 32 * 32, kMW3 = kMB2 + 32, kMB3 = kMW3 + 32;
 static_assert(kMB3 + 1 == PEL_MLP_PARAMS, "parameter layout");
@@ -562,50 +560,109 @@ static_assert(kMB3 + 1 == PEL_MLP_PARAMS, "parameter layout");
 struct MlpRegs {
     bf16x4_t w1[2];     // layer-1 A operands (hidden block mb, features 4 (l/16) .. +3)
     bf16x4_t w2[2][2];  // layer-2 A operands [hidden block][input block]
-    float b1[2][4], b2[2][4], w3[2][4], b3;
+    f32x4_t b1[2], b2[2];  // biases of this lane's four D rows: the accumulators' start
+    float w3[2][4], b3;
 };
 
-__device__ void load_mlp(const float *__restrict__ p, MlpRegs &r) {
-    const int l = threadIdx.x & 63, row = l & 15, kq = 4 * (l >> 4);
-#pragma unroll
+// The network as each lane's MFMA operands, laid out [lane][kMlpStride words]
+// so that a lane loads its registers with ten 16-byte loads from one base
+// address (immediate offsets): words 0-3 the two layer-1 A operands, 4-11 the
+// four layer-2 A operands (bf16 pairs), 12-19 b1, 20-27 b2, 28-35 w3 (this
+// lane's four rows of each hidden block), 36 b3, 37-39 zero.
+constexpr int kMlpWords = 37, kMlpStride = 40;
+
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+// Two fp32 values rounded to bf16 (nearest even; one v_cvt_pk_bf16_f32), low first.
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
+    bf16x2_t v;
+    v[0] = (__bf16)a;
+    v[1] = (__bf16)b;
+    return __builtin_bit_cast(uint32_t, v);
+}
+
+// One wave: lane l writes its operand words (weights rounded to bf16 once).
+__global__ void __launch_bounds__(64) mlp_frag_kernel(const float *__restrict__ p, uint32_t *__restrict__ frag) {
+    const int l = threadIdx.x, row = l & 15, kq = 4 * (l >> 4);
+    auto put = [&](int w, uint32_t x) { frag[l * kMlpStride + w] = x; };
     for (int mb = 0; mb < 2; ++mb) {
-#pragma unroll
+        float a1[4], a2[2][4];
         for (int j = 0; j < 4; ++j) {
             const int k = kq + j;
-            r.w1[mb][j] = (short)(k < 8 ? f32_to_bf16(p[kMW1 + (16 * mb + row) * 8 + k]) : 0);
-#pragma unroll
-            for (int sb = 0; sb < 2; ++sb) r.w2[mb][sb][j] = (short)f32_to_bf16(p[kMW2 + (16 * mb + row) * 32 + 16 * sb + k]);
+            a1[j] = k < 8 ? p[kMW1 + (16 * mb + row) * 8 + k] : 0.f;
+            for (int sb = 0; sb < 2; ++sb) a2[sb][j] = p[kMW2 + (16 * mb + row) * 32 + 16 * sb + k];
         }
-#pragma unroll
+        put(2 * mb, pack_bf16x2(a1[0], a1[1]));
+        put(2 * mb + 1, pack_bf16x2(a1[2], a1[3]));
+        for (int sb = 0; sb < 2; ++sb) {
+            put(4 + 4 * mb + 2 * sb, pack_bf16x2(a2[sb][0], a2[sb][1]));
+            put(4 + 4 * mb + 2 * sb + 1, pack_bf16x2(a2[sb][2], a2[sb][3]));
+        }
         for (int i = 0; i < 4; ++i) {
             const int h = 16 * mb + kq + i;
-            r.b1[mb][i] = p[kMB1 + h];
-            r.b2[mb][i] = p[kMB2 + h];
-            r.w3[mb][i] = p[kMW3 + h];
+            put(12 + 4 * mb + i, __float_as_uint(p[kMB1 + h]));
+            put(20 + 4 * mb + i, __float_as_uint(p[kMB2 + h]));
+            put(28 + 4 * mb + i, __float_as_uint(p[kMW3 + h]));
         }
     }
-    r.b3 = p[kMB3];
+    put(36, __float_as_uint(p[kMB3]));
+    for (int w = kMlpWords; w < kMlpStride; ++w) put(w, 0u);
+}
+
+__device__ __forceinline__ bf16x4_t unpack_bf16x4(uint32_t lo, uint32_t hi) {
+    bf16x4_t x;
+    x[0] = (short)(lo & 0xFFFFu);
+    x[1] = (short)(lo >> 16);
+    x[2] = (short)(hi & 0xFFFFu);
+    x[3] = (short)(hi >> 16);
+    return x;
+}
+
+__device__ void load_mlp(const uint32_t *__restrict__ frag, MlpRegs &r) {
+    const uint4 *__restrict__ base = reinterpret_cast<const uint4 *>(frag + (threadIdx.x & 63) * kMlpStride);
+    uint32_t w[kMlpStride];
+#pragma unroll
+    for (int i = 0; i < kMlpStride / 4; ++i) {
+        const uint4 q = base[i];
+        w[4 * i] = q.x;
+        w[4 * i + 1] = q.y;
+        w[4 * i + 2] = q.z;
+        w[4 * i + 3] = q.w;
+    }
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+        r.w1[mb] = unpack_bf16x4(w[2 * mb], w[2 * mb + 1]);
+#pragma unroll
+        for (int sb = 0; sb < 2; ++sb) r.w2[mb][sb] = unpack_bf16x4(w[4 + 4 * mb + 2 * sb], w[5 + 4 * mb + 2 * sb]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            r.b1[mb][i] = __uint_as_float(w[12 + 4 * mb + i]);
+            r.b2[mb][i] = __uint_as_float(w[20 + 4 * mb + i]);
+            r.w3[mb][i] = __uint_as_float(w[28 + 4 * mb + i]);
+        }
+    }
+    r.b3 = __uint_as_float(w[36]);
 }
 
 // Logit z of the 16 variables whose features this wave holds (lane l: variable
 // l % 16, features 4 (l/16) .. +3; lanes 32-63 hold the zero padding of K).
 // Every lane returns z of its variable.  All 64 lanes must be active.
 __device__ __forceinline__ float mlp_logit(const MlpRegs &r, bf16x4_t x) {
-    const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
     bf16x4_t h1[2];
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
-        const f32x4_t d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w1[mb], x, zero, 0, 0, 0);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) h1[mb][i] = (short)f32_to_bf16(fmaxf(d[i] + r.b1[mb][i], 0.f));
+        // the accumulator starts at the bias: relu(b1 + W1 x), rounded to bf16
+        const f32x4_t d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w1[mb], x, r.b1[mb], 0, 0, 0);
+        h1[mb] = unpack_bf16x4(pack_bf16x2(fmaxf(d[0], 0.f), fmaxf(d[1], 0.f)),
+                               pack_bf16x2(fmaxf(d[2], 0.f), fmaxf(d[3], 0.f)));
     }
     float part = 0.f;
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
-        f32x4_t d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w2[mb][0], h1[0], zero, 0, 0, 0);
+        f32x4_t d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w2[mb][0], h1[0], r.b2[mb], 0, 0, 0);
         d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w2[mb][1], h1[1], d, 0, 0, 0);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) part += r.w3[mb][i] * fmaxf(d[i] + r.b2[mb][i], 0.f);
+        for (int i = 0; i < 4; ++i) part = __builtin_fmaf(r.w3[mb][i], fmaxf(d[i], 0.f), part);
     }
     part += __shfl_xor(part, 16, 64);
     part += __shfl_xor(part, 32, 64);
@@ -623,29 +680,28 @@ __device__ __forceinline__ float clamp4(float x) { return fminf(fmaxf(x, -4.f), 
 //   7 r / 4
 // Every lane computes the eight features of one variable (a wave takes a
 // 64-variable slice).  Then, with parameters, the network gives every variable
-// its strength for leg r (bf16 in LDS): the slice is evaluated as four
-// 16-variable MFMA groups, whose B operands (lane l: variable l % 16, features
-// 4 (l / 16) .. +3, zero for l >= 32) are gathered from the owning lanes with
-// ds_bpermute; lane l's own logit comes out of group l / 16.  The flip counts
-// are cleared for the next leg.  The network does not replace the leg's random
-// draw u (the disorder that lets successive legs explore differently): it
-// shifts it, gamma = clamp(u + (hi - lo) (sigmoid(z) - 1/2), lo, hi), so a
-// network that outputs z = 0 reproduces the uniform legs exactly.  Optionally
-// records features, hard decisions and the strengths used in the caller's
-// order.
-__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
-    return (uint32_t)f32_to_bf16(a) | ((uint32_t)f32_to_bf16(b) << 16);
-}
-
+// its strength shift for leg r: the slice is evaluated as four 16-variable
+// MFMA groups, whose B operands (lane l: variable l % 16, features 4 (l / 16)
+// .. +3, zero for l >= 32) are gathered from the owning lanes with
+// ds_bpermute; lane l's own logit comes out of group l / 16.  The shift is
+// (hi - lo) (sigmoid(z) - 1/2), kept in LDS as the nearest multiple of
+// (hi - lo) / 254 (int8), and the leg uses gamma = clamp(u + shift, lo, hi)
+// with u the leg's own random draw, whose disorder lets successive legs
+// explore differently; a network that outputs z = 0 reproduces the uniform
+// legs exactly.  known_valid: the state satisfies every check (the previous
+// leg converged), so feature 6 is 0 without a parity pass.  apply: evaluate
+// the network and shift the leg (else the leg keeps its uniform row).  The flip counts
+// are cleared for the next leg.  Optionally records features and hard
+// decisions in the caller's order.
 template <int BLOCK, int PLACE>
 __device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int r, const uint8_t *__restrict__ syn,
-                             GammaSrc gs) {
+                             bool known_valid, bool apply) {
     const int C = a.g.C, V = a.g.V, L = a.learned_legs;
-    MlpRegs mr;
-    if (a.mlp) load_mlp(a.mlp, mr);
-    for (int i = threadIdx.x; i <= C; i += BLOCK)
-        *lds_at<uint8_t>(a.unsat_lds + (uint32_t)i) = (uint8_t)(i < C ? parity_row<PLACE>(a, m, i, syn) : 0);
-    __syncthreads();
+    if (!known_valid) {
+        for (int i = threadIdx.x; i <= C; i += BLOCK)
+            *lds_at<uint8_t>(a.unsat_lds + (uint32_t)i) = (uint8_t)(i < C ? parity_row<PLACE>(a, m, i, syn) : 0);
+        __syncthreads();
+    }
     const int lane = threadIdx.x & 63, q = lane >> 4;
     // v0 is uniform per wave, so every lane of a wave runs the same rounds (the
     // MFMAs need all 64 lanes); lanes past V compute a copy of variable V - 1
@@ -661,25 +717,25 @@ __device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int
         const int2 vs = var_slice(a, vc);
         float mn = INFINITY;
         int uns = 0;
-        // the slice's columns in rounds of four (loads, then code words, then
+        // the slice's columns in rounds of eight (loads, then code words, then
         // messages); slots past the slice width read the sentinel check, slots
         // past this variable's degree are not counted
         const uint2 pad = make_uint2(4u * (uint32_t)C, m.sbb << 5);
-        for (int k0 = 0; k0 < vs.y; k0 += 4) {
-            uint2 e[4];
+        for (int k0 = 0; k0 < vs.y; k0 += 8) {
+            uint2 e[8];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < 8; ++u)
                 e[u] = k0 + u < vs.y ? a.g.var_ent[vs.x + (k0 + u) * kSlice + (vc & 63)] : pad;
-            float msg[4];
-            int un[4];
+            float msg[8];
+            int un[8];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < 8; ++u) {
                 const uint32_t w = m.sb_ent(e[u].y);
                 msg[u] = m.cs_msg(e[u].x + (__builtin_amdgcn_ubfe(w, e[u].y, 2u) << m.cs_shift));
-                un[u] = *lds_at<uint8_t>(a.unsat_lds + (e[u].x >> 2));
+                un[u] = known_valid ? 0 : (int)*lds_at<uint8_t>(a.unsat_lds + (e[u].x >> 2));
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < 8; ++u) {
                 if (k0 + u < deg) {
                     mn = fminf(mn, msg[u]);
                     uns += un[u];
@@ -687,13 +743,14 @@ __device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int
             }
         }
         float f[8];
+        const float inv_deg = deg > 0 ? 1.f / (float)deg : 0.f;
         f[0] = l0 * 0.125f;
         f[1] = clamp4(post * 0.0625f);
         f[2] = (float)*lds_at<uint8_t>(a.flips_lds + (uint32_t)vc) * 0.125f;
-        f[3] = deg > 0 ? clamp4(ls.y / (float)deg * 0.125f) : 0.f;
+        f[3] = clamp4(ls.y * inv_deg * 0.125f);
         f[4] = deg > 0 ? clamp4(mn * 0.125f) : 0.f;
         f[5] = (float)deg * 0.125f;
-        f[6] = deg > 0 ? (float)uns / (float)deg : 0.f;
+        f[6] = (float)uns * inv_deg;
         f[7] = (float)r * 0.25f;
         if (valid) {
             const size_t row = ((size_t)b * L + r) * V + a.g.vperm[v];
@@ -703,7 +760,11 @@ __device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int
             }
             if (a.hard_out) a.hard_out[row] = post <= 0.f ? 1 : 0;
         }
-        if (a.mlp) {
+        if (a.mlp && apply) {
+            // the operands are reloaded every round (coalesced, L1-resident) rather
+            // than held across the gathers above
+            MlpRegs mr;
+            load_mlp(a.mlp_frag, mr);
             const uint32_t p01 = pack_bf16x2(f[0], f[1]), p23 = pack_bf16x2(f[2], f[3]);
             const uint32_t p45 = pack_bf16x2(f[4], f[5]), p67 = pack_bf16x2(f[6], f[7]);
             float z = 0.f;
@@ -714,29 +775,25 @@ __device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int
                 const uint32_t b01 = (uint32_t)__shfl((int)p45, src, 64), b23 = (uint32_t)__shfl((int)p67, src, 64);
                 const uint32_t x01 = q == 0 ? a01 : (q == 1 ? b01 : 0u);
                 const uint32_t x23 = q == 0 ? a23 : (q == 1 ? b23 : 0u);
-                bf16x4_t x;
-                x[0] = (short)(x01 & 0xFFFFu);
-                x[1] = (short)(x01 >> 16);
-                x[2] = (short)(x23 & 0xFFFFu);
-                x[3] = (short)(x23 >> 16);
-                const float zg = mlp_logit(mr, x);
+                const float zg = mlp_logit(mr, unpack_bf16x4(x01, x23));
                 z = q == grp ? zg : z;
             }
-            // shift of the leg's own draw: (hi - lo) (sigmoid(z) - 1/2); zero at z = 0
-            const float d = (a.learned_hi - a.learned_lo) * (1.f / (1.f + expf(-z)) - 0.5f);
-            if (valid) {
-                *lds_at<uint16_t>(a.gamma_lds + 2u * (uint32_t)v) = f32_to_bf16(d);
-                if (a.gamma_out) {
-                    gs.lds = a.gamma_lds;
-                    gs.lo = a.learned_lo;
-                    gs.hi = a.learned_hi;
-                    a.gamma_out[((size_t)b * L + r) * V + a.g.vperm[v]] = gs(v);
-                }
-            }
+            // shift of the leg's own draw in steps of (hi - lo) / 254: 254 (sigmoid(z) - 1/2)
+            // rounded to the nearest integer (ties to even), in [-127, 127]
+            const float steps = __builtin_rintf(254.f * (1.f / (1.f + expf(-z)) - 0.5f));
+            if (valid) *lds_at<int8_t>(a.gamma_lds + (uint32_t)v) = (int8_t)(int)steps;
         }
         if (valid) *lds_at<uint8_t>(a.flips_lds + (uint32_t)v) = 0;
     }
     __syncthreads();
+}
+
+// The strengths a learned leg uses (shifted or its uniform row), in the
+// caller's order, for gamma_out.
+template <int BLOCK>
+__device__ void record_strengths(const KArgs &a, int b, int r, const GammaSrc &gs) {
+    const int V = a.g.V, L = a.learned_legs;
+    for (int v = threadIdx.x; v < V; v += BLOCK) a.gamma_out[((size_t)b * L + r) * V + a.g.vperm[v]] = gs(v);
 }
 
 // check_and_select_kernel (kernels.py:657-741) for a shot whose hard decision
@@ -767,11 +824,13 @@ __device__ void select_solution(const KArgs &a, const ShotMem<PLACE> &m, const f
 }
 
 // One leg of up to T iterations.  The shot leaves the leg at its first valid
-// solution (freeze_finished_lanes_kernel with sought = found + 1).
-template <int BLOCK, int PLACE, bool LEARN = false>
+// solution (freeze_finished_lanes_kernel with sought = found + 1).  FLIPS:
+// count each variable's hard-decision flips (a feature of the learned
+// strengths; only legs whose end state feeds the network count them).
+template <int BLOCK, int PLACE, bool FLIPS = false>
 __device__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m, int T, const GammaSrc &gs,
                         const float *__restrict__ wj, const uint8_t *__restrict__ syn, uint8_t *err,
-                        ShotState &st, double *s_red, int *s_flag, bool flips_on = false) {
+                        ShotState &st, double *s_red, int *s_flag) {
     const int C = a.g.C, V = a.g.V;
     for (int t = 0; t < T; ++t) {
         const float alpha = a.use_alpha ? a.alpha_t[t] : 1.0f;
@@ -786,7 +845,7 @@ __device__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m, int T, const Ga
             select_solution<BLOCK, PLACE>(a, m, wj, err, st, s_red);
             return true;
         }
-        for (int v = threadIdx.x; v < V; v += BLOCK) var_update<PLACE, LEARN>(a, m, v, gs, flips_on);
+        for (int v = threadIdx.x; v < V; v += BLOCK) var_update<PLACE, FLIPS>(a, m, v, gs);
         __syncthreads();
         st.iters += 1;
         st.any_iter = 1;
@@ -849,15 +908,19 @@ __device__ bool first_leg(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &s
     m.l0g = io.l0;
     __syncthreads();
 
-    st = ShotState{INFINITY, 0, 0, 0};
+    st = ShotState{INFINITY, 0, 0, 0, 0, 0};
     GammaSrc g0{nullptr, nullptr, a.plain ? 0.f : a.gamma0};
     if (!a.plain && a.init_gamma) {
         g0.row = a.init_gamma + (size_t)b * V;
         g0.perm = a.g.vperm;
     }
-    if (st.found < a.stop_nconv)
-        run_leg<BLOCK, PLACE, LEARN>(a, m, a.pre_iter, g0, io.wj, io.syn, io.err, st, s_red, s_flag,
-                                     LEARN && a.learned_legs > 0);
+    if (st.found < a.stop_nconv) {
+        // the network reads the flips of this leg
+        if (LEARN && a.learned_legs > 0)
+            st.last_ok = run_leg<BLOCK, PLACE, true>(a, m, a.pre_iter, g0, io.wj, io.syn, io.err, st, s_red, s_flag);
+        else
+            st.last_ok = run_leg<BLOCK, PLACE, false>(a, m, a.pre_iter, g0, io.wj, io.syn, io.err, st, s_red, s_flag);
+    }
     // first_leg_hard_dec (decoder.py:3597-3598): kept in err while no solution exists.
     if (!a.last_leg && st.found == 0) write_hard<BLOCK, PLACE>(a, m, io.err, st.any_iter);
     return !a.skip_relay_legs && a.num_sets > 0 && st.found < a.stop_nconv;
@@ -886,18 +949,38 @@ __device__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &
     if (run) {
         for (int leg = 0; leg < a.num_sets && st.found < a.stop_nconv; ++leg) {
             GammaSrc gl = leg_gamma(a, b, leg);
+            bool count = false;
             if (LEARN && leg < a.learned_legs) {
-                // a re-queued shot had leg 0's features and strengths computed before it
-                // was parked (the parked state is the posterior only)
-                if (!(leg == 0 && leg0_ready)) leg_features<BLOCK, PLACE>(a, b, m, leg, io.syn, gl);
-                if (a.mlp) {
+                // The network runs once per shot, on the state after the pre-iter leg
+                // (the start of relay leg 0), and only when that state is a valid
+                // solution; its per-variable shifts then apply to every learned leg
+                // that starts from a valid solution (the previous leg converged).  A
+                // leg that starts from a state violating some check runs on its
+                // uniform row, the exploration such a state needs.  Traces record the
+                // features at the start of every learned leg.  A re-queued shot (its
+                // first leg converged) had the network run before it was parked.
+                const bool trace = a.feat_out || a.hard_out;
+                if (leg == 0) {
+                    st.shifted = leg0_ready ? st.shifted : (a.mlp && st.last_ok);
+                    if (!leg0_ready && (st.shifted || trace))
+                        leg_features<BLOCK, PLACE>(a, b, m, 0, io.syn, st.last_ok != 0, st.shifted != 0);
+                } else if (trace) {
+                    leg_features<BLOCK, PLACE>(a, b, m, leg, io.syn, st.last_ok != 0, false);
+                }
+                if (st.shifted && st.last_ok) {
                     gl.lds = a.gamma_lds;
                     gl.lo = a.learned_lo;
                     gl.hi = a.learned_hi;
                 }
+                if (a.gamma_out) record_strengths<BLOCK>(a, b, leg, gl);
+                count = trace && leg + 1 < a.learned_legs;
             }
-            run_leg<BLOCK, PLACE, LEARN>(a, m, a.set_max_iter, gl, io.wj, io.syn, io.err, st, s_red, s_flag,
-                                         LEARN && leg + 1 < a.learned_legs);
+            if (count)
+                st.last_ok = run_leg<BLOCK, PLACE, true>(a, m, a.set_max_iter, gl, io.wj, io.syn, io.err, st, s_red,
+                                                         s_flag);
+            else
+                st.last_ok = run_leg<BLOCK, PLACE, false>(a, m, a.set_max_iter, gl, io.wj, io.syn, io.err, st, s_red,
+                                                          s_flag);
             if (leg == 0 && a.pre_iter == 0 && !a.last_leg && st.found == 0)
                 write_hard<BLOCK, PLACE>(a, m, io.err, st.any_iter);  // decoder.py:3809-3810
         }
@@ -929,7 +1012,7 @@ __device__ void requeue_shot(const KArgs &a, int b, const ShotMem<PLACE> &m, con
     for (int v = threadIdx.x; v < V; v += BLOCK) {
         const float2 ls = m.ls_pos(v);
         post[v] = ls.x + ls.y;
-        if (LEARN && a.mlp) a.rq_gamma[(size_t)b * V + v] = *lds_at<uint16_t>(a.gamma_lds + 2u * (uint32_t)v);
+        if (LEARN && a.mlp) a.rq_gamma[(size_t)b * V + v] = *lds_at<int8_t>(a.gamma_lds + (uint32_t)v);
     }
     __syncthreads();
     if (opaque_tid() == 0) {
@@ -952,7 +1035,7 @@ __device__ void resume_shot(const KArgs &a, int b, ShotMem<PLACE> &m) {
         if (PLACE & (1 << kL0)) m.l0_set(v, io.l0[a.g.vperm[v]]);
         if (LEARN) {
             *lds_at<uint8_t>(a.flips_lds + (uint32_t)v) = 0;  // leg 0's flips start from zero
-            if (a.mlp) *lds_at<uint16_t>(a.gamma_lds + 2u * (uint32_t)v) = a.rq_gamma[(size_t)b * V + v];
+            if (a.mlp) *lds_at<int8_t>(a.gamma_lds + (uint32_t)v) = a.rq_gamma[(size_t)b * V + v];
         }
     }
     m.l0g = io.l0;
@@ -1018,7 +1101,7 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
     // The leader writes the finished shot's scalars and takes the next ticket in
     // one branch: first-leg tickets in queue order, then re-queued shots.
     int prev = -1;
-    ShotState st{0.f, 0, 0, 0};
+    ShotState st{0.f, 0, 0, 0, 0, 0};
     for (;;) {
         if (opaque_tid() == 0) {
             const int64_t now = a.shot_clock ? (int64_t)wall_clock64() : 0;
@@ -1053,8 +1136,9 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
             more = first_leg<BLOCK, PLACE, LEARN>(a, b, m, st, s_red, s_flag);
             if (a.requeue_below > 0) {
                 if (more && st.found > 0 && st.iters < a.requeue_below) {
-                    if (LEARN && a.learned_legs > 0)
-                        leg_features<BLOCK, PLACE>(a, b, m, 0, a.syn + (size_t)b * a.g.C, leg_gamma(a, b, 0));
+                    // the first leg converged: relay leg 0 starts from a valid solution
+                    if (LEARN && a.learned_legs > 0 && (a.mlp || a.feat_out || a.hard_out))
+                        leg_features<BLOCK, PLACE>(a, b, m, 0, a.syn + (size_t)b * a.g.C, true, a.mlp != nullptr);
                     requeue_shot<BLOCK, PLACE, LEARN>(a, b, m, st);
                     prev = -1;
                     continue;
@@ -1065,7 +1149,9 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
         } else {
             resume_shot<BLOCK, PLACE, LEARN>(a, b, m);
             const int4 rs = uniform_i4(a.rq_state[b]);
-            st = ShotState{__int_as_float(rs.x), rs.y, rs.z, rs.w};
+            // parked shots are those whose first leg converged; the network ran before
+            // they were parked
+            st = ShotState{__int_as_float(rs.x), rs.y, rs.z, rs.w, 1, a.mlp != nullptr};
         }
         relay_legs<BLOCK, PLACE, LEARN>(a, b, m, st, more, s_red, s_flag, stage == 2);
         prev = b;
@@ -1215,9 +1301,10 @@ namespace {
 
 // Workspace layout: [0,256) queue counters | order [B] | key [B] | rq_slot [B]
 // | rq_state [B] | rq_post [B,V] (only when shots are re-queued) | rq_gamma
-// [B,V] bf16 (re-queue with learned strengths) | spill slots
+// [B,V] int8 (re-queue with learned strengths) | mlp_frag (learned plans) |
+// spill slots
 struct WsLayout {
-    int64_t order, key, rq_slot, rq_state, rq_post, rq_gamma, spill, total_fixed;
+    int64_t order, key, rq_slot, rq_state, rq_post, rq_gamma, mlp_frag, spill, total_fixed;
 };
 
 WsLayout ws_layout(const pel_graph *g, int64_t B, bool requeue, bool learn) {
@@ -1229,7 +1316,8 @@ WsLayout ws_layout(const pel_graph *g, int64_t B, bool requeue, bool learn) {
     w.rq_state = align_up(w.rq_slot + 4 * nb, 256);
     w.rq_post = align_up(w.rq_state + 16 * nb, 256);
     w.rq_gamma = align_up(w.rq_post + (requeue ? 4 * nb * g->V : 0), 256);
-    w.spill = align_up(w.rq_gamma + (requeue && learn ? 2 * nb * g->V : 0), 256);
+    w.mlp_frag = align_up(w.rq_gamma + (requeue && learn ? nb * g->V : 0), 256);
+    w.spill = align_up(w.mlp_frag + (learn ? 4LL * kMlpStride * 64 : 0), 256);
     w.total_fixed = w.spill;
     return w;
 }
@@ -1265,13 +1353,13 @@ void place_arrays(const StateSizes &st, int place, int block, bool learn, uint32
     *scratch = (uint32_t)*lds;
     *lds += align_up(12LL * (block / 64) + 8, 16) + 16LL * st.nrs + align_up(8LL * st.nvs, 16);
     LearnLds l{0, 0, 0};
-    if (learn) {  // flip counts u8 [V+1], violated checks u8 [C+1], strengths bf16 [V]
+    if (learn) {  // flip counts u8 [V+1], violated checks u8 [C+1], strength shifts i8 [V]
         l.flips = (uint32_t)*lds;
         *lds += align_up(st.V + 1, 16);
         l.unsat = (uint32_t)*lds;
         *lds += align_up(st.C + 1, 16);
         l.gamma = (uint32_t)*lds;
-        *lds += align_up(2 * st.V, 16);
+        *lds += align_up(st.V, 16);
     }
     if (ll) *ll = l;
 }
@@ -1693,7 +1781,8 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     a.rq_slot = reinterpret_cast<int32_t *>(ws + w.rq_slot);
     a.rq_state = reinterpret_cast<int4 *>(ws + w.rq_state);
     a.rq_post = plan->requeue_below > 0 ? reinterpret_cast<float *>(ws + w.rq_post) : nullptr;
-    a.rq_gamma = plan->requeue_below > 0 && learn ? reinterpret_cast<uint16_t *>(ws + w.rq_gamma) : nullptr;
+    a.rq_gamma = plan->requeue_below > 0 && learn ? reinterpret_cast<int8_t *>(ws + w.rq_gamma) : nullptr;
+    a.mlp_frag = learn ? reinterpret_cast<const uint32_t *>(ws + w.mlp_frag) : nullptr;
     a.mlp = b->mlp;
     a.learned_legs = learn ? b->learned_legs : 0;
     a.feat_out = b->feat_out;
@@ -1717,6 +1806,11 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     PEL_HIP(hipGetLastError());
     hipLaunchKernelGGL(order_shots_kernel, dim3(1), dim3(kOrderBlock), 0, s, key, b->B, order, a.ctrl);
     PEL_HIP(hipGetLastError());
+    if (b->mlp) {  // the network as per-lane MFMA operands, once per decode
+        hipLaunchKernelGGL(mlp_frag_kernel, dim3(1), dim3(64), 0, s, b->mlp,
+                           reinterpret_cast<uint32_t *>(ws + w.mlp_frag));
+        PEL_HIP(hipGetLastError());
+    }
     if (b->ev_begin) PEL_HIP(hipEventRecord(static_cast<hipEvent_t>(b->ev_begin), s));
     KernelFn fn = pick_kernel(plan->block, plan->placement, learn);
     hipLaunchKernelGGL(fn, dim3(plan->grid), dim3(plan->block), (size_t)plan->lds_bytes, s, a);

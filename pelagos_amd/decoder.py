@@ -569,15 +569,22 @@ class RelayBPDecoder:
                      self.gamma_table(B), None, None, errors, weights, iterations, found_count, None,
                      self.pre_iter, False, B, learn, workspace=workspace, stream=stream)
 
-    def collect_traces(self, syndromes: torch.Tensor, legs: int) -> Dict[str, torch.Tensor]:
-        """Decode as ``decode`` does (uniform-draw legs, no learned strengths)
-        and record, at the start of relay legs 0..legs-1, the learned network's
-        features of every variable (``features`` [B, legs, V, 8]) and the hard
-        decision (``hard`` [B, legs, V]; 255 where the shot stopped earlier)."""
+    def collect_traces(self, syndromes: torch.Tensor, legs: int, with_network: bool = False) -> Dict[str, torch.Tensor]:
+        """Decode as ``decode`` does and record, at the start of relay legs
+        0..legs-1, the learned network's features of every variable
+        (``features`` [B, legs, V, 8]) and the hard decision (``hard``
+        [B, legs, V]; 255 where the shot stopped earlier).  The legs run on
+        their uniform draws unless ``with_network`` (then this decoder's
+        network shifts those of them that start from a valid solution)."""
         B = int(syndromes.shape[0])
         feats = torch.zeros(B, legs, self.V, 8, dtype=torch.float32, device=self.device)
         hard = torch.full((B, legs, self.V), 255, dtype=torch.uint8, device=self.device)
-        out = self.decode(syndromes, _records=dict(mlp=None, legs=int(legs), feat_out=feats, hard_out=hard))
+        mlp = None
+        if with_network:
+            if self.learned_params is None:
+                raise ValueError("collect_traces(with_network=True) needs learned_gamma")
+            mlp = self.learned_params
+        out = self.decode(syndromes, _records=dict(mlp=mlp, legs=int(legs), feat_out=feats, hard_out=hard))
         out["features"] = feats
         out["hard"] = hard
         return out

@@ -147,14 +147,15 @@ def leg_iterations(dec, syndromes: torch.Tensor, legs: int) -> np.ndarray:
     ``dec`` (0 for a leg the shot never started): the differences of the
     iteration counts of decodes stopped after r and r + 1 relay legs, whose
     legs are the full decode's own (each leg's draw depends on the leg only)."""
-    saved = dec.num_sets
+    saved, saved_l = dec.num_sets, dec.learned_legs
     tot = []
     try:
         for r in range(legs + 1):
             dec.num_sets = min(r, saved)
+            dec.learned_legs = min(saved_l, dec.num_sets)   # learned legs among the first r
             tot.append(dec.decode(syndromes)["iterations"].cpu().numpy().astype(np.int64))
     finally:
-        dec.num_sets = saved
+        dec.num_sets, dec.learned_legs = saved, saved_l
     return np.diff(np.stack(tot, 1), axis=1)
 
 
@@ -212,6 +213,80 @@ def training_set(dec, syndromes: np.ndarray, legs: int, max_samples: int, seed: 
     keep = torch.randperm(x.shape[0], generator=gen)[:max_samples].to(dev)
     return dict(x=x[keep].contiguous(), y=y[keep].contiguous(), w=w[keep].contiguous(), legs_started=n_legs,
                 legs_converged=n_ok, converged_from_violated=n_hard, candidates=int(x.shape[0]))
+
+
+def constant_params(sigma: float) -> np.ndarray:
+    """A network whose output is sigmoid(z) = sigma for every input (every
+    weight 0, b3 = logit(sigma)): it shifts every variable's draw by
+    (hi - lo) (sigma - 1/2)."""
+    p = np.zeros(N_PARAMS, np.float32)
+    p[-1] = np.float32(np.log(sigma / (1.0 - sigma)))
+    return p
+
+
+def rwr_training_set(make_decoder, syndromes: np.ndarray, errors: np.ndarray, legs: int,
+                     sigmas=(0.5, 2 / 3, 5 / 6, 0.99), tau: float = 1.0, max_samples: int = 2_000_000,
+                     seed: int = 1, batch: int = 512) -> Dict[str, torch.Tensor]:
+    """Reward-weighted regression over candidate shifts.  For each candidate
+    sigma, ``make_decoder(params, legs)`` gives a decoder whose network is the
+    constant ``sigma``: every shot whose pre-iter leg ended at a valid solution
+    has the draws of its learned relay legs 0..legs-1 shifted by
+    (hi - lo) (sigma - 1/2), as in every learned decode (the network runs once,
+    on the state after the pre-iter leg).  Such a shot contributes the features
+    of its variables at that state, with target sigma and weight
+    exp(-(m - 1) / tau), m the mean length of the learned legs it ran
+    (``leg_iterations``), times 1 if it was decoded to ``errors`` (the true
+    error pattern) and 0 otherwise.  The fitted sigmoid is then, for each
+    state, the reward-weighted mean of the candidates: the shift under which
+    legs from such states converge fastest without changing the decoding.  A
+    uniform sample of at most ``max_samples`` variables is kept."""
+    xs, ys, ws = [], [], []
+    stats = {"shots_shifted": 0, "shots_correct": {}, "mean_leg_iterations": {}}
+    for sg in sigmas:
+        dec = make_decoder(constant_params(sg), legs)
+        dev = dec.device
+        H = dec.H_csr.tocsr().astype(np.int32)
+        n_it, n_leg, n_ok, n_sh = 0, 0, 0, 0
+        for i in range(0, syndromes.shape[0], batch):
+            syn = syndromes[i:i + batch]
+            syn_dev = torch.from_numpy(np.ascontiguousarray(syn)).to(dev)
+            out = dec.collect_traces(syn_dev, legs, with_network=True)
+            hard = out["hard"].cpu().numpy()[:, 0]                         # [b, V] at relay leg 0
+            reached = hard[:, 0] != 255
+            shifted = np.zeros_like(reached)
+            shifted[reached] = _syndrome_ok(H, np.where(hard[:, None, :] == 255, 0, hard[:, None, :]),
+                                            syn)[reached, 0]
+            correct = (out["errors"].cpu().numpy() == errors[i:i + batch]).all(1)
+            its = leg_iterations(dec, syn_dev, legs)                       # [b, legs]
+            ran = np.maximum((its > 0).sum(1), 1)
+            mean_it = its.sum(1) / ran
+            w = np.where(shifted & correct, np.exp(-(mean_it - 1.0) / tau), 0.0).astype(np.float32)
+            idx = np.nonzero(shifted)[0]
+            n_sh += int(idx.size)
+            n_ok += int((shifted & correct).sum())
+            n_it += int(its[idx].sum())
+            n_leg += int(ran[idx].sum())
+            if idx.size == 0:
+                continue
+            f = out["features"][torch.from_numpy(idx).to(dev), 0]          # [k, V, 8]
+            wt = torch.from_numpy(w[idx]).to(dev)
+            xs.append(f.reshape(-1, N_FEATURES))
+            ys.append(torch.full((f.shape[0] * f.shape[1],), float(sg), dtype=torch.float32, device=dev))
+            ws.append(wt[:, None].expand(-1, f.shape[1]).reshape(-1))
+        key = round(float(sg), 4)
+        stats["mean_leg_iterations"][key] = round(n_it / max(n_leg, 1), 3)
+        stats["shots_correct"][key] = n_ok
+        stats["shots_shifted"] = n_sh
+        del dec
+    x, y, w = torch.cat(xs), torch.cat(ys), torch.cat(ws)
+    keep_mask = w > 0
+    x, y, w = x[keep_mask], y[keep_mask], w[keep_mask]
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    keep = torch.randperm(x.shape[0], generator=gen)[:max_samples].to(x.device)
+    w = w[keep]
+    w = w / w.mean()
+    stats["candidates"] = int(keep_mask.sum())
+    return dict(x=x[keep].contiguous(), y=y[keep].contiguous(), w=w.contiguous(), **stats)
 
 
 def default_weights_path() -> Path:

@@ -2,16 +2,18 @@
 compare the learned form with uniform-draw legs.
 
     python -m pelagos_amd.train_gamma [--out PATH] [--train-shots 512] [--legs 4]
-        [--samples 2000000] [--epochs 6] [--eval-shots 65536] [--learned-legs 1 2 4]
+        [--samples 2000000] [--epochs 6] [--eval-shots 65536] [--learned-legs 4]
+        [--method rwr|mean-draw]
 
-Training traces come from the plain decoder (uniform-draw legs) on shots whose
-errors are drawn at the workload's priors scaled to a mean of 0.003 (seed 1):
-features at the start of relay legs 0..legs-1 of every variable, target = the
-variable's draw in that leg, kept for legs that converged
-(pelagos_amd.learned.training_set).  The evaluation decodes fresh
-shots (seed 2) drawn at the workload's own priors with both forms and prints,
-per form, decoded shots/s (HIP-event kernel time and wall time per
-4096-shot batch), mean iterations, and the frame-error and
+Training shots have errors drawn at the workload's priors scaled to a mean of
+0.003 (seed 1).  The default method (rwr, pelagos_amd.learned.rwr_training_set)
+decodes them once per candidate shift with a constant network on the learned
+legs and regresses the network onto the candidate shifts, each leg weighted by
+how quickly it converged; features are those at the start of each learned leg
+that starts from a valid solution (the only legs a network shifts).  The
+evaluation decodes fresh shots (seed 2) drawn at the workload's own priors with
+both forms and prints, per form, decoded shots/s (HIP-event kernel time and
+wall time per 4096-shot batch), mean iterations, and the frame-error and
 synthetic-observable error rates with Wilson 95% intervals.  One JSON line per
 result.
 """
@@ -75,7 +77,7 @@ def main() -> int:
     ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--eval-shots", type=int, default=65536)
-    ap.add_argument("--learned-legs", type=int, nargs="+", default=[1, 2, 4])
+    ap.add_argument("--learned-legs", type=int, nargs="+", default=[4])
     ap.add_argument("--skip-train", action="store_true", help="evaluate the weights already at --out")
     ap.add_argument("--hard-weight", type=float, default=4.0,
                     help="weight of legs that started from a state violating some check")
@@ -86,7 +88,13 @@ def main() -> int:
     ap.add_argument("--fast-iters", type=int, default=None,
                     help="a leg that starts from a valid solution counts as successful only when it "
                          "converges within this many iterations (default: any)")
-    ap.add_argument("--save-legs", type=int, default=1, help="learned legs recorded with the weights")
+    ap.add_argument("--save-legs", type=int, default=4, help="learned legs recorded with the weights")
+    ap.add_argument("--method", choices=("rwr", "mean-draw"), default="rwr",
+                    help="rwr: reward-weighted regression over constant candidate shifts; mean-draw: the "
+                         "mean uniform draw of successful plain legs")
+    ap.add_argument("--sigmas", type=float, nargs="+", default=[0.5, 2 / 3, 5 / 6, 0.99],
+                    help="candidate network outputs (shift (hi - lo) (sigma - 1/2)) for rwr")
+    ap.add_argument("--tau", type=float, default=1.0, help="rwr weight exp(-(leg iterations - 1) / tau)")
     ap.add_argument("--const-gamma", type=float, nargs="*", default=[],
                     help="also evaluate legs with this constant strength for every variable")
     args = ap.parse_args()
@@ -98,13 +106,24 @@ def main() -> int:
         scale = args.prior_mean / float(p.mean())
         err = codes.sample_errors(p, args.train_shots, seed=1, scale=scale)
         syn = codes.syndromes_of(H, err)
-        dec = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=1, device=[0], **RELAY)
         t0 = time.perf_counter()
-        ts = learned.training_set(dec, syn, args.legs, args.samples, seed=1, hard_weight=args.hard_weight,
-                                  easy_weight=args.easy_weight, fast_iters=args.fast_iters)
-        log(f"[train] traces: {ts['x'].shape[0]} samples of {ts['candidates']} from {ts['legs_converged']} "
-            f"converged legs of {ts['legs_started']} ({ts['converged_from_violated']} from a violated state) "
-            f"in {time.perf_counter() - t0:.1f} s")
+        if args.method == "rwr":
+            def make_decoder(params, legs):
+                return RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=1, device=[0],
+                                      learned_gamma=params, learned_legs=legs, **RELAY)
+
+            ts = learned.rwr_training_set(make_decoder, syn, err, args.legs, sigmas=args.sigmas, tau=args.tau,
+                                          max_samples=args.samples, seed=1)
+            info = {k: ts[k] for k in ("shots_shifted", "shots_correct", "mean_leg_iterations", "candidates")}
+            info.update(method="rwr", sigmas=args.sigmas, tau=args.tau)
+        else:
+            dec = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=1, device=[0], **RELAY)
+            ts = learned.training_set(dec, syn, args.legs, args.samples, seed=1, hard_weight=args.hard_weight,
+                                      easy_weight=args.easy_weight, fast_iters=args.fast_iters)
+            info = {k: ts[k] for k in ("legs_started", "legs_converged", "converged_from_violated", "candidates")}
+            info.update(method="mean-draw", hard_weight=args.hard_weight, easy_weight=args.easy_weight,
+                        fast_iters=args.fast_iters)
+        log(f"[train] {ts['x'].shape[0]} samples in {time.perf_counter() - t0:.1f} s: {info}")
         net = learned.GammaMLP(seed=1)
         t0 = time.perf_counter()
         losses = net.fit(ts["x"], ts["y"], ts["w"], epochs=args.epochs, lr=args.lr, log=log)
@@ -112,13 +131,9 @@ def main() -> int:
         train_s = time.perf_counter() - t0
         out_path.parent.mkdir(parents=True, exist_ok=True)
         net.save(out_path, learned_legs=args.save_legs)
-        print(json.dumps({"train": {"samples": int(ts["x"].shape[0]), "candidates": ts["candidates"],
-                                    "legs_started": ts["legs_started"], "legs_converged": ts["legs_converged"],
-                                    "converged_from_violated": ts["converged_from_violated"], "epochs": args.epochs,
+        print(json.dumps({"train": {"samples": int(ts["x"].shape[0]), **info, "epochs": args.epochs,
                                     "losses": [round(x, 5) for x in losses], "seconds": round(train_s, 2),
-                                    "prior_scale": round(scale, 4), "hard_weight": args.hard_weight,
-                                    "easy_weight": args.easy_weight, "fast_iters": args.fast_iters,
-                                    "weights": str(out_path)}}), flush=True)
+                                    "prior_scale": round(scale, 4), "weights": str(out_path)}}), flush=True)
     net = learned.GammaMLP.load(out_path)
 
     err = codes.sample_errors(p, args.eval_shots, seed=2)
