@@ -113,38 +113,61 @@ def _pmc_rows(folder: str):
 
 def traffic_record(fetch_dir: str, write_dir: str, cfetch_dir: str, cwrite_dir: str) -> int:
     """HBM bytes per decode launch from FETCH_SIZE / WRITE_SIZE (KiB per
-    dispatch), each scaled by what the same counter reads for a device copy of
-    known size (the copy reads and writes copy_probe_bytes)."""
-    def kib(rows, kernel_sub, counter):
-        vals = [float(r["Counter_Value"]) for r in rows
-                if kernel_sub in r["Kernel_Name"] and r["Counter_Name"] == counter]
-        return vals
+    dispatch, averaged over the run's launches of each kernel variant: plain
+    and learned), each scaled by what the same counter reads for a device copy
+    of known size (the copy reads and writes copy_probe_bytes)."""
+    def kib(rows, pred, counter):
+        return [float(r["Counter_Value"]) for r in rows if pred(r["Kernel_Name"]) and r["Counter_Name"] == counter]
 
-    dec_f = kib(_pmc_rows(fetch_dir), "relay_decode_kernel", "FETCH_SIZE")
-    dec_w = kib(_pmc_rows(write_dir), "relay_decode_kernel", "WRITE_SIZE")
-    cp_f = kib(_pmc_rows(cfetch_dir), "copyBuffer", "FETCH_SIZE")
-    cp_w = kib(_pmc_rows(cwrite_dir), "copyBuffer", "WRITE_SIZE")
     copy_bytes = 1024 << 20
+    cp_f = kib(_pmc_rows(cfetch_dir), lambda k: "copyBuffer" in k, "FETCH_SIZE")
+    cp_w = kib(_pmc_rows(cwrite_dir), lambda k: "copyBuffer" in k, "WRITE_SIZE")
     f_scale = copy_bytes / (sum(cp_f) / len(cp_f) * 1024)
     w_scale = copy_bytes / (sum(cp_w) / len(cp_w) * 1024)
-    fetch = sum(dec_f) / len(dec_f) * 1024 * f_scale
-    write = sum(dec_w) / len(dec_w) * 1024 * w_scale
-    rec = {
-        "kernel": "relay_decode_kernel",
-        "workload": "banded 936x8785, 4096 shots, one launch",
-        "fetch_kib_raw": sum(dec_f) / len(dec_f),
-        "write_kib_raw": sum(dec_w) / len(dec_w),
-        "copy_probe_bytes": copy_bytes,
-        "copy_fetch_kib_raw": sum(cp_f) / len(cp_f),
-        "copy_write_kib_raw": sum(cp_w) / len(cp_w),
-        "fetch_scale": round(f_scale, 4),
-        "write_scale": round(w_scale, 4),
-        "hbm_bytes_per_launch": int(fetch + write),
-    }
+    rec = {"kernel": "relay_decode_kernel", "workload": "banded 936x8785, 4096 shots, one launch",
+           "copy_probe_bytes": copy_bytes, "copy_fetch_kib_raw": sum(cp_f) / len(cp_f),
+           "copy_write_kib_raw": sum(cp_w) / len(cp_w), "fetch_scale": round(f_scale, 4),
+           "write_scale": round(w_scale, 4)}
+    frows, wrows = _pmc_rows(fetch_dir), _pmc_rows(write_dir)
+    for form, flag in (("plain", "false>"), ("learned", "true>")):
+        pred = (lambda k, flag=flag: "relay_decode_kernel" in k and flag in k)
+        f, w = kib(frows, pred, "FETCH_SIZE"), kib(wrows, pred, "WRITE_SIZE")
+        if not f or not w:
+            continue
+        rec[form] = {"launches": len(f), "fetch_kib_raw": sum(f) / len(f), "write_kib_raw": sum(w) / len(w),
+                     "hbm_bytes_per_launch": int(sum(f) / len(f) * 1024 * f_scale + sum(w) / len(w) * 1024 * w_scale)}
     out = ROOT / "profiles" / "relay_decode_banded936x8785_traffic.json"
     out.write_text(json.dumps(rec, indent=1) + "\n")
     print(json.dumps(rec))
     return 0
+
+
+def roofline(iters_per_launch: int, b_iter: int, kern_ms, elapsed: float, steps: int, seq_ms: float, depth: int,
+             traffic, kernel: str) -> dict:
+    """The decode kernel's roofline object: algorithmic bytes per launch
+    (B_iter x shot-iterations) over the average launch duration (HIP events
+    on each launch's stream) in the timed steps, beside the aggregate rate
+    over the timed region and the rate of launches one at a time."""
+    kern_avg = sum(kern_ms) / max(len(kern_ms), 1)
+    bytes_launch = iters_per_launch * b_iter
+    achieved = bytes_launch / (kern_avg * 1e-3) / 1e9
+    aggregate = bytes_launch * steps / elapsed / 1e9
+    isolated = bytes_launch / (seq_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
+            "kernel_ms": round(kern_avg, 4), "launches_in_flight": depth, "bytes_per_iteration": b_iter,
+            "iterations_per_launch": iters_per_launch, "aggregate_achieved": round(aggregate, 1),
+            "aggregate_frac": round(aggregate / HBM_PEAK_GBS, 4), "isolated_kernel_ms": round(seq_ms, 4),
+            "isolated_achieved": round(isolated, 1), "isolated_frac": round(isolated / HBM_PEAK_GBS, 4)}
+
+
+def _traffic(form: str):
+    tfile = ROOT / "profiles" / "relay_decode_banded936x8785_traffic.json"
+    try:
+        rec = json.loads(tfile.read_text())
+        return rec.get(form, {}).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
 
 
 def measure(dec, syn, steps: int, warmup: int, depth: int) -> dict:
@@ -246,6 +269,10 @@ def learned_form(H, p, err, syn, syn_np, args, plain_rate, plain_ler) -> dict:
     logical = (~conv) | ((O @ (errors ^ err).T.astype(np.int64)) % 2).any(0)
     le = wilson(int(logical.sum()), errors.shape[0])
     rate = m["rate"]
+    seq_ms = sum(seq["kernel_ms"]) / len(seq["kernel_ms"])
+    roof = roofline(int(out["iterations"].long().sum()), bytes_per_iteration(dec.C, dec.V, dec.E), m["kernel_ms"],
+                    m["elapsed"], args.steps, seq_ms, args.depth, _traffic("learned"),
+                    "relay_decode_kernel<1024, 15, true> (learned strengths)")
     big = ler_eval(dec, H, p)
     return {"shots_per_s": round(rate, 1), "kernel_ms": round(sum(m["kernel_ms"]) / len(m["kernel_ms"]), 4),
             "ler_65536": big,
@@ -256,7 +283,7 @@ def learned_form(H, p, err, syn, syn_np, args, plain_rate, plain_ler) -> dict:
             "mean_iterations": round(float(out["iterations"].float().mean()), 3),
             "synthetic_observable_ler": [round(x, 5) for x in le],
             "matches_decode": same_outputs(m["slots"] + seq["slots"], ref),
-            "weights": str(wpath.relative_to(ROOT))}
+            "weights": str(wpath.relative_to(ROOT)), "roofline": roof}
 
 
 THREE_CODES = {
@@ -325,6 +352,7 @@ def batched_graph(scales=(0.001, 0.003, 0.01)) -> dict:
 def serving_runs(seconds: float, rates=(2_000, 10_000, 50_000, 100_000, 200_000)) -> dict:
     """The continuous-batching server on the three codes (prior scale 0.003,
     requests of 1-256 shots, code chosen uniformly, Poisson arrivals)."""
+    import numpy as np
     import torch
 
     from pelagos_amd import RelayBPDecoder, codes, learned
@@ -338,7 +366,9 @@ def serving_runs(seconds: float, rates=(2_000, 10_000, 50_000, 100_000, 200_000)
         kw = {}
         if k == "banded936" and wpath.exists():
             net = net or learned.GammaMLP.load(wpath)
-            kw = dict(learned_gamma=net, learned_legs=1)
+            with np.load(wpath, allow_pickle=False) as f:
+                legs = int(f["learned_legs"]) if "learned_legs" in f.files else 1
+            kw = dict(learned_gamma=net, learned_legs=legs)
         decs[k] = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0], **RELAY, **kw)
     pools = {}
     for k, (H, p) in cs.items():
@@ -430,9 +460,8 @@ def main() -> int:
     errors = out["errors"].cpu().numpy()
     total_iters = int(iters.sum())
     b_iter = bytes_per_iteration(C, V, E)
-    achieved_gbs = total_iters * b_iter / (kern_avg_ms * 1e-3) / 1e9
-    aggregate_gbs = total_iters * b_iter * args.steps / elapsed / 1e9
-    isolated_gbs = total_iters * b_iter / (seq_ms * 1e-3) / 1e9
+    roof_plain = roofline(total_iters, b_iter, kern_ms, elapsed, args.steps, seq_ms, args.depth, _traffic("plain"),
+                          "relay_decode_kernel<1024, 15, false> (uniform-draw legs)")
     shots_per_s = m["rate"]
 
     # Quality: frame errors and a seeded synthetic-observable logical error rate.
@@ -444,14 +473,6 @@ def main() -> int:
     logical = (~converged) | ((O.astype(np.int64) @ resid.T.astype(np.int64)) % 2).any(0)
     fe = wilson(int(frame_err.sum()), args.shots)
     le = wilson(int(logical.sum()), args.shots)
-
-    traffic = None
-    tfile = ROOT / "profiles" / "relay_decode_banded936x8785_traffic.json"
-    if tfile.exists():
-        try:
-            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
 
     result = {
         "metric": "decoded shots/s at matched LER (learned vs plain gamma) + p99 latency, 1 GPU",
@@ -476,24 +497,7 @@ def main() -> int:
             "gamma0": RELAY["gamma0"], "gamma_dist_interval": list(RELAY["gamma_dist_interval"]),
             "block": plan.block, "grid": plan.grid, "lds_bytes": plan.lds_bytes,
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved_gbs, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "kernel": "relay_decode_kernel",
-            "kernel_ms": round(kern_avg_ms, 4),
-            "launches_in_flight": args.depth,
-            "bytes_per_iteration": b_iter,
-            "iterations_per_launch": total_iters,
-            "aggregate_achieved": round(aggregate_gbs, 1),
-            "aggregate_frac": round(aggregate_gbs / HBM_PEAK_GBS, 4),
-            "isolated_kernel_ms": round(seq_ms, 4),
-            "isolated_achieved": round(isolated_gbs, 1),
-            "isolated_frac": round(isolated_gbs / HBM_PEAK_GBS, 4),
-        },
+        "roofline": roof_plain,
         "quality": {
             "mean_iterations": round(float(iters.mean()), 3),
             "converged_frac": round(float(converged.mean()), 5),
@@ -514,6 +518,9 @@ def main() -> int:
         if lr.get("ler_within_plain_ci") and lr.get("shots_per_s", 0) > shots_per_s:
             result["value"] = lr["shots_per_s"]
             result["value_form"] = "learned"
+            result["ms_per_step"] = round(args.shots / lr["shots_per_s"] * 1e3, 3)
+            result["roofline"] = lr.pop("roofline")
+            result["plain"]["roofline"] = roof_plain
     if not args.no_batched:
         result["batched"] = batched_graph()
     if not args.no_serving:

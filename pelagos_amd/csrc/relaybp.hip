@@ -693,9 +693,10 @@ __device__ __forceinline__ float clamp4(float x) { return fminf(fmaxf(x, -4.f), 
 // the network and shift the leg (else the leg keeps its uniform row).  The flip counts
 // are cleared for the next leg.  Optionally records features and hard
 // decisions in the caller's order.
-template <int BLOCK, int PLACE>
-__device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int r, const uint8_t *__restrict__ syn,
-                             bool known_valid, bool apply) {
+template <int BLOCK, int PLACE, bool KNOWN_VALID>
+__device__ void leg_features_t(const KArgs &a, int b, const ShotMem<PLACE> &m, int r, const uint8_t *__restrict__ syn,
+                               bool apply) {
+    constexpr bool known_valid = KNOWN_VALID;
     const int C = a.g.C, V = a.g.V, L = a.learned_legs;
     if (!known_valid) {
         for (int i = threadIdx.x; i <= C; i += BLOCK)
@@ -724,8 +725,12 @@ __device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int
         for (int k0 = 0; k0 < vs.y; k0 += 8) {
             uint2 e[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-                e[u] = k0 + u < vs.y ? a.g.var_ent[vs.x + (k0 + u) * kSlice + (vc & 63)] : pad;
+            for (int u = 0; u < 8; ++u) {
+                // load from a slot inside the slice and select the value (selecting
+                // between two addresses would make the load a flat one)
+                const uint2 ld = a.g.var_ent[vs.x + min(k0 + u, vs.y - 1) * kSlice + (vc & 63)];
+                e[u] = k0 + u < vs.y ? ld : pad;
+            }
             float msg[8];
             int un[8];
 #pragma unroll
@@ -786,6 +791,13 @@ __device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int
         if (valid) *lds_at<uint8_t>(a.flips_lds + (uint32_t)v) = 0;
     }
     __syncthreads();
+}
+
+template <int BLOCK, int PLACE>
+__device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int r, const uint8_t *__restrict__ syn,
+                             bool known_valid, bool apply) {
+    if (known_valid) leg_features_t<BLOCK, PLACE, true>(a, b, m, r, syn, apply);
+    else leg_features_t<BLOCK, PLACE, false>(a, b, m, r, syn, apply);
 }
 
 // The strengths a learned leg uses (shifted or its uniform row), in the

@@ -237,3 +237,38 @@ def test_gpu_leg_iterations_match_truncated_decoders():
         np.testing.assert_array_equal(first + its[:, :r].sum(1), dr.decode(s)["iterations"].cpu().numpy())
     assert (its >= 0).all() and (first + its.sum(1) <= full).all()
     assert dec.num_sets == 12
+
+
+@pytest.mark.gpu
+def test_gpu_trained_network_shortens_the_relay_legs():
+    """What the shipped network gains on the 936 x 8785 circuit-shaped
+    workload: the same shots take far fewer BP iterations than with the
+    uniform legs (the relay legs that start from a valid solution converge in
+    about two iterations instead of six), every claimed solution satisfies its
+    syndrome, and the decoding agrees with the truth as often as the plain
+    form's."""
+    from pelagos_amd import RelayBPDecoder
+
+    path = learned.default_weights_path()
+    with np.load(path, allow_pickle=False) as f:
+        legs = int(f["learned_legs"])
+    H, p = codes.banded_circuit_code(seed=0)
+    err = codes.sample_errors(p, 2048, seed=21)
+    syn = codes.syndromes_of(H, err)
+    S = torch.from_numpy(syn).cuda().contiguous()
+    relay = dict(pre_iter=80, num_sets=100, set_max_iter=60, gamma0=0.65, gamma_dist_interval=(-0.24, 0.66),
+                 stop_nconv=5, data_type="float32", algo="relay", seed=0, device=[0])
+    plain = RelayBPDecoder(H, p, **relay).decode(S)
+    dec = RelayBPDecoder(H, p, learned_gamma=learned.GammaMLP.load(path), learned_legs=legs, **relay)
+    print("[gpu] trained network", flush=True)
+    out = dec.decode(S)
+    torch.cuda.synchronize()
+    it_p = plain["iterations"].float().mean().item()
+    it_l = out["iterations"].float().mean().item()
+    assert it_l < 0.7 * it_p, (it_l, it_p)
+    e = out["errors"].cpu().numpy()
+    ok = out["found_count"].cpu().numpy() > 0
+    assert (codes.syndromes_of(H, e)[ok] == syn[ok]).all()
+    wrong_l = int((e != err).any(1).sum())
+    wrong_p = int((plain["errors"].cpu().numpy() != err).any(1).sum())
+    assert wrong_l <= wrong_p + 1, (wrong_l, wrong_p)
