@@ -35,8 +35,10 @@ extern "C" {
 typedef struct pel_graph pel_graph;
 
 /* Learned memory-strength proposal: a per-variable MLP 8 -> 32 -> 32 -> 1
- * (relu, relu, then gamma = -0.24 + 0.9 sigmoid(z)).  Parameters are one fp32
- * vector: W1[32][8], b1[32], W2[32][32], b2[32], w3[32], b3. */
+ * (relu, relu, then a logit z).  Each variable's output is a shift
+ * (hi - lo) (sigmoid(z) - 1/2) of a learned leg's uniform draw u:
+ * gamma = clamp(u + shift, lo, hi).  Parameters are one fp32 vector:
+ * W1[32][8], b1[32], W2[32][32], b2[32], w3[32], b3. */
 #define PEL_MLP_FEATURES 8
 #define PEL_MLP_HIDDEN 32
 #define PEL_MLP_PARAMS (32 * 8 + 32 + 32 * 32 + 32 + 32 + 1)
@@ -83,15 +85,18 @@ typedef struct {
     void *ev_end;                /* decode kernel and right after it (kernel-only timing) */
     /* Learned memory strengths (plans made with PEL_PLAN_LEARNED; no counterpart in the reference,
      * whose plug points for them are explicit_gammas / init_gamma, decoder.py:480-491, 2322). */
-    const float *mlp;            /* network parameters [PEL_MLP_PARAMS], or NULL: relay legs
-                                    0 .. learned_legs-1 then shift their gamma_legs rows per
-                                    variable by the network's output (see learned_lo) */
+    const float *mlp;            /* network parameters [PEL_MLP_PARAMS], or NULL.  The network
+                                    runs once per shot, on the state after the pre-iter leg when
+                                    that state satisfies every check, and its per-variable shifts
+                                    apply to those of relay legs 0 .. learned_legs-1 that start
+                                    from a valid solution (the others keep their uniform rows) */
     int32_t learned_legs;
     float *feat_out;             /* out [B][learned_legs][V][8] or NULL: the network's features at
                                     the start of each of those legs (also without parameters) */
     uint8_t *hard_out;           /* out [B][learned_legs][V] or NULL: hard decision there; rows of
                                     legs a shot never reached are left untouched */
-    float *gamma_out;            /* out [B][learned_legs][V] or NULL: the strengths used */
+    float *gamma_out;            /* out [B][learned_legs][V] or NULL: the strengths each of those
+                                    legs used (shifted or its uniform row) */
     float learned_lo, learned_hi; /* strength interval: a learned leg uses clamp(u + (hi - lo)
                                     (sigmoid(z) - 1/2), lo, hi) with u that leg's gamma_legs row */
 } pel_batch;
@@ -163,9 +168,9 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
                void *stream);
 
 /* ---- learned memory strengths (no counterpart in the reference) ----------
- * Training entry points for the MLP above.  x [n][8] features, y [n] targets
- * in [0,1] (1: the variable's hard decision is correct), w [n] sample
- * weights.  All arrays are device memory; work is ordered on `stream`. */
+ * Training entry points for the MLP above.  x [n][8] features, y [n] soft
+ * targets for sigmoid(z) in [0,1], w [n] sample weights.  All arrays are
+ * device memory; work is ordered on `stream`. */
 
 /* PEL_MLP_PARAMS. */
 int pel_mlp_param_count(void);
