@@ -139,6 +139,8 @@ struct KArgs {
     uint32_t off[kNumArrays];  // byte offsets of the state arrays (LDS or spill slot)
     uint32_t scratch_off;      // LDS byte offset of the workgroup's reduction scratch
     uint32_t rtab, vtab;       // LDS byte addresses of the row and variable slice descriptor tables
+    uint32_t rowinfo;          // LDS byte address of the shot's row flags (u8 by row position: bit 0 the
+                               // syndrome, bit 1 degree > 0, bit 2 degree == 1)
     // learned memory strengths (plans made with PEL_PLAN_LEARNED)
     const float *mlp;          // network parameters, or null (then features are only recorded)
     int32_t learned_legs;      // relay legs 0 .. learned_legs-1 get per-variable strengths
@@ -341,6 +343,19 @@ __device__ __forceinline__ int2 var_slice(const KArgs &a, int v) {
     return make_int2(uniform_i32(__float_as_int(r.x)), uniform_i32(__float_as_int(r.y)));
 }
 
+// The shot's row flags, read by every check pass from LDS: written once per
+// shot (syndrome bit, degree > 0, degree == 1 by row position) so that a check
+// pass does not start with dependent global loads of the syndrome and the row
+// degree, whose latency every wave would wait out.
+template <int BLOCK>
+__device__ void row_info_init(const KArgs &a, const uint8_t *__restrict__ syn) {
+    for (int pos = threadIdx.x; pos < a.g.C; pos += BLOCK) {
+        const int deg = a.g.row_deg[pos];
+        *lds_at<uint8_t>(a.rowinfo + (uint32_t)pos) =
+            (uint8_t)((syn[a.g.row_orig[pos]] & 1u) | (deg > 0 ? 2u : 0u) | (deg == 1 ? 4u : 0u));
+    }
+}
+
 // Check pass for row position pos (kernels.py:1128-1215).  FIRST: messages were
 // just reset (mu_e = lam0_v, kernels.py:981-1000), so nothing is reconstructed
 // and no parity is tested.  Otherwise reconstructs mu_e = lam_mixed + (S - nu_e)
@@ -363,8 +378,8 @@ __device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m
                                          const uint8_t *__restrict__ syn) {
     const int4 rs = row_slice(a, pos);  // {ent, word, width, 0}
     const int lane = pos & 63;
-    const int deg = a.g.row_deg[pos];
-    const uint32_t s_i = syn[a.g.row_orig[pos]] & 1u;
+    const uint32_t info = *lds_at<uint8_t>(a.rowinfo + (uint32_t)pos);
+    const uint32_t s_i = info & 1u;
     const uint32_t wbase = (uint32_t)(rs.y + lane);
     const uint32_t lane_off = 4u * (uint32_t)lane;
     const uint32_t qrow = 4u * (uint32_t)pos;  // this row's messages CS[0..3][pos]
@@ -427,12 +442,12 @@ __device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m
             neg ^= __popc(wnew);
         }
     }
-    if (deg > 0) {
+    if (info & 2u) {
         // Mark the edge holding the minimum (after the sign words above).
         const uint32_t aw = wbase + (argk >> 4) * kSlice;
         m.sb_set(aw, m.sb_get(aw) | (2u << (2u * (argk & 15u))));
         float o1 = m1, o2 = m2;
-        if (deg == 1) o1 = o2 = 0.f;  // degree-1 checks send zero (kernels.py:1192-1193)
+        if (info & 4u) o1 = o2 = 0.f;  // degree-1 checks send zero (kernels.py:1192-1193)
         if (a.use_alpha) {
             o1 = alpha * o1;
             o2 = alpha * o2;
@@ -456,7 +471,7 @@ template <int PLACE>
 __device__ __forceinline__ int parity_row(const KArgs &a, const ShotMem<PLACE> &m, int pos,
                                           const uint8_t *__restrict__ syn) {
     const int4 rs = row_slice(a, pos);
-    bool hpar = (syn[a.g.row_orig[pos]] & 1u) != 0u;
+    bool hpar = (*lds_at<uint8_t>(a.rowinfo + (uint32_t)pos) & 1u) != 0u;
     const uint32_t lane_off = 4u * (uint32_t)(pos & 63);
     for (int k0 = 0; k0 < rs.z; k0 += kChunk) {
         const uint32_t *__restrict__ chunk = a.g.row_ent + rs.x + (int64_t)k0 * kSlice;
@@ -917,6 +932,7 @@ __device__ bool first_leg(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &s
         if (LEARN) *lds_at<uint8_t>(a.flips_lds + (uint32_t)v) = 0;
         io.err[j] = 0;
     }
+    row_info_init<BLOCK>(a, io.syn);
     m.l0g = io.l0;
     __syncthreads();
 
@@ -1050,6 +1066,7 @@ __device__ void resume_shot(const KArgs &a, int b, ShotMem<PLACE> &m) {
             if (a.mlp) *lds_at<int8_t>(a.gamma_lds + (uint32_t)v) = a.rq_gamma[(size_t)b * V + v];
         }
     }
+    row_info_init<BLOCK>(a, io.syn);
     m.l0g = io.l0;
     __syncthreads();
 }
@@ -1363,7 +1380,7 @@ void place_arrays(const StateSizes &st, int place, int block, bool learn, uint32
     // scratch: s_red doubles, s_flag ints, s_next[2]; then the slice descriptor
     // tables: rows {ent, word, width, 0}, variables {ent, width}
     *scratch = (uint32_t)*lds;
-    *lds += align_up(12LL * (block / 64) + 8, 16) + 16LL * st.nrs + align_up(8LL * st.nvs, 16);
+    *lds += align_up(12LL * (block / 64) + 8, 16) + 16LL * st.nrs + align_up(8LL * st.nvs, 16) + align_up(st.C, 16);
     LearnLds l{0, 0, 0};
     if (learn) {  // flip counts u8 [V+1], violated checks u8 [C+1], strength shifts i8 [V]
         l.flips = (uint32_t)*lds;
@@ -1811,6 +1828,7 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     a.scratch_off = scratch;
     a.rtab = row_table_addr(scratch, plan->block);
     a.vtab = a.rtab + (uint32_t)(16LL * st.nrs);
+    a.rowinfo = a.vtab + (uint32_t)align_up(8LL * st.nvs, 16);
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int key_blocks = (int)((b->B + kOrderBlock / 64 - 1) / (kOrderBlock / 64));
     hipLaunchKernelGGL(syndrome_key_kernel, dim3(key_blocks), dim3(kOrderBlock), 0, s, b->syndromes, b->B, g->C,
