@@ -349,16 +349,18 @@ def batched_graph(scales=(0.001, 0.003, 0.01)) -> dict:
     return {"codes": {k: {"shots": v[0], "code": v[1]} for k, v in THREE_CODES.items()}, "launches": res}
 
 
-def serving_runs(seconds: float, rates=(2_000, 10_000, 50_000, 100_000, 200_000)) -> dict:
-    """The continuous-batching server on the three codes (prior scale 0.003,
-    requests of 1-256 shots, code chosen uniformly, Poisson arrivals)."""
+def serving_runs(seconds: float, prior_scale: float = 0.003,
+                 rates=(2_000, 10_000, 50_000, 100_000, 200_000)) -> dict:
+    """The continuous-batching server on the three codes (priors scaled to mean
+    prior_scale, requests of 1-256 shots, code chosen uniformly, Poisson
+    arrivals)."""
     import numpy as np
     import torch
 
     from pelagos_amd import RelayBPDecoder, codes, learned
     from pelagos_amd.serving import DecodeServer, poisson_trace, run_trace
 
-    cs = _three_codes(0.003)
+    cs = _three_codes(prior_scale)
     net = None
     wpath = learned.default_weights_path()
     decs = {}
@@ -381,9 +383,9 @@ def serving_runs(seconds: float, rates=(2_000, 10_000, 50_000, 100_000, 200_000)
         st = run_trace(server, trace, pools)
         st["rate"] = rate
         out.append(st)
-        print(f"[bench] serving {rate}/s: p50 {st['p50_ms']} ms p99 {st['p99_ms']} ms "
+        print(f"[bench] serving p={prior_scale} {rate}/s: p50 {st['p50_ms']} ms p99 {st['p99_ms']} ms "
               f"{st['shots_per_s']} shots/s", file=sys.stderr, flush=True)
-    return {"codes": list(cs), "seconds_per_rate": seconds, "request_shots": "1-256 uniform",
+    return {"codes": list(cs), "prior_scale": prior_scale, "seconds_per_rate": seconds, "request_shots": "1-256 uniform",
             "learned_form_on": "banded936" if net is not None else None, "hbm_budget_gb": 64, "slots": 4,
             "rates": out}
 
@@ -399,7 +401,7 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-strength form")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-latency runs")
     ap.add_argument("--no-batched", action="store_true", help="skip the three-code graph launch")
-    ap.add_argument("--serving-seconds", type=float, default=1.5,
+    ap.add_argument("--serving-seconds", type=float, default=10.0,
                     help="length of each serving run (one per arrival rate)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -524,8 +526,9 @@ def main() -> int:
     if not args.no_batched:
         result["batched"] = batched_graph()
     if not args.no_serving:
-        result["serving"] = serving_runs(args.serving_seconds)
-        result["p99_latency_ms"] = {str(r["rate"]): r["p99_ms"] for r in result["serving"]["rates"]}
+        result["serving"] = {f"prior_{sc}": serving_runs(args.serving_seconds, sc) for sc in (0.001, 0.003)}
+        result["p99_latency_ms"] = {f"prior_{sc}": {str(r["rate"]): r["p99_ms"] for r in sv["rates"]}
+                                    for sc, sv in zip((0.001, 0.003), result["serving"].values())}
     if not args.no_cpu:
         lam = utils.compute_log_prior_ratios(p)
         gtab = dec.gamma_table(args.shots).cpu().numpy()
