@@ -89,10 +89,18 @@ def cpu_baseline(dec, syn_np, lam, gtab, target_s: float = 10.0):
     t1 = time.perf_counter()
     oracle.decode(g.chk_ptr, g.edge_var, g.V, syn_np[:n1], lam, gtab, nthreads=1, **kw)
     dt1 = time.perf_counter() - t1
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": round(done / dt, 1),
         "unit": "shots/s",
         "cores": threads,
+        "nproc": os.cpu_count(),
+        "cpu_model": model,
         "kind": "port",
         "sample": f"all {B} shots x {reps} passes ({dt:.1f} s), C oracle, OpenMP one shot per thread; "
                   f"single thread: {n1 / dt1:.1f} shots/s on the first {n1} shots",
