@@ -1457,6 +1457,69 @@ void balance_row_slices(const int32_t *chk_ptr, int32_t C, std::vector<int32_t> 
     rorig.swap(out);
 }
 
+// Positions of the variables within each degree class.  At slot k of a row
+// slice, a check pass reads the 8-byte state of the k-th variable of each of
+// its 64 rows.  An LDS read of 8-byte words serves 16 lanes per pass, and two
+// lanes of a pass collide when their variables' positions are equal mod 16 (the
+// state array is 8-byte entries over 32 four-byte banks) unless they read the
+// same entry.  Variables of equal degree are interchangeable (same slice
+// width), so each in turn takes the residue mod 16 that the fewest variables
+// it is read together with already hold, among the residues its class still
+// has positions for; the class's positions that are equal to r mod 16 then go
+// to its variables of residue r in their previous order.
+void bank_spread_vars(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, int32_t V,
+                      const std::vector<int32_t> &rorig, const std::vector<int32_t> &vdeg0,
+                      std::vector<int32_t> &vperm) {
+    std::vector<std::vector<int32_t>> members;  // read group -> variables
+    std::vector<std::vector<int32_t>> vgroups(V);
+    const int32_t nrs = (C + kSlice - 1) / kSlice;
+    for (int32_t sl = 0; sl < nrs; ++sl) {
+        int32_t width = 0;
+        for (int32_t p = sl * kSlice; p < std::min(C, (sl + 1) * kSlice); ++p)
+            width = std::max(width, chk_ptr[rorig[p] + 1] - chk_ptr[rorig[p]]);
+        for (int32_t g = 0; g < kSlice / 16; ++g)
+            for (int32_t k = 0; k < width; ++k) {
+                std::vector<int32_t> mem;
+                for (int32_t l = 16 * g; l < 16 * (g + 1); ++l) {
+                    const int32_t p = sl * kSlice + l;
+                    if (p >= C) break;
+                    const int32_t row = rorig[p];
+                    if (k < chk_ptr[row + 1] - chk_ptr[row]) mem.push_back(chk_var[chk_ptr[row] + k]);
+                }
+                if (mem.size() < 2) continue;
+                const int32_t id = (int32_t)members.size();
+                for (int32_t v : mem) vgroups[v].push_back(id);
+                members.push_back(std::move(mem));
+            }
+    }
+    std::vector<int8_t> res(V, -1);
+    for (int32_t a = 0; a < V;) {
+        int32_t b = a;
+        while (b < V && vdeg0[vperm[b]] == vdeg0[vperm[a]]) ++b;
+        int32_t cap[16] = {0};
+        for (int32_t p = a; p < b; ++p) cap[p & 15]++;
+        for (int32_t i = a; i < b; ++i) {
+            const int32_t v = vperm[i];
+            int32_t cnt[16] = {0};
+            for (int32_t g : vgroups[v])
+                for (int32_t u : members[g])
+                    if (u != v && res[u] >= 0) cnt[res[u]]++;
+            int32_t best = -1;
+            for (int32_t r = 0; r < 16; ++r)
+                if (cap[r] > 0 && (best < 0 || cnt[r] < cnt[best])) best = r;
+            res[v] = (int8_t)best;
+            cap[best]--;
+        }
+        std::vector<int32_t> next[16];
+        for (int32_t i = a; i < b; ++i) next[res[vperm[i]]].push_back(vperm[i]);
+        int32_t taken[16] = {0};
+        std::vector<int32_t> out(b - a);
+        for (int32_t p = a; p < b; ++p) out[p - a] = next[p & 15][(size_t)taken[p & 15]++];
+        std::copy(out.begin(), out.end(), vperm.begin() + a);
+        a = b;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1500,6 +1563,7 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
     std::vector<int32_t> vperm(V), vposv(V), vdeg(V);
     std::iota(vperm.begin(), vperm.end(), 0);
     std::stable_sort(vperm.begin(), vperm.end(), [&](int32_t x, int32_t y) { return vdeg0[x] > vdeg0[y]; });
+    bank_spread_vars(chk_ptr, chk_var, C, V, rorig, vdeg0, vperm);
     for (int32_t v = 0; v < V; ++v) {
         vposv[vperm[v]] = v;
         vdeg[v] = vdeg0[vperm[v]];
