@@ -121,16 +121,20 @@ def _decoder(H, p, **kw):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("requeue", [0, None])
-def test_gpu_learned_decode_matches_oracle_given_its_strengths(requeue):
+@pytest.mark.parametrize("requeue,placement", [(0, None), (None, None), (None, 7)])
+def test_gpu_learned_decode_matches_oracle_given_its_strengths(requeue, placement):
     """The learned decode equals the CPU oracle run with the strengths the
-    network gave (per shot, per learned leg) and the uniform rows after them."""
+    network gave (per shot, per learned leg) and the uniform rows after them,
+    with and without re-queueing, with the priors in LDS (placement 15, the
+    default here) and read from L2 (placement 7)."""
     H, p, err, syn = _banded(96, seed=11)
     L = 3
-    dec = _decoder(H, p, learned_gamma=learned.GammaMLP(seed=2), learned_legs=L)
-    dec.requeue_below = requeue
+    kw = {} if placement is None else {"state_placement": placement}
+    dec = _decoder(H, p, learned_gamma=learned.GammaMLP(seed=2), learned_legs=L, **kw)
+    if requeue is not None:
+        dec.requeue_below = requeue
     dec.record_learned_gammas = True
-    print(f"[gpu] learned decode requeue={requeue}", flush=True)
+    print(f"[gpu] learned decode requeue={requeue} placement={placement}", flush=True)
     out = dec.decode(torch.from_numpy(syn).cuda())
     torch.cuda.synchronize()
     gl = dec.learned_gammas.cpu().numpy()                      # [B, L, V]
