@@ -117,6 +117,8 @@ struct KArgs {
     const float *prior;
     const float *wj;
     int32_t prior_batched;
+    const float *wj_pos;     // wj and the priors in variable-position order (one copy for the
+    const float *prior_pos;  // batch), or null when they are per shot
     const float *gamma_legs;
     int32_t gamma_per_row;
     int32_t gamma_by_position;  // gamma_legs rows are in variable-position order
@@ -305,7 +307,7 @@ __device__ __forceinline__ T load_ell(const T *__restrict__ chunk, uint32_t lane
 }
 
 template <int BLOCK>
-__device__ double block_sum(double v, double *s_red) {
+__device__ __forceinline__ double block_sum(double v, double *s_red) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     const int wid = threadIdx.x >> 6;
@@ -348,12 +350,70 @@ __device__ __forceinline__ int2 var_slice(const KArgs &a, int v) {
 // pass does not start with dependent global loads of the syndrome and the row
 // degree, whose latency every wave would wait out.
 template <int BLOCK>
-__device__ void row_info_init(const KArgs &a, const uint8_t *__restrict__ syn) {
+__device__ __forceinline__ void row_info_init(const KArgs &a, const uint8_t *__restrict__ syn) {
     for (int pos = threadIdx.x; pos < a.g.C; pos += BLOCK) {
         const int deg = a.g.row_deg[pos];
         *lds_at<uint8_t>(a.rowinfo + (uint32_t)pos) =
             (uint8_t)((syn[a.g.row_orig[pos]] & 1u) | (deg > 0 ? 2u : 0u) | (deg == 1 ? 4u : 0u));
     }
+}
+
+// Edge codes: 16 slots of a row share one 32-bit word, slot k's 2-bit code at
+// bits 31-2k (sign of the incoming message mu_e) and 30-2k (the edge holds the
+// row's minimum), so CS plane code*stride holds, for row r:
+//   0: +m1 * par   1: +m2 * par   2: -m1 * par   3: -m2 * par.
+//
+// One chunk of kChunk edge slots of a row: gathers each slot's variable state
+// (and, after the first iteration, its previous check message, selected by the
+// slot's code in wcur, whose bits 15-2u.. are slot u's), then updates the
+// running minimum, second minimum, argmin and hard-decision parity.  Each slot
+// shifts `bits` left by two and brings its sign in at bit 1 (v_alignbit takes
+// mu's bits 31 and 30; bit 30 is cleared afterwards).  After the first
+// iteration mu = lam_mixed + (S - nu) is never -0 (S starts at +0 and sums
+// from +0, so S - nu and with it mu can be -0 only if lam_mixed and S were),
+// so its sign bit is exactly mu < 0 for the finite states that finite priors
+// give.  The first iteration's mu is the prior itself, whose sign is taken by
+// comparison.
+template <bool FIRST, int PLACE>
+__device__ __forceinline__ uint32_t check_chunk(const KArgs &a, const ShotMem<PLACE> &m, const uint32_t (&e)[kChunk],
+                                                int k0, uint32_t wcur, uint32_t qrow, uint32_t bits, float &m1,
+                                                float &m2, uint32_t &argk, uint64_t &hpar) {
+    float x[kChunk], y[kChunk], nuo[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+        if (FIRST) {
+            if (PLACE & (1 << kL0)) {
+                x[u] = m.l0_ent(e[u]);
+            } else {
+                const int v = (int)((e[u] - m.lsb) >> 3);
+                x[u] = v < a.g.V ? m.l0g[a.g.vperm[v]] : INFINITY;
+            }
+            y[u] = 0.f;
+            nuo[u] = 0.f;
+        } else {
+            const float2 ls = m.ls_ent(e[u]);
+            x[u] = ls.x;
+            y[u] = ls.y;
+            nuo[u] = m.cs_msg((__builtin_amdgcn_ubfe(wcur, 14u - 2u * (uint32_t)u, 2u) << m.cs_shift) | qrow);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+        float mu = x[u];
+        if (FIRST) {
+            bits = (bits << 2) | (mu < 0.f ? 2u : 0u);
+        } else {
+            hpar ^= __ballot(x[u] + y[u] <= 0.f);
+            mu = x[u] + (y[u] - nuo[u]);
+            bits = __builtin_amdgcn_alignbit(bits, __float_as_uint(mu), 30u);
+        }
+        const float am = fabsf(mu);
+        const bool lt = am < m1;
+        m2 = __builtin_amdgcn_fmed3f(m1, am, m2);
+        argk = lt ? (uint32_t)(k0 + u) : argk;
+        m1 = lt ? am : m1;
+    }
+    return bits;
 }
 
 // Check pass for row position pos (kernels.py:1128-1215).  FIRST: messages were
@@ -364,10 +424,11 @@ __device__ void row_info_init(const KArgs &a, const uint8_t *__restrict__ syn) {
 // the check pass).
 //
 // The wave walks its slice in chunks of kChunk edges up to the slice's widest
-// row: the chunk's index loads are issued together, then the LDS gathers, then
-// the arithmetic.  Slots past the row's end hold the sentinel variable (LLR
-// +inf), whose message is +inf: positive, never below the running minimum and
-// never a hard-decision 1, so it changes nothing.
+// row, two chunks (one 16-slot code word) per step.  The index loads of each
+// chunk are issued one chunk ahead, so their L2 latency overlaps the previous
+// chunk's gathers and arithmetic.  Slots past the row's end hold the sentinel
+// variable (LLR +inf), whose message is +inf: positive, never below the
+// running minimum and never a hard-decision 1, so it changes nothing.
 //
 // The running reduction keeps the reference's tie semantics (kernels.py:
 // 1160-1178): the argmin edge is the first to attain the minimum and m2 equals
@@ -383,69 +444,44 @@ __device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m
     const uint32_t wbase = (uint32_t)(rs.y + lane);
     const uint32_t lane_off = 4u * (uint32_t)lane;
     const uint32_t qrow = 4u * (uint32_t)pos;  // this row's messages CS[0..3][pos]
+    const uint32_t *__restrict__ ent = a.g.row_ent + rs.x;
     float m1 = INFINITY, m2 = INFINITY;
-    uint32_t argk = 0, argu = 0;
+    uint32_t argk = 0;
     uint64_t hpar = 0;  // lane mask: parity of hard decisions so far
-    uint32_t neg = 0, wold = 0, wnew = 0;
-    for (int k0 = 0; k0 < rs.z; k0 += kChunk) {
+    uint32_t neg = 0;
+    uint32_t ea[kChunk], eb[kChunk];
+    if (rs.z > 0) {
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) ea[u] = load_ell(ent, lane_off, u);
+    }
+    for (int k0 = 0; k0 < rs.z; k0 += 2 * kChunk) {
         const uint32_t word = wbase + (uint32_t)(k0 >> 4) * kSlice;
-        const uint32_t half = (uint32_t)(k0 & 8) * 2u;  // bit offset of this chunk in its word
-        if ((k0 & 15) == 0) {
-            if (!FIRST) wold = m.sb_get(word);
-            wnew = 0;
+        const uint32_t wold = FIRST ? 0u : m.sb_get(word);
+        const bool two = k0 + kChunk < rs.z;
+        if (two) {
+            const uint32_t *__restrict__ c = ent + (int64_t)(k0 + kChunk) * kSlice;
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) eb[u] = load_ell(c, lane_off, u);
         }
-        const uint32_t *__restrict__ chunk = a.g.row_ent + rs.x + (int64_t)k0 * kSlice;
-        uint32_t e[kChunk];
+        uint32_t bits = check_chunk<FIRST, PLACE>(a, m, ea, k0, wold >> 16, qrow, 0u, m1, m2, argk, hpar);
+        if (two) {
+            if (k0 + 2 * kChunk < rs.z) {
+                const uint32_t *__restrict__ c = ent + (int64_t)(k0 + 2 * kChunk) * kSlice;
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u) e[u] = load_ell(chunk, lane_off, u);
-        float x[kChunk], y[kChunk], nuo[kChunk];
-        const uint32_t wcur = FIRST ? 0u : (wold >> half);  // slot u's code at bit 2u
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-            if (FIRST) {
-                if (PLACE & (1 << kL0)) {
-                    x[u] = m.l0_ent(e[u]);
-                } else {
-                    const int v = (int)((e[u] - m.lsb) >> 3);
-                    x[u] = v < a.g.V ? m.l0g[a.g.vperm[v]] : INFINITY;
-                }
-                y[u] = 0.f;
-                nuo[u] = 0.f;
-            } else {
-                const float2 ls = m.ls_ent(e[u]);
-                x[u] = ls.x;
-                y[u] = ls.y;
-                nuo[u] = m.cs_msg((__builtin_amdgcn_ubfe(wcur, 2u * (uint32_t)u, 2u) << m.cs_shift) | qrow);
+                for (int u = 0; u < kChunk; ++u) ea[u] = load_ell(c, lane_off, u);
             }
+            bits = check_chunk<FIRST, PLACE>(a, m, eb, k0 + kChunk, wold, qrow, bits, m1, m2, argk, hpar);
+        } else {
+            bits <<= 16;  // a word's first chunk holds its high half
         }
-        uint32_t bits = 0;
-        bool imp = false;
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-            float mu = x[u];
-            if (!FIRST) {
-                hpar ^= __ballot(x[u] + y[u] <= 0.f);
-                mu = x[u] + (y[u] - nuo[u]);
-            }
-            bits |= (mu < 0.f ? 1u : 0u) << (2u * (uint32_t)u);
-            const float am = fabsf(mu);
-            const bool lt = am < m1;
-            m2 = __builtin_amdgcn_fmed3f(m1, am, m2);
-            argu = lt ? (uint32_t)u : argu;
-            imp |= lt;
-            m1 = lt ? am : m1;
-        }
-        if (imp) argk = (uint32_t)k0 + argu;
-        wnew |= bits << half;
-        if (((k0 + kChunk) & 15) == 0 || k0 + kChunk >= rs.z) {
-            m.sb_set(word, wnew);
-            neg ^= __popc(wnew);
-        }
+        bits &= 0xAAAAAAAAu;  // the signs; the minimum flags start clear
+        m.sb_set(word, bits);
+        neg ^= __popc(bits);
     }
     if (info & 2u) {
         // Mark the edge holding the minimum (after the sign words above).
         const uint32_t aw = wbase + (argk >> 4) * kSlice;
-        m.sb_set(aw, m.sb_get(aw) | (2u << (2u * (argk & 15u))));
+        m.sb_set(aw, m.sb_get(aw) | (1u << (30u - 2u * (argk & 15u))));
         float o1 = m1, o2 = m2;
         if (info & 4u) o1 = o2 = 0.f;  // degree-1 checks send zero (kernels.py:1192-1193)
         if (a.use_alpha) {
@@ -458,7 +494,7 @@ __device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m
         }
         const uint32_t par = ((neg ^ s_i) & 1u) << 31;
         const uint32_t p1 = __float_as_uint(o1) ^ par, p2 = __float_as_uint(o2) ^ par;
-        m.cs_set(pos, make_float4(__uint_as_float(p1), __uint_as_float(p1 ^ 0x80000000u), __uint_as_float(p2),
+        m.cs_set(pos, make_float4(__uint_as_float(p1), __uint_as_float(p2), __uint_as_float(p1 ^ 0x80000000u),
                                       __uint_as_float(p2 ^ 0x80000000u)));
     }
     const bool hp = (hpar >> (threadIdx.x & 63)) & 1ull;
@@ -565,6 +601,7 @@ __device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> 
 // data movement between layers.
 
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 
@@ -633,7 +670,7 @@ __device__ __forceinline__ bf16x4_t unpack_bf16x4(uint32_t lo, uint32_t hi) {
     return x;
 }
 
-__device__ void load_mlp(const uint32_t *__restrict__ frag, MlpRegs &r) {
+__device__ __forceinline__ void load_mlp(const uint32_t *__restrict__ frag, MlpRegs &r) {
     const uint4 *__restrict__ base = reinterpret_cast<const uint4 *>(frag + (threadIdx.x & 63) * kMlpStride);
     uint32_t w[kMlpStride];
 #pragma unroll
@@ -659,29 +696,52 @@ __device__ void load_mlp(const uint32_t *__restrict__ frag, MlpRegs &r) {
     r.b3 = __uint_as_float(w[36]);
 }
 
-// Logit z of the 16 variables whose features this wave holds (lane l: variable
-// l % 16, features 4 (l/16) .. +3; lanes 32-63 hold the zero padding of K).
-// Every lane returns z of its variable.  All 64 lanes must be active.
-__device__ __forceinline__ float mlp_logit(const MlpRegs &r, bf16x4_t x) {
-    bf16x4_t h1[2];
+// Logits of G groups of 16 variables (group g: lane l holds variable l % 16,
+// features 4 (l/16) .. +3 in x[g]; lanes 32-63 hold the zero padding of K).
+// Every lane gets z[g] of its variable of group g.  The groups are evaluated
+// layer by layer, so their matrix-core chains overlap.  All 64 lanes must be
+// active.
+template <int G>
+__device__ __forceinline__ void mlp_logits(const MlpRegs &r, const bf16x4_t (&x)[G], float (&z)[G]) {
+    bf16x4_t h1[G][2];
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
-        // the accumulator starts at the bias: relu(b1 + W1 x), rounded to bf16
-        const f32x4_t d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w1[mb], x, r.b1[mb], 0, 0, 0);
-        h1[mb] = unpack_bf16x4(pack_bf16x2(fmaxf(d[0], 0.f), fmaxf(d[1], 0.f)),
-                               pack_bf16x2(fmaxf(d[2], 0.f), fmaxf(d[3], 0.f)));
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            // the accumulator starts at the bias: relu(b1 + W1 x), rounded to bf16
+            const f32x4_t d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w1[mb], x[g], r.b1[mb], 0, 0, 0);
+            h1[g][mb] = unpack_bf16x4(pack_bf16x2(fmaxf(d[0], 0.f), fmaxf(d[1], 0.f)),
+                                      pack_bf16x2(fmaxf(d[2], 0.f), fmaxf(d[3], 0.f)));
+        }
     }
-    float part = 0.f;
+    float part[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) part[g] = 0.f;
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
-        f32x4_t d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w2[mb][0], h1[0], r.b2[mb], 0, 0, 0);
-        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w2[mb][1], h1[1], d, 0, 0, 0);
+        // layer 2 in one K = 32 product: lane group q's eight K slots are hidden
+        // units 4q .. 4q+3 and 16+4q .. 16+4q+3 in both operands (its two layer-1
+        // output rows and the matching weight columns), so the sum runs over
+        // all 32 hidden units whatever order the matrix core takes them in
+        const bf16x8_t wa = __builtin_bit_cast(bf16x8_t, __builtin_shufflevector(r.w2[mb][0], r.w2[mb][1], 0, 1, 2, 3, 4, 5, 6, 7));
+        f32x4_t d[G];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) part = __builtin_fmaf(r.w3[mb][i], fmaxf(d[i], 0.f), part);
+        for (int g = 0; g < G; ++g) {
+            const bf16x8_t hb = __builtin_bit_cast(bf16x8_t, __builtin_shufflevector(h1[g][0], h1[g][1], 0, 1, 2, 3, 4, 5, 6, 7));
+            d[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, hb, r.b2[mb], 0, 0, 0);
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) part[g] = __builtin_fmaf(r.w3[mb][i], fmaxf(d[g][i], 0.f), part[g]);
+        }
     }
-    part += __shfl_xor(part, 16, 64);
-    part += __shfl_xor(part, 32, 64);
-    return part + r.b3;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        part[g] += __shfl_xor(part[g], 16, 64);
+        part[g] += __shfl_xor(part[g], 32, 64);
+        z[g] = part[g] + r.b3;
+    }
 }
 
 __device__ __forceinline__ float clamp4(float x) { return fminf(fmaxf(x, -4.f), 4.f); }
@@ -708,8 +768,71 @@ __device__ __forceinline__ float clamp4(float x) { return fminf(fmaxf(x, -4.f), 
 // the network and shift the leg (else the leg keeps its uniform row).  The flip counts
 // are cleared for the next leg.  Optionally records features and hard
 // decisions in the caller's order.
+constexpr int kFeatBatch = 1;  // slices per wave whose features are gathered together
+
+// What the features of variable position vc are computed from (see
+// leg_features_t): its state, degree, prior and the smallest of its incoming
+// messages (and the count of its violated checks).
+template <int PLACE, bool KNOWN_VALID>
+struct VarFeatures {
+    float2 ls;
+    float l0, mn;
+    int deg, uns;
+    int2 vs;
+    // Columns k0 .. k0 + W - 1 of the slice (W <= its width): the W index loads
+    // are issued together (uniform base, the lane's offset, immediates), then
+    // the code words, then the messages; slots past this variable's degree are
+    // not counted.
+    template <int W>
+    __device__ __forceinline__ void gather(const KArgs &a, const ShotMem<PLACE> &m, int vc, int k0) {
+        const uint2 *__restrict__ base = a.g.var_ent + vs.x + (int64_t)k0 * kSlice;
+        const uint32_t lane_off = 8u * (uint32_t)(vc & 63);
+        uint2 e[W];
+#pragma unroll
+        for (int u = 0; u < W; ++u) e[u] = load_ell(base, lane_off, u);
+        float msg[W];
+        int un[W];
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            const uint32_t w = m.sb_ent(e[u].y);
+            msg[u] = m.cs_msg(e[u].x + (__builtin_amdgcn_ubfe(w, e[u].y, 2u) << m.cs_shift));
+            un[u] = KNOWN_VALID ? 0 : (int)*lds_at<uint8_t>(a.unsat_lds + (e[u].x >> 2));
+        }
+#pragma unroll
+        for (int u = 0; u < W; ++u) {  // selects, not branches: no load is sunk into a branch
+            const bool in = k0 + u < deg;
+            mn = fminf(mn, in ? msg[u] : INFINITY);
+            uns += in ? un[u] : 0;
+        }
+    }
+    __device__ __forceinline__ void start(const KArgs &a, const ShotMem<PLACE> &m, int vc) {
+        ls = m.ls_pos(vc);
+        deg = a.g.var_deg[vc];
+        l0 = (PLACE & (1 << kL0)) ? m.l0_pos(vc) : m.l0g[a.g.vperm[vc]];
+        vs = var_slice(a, vc);
+        mn = INFINITY;
+        uns = 0;
+        switch (vs.y) {
+            case 0: break;
+            case 1: gather<1>(a, m, vc, 0); break;
+            case 2: gather<2>(a, m, vc, 0); break;
+            case 3: gather<3>(a, m, vc, 0); break;
+            case 4: gather<4>(a, m, vc, 0); break;
+            case 5: gather<5>(a, m, vc, 0); break;
+            case 6: gather<6>(a, m, vc, 0); break;
+            case 7: gather<7>(a, m, vc, 0); break;
+            case 8: gather<8>(a, m, vc, 0); break;
+            default: {
+                int k0 = 0;
+                for (; k0 + 4 <= vs.y; k0 += 4) gather<4>(a, m, vc, k0);
+                for (; k0 < vs.y; ++k0) gather<1>(a, m, vc, k0);
+            }
+        }
+    }
+};
+
 template <int BLOCK, int PLACE, bool KNOWN_VALID>
-__device__ void leg_features_t(const KArgs &a, int b, const ShotMem<PLACE> &m, int r, const uint8_t *__restrict__ syn,
+__device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const ShotMem<PLACE> &m, int r, const uint8_t *__restrict__ syn,
                                bool apply) {
     constexpr bool known_valid = KNOWN_VALID;
     const int C = a.g.C, V = a.g.V, L = a.learned_legs;
@@ -719,97 +842,87 @@ __device__ void leg_features_t(const KArgs &a, int b, const ShotMem<PLACE> &m, i
         __syncthreads();
     }
     const int lane = threadIdx.x & 63, q = lane >> 4;
-    // v0 is uniform per wave, so every lane of a wave runs the same rounds (the
-    // MFMAs need all 64 lanes); lanes past V compute a copy of variable V - 1
-    // (in the same, last slice) and store nothing.
-    for (int v0 = threadIdx.x & ~63; v0 < V; v0 += BLOCK) {
-        const int v = v0 + lane;
-        const bool valid = v < V;
-        const int vc = valid ? v : V - 1;
-        const float2 ls = m.ls_pos(vc);
-        const float post = ls.x + ls.y;
-        const int deg = a.g.var_deg[vc];
-        const float l0 = (PLACE & (1 << kL0)) ? m.l0_pos(vc) : m.l0g[a.g.vperm[vc]];
-        const int2 vs = var_slice(a, vc);
-        float mn = INFINITY;
-        int uns = 0;
-        // the slice's columns in rounds of eight (loads, then code words, then
-        // messages); slots past the slice width read the sentinel check, slots
-        // past this variable's degree are not counted
-        const uint2 pad = make_uint2(4u * (uint32_t)C, m.sbb << 5);
-        for (int k0 = 0; k0 < vs.y; k0 += 8) {
-            uint2 e[8];
+    // the network's operands, loaded once for the whole pass
+    const bool net = a.mlp && apply;
+    MlpRegs mr;
+    if (net) load_mlp(a.mlp_frag, mr);
+    // A wave takes kFeatBatch of its slices at a time (v0 is uniform, so every
+    // lane of a wave runs the same rounds; the MFMAs need all 64 lanes): the
+    // slices' features are gathered together, so their load latencies overlap,
+    // and the network's operands are loaded once per pass.  Lanes past V, and
+    // batch slots past the last slice, compute a copy of variable V - 1 (in the
+    // last slice) and store nothing.
+    for (int vb = threadIdx.x & ~63; vb < V; vb += kFeatBatch * BLOCK) {
+        VarFeatures<PLACE, KNOWN_VALID> vf[kFeatBatch];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                // load from a slot inside the slice and select the value (selecting
-                // between two addresses would make the load a flat one)
-                const uint2 ld = a.g.var_ent[vs.x + min(k0 + u, vs.y - 1) * kSlice + (vc & 63)];
-                e[u] = k0 + u < vs.y ? ld : pad;
-            }
-            float msg[8];
-            int un[8];
+        for (int j = 0; j < kFeatBatch; ++j) vf[j].start(a, m, min(vb + j * BLOCK + lane, V - 1));
+        uint32_t pk[kFeatBatch][4];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const uint32_t w = m.sb_ent(e[u].y);
-                msg[u] = m.cs_msg(e[u].x + (__builtin_amdgcn_ubfe(w, e[u].y, 2u) << m.cs_shift));
-                un[u] = known_valid ? 0 : (int)*lds_at<uint8_t>(a.unsat_lds + (e[u].x >> 2));
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                if (k0 + u < deg) {
-                    mn = fminf(mn, msg[u]);
-                    uns += un[u];
+        for (int j = 0; j < kFeatBatch; ++j) {
+            const VarFeatures<PLACE, KNOWN_VALID> &x = vf[j];
+            const int v = vb + j * BLOCK + lane;
+            const int vc = min(v, V - 1);
+            float f[8];
+            const float inv_deg = x.deg > 0 ? 1.f / (float)x.deg : 0.f;
+            const float post = x.ls.x + x.ls.y;
+            f[0] = x.l0 * 0.125f;
+            f[1] = clamp4(post * 0.0625f);
+            f[2] = (float)*lds_at<uint8_t>(a.flips_lds + (uint32_t)vc) * 0.125f;
+            f[3] = clamp4(x.ls.y * inv_deg * 0.125f);
+            f[4] = x.deg > 0 ? clamp4(x.mn * 0.125f) : 0.f;
+            f[5] = (float)x.deg * 0.125f;
+            f[6] = (float)x.uns * inv_deg;
+            f[7] = (float)r * 0.25f;
+            if (v < V && (a.feat_out || a.hard_out)) {
+                const size_t row = ((size_t)b * L + r) * V + a.g.vperm[v];
+                if (a.feat_out) {
+                    *reinterpret_cast<float4 *>(a.feat_out + row * 8) = make_float4(f[0], f[1], f[2], f[3]);
+                    *reinterpret_cast<float4 *>(a.feat_out + row * 8 + 4) = make_float4(f[4], f[5], f[6], f[7]);
                 }
+                if (a.hard_out) a.hard_out[row] = post <= 0.f ? 1 : 0;
             }
+            pk[j][0] = pack_bf16x2(f[0], f[1]);
+            pk[j][1] = pack_bf16x2(f[2], f[3]);
+            pk[j][2] = pack_bf16x2(f[4], f[5]);
+            pk[j][3] = pack_bf16x2(f[6], f[7]);
         }
-        float f[8];
-        const float inv_deg = deg > 0 ? 1.f / (float)deg : 0.f;
-        f[0] = l0 * 0.125f;
-        f[1] = clamp4(post * 0.0625f);
-        f[2] = (float)*lds_at<uint8_t>(a.flips_lds + (uint32_t)vc) * 0.125f;
-        f[3] = clamp4(ls.y * inv_deg * 0.125f);
-        f[4] = deg > 0 ? clamp4(mn * 0.125f) : 0.f;
-        f[5] = (float)deg * 0.125f;
-        f[6] = (float)uns * inv_deg;
-        f[7] = (float)r * 0.25f;
-        if (valid) {
-            const size_t row = ((size_t)b * L + r) * V + a.g.vperm[v];
-            if (a.feat_out) {
-                *reinterpret_cast<float4 *>(a.feat_out + row * 8) = make_float4(f[0], f[1], f[2], f[3]);
-                *reinterpret_cast<float4 *>(a.feat_out + row * 8 + 4) = make_float4(f[4], f[5], f[6], f[7]);
-            }
-            if (a.hard_out) a.hard_out[row] = post <= 0.f ? 1 : 0;
-        }
-        if (a.mlp && apply) {
-            // the operands are reloaded every round (coalesced, L1-resident) rather
-            // than held across the gathers above
-            MlpRegs mr;
-            load_mlp(a.mlp_frag, mr);
-            const uint32_t p01 = pack_bf16x2(f[0], f[1]), p23 = pack_bf16x2(f[2], f[3]);
-            const uint32_t p45 = pack_bf16x2(f[4], f[5]), p67 = pack_bf16x2(f[6], f[7]);
-            float z = 0.f;
+        if (net) {
 #pragma unroll
-            for (int grp = 0; grp < 4; ++grp) {
-                const int src = 16 * grp + (lane & 15);
-                const uint32_t a01 = (uint32_t)__shfl((int)p01, src, 64), a23 = (uint32_t)__shfl((int)p23, src, 64);
-                const uint32_t b01 = (uint32_t)__shfl((int)p45, src, 64), b23 = (uint32_t)__shfl((int)p67, src, 64);
-                const uint32_t x01 = q == 0 ? a01 : (q == 1 ? b01 : 0u);
-                const uint32_t x23 = q == 0 ? a23 : (q == 1 ? b23 : 0u);
-                const float zg = mlp_logit(mr, unpack_bf16x4(x01, x23));
-                z = q == grp ? zg : z;
+            for (int j = 0; j < kFeatBatch; ++j) {
+                const int v = vb + j * BLOCK + lane;
+                if (uniform_i32(vb + j * BLOCK) >= V) break;
+                bf16x4_t xg[4];
+#pragma unroll
+                for (int grp = 0; grp < 4; ++grp) {
+                    const int src = 16 * grp + (lane & 15);
+                    const uint32_t a01 = (uint32_t)__shfl((int)pk[j][0], src, 64);
+                    const uint32_t a23 = (uint32_t)__shfl((int)pk[j][1], src, 64);
+                    const uint32_t b01 = (uint32_t)__shfl((int)pk[j][2], src, 64);
+                    const uint32_t b23 = (uint32_t)__shfl((int)pk[j][3], src, 64);
+                    const uint32_t x01 = q == 0 ? a01 : (q == 1 ? b01 : 0u);
+                    const uint32_t x23 = q == 0 ? a23 : (q == 1 ? b23 : 0u);
+                    xg[grp] = unpack_bf16x4(x01, x23);
+                }
+                float zg[4];
+                mlp_logits<4>(mr, xg, zg);
+                const float z = q == 0 ? zg[0] : (q == 1 ? zg[1] : (q == 2 ? zg[2] : zg[3]));
+                // shift of the leg's own draw in steps of (hi - lo) / 254: 254 (sigmoid(z) - 1/2)
+                // rounded to the nearest integer (ties to even), in [-127, 127]
+                const float steps = __builtin_rintf(254.f * (1.f / (1.f + expf(-z)) - 0.5f));
+                if (v < V) *lds_at<int8_t>(a.gamma_lds + (uint32_t)v) = (int8_t)(int)steps;
             }
-            // shift of the leg's own draw in steps of (hi - lo) / 254: 254 (sigmoid(z) - 1/2)
-            // rounded to the nearest integer (ties to even), in [-127, 127]
-            const float steps = __builtin_rintf(254.f * (1.f / (1.f + expf(-z)) - 0.5f));
-            if (valid) *lds_at<int8_t>(a.gamma_lds + (uint32_t)v) = (int8_t)(int)steps;
         }
-        if (valid) *lds_at<uint8_t>(a.flips_lds + (uint32_t)v) = 0;
+#pragma unroll
+        for (int j = 0; j < kFeatBatch; ++j) {
+            const int v = vb + j * BLOCK + lane;
+            if (v < V) *lds_at<uint8_t>(a.flips_lds + (uint32_t)v) = 0;
+        }
     }
     __syncthreads();
 }
 
 template <int BLOCK, int PLACE>
-__device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int r, const uint8_t *__restrict__ syn,
+__device__ __forceinline__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int r, const uint8_t *__restrict__ syn,
                              bool known_valid, bool apply) {
     if (known_valid) leg_features_t<BLOCK, PLACE, true>(a, b, m, r, syn, apply);
     else leg_features_t<BLOCK, PLACE, false>(a, b, m, r, syn, apply);
@@ -818,34 +931,68 @@ __device__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int
 // The strengths a learned leg uses (shifted or its uniform row), in the
 // caller's order, for gamma_out.
 template <int BLOCK>
-__device__ void record_strengths(const KArgs &a, int b, int r, const GammaSrc &gs) {
+__device__ __forceinline__ void record_strengths(const KArgs &a, int b, int r, const GammaSrc &gs) {
     const int V = a.g.V, L = a.learned_legs;
     for (int v = threadIdx.x; v < V; v += BLOCK) a.gamma_out[((size_t)b * L + r) * V + a.g.vperm[v]] = gs(v);
+}
+
+constexpr int kBatchU = 4;  // positions per thread handled together by the per-shot loops
+
+// f(j, state) for every caller-order index j this thread owns, with the state
+// of its position; kBatchU at a time, so that their index loads and state
+// reads are in flight together.
+template <int BLOCK, int PLACE, typename F>
+__device__ __forceinline__ void for_each_caller_index(const KArgs &a, const ShotMem<PLACE> &m, F f) {
+    const int V = a.g.V;
+    for (int j0 = threadIdx.x; j0 < V; j0 += kBatchU * BLOCK) {
+        int v[kBatchU];
+        float2 ls[kBatchU];
+#pragma unroll
+        for (int u = 0; u < kBatchU; ++u) v[u] = a.g.vpos[min(j0 + u * BLOCK, V - 1)];
+#pragma unroll
+        for (int u = 0; u < kBatchU; ++u) ls[u] = m.ls_pos(v[u]);
+#pragma unroll
+        for (int u = 0; u < kBatchU; ++u)
+            if (j0 + u * BLOCK < V) f(j0 + u * BLOCK, ls[u]);
+    }
 }
 
 // check_and_select_kernel (kernels.py:657-741) for a shot whose hard decision
 // satisfies every check: weight = sum of wj over bits with post <= 0 (double
 // accumulation, rounded once), strict improvement or first solution wins.
 template <int BLOCK, int PLACE>
-__device__ void select_solution(const KArgs &a, const ShotMem<PLACE> &m, const float *__restrict__ wj,
+__device__ __forceinline__ void select_solution(const KArgs &a, const ShotMem<PLACE> &m, const float *__restrict__ wj,
                                 uint8_t *err, ShotState &st, double *s_red) {
     const int V = a.g.V;
     double acc = 0.0;
-    // position order: contiguous state reads; wj is gathered only for the few
-    // bits that are set.  The double sum of n fp32 weights is exact, whatever
-    // the order, while the set weights span less than 2^(29 - log2 n) in
-    // magnitude (2^15 at n = V = 8785).
-    for (int v = threadIdx.x; v < V; v += BLOCK) {
-        const float2 ls = m.ls_pos(v);
-        if (ls.x + ls.y <= 0.f) acc += (double)wj[a.g.vperm[v]];
+    // Position order: contiguous state reads.  The double sum of n fp32
+    // weights is exact, whatever the order, while the set weights span less
+    // than 2^(29 - log2 n) in magnitude (2^15 at n = V = 8785).
+    if (a.wj_pos) {
+        // weights in position order too: kBatchU contiguous loads per thread in
+        // flight together, and a select rather than a branch
+        for (int v0 = threadIdx.x; v0 < V; v0 += kBatchU * BLOCK) {
+            float2 ls[kBatchU];
+            float w[kBatchU];
+#pragma unroll
+            for (int u = 0; u < kBatchU; ++u) {
+                ls[u] = m.ls_pos(min(v0 + u * BLOCK, V));  // position V: the sentinel (+inf, never set)
+                w[u] = a.wj_pos[min(v0 + u * BLOCK, V - 1)];
+            }
+#pragma unroll
+            for (int u = 0; u < kBatchU; ++u) acc += ls[u].x + ls[u].y <= 0.f ? (double)w[u] : 0.0;
+        }
+    } else {
+        // per-shot weights in the caller's order: gathered only for the few set bits
+        for (int v = threadIdx.x; v < V; v += BLOCK) {
+            const float2 ls = m.ls_pos(v);
+            if (ls.x + ls.y <= 0.f) acc += (double)wj[a.g.vperm[v]];
+        }
     }
     const float w = (float)block_sum<BLOCK>(acc, s_red);
     if (st.best_w >= 1e20f || w < st.best_w) {
         st.best_w = w;
-        for (int j = threadIdx.x; j < V; j += BLOCK) {
-            const float2 ls = m.ls_pos(a.g.vpos[j]);
-            err[j] = (ls.x + ls.y <= 0.f) ? 1 : 0;
-        }
+        for_each_caller_index<BLOCK, PLACE>(a, m, [&](int j, float2 ls) { err[j] = (ls.x + ls.y <= 0.f) ? 1 : 0; });
     }
     st.found += 1;
 }
@@ -855,7 +1002,7 @@ __device__ void select_solution(const KArgs &a, const ShotMem<PLACE> &m, const f
 // count each variable's hard-decision flips (a feature of the learned
 // strengths; only legs whose end state feeds the network count them).
 template <int BLOCK, int PLACE, bool FLIPS = false>
-__device__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m, int T, const GammaSrc &gs,
+__device__ __forceinline__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m, int T, const GammaSrc &gs,
                         const float *__restrict__ wj, const uint8_t *__restrict__ syn, uint8_t *err,
                         ShotState &st, double *s_red, int *s_flag) {
     const int C = a.g.C, V = a.g.V;
@@ -890,15 +1037,11 @@ __device__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m, int T, const Ga
 }
 
 template <int BLOCK, int PLACE>
-__device__ void write_hard(const KArgs &a, const ShotMem<PLACE> &m, uint8_t *err, bool any_iter) {
-    for (int j = threadIdx.x; j < a.g.V; j += BLOCK) {
-        uint8_t h = 0;
-        if (any_iter) {
-            const float2 ls = m.ls_pos(a.g.vpos[j]);
-            h = (ls.x + ls.y <= 0.f) ? 1 : 0;
-        }
-        err[j] = h;
-    }
+__device__ __forceinline__ void write_hard(const KArgs &a, const ShotMem<PLACE> &m, uint8_t *err, bool any_iter) {
+    if (any_iter)
+        for_each_caller_index<BLOCK, PLACE>(a, m, [&](int j, float2 ls) { err[j] = (ls.x + ls.y <= 0.f) ? 1 : 0; });
+    else
+        for (int j = threadIdx.x; j < a.g.V; j += BLOCK) err[j] = 0;
 }
 
 // Per-shot slices of the batch arrays.
@@ -919,18 +1062,28 @@ __device__ __forceinline__ ShotIO shot_io(const KArgs &a, int b) {
 // (prior or init_posterior), S = 0, so that lam_mixed + S == post exactly.
 // Returns true when relay legs remain to run.
 template <int BLOCK, int PLACE, bool LEARN>
-__device__ bool first_leg(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &st, double *s_red,
+__device__ __forceinline__ bool first_leg(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &st, double *s_red,
                           int *s_flag) {
     const int V = a.g.V;
     const ShotIO io = shot_io(a, b);
-    for (int j = threadIdx.x; j < V; j += BLOCK) {
-        const int v = a.g.vpos[j];
-        const float lj = io.l0[j];
-        const float p0 = a.init_post ? a.init_post[(size_t)b * V + j] : lj;
-        m.ls_set(v, make_float2(p0, 0.f));
-        if (PLACE & (1 << kL0)) m.l0_set(v, lj);
-        if (LEARN) *lds_at<uint8_t>(a.flips_lds + (uint32_t)v) = 0;
-        io.err[j] = 0;
+    for (int j0 = threadIdx.x; j0 < V; j0 += kBatchU * BLOCK) {  // kBatchU per thread at a time
+        int v[kBatchU];
+        float lj[kBatchU], p0[kBatchU];
+#pragma unroll
+        for (int u = 0; u < kBatchU; ++u) {
+            const int j = min(j0 + u * BLOCK, V - 1);
+            v[u] = a.g.vpos[j];
+            lj[u] = io.l0[j];
+            p0[u] = a.init_post ? a.init_post[(size_t)b * V + j] : lj[u];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatchU; ++u) {
+            if (j0 + u * BLOCK >= V) break;
+            m.ls_set(v[u], make_float2(p0[u], 0.f));
+            if (PLACE & (1 << kL0)) m.l0_set(v[u], lj[u]);
+            if (LEARN) *lds_at<uint8_t>(a.flips_lds + (uint32_t)v[u]) = 0;
+            io.err[j0 + u * BLOCK] = 0;
+        }
     }
     row_info_init<BLOCK>(a, io.syn);
     m.l0g = io.l0;
@@ -970,7 +1123,7 @@ __device__ __forceinline__ GammaSrc leg_gamma(const KArgs &a, int b, int leg) {
 // Stage 2: relay legs until stop_nconv solutions (decoder.py:3655-3835), then
 // the unconverged correction and the optional last posterior.
 template <int BLOCK, int PLACE, bool LEARN>
-__device__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &st, bool run, double *s_red,
+__device__ __forceinline__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &st, bool run, double *s_red,
                            int *s_flag, bool leg0_ready = false) {
     const int V = a.g.V;
     const ShotIO io = shot_io(a, b);
@@ -1017,10 +1170,7 @@ __device__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &
     if (a.last_leg && st.found == 0) write_hard<BLOCK, PLACE>(a, m, io.err, st.any_iter);
     if (a.last_post) {
         float *lp = a.last_post + (size_t)b * V;
-        for (int j = threadIdx.x; j < V; j += BLOCK) {
-            const float2 ls = m.ls_pos(a.g.vpos[j]);
-            lp[j] = ls.x + ls.y;
-        }
+        for_each_caller_index<BLOCK, PLACE>(a, m, [&](int j, float2 ls) { lp[j] = ls.x + ls.y; });
     }
 }
 
@@ -1034,7 +1184,7 @@ __device__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &
 // so one fence (one L2 write-back) covers the whole workgroup.  A fence in
 // every thread would write back and invalidate the XCD's L2 once per wave.
 template <int BLOCK, int PLACE, bool LEARN>
-__device__ void requeue_shot(const KArgs &a, int b, const ShotMem<PLACE> &m, const ShotState &st) {
+__device__ __forceinline__ void requeue_shot(const KArgs &a, int b, const ShotMem<PLACE> &m, const ShotState &st) {
     const int V = a.g.V;
     float *post = a.rq_post + (size_t)b * V;
     for (int v = threadIdx.x; v < V; v += BLOCK) {
@@ -1054,16 +1204,30 @@ __device__ void requeue_shot(const KArgs &a, int b, const ShotMem<PLACE> &m, con
 
 // Resume a re-queued shot: posterior back as lam_mixed = post, S = 0.
 template <int BLOCK, int PLACE, bool LEARN>
-__device__ void resume_shot(const KArgs &a, int b, ShotMem<PLACE> &m) {
+__device__ __forceinline__ void resume_shot(const KArgs &a, int b, ShotMem<PLACE> &m) {
     const int V = a.g.V;
     const ShotIO io = shot_io(a, b);
     const float *post = a.rq_post + (size_t)b * V;
-    for (int v = threadIdx.x; v < V; v += BLOCK) {
-        m.ls_set(v, make_float2(post[v], 0.f));
-        if (PLACE & (1 << kL0)) m.l0_set(v, io.l0[a.g.vperm[v]]);
-        if (LEARN) {
-            *lds_at<uint8_t>(a.flips_lds + (uint32_t)v) = 0;  // leg 0's flips start from zero
-            if (a.mlp) *lds_at<int8_t>(a.gamma_lds + (uint32_t)v) = a.rq_gamma[(size_t)b * V + v];
+    for (int v0 = threadIdx.x; v0 < V; v0 += kBatchU * BLOCK) {  // kBatchU per thread at a time
+        float p[kBatchU], l[kBatchU];
+        int8_t gq[kBatchU];
+#pragma unroll
+        for (int u = 0; u < kBatchU; ++u) {
+            const int v = min(v0 + u * BLOCK, V - 1);
+            p[u] = post[v];
+            if (PLACE & (1 << kL0)) l[u] = a.prior_pos ? a.prior_pos[v] : io.l0[a.g.vperm[v]];
+            if (LEARN && a.mlp) gq[u] = a.rq_gamma[(size_t)b * V + v];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatchU; ++u) {
+            const int v = v0 + u * BLOCK;
+            if (v >= V) break;
+            m.ls_set(v, make_float2(p[u], 0.f));
+            if (PLACE & (1 << kL0)) m.l0_set(v, l[u]);
+            if (LEARN) {
+                *lds_at<uint8_t>(a.flips_lds + (uint32_t)v) = 0;  // leg 0's flips start from zero
+                if (a.mlp) *lds_at<int8_t>(a.gamma_lds + (uint32_t)v) = gq[u];
+            }
         }
     }
     row_info_init<BLOCK>(a, io.syn);
@@ -1076,7 +1240,7 @@ __device__ void resume_shot(const KArgs &a, int b, ShotMem<PLACE> &m) {
 // leg of bounded length, and each workgroup holding one is running, so once
 // all B have left (kCtrlLeft == B) the number published is final and the wait
 // ends.
-__device__ int wait_requeued(const KArgs &a, int t2) {
+__device__ __forceinline__ int wait_requeued(const KArgs &a, int t2) {
     if (t2 >= a.B) return a.B;
     for (;;) {
         const int s = __hip_atomic_load(&a.rq_slot[t2], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
@@ -1333,7 +1497,7 @@ namespace {
 // [B,V] int8 (re-queue with learned strengths) | mlp_frag (learned plans) |
 // spill slots
 struct WsLayout {
-    int64_t order, key, rq_slot, rq_state, rq_post, rq_gamma, mlp_frag, spill, total_fixed;
+    int64_t order, key, rq_slot, rq_state, rq_post, rq_gamma, mlp_frag, by_pos, spill, total_fixed;
 };
 
 WsLayout ws_layout(const pel_graph *g, int64_t B, bool requeue, bool learn) {
@@ -1346,7 +1510,8 @@ WsLayout ws_layout(const pel_graph *g, int64_t B, bool requeue, bool learn) {
     w.rq_post = align_up(w.rq_state + 16 * nb, 256);
     w.rq_gamma = align_up(w.rq_post + (requeue ? 4 * nb * g->V : 0), 256);
     w.mlp_frag = align_up(w.rq_gamma + (requeue && learn ? nb * g->V : 0), 256);
-    w.spill = align_up(w.mlp_frag + (learn ? 4LL * kMlpStride * 64 : 0), 256);
+    w.by_pos = align_up(w.mlp_frag + (learn ? 4LL * kMlpStride * 64 : 0), 256);  // wj, priors by position
+    w.spill = align_up(w.by_pos + 2 * align_up(4LL * g->V, 256), 256);
     w.total_fixed = w.spill;
     return w;
 }
@@ -1613,7 +1778,8 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
         for (int32_t e = chk_ptr[i]; e < chk_ptr[i + 1]; ++e) {
             const int32_t v = vposv[chk_var[e]], k = e - chk_ptr[i];
             const int64_t word = (int64_t)rs[s].y + (k / kEdgesPerWord) * kSlice + lane;
-            const uint32_t sbaddr = (uint32_t)((cs_bytes + 4 * word) << 5) | (uint32_t)(2 * (k % kEdgesPerWord));
+            const uint32_t sbaddr =
+                (uint32_t)((cs_bytes + 4 * word) << 5) | (uint32_t)(2 * (kEdgesPerWord - 1 - k % kEdgesPerWord));
             var_ent[vs[v / kSlice].x + (int64_t)fill[v]++ * kSlice + (v % kSlice)] = make_uint2(4u * (uint32_t)p, sbaddr);
         }
     }
@@ -1854,6 +2020,11 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     a.prior = b->prior_llr;
     a.wj = b->wj;
     a.prior_batched = b->prior_batched;
+    uint8_t *ws = static_cast<uint8_t *>(b->workspace);
+    if (!b->prior_batched) {
+        a.wj_pos = reinterpret_cast<const float *>(ws + w.by_pos);
+        a.prior_pos = reinterpret_cast<const float *>(ws + w.by_pos + align_up(4LL * g->V, 256));
+    }
     a.gamma_legs = b->gamma_legs;
     a.gamma_per_row = b->gamma_per_row;
     a.gamma_by_position = b->gamma_by_position;
@@ -1865,7 +2036,6 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     a.found = b->found_count;
     a.last_post = b->last_posterior;
     a.shot_clock = b->shot_clock;
-    uint8_t *ws = static_cast<uint8_t *>(b->workspace);
     a.ctrl = reinterpret_cast<int32_t *>(ws);
     int32_t *order = reinterpret_cast<int32_t *>(ws + w.order);
     int32_t *key = reinterpret_cast<int32_t *>(ws + w.key);
@@ -1900,6 +2070,15 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     PEL_HIP(hipGetLastError());
     hipLaunchKernelGGL(order_shots_kernel, dim3(1), dim3(kOrderBlock), 0, s, key, b->B, order, a.ctrl);
     PEL_HIP(hipGetLastError());
+    if (!b->prior_batched) {  // the shared weights and priors in position order, once per decode
+        const unsigned pblocks = (unsigned)std::min<int64_t>((g->V + 255) / 256, 1024);
+        hipLaunchKernelGGL(permute_vars_kernel, dim3(pblocks), dim3(256), 0, s, b->wj, const_cast<float *>(a.wj_pos),
+                           g->d.vperm, g->V, (int64_t)g->V);
+        PEL_HIP(hipGetLastError());
+        hipLaunchKernelGGL(permute_vars_kernel, dim3(pblocks), dim3(256), 0, s, b->prior_llr,
+                           const_cast<float *>(a.prior_pos), g->d.vperm, g->V, (int64_t)g->V);
+        PEL_HIP(hipGetLastError());
+    }
     if (b->mlp) {  // the network as per-lane MFMA operands, once per decode
         hipLaunchKernelGGL(mlp_frag_kernel, dim3(1), dim3(64), 0, s, b->mlp,
                            reinterpret_cast<uint32_t *>(ws + w.mlp_frag));
