@@ -176,6 +176,8 @@ typedef float vec4f __attribute__((ext_vector_type(4)));
 template <int PLACE>
 struct ShotMem {
     uint8_t *ls, *cs, *sb;  // array bases (LDS or workspace slot)
+    vec4f *first;           // fast placements: the workgroup's copy of LDS [0, first_bytes) (CS and SB)
+    int32_t first_words;    // after a shot's first check pass, in 16-byte words
     const float *l0g;   // the caller's prior array (caller's order) when kL0 is clear
     uint32_t lsb, sbb, l0b;  // canonical LDS addresses of LS, SB and L0
     uint32_t cs_shift;       // CS plane stride = 1 << cs_shift bytes
@@ -267,6 +269,7 @@ struct ShotState {
     int32_t any_iter;
     int32_t last_ok;  // the last leg ended at a valid solution (its final state satisfies every check)
     int32_t shifted;  // the network gave this shot strength shifts for its learned legs
+    int32_t first_ok; // the workgroup's copy of this shot's first check pass is written
 };
 
 // Values every thread of the workgroup holds identically (LDS broadcasts,
@@ -842,14 +845,11 @@ __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const Shot
         __syncthreads();
     }
     const int lane = threadIdx.x & 63, q = lane >> 4;
-    // the network's operands, loaded once for the whole pass
     const bool net = a.mlp && apply;
-    MlpRegs mr;
-    if (net) load_mlp(a.mlp_frag, mr);
     // A wave takes kFeatBatch of its slices at a time (v0 is uniform, so every
     // lane of a wave runs the same rounds; the MFMAs need all 64 lanes): the
     // slices' features are gathered together, so their load latencies overlap,
-    // and the network's operands are loaded once per pass.  Lanes past V, and
+    // and the network's operands are loaded once per batch.  Lanes past V, and
     // batch slots past the last slice, compute a copy of variable V - 1 (in the
     // last slice) and store nothing.
     for (int vb = threadIdx.x & ~63; vb < V; vb += kFeatBatch * BLOCK) {
@@ -887,6 +887,10 @@ __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const Shot
             pk[j][3] = pack_bf16x2(f[6], f[7]);
         }
         if (net) {
+            // the operands are reloaded every slice (coalesced, L1-resident)
+            // rather than held across the gathers above
+            MlpRegs mr;
+            load_mlp(a.mlp_frag, mr);
 #pragma unroll
             for (int j = 0; j < kFeatBatch; ++j) {
                 const int v = vb + j * BLOCK + lane;
@@ -1009,12 +1013,27 @@ __device__ __forceinline__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m,
     for (int t = 0; t < T; ++t) {
         const float alpha = a.use_alpha ? a.alpha_t[t] : 1.0f;
         int bad = 0;
-        if (t == 0) {
+        // A leg's first check pass starts from messages reset to the priors
+        // (kernels.py:981-1000), so its output (CS and SB) depends only on the
+        // shot's priors, syndrome and alpha_t[0]: every leg of a shot computes
+        // the same words.  In the fast placements (CS and SB at LDS [0, lsb))
+        // the first one is copied to the workgroup's slot and later legs copy it
+        // back.  Each thread restores exactly the words it saved.
+        const bool restore = t == 0 && ShotMem<PLACE>::kFast && st.first_ok;
+        if (restore) {
+            for (int i = threadIdx.x; i < m.first_words; i += BLOCK) *lds_at<vec4f>(16u * (uint32_t)i) = m.first[i];
+        } else if (t == 0) {
             for (int i = threadIdx.x; i < C; i += BLOCK) check_row<true, PLACE>(a, m, i, alpha, syn);
         } else {
             for (int i = threadIdx.x; i < C; i += BLOCK) bad |= check_row<false, PLACE>(a, m, i, alpha, syn);
         }
         bad = block_or<BLOCK>(bad, s_flag);
+        if (t == 0 && ShotMem<PLACE>::kFast && !st.first_ok) {
+            // read before the barrier that ends this iteration; the next check
+            // pass rewrites CS and SB only after it
+            for (int i = threadIdx.x; i < m.first_words; i += BLOCK) m.first[i] = *lds_at<vec4f>(16u * (uint32_t)i);
+            st.first_ok = 1;
+        }
         if (t > 0 && !bad) {  // iteration t-1 converged
             select_solution<BLOCK, PLACE>(a, m, wj, err, st, s_red);
             return true;
@@ -1089,7 +1108,7 @@ __device__ __forceinline__ bool first_leg(const KArgs &a, int b, ShotMem<PLACE> 
     m.l0g = io.l0;
     __syncthreads();
 
-    st = ShotState{INFINITY, 0, 0, 0, 0, 0};
+    st = ShotState{INFINITY, 0, 0, 0, 0, 0, 0};
     GammaSrc g0{nullptr, nullptr, a.plain ? 0.f : a.gamma0};
     if (!a.plain && a.init_gamma) {
         g0.row = a.init_gamma + (size_t)b * V;
@@ -1272,6 +1291,8 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
     m.sbb = a.g.sbb;
     m.l0b = a.off[kL0];
     m.cs_shift = a.g.cs_shift;
+    m.first = reinterpret_cast<vec4f *>(slot);
+    m.first_words = (int32_t)(a.g.lsb / 16u);  // CS at 0, SB at sbb, LS at lsb
     // This workgroup's copy of the slice descriptor tables.
     {
         const int nrs = (a.g.C + kSlice - 1) / kSlice, nvs = (a.g.V + kSlice - 1) / kSlice;
@@ -1294,7 +1315,7 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
     // The leader writes the finished shot's scalars and takes the next ticket in
     // one branch: first-leg tickets in queue order, then re-queued shots.
     int prev = -1;
-    ShotState st{0.f, 0, 0, 0, 0, 0};
+    ShotState st{0.f, 0, 0, 0, 0, 0, 0};
     for (;;) {
         if (opaque_tid() == 0) {
             const int64_t now = a.shot_clock ? (int64_t)wall_clock64() : 0;
@@ -1344,7 +1365,7 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
             const int4 rs = uniform_i4(a.rq_state[b]);
             // parked shots are those whose first leg converged; the network ran before
             // they were parked
-            st = ShotState{__int_as_float(rs.x), rs.y, rs.z, rs.w, 1, a.mlp != nullptr};
+            st = ShotState{__int_as_float(rs.x), rs.y, rs.z, rs.w, 1, a.mlp != nullptr, 0};
         }
         relay_legs<BLOCK, PLACE, LEARN>(a, b, m, st, more, s_red, s_flag, stage == 2);
         prev = b;
@@ -1529,6 +1550,7 @@ void place_arrays(const StateSizes &st, int place, int block, bool learn, uint32
         off[kL0] = (uint32_t)(st.sz[kCS] + st.sz[kSB] + st.sz[kLS]);
         *lds = st.sz[kCS] + st.sz[kSB] + st.sz[kLS] + ((place & (1 << kL0)) ? st.sz[kL0] : 0);
         if (!(place & (1 << kL0))) off[kL0] = 0;
+        *spill = st.sz[kCS] + st.sz[kSB];  // the slot holds the copy of a shot's first check pass
     } else {
         for (int k = 0; k < kNumArrays; ++k) {
             if (place & (1 << k)) {
