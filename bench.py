@@ -1,14 +1,18 @@
 """Decoded shots/s of Relay-BP-S on one MI355X (936 x 8785 banded circuit-shaped code),
 plain and with learned strengths, plus serving latency.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--shots 4096] [--depth 2] [--no-cpu]
-                    [--no-learned] [--no-serving] [--no-batched]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--shots 4096] [--depth 3] [--requeue-below 0]
+                    [--no-cpu] [--no-learned] [--no-serving] [--no-batched]
 
 A step decodes the whole 4096-shot batch (syndromes already resident in HBM)
 into its own output buffers.  The timed steps run with ``--depth`` launches in
 flight: step i is issued on stream i % depth (each stream with its own
 workspace and outputs), so a launch's last few slow shots overlap the next
-launch instead of leaving the GPU idle.  Every step's results are checked
+launches instead of leaving the GPU idle.  With launches overlapping, the
+timed steps skip the library's re-queue of fast first legs (a single launch's
+tail remedy, which parks and reloads every such shot): ``--requeue-below``
+sets the plan's threshold for them (default 0, off).  The one-launch-at-a-time
+figures use the default plan.  Every step's results are checked
 against a plain ``decode()`` of the batch.  Prints ONE JSON line (rank 0)
 with: the throughput of the plain form (uniform-draw legs) and, when trained
 weights are present, of the learned form on the same shots with both error
@@ -267,8 +271,11 @@ def learned_form(H, p, err, syn, syn_np, args, plain_rate, plain_ler) -> dict:
     net = learned.GammaMLP.load(wpath)
     dec = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0], learned_gamma=net,
                          learned_legs=legs, **RELAY)
+    dec_t = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0], learned_gamma=net,
+                           learned_legs=legs, **RELAY)
+    dec_t.requeue_below = args.requeue_below
     ref = dec.decode(syn)
-    m = measure(dec, syn, args.steps, args.warmup, args.depth)
+    m = measure(dec_t, syn, args.steps, args.warmup, args.depth)
     seq = measure(dec, syn, max(3, args.steps // 4), 1, 1)
     out = m["out"]
     errors = out["errors"].cpu().numpy()
@@ -401,10 +408,12 @@ def serving_runs(seconds: float, prior_scale: float = 0.003,
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--shots", type=int, default=4096)
-    ap.add_argument("--depth", type=int, default=2, help="decode launches in flight in the timed steps")
+    ap.add_argument("--depth", type=int, default=3, help="decode launches in flight in the timed steps")
+    ap.add_argument("--requeue-below", type=int, default=0,
+                    help="re-queue threshold of the timed steps' plan (0: off; the library default is 16)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-strength form")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-latency runs")
@@ -449,14 +458,18 @@ def main() -> int:
     syn_np = codes.syndromes_of(H, err)
     dec = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0],
                          block_size=args.block or None, **RELAY)
+    dec_t = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0],
+                           block_size=args.block or None, **RELAY)
+    dec_t.requeue_below = args.requeue_below
     C, V, E = dec.C, dec.V, dec.E
     syn = torch.from_numpy(syn_np).to(dev).contiguous()
     plan = dec.plan(args.shots)
     print(f"[bench] banded {C}x{V} E={E} shots={args.shots} block={plan.block} grid={plan.grid} "
-          f"placement={plan.placement} lds={plan.lds_bytes} depth={args.depth}", file=sys.stderr, flush=True)
+          f"placement={plan.placement} lds={plan.lds_bytes} depth={args.depth} requeue_below={args.requeue_below}",
+          file=sys.stderr, flush=True)
 
     ref = dec.decode(syn)
-    m = measure(dec, syn, args.steps, args.warmup, args.depth)   # the timed region
+    m = measure(dec_t, syn, args.steps, args.warmup, args.depth)  # the timed region
     seq = measure(dec, syn, max(3, args.steps // 4), 1, 1)        # one launch at a time
     elapsed = m["elapsed"]
     kern_ms = m["kernel_ms"]
@@ -506,6 +519,7 @@ def main() -> int:
             "set_max_iter": RELAY["set_max_iter"], "stop_nconv": RELAY["stop_nconv"],
             "gamma0": RELAY["gamma0"], "gamma_dist_interval": list(RELAY["gamma_dist_interval"]),
             "block": plan.block, "grid": plan.grid, "lds_bytes": plan.lds_bytes,
+            "launches_in_flight": args.depth, "requeue_below": args.requeue_below,
         },
         "roofline": roof_plain,
         "quality": {

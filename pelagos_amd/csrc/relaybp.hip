@@ -636,6 +636,15 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
     return __builtin_bit_cast(uint32_t, v);
 }
 
+// max(x, 0) of both bf16 halves of a word (v_pk_max_i16): the sign bit makes
+// every negative bf16 (and -0) a negative 16-bit integer.
+__device__ __forceinline__ uint32_t relu_bf16x2(uint32_t w) {
+    typedef short s16x2_t __attribute__((ext_vector_type(2)));
+    const s16x2_t v = __builtin_bit_cast(s16x2_t, w);
+    const s16x2_t z = {0, 0};
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(v, z));
+}
+
 // One wave: lane l writes its operand words (weights rounded to bf16 once).
 __global__ void __launch_bounds__(64) mlp_frag_kernel(const float *__restrict__ p, uint32_t *__restrict__ frag) {
     const int l = threadIdx.x, row = l & 15, kq = 4 * (l >> 4);
@@ -664,13 +673,10 @@ __global__ void __launch_bounds__(64) mlp_frag_kernel(const float *__restrict__ 
     for (int w = kMlpWords; w < kMlpStride; ++w) put(w, 0u);
 }
 
+// Two words of bf16 pairs as one operand (element 0 in the low half of lo): a
+// reinterpretation, no instructions.
 __device__ __forceinline__ bf16x4_t unpack_bf16x4(uint32_t lo, uint32_t hi) {
-    bf16x4_t x;
-    x[0] = (short)(lo & 0xFFFFu);
-    x[1] = (short)(lo >> 16);
-    x[2] = (short)(hi & 0xFFFFu);
-    x[3] = (short)(hi >> 16);
-    return x;
+    return __builtin_bit_cast(bf16x4_t, make_uint2(lo, hi));
 }
 
 __device__ __forceinline__ void load_mlp(const uint32_t *__restrict__ frag, MlpRegs &r) {
@@ -713,8 +719,9 @@ __device__ __forceinline__ void mlp_logits(const MlpRegs &r, const bf16x4_t (&x)
         for (int g = 0; g < G; ++g) {
             // the accumulator starts at the bias: relu(b1 + W1 x), rounded to bf16
             const f32x4_t d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w1[mb], x[g], r.b1[mb], 0, 0, 0);
-            h1[g][mb] = unpack_bf16x4(pack_bf16x2(fmaxf(d[0], 0.f), fmaxf(d[1], 0.f)),
-                                      pack_bf16x2(fmaxf(d[2], 0.f), fmaxf(d[3], 0.f)));
+            // relu after rounding: a bf16 is negative exactly when its 16-bit pattern
+            // is, so a packed signed max with 0 clamps both halves at once
+            h1[g][mb] = unpack_bf16x4(relu_bf16x2(pack_bf16x2(d[0], d[1])), relu_bf16x2(pack_bf16x2(d[2], d[3])));
         }
     }
     float part[G];
@@ -912,7 +919,10 @@ __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const Shot
                 const float z = q == 0 ? zg[0] : (q == 1 ? zg[1] : (q == 2 ? zg[2] : zg[3]));
                 // shift of the leg's own draw in steps of (hi - lo) / 254: 254 (sigmoid(z) - 1/2)
                 // rounded to the nearest integer (ties to even), in [-127, 127]
-                const float steps = __builtin_rintf(254.f * (1.f / (1.f + expf(-z)) - 0.5f));
+                // (hardware exp2 and reciprocal: within a few ulp of the correctly rounded
+                // sigmoid, which moves a shift by one step only at a rounding tie)
+                const float sig = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-z * 1.4426950408889634f));
+                const float steps = __builtin_rintf(254.f * (sig - 0.5f));
                 if (v < V) *lds_at<int8_t>(a.gamma_lds + (uint32_t)v) = (int8_t)(int)steps;
             }
         }
