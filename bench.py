@@ -268,11 +268,12 @@ def learned_form(H, p, err, syn, syn_np, args, plain_rate, plain_ler) -> dict:
         return {"skipped": f"no trained weights at {wpath.relative_to(ROOT)}"}
     with np.load(wpath, allow_pickle=False) as f:
         legs = int(f["learned_legs"]) if "learned_legs" in f.files else 1
+        damp = float(f["learned_last_damp"]) if "learned_last_damp" in f.files else 0.0
     net = learned.GammaMLP.load(wpath)
     dec = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0], learned_gamma=net,
-                         learned_legs=legs, **RELAY)
+                         learned_legs=legs, learned_last_damp=damp, **RELAY)
     dec_t = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0], learned_gamma=net,
-                           learned_legs=legs, **RELAY)
+                           learned_legs=legs, learned_last_damp=damp, **RELAY)
     dec_t.requeue_below = args.requeue_below
     ref = dec.decode(syn)
     m = measure(dec_t, syn, args.steps, args.warmup, args.depth)
@@ -292,7 +293,7 @@ def learned_form(H, p, err, syn, syn_np, args, plain_rate, plain_ler) -> dict:
     return {"shots_per_s": round(rate, 1), "kernel_ms": round(sum(m["kernel_ms"]) / len(m["kernel_ms"]), 4),
             "ler_65536": big,
             "ler_within_plain_ci": bool(plain_ler[1] <= big["ler"][0] <= plain_ler[2]),
-            "learned_legs": legs, "speedup_vs_plain": round(rate / plain_rate, 3),
+            "learned_legs": legs, "learned_last_damp": damp, "speedup_vs_plain": round(rate / plain_rate, 3),
             "sequential_shots_per_s": round(seq["rate"], 1),
             "sequential_kernel_ms": round(sum(seq["kernel_ms"]) / len(seq["kernel_ms"]), 4),
             "mean_iterations": round(float(out["iterations"].float().mean()), 3),
@@ -385,7 +386,8 @@ def serving_runs(seconds: float, prior_scale: float = 0.003,
             net = net or learned.GammaMLP.load(wpath)
             with np.load(wpath, allow_pickle=False) as f:
                 legs = int(f["learned_legs"]) if "learned_legs" in f.files else 1
-            kw = dict(learned_gamma=net, learned_legs=legs)
+                damp = float(f["learned_last_damp"]) if "learned_last_damp" in f.files else 0.0
+            kw = dict(learned_gamma=net, learned_legs=legs, learned_last_damp=damp)
         decs[k] = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0], **RELAY, **kw)
     pools = {}
     for k, (H, p) in cs.items():

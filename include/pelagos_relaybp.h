@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define PEL_ABI_VERSION 3
+#define PEL_ABI_VERSION 4
 
 /* status codes */
 #define PEL_OK 0
@@ -99,6 +99,8 @@ typedef struct {
                                     legs used (shifted or its uniform row) */
     float learned_lo, learned_hi; /* strength interval: a learned leg uses clamp(u + (hi - lo)
                                     (sigmoid(z) - 1/2), lo, hi) with u that leg's gamma_legs row */
+    float learned_last_damp;     /* in [0, 1]: the last learned leg (learned_legs - 1) takes the shift
+                                    times 1 - learned_last_damp (0: the full shift; 1: its uniform row) */
 } pel_batch;
 
 /* Launch plan for one (graph, B) pair: chosen by pel_plan_decode. */

@@ -13,7 +13,7 @@ from typing import Optional
 
 LIB_PATH = Path(__file__).resolve().parent / "libpelagos_relaybp.so"
 
-PEL_ABI_VERSION = 3
+PEL_ABI_VERSION = 4
 PEL_PLAN_LEARNED = 1
 PEL_MLP_PARAMS = 32 * 8 + 32 + 32 * 32 + 32 + 32 + 1
 PEL_OK = 0
@@ -69,6 +69,7 @@ class PelBatch(ctypes.Structure):
         ("gamma_out", _vp),
         ("learned_lo", ctypes.c_float),
         ("learned_hi", ctypes.c_float),
+        ("learned_last_damp", ctypes.c_float),
     ]
 
 

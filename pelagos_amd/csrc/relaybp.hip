@@ -152,6 +152,7 @@ struct KArgs {
     uint32_t flips_lds, unsat_lds, gamma_lds;  // LDS addresses: flip counts u8 [V+1], violated checks
                                                // u8 [C+1], learned shifts bf16 [V]
     float learned_lo, learned_hi;              // strength interval of the relay legs
+    float last_scale;                          // the shift's factor on the last learned leg
     int8_t *rq_gamma;          // [B][V] relay-leg-0 strength shifts of re-queued shots (by position)
     const uint32_t *mlp_frag;  // [64][kMlpStride] the network as each lane's MFMA operands
 };
@@ -250,6 +251,7 @@ struct GammaSrc {
     uint32_t lds = 0;     // non-zero: learned per-variable shifts (int8 steps of (hi - lo) / 254, by
     float lo = 0.f, hi = 0.f;  // position) at this LDS address, added to the row's strength and
                                // clamped to [lo, hi]
+    float scale = 1.f;         // factor on the shifts (the last learned leg's damping)
     __device__ __forceinline__ float base(int v) const {
         if (!row) return konst;
         return perm ? row[perm[v]] : row[v];
@@ -257,7 +259,7 @@ struct GammaSrc {
     __device__ __forceinline__ float operator()(int v) const {
         const float u = base(v);
         if (!lds) return u;
-        const float d = (float)*lds_at<int8_t>(lds + (uint32_t)v) * ((hi - lo) * (1.f / 254.f));
+        const float d = (float)*lds_at<int8_t>(lds + (uint32_t)v) * ((hi - lo) * (scale / 254.f));
         return fminf(fmaxf(u + d, lo), hi);
     }
 };
@@ -1181,6 +1183,7 @@ __device__ __forceinline__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE>
                     gl.lds = a.gamma_lds;
                     gl.lo = a.learned_lo;
                     gl.hi = a.learned_hi;
+                    gl.scale = leg == a.learned_legs - 1 ? a.last_scale : 1.f;
                 }
                 if (a.gamma_out) record_strengths<BLOCK>(a, b, leg, gl);
                 count = trace && leg + 1 < a.learned_legs;
@@ -2012,6 +2015,8 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     if (b->gamma_out && !b->mlp) return fail(PEL_EINVAL, "pel_decode: gamma_out needs network parameters");
     if (b->mlp && !(b->learned_lo <= b->learned_hi))
         return fail(PEL_EINVAL, "pel_decode: learned_lo must not exceed learned_hi");
+    if (!(b->learned_last_damp >= 0.f && b->learned_last_damp <= 1.f))
+        return fail(PEL_EINVAL, "pel_decode: learned_last_damp must lie in [0, 1]");
     if (b->mlp && (p->skip_relay_legs || p->num_sets < b->learned_legs))
         return fail(PEL_EINVAL, "pel_decode: learned legs need that many relay legs");
     int dev = 0;
@@ -2088,6 +2093,7 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     a.gamma_lds = ll.gamma;
     a.learned_lo = b->learned_lo;
     a.learned_hi = b->learned_hi;
+    a.last_scale = 1.f - b->learned_last_damp;
     a.spill = ws + w.spill;
     a.spill_stride = plan->spill_stride;
     for (int k = 0; k < kNumArrays; ++k) a.off[k] = offs[k];

@@ -121,6 +121,8 @@ class RelayBPDecoder:
     parameter vector) with ``learned_legs = L > 0`` gives relay legs 0..L-1
     per-variable memory strengths from the network, evaluated inside the decode
     kernel; the remaining legs use the uniform draws as before.
+    ``learned_last_damp`` in [0, 1] scales the network's shift on leg L-1 by
+    1 - learned_last_damp (a milder last learned leg).
     """
 
     def __init__(
@@ -157,6 +159,7 @@ class RelayBPDecoder:
         state_placement: Optional[int] = None,
         learned_gamma=None,
         learned_legs: int = 0,
+        learned_last_damp: float = 0.0,
     ):
         # Validation in the reference's order and wording (decoder.py:362-512).
         if unconverged_correction not in UNCONVERGED_CORRECTIONS:
@@ -302,6 +305,9 @@ class RelayBPDecoder:
         self.learned_legs = int(learned_legs)
         if self.learned_legs < 0:
             raise ValueError("learned_legs must be >= 0")
+        self.learned_last_damp = float(learned_last_damp)
+        if not 0.0 <= self.learned_last_damp <= 1.0:
+            raise ValueError("learned_last_damp must lie in [0, 1]")
         self.learned_params: Optional[torch.Tensor] = None
         if learned_gamma is not None:
             prm = getattr(learned_gamma, "params", learned_gamma)
@@ -647,6 +653,7 @@ class RelayBPDecoder:
             batch.gamma_out = _ptr(learn.get("gamma_out"))
             batch.learned_lo = float(self.gamma_dist_interval[0])
             batch.learned_hi = float(self.gamma_dist_interval[1])
+            batch.learned_last_damp = float(learn.get("last_damp", self.learned_last_damp))
         cur = stream if stream is not None else torch.cuda.current_stream(self.device)
         ev = self.kernel_events
         if ev is not None:

@@ -125,8 +125,10 @@ class GammaMLP:
         return losses
 
     # -- persistence ---------------------------------------------------------------------------
-    def save(self, path, learned_legs: Optional[int] = None) -> None:
+    def save(self, path, learned_legs: Optional[int] = None, learned_last_damp: Optional[float] = None) -> None:
         extra = {} if learned_legs is None else {"learned_legs": np.int32(learned_legs)}
+        if learned_last_damp is not None:
+            extra["learned_last_damp"] = np.float32(learned_last_damp)
         np.savez(path, params=self.params.cpu().numpy(), **extra)
 
     @classmethod

@@ -153,6 +153,53 @@ def test_gpu_learned_decode_matches_oracle_given_its_strengths(requeue, placemen
 
 
 @pytest.mark.gpu
+def test_gpu_last_learned_leg_damping():
+    """learned_last_damp scales the shift of the last learned leg only: with
+    damp 1 a fourth learned leg is a uniform leg (the decode equals three
+    learned legs bit for bit); with damp 1/2 the decode equals the oracle fed
+    the strengths it recorded, and leg 3's unclamped strengths sit at half of
+    leg 0's shift from their own draws."""
+    from pelagos_amd import RelayBPDecoder
+
+    def make(**kw):  # five solutions: the pre-iter leg and four relay legs
+        return RelayBPDecoder(H, p, pre_iter=30, num_sets=12, set_max_iter=20, gamma0=0.65,
+                              gamma_dist_interval=(-0.24, 0.66), stop_nconv=5, data_type="float32",
+                              algo="relay", seed=3, device=[0], **kw)
+
+    H, p, err, syn = _banded(96, seed=15)
+    S = torch.from_numpy(syn).cuda()
+    net = learned.GammaMLP(seed=2)
+    print("[gpu] last-leg damping", flush=True)
+    three = make(learned_gamma=net, learned_legs=3).decode(S)
+    full_damp = make(learned_gamma=net, learned_legs=4, learned_last_damp=1.0).decode(S)
+    for k in ("errors", "iterations", "found_count", "weights"):
+        assert torch.equal(three[k], full_damp[k]), k
+    L = 4
+    dec = make(learned_gamma=net, learned_legs=L, learned_last_damp=0.5)
+    dec.record_learned_gammas = True
+    out = dec.decode(S)
+    torch.cuda.synchronize()
+    gl = dec.learned_gammas.cpu().numpy()                      # [B, L, V]
+    B, V = syn.shape[0], H.shape[1]
+    rows = dec.gamma_table(B).cpu().numpy()                    # [R, V]
+    table = np.broadcast_to(rows[:, None, :], (rows.shape[0], B, V)).copy()
+    table[:L] = np.transpose(gl, (1, 0, 2))
+    lam = dec.log_prior_ratios_per_gpu[0].reshape(-1).cpu().numpy()
+    ref = oracle.decode(dec.graph.chk_ptr, dec.graph.edge_var, V, syn, lam, table, pre_iter=30, num_sets=12,
+                        set_max_iter=20, stop_nconv=5, gamma0=0.65, gamma_per_row=True)
+    for k in ("errors", "iterations", "found_count", "weights"):
+        np.testing.assert_array_equal(out[k].cpu().numpy(), ref[k], err_msg=k)
+    lo, hi = np.float32(-0.24), np.float32(0.66)
+    d0 = gl[:, 0] - rows[0][None, :]
+    d3 = gl[:, 3] - rows[3][None, :]
+    ran = (gl[:, 3] != 0).any(1)                # rows of legs a shot never reached stay 0
+    reached = ran & (np.abs(d3) > 0).any(1)      # shots whose leg 3 ran shifted (so did their leg 0)
+    inner = reached[:, None] & (gl[:, 0] > lo) & (gl[:, 0] < hi) & (gl[:, 3] > lo) & (gl[:, 3] < hi)
+    assert inner.sum() > 100
+    np.testing.assert_allclose(d3[inner], 0.5 * d0[inner], atol=2e-6)
+
+
+@pytest.mark.gpu
 def test_gpu_inference_matches_the_network_in_bf16():
     """Strengths of relay leg 0 (features from the pre-iter leg, the same in
     the plain and the learned decode) against the restatement with bf16
@@ -276,3 +323,10 @@ def test_gpu_trained_network_shortens_the_relay_legs():
     wrong_l = int((e != err).any(1).sum())
     wrong_p = int((plain["errors"].cpu().numpy() != err).any(1).sum())
     assert wrong_l <= wrong_p + 1, (wrong_l, wrong_p)
+
+
+def test_last_damp_outside_the_unit_interval_is_rejected():
+    H, p, err, syn = _banded(4, seed=1)
+    for bad in (-0.1, 1.5):
+        with pytest.raises(ValueError):
+            _decoder(H, p, learned_last_damp=bad)
