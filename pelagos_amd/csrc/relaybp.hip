@@ -780,7 +780,10 @@ __device__ __forceinline__ float clamp4(float x) { return fminf(fmaxf(x, -4.f), 
 // the network and shift the leg (else the leg keeps its uniform row).  The flip counts
 // are cleared for the next leg.  Optionally records features and hard
 // decisions in the caller's order.
-constexpr int kFeatBatch = 1;  // slices per wave whose features are gathered together
+// Slices per wave whose features are gathered before their network evaluation.
+// More than one overlaps their loads, but spills the learned kernel's registers
+// (it sits at the 128-VGPR limit of 16 waves per CU), so it stays 1.
+constexpr int kFeatBatch = 1;
 
 // What the features of variable position vc are computed from (see
 // leg_features_t): its state, degree, prior and the smallest of its incoming
@@ -856,11 +859,11 @@ __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const Shot
     const int lane = threadIdx.x & 63, q = lane >> 4;
     const bool net = a.mlp && apply;
     // A wave takes kFeatBatch of its slices at a time (v0 is uniform, so every
-    // lane of a wave runs the same rounds; the MFMAs need all 64 lanes): the
-    // slices' features are gathered together, so their load latencies overlap,
-    // and the network's operands are loaded once per batch.  Lanes past V, and
-    // batch slots past the last slice, compute a copy of variable V - 1 (in the
-    // last slice) and store nothing.
+    // lane of a wave runs the same rounds; the MFMAs need all 64 lanes), and
+    // the network's operands are loaded once per batch (from L1: holding them
+    // across the gathers would spill).  Lanes past V, and batch slots past the
+    // last slice, compute a copy of variable V - 1 (in the last slice) and store
+    // nothing.
     for (int vb = threadIdx.x & ~63; vb < V; vb += kFeatBatch * BLOCK) {
         VarFeatures<PLACE, KNOWN_VALID> vf[kFeatBatch];
 #pragma unroll
