@@ -2,7 +2,7 @@
 plain and with learned strengths, plus serving latency.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--shots 4096] [--depth 3] [--requeue-below 0]
-                    [--no-cpu] [--no-learned] [--no-serving] [--no-batched]
+                    [--no-cpu] [--no-learned] [--no-damped] [--no-serving] [--no-batched]
 
 A step decodes the whole 4096-shot batch (syndromes already resident in HBM)
 into its own output buffers.  The timed steps run with ``--depth`` launches in
@@ -17,7 +17,8 @@ against a plain ``decode()`` of the batch.  Prints ONE JSON line (rank 0)
 with: the throughput of the plain form (uniform-draw legs) and, when trained
 weights are present, of the learned form on the same shots with both error
 rates; ``value`` is the faster form whose synthetic-observable error rate
-lies in the plain form's 95% interval.  Also the decode kernel's roofline
+lies in the plain form's 95% interval.  A learned variant with a fourth
+learned leg at half shift (not the default) is reported beside them.  Also the decode kernel's roofline
 figures (algorithmic edge-message bytes over the per-launch kernel time
 measured with HIP events on each launch's stream, the aggregate rate, and the
 same for launches one at a time), one graph launch over three codes, the
@@ -256,8 +257,10 @@ def same_outputs(slots, ref) -> bool:
     return all(torch.equal(o[k], ref[k]) for o in slots for k in ("errors", "weights", "iterations", "found_count"))
 
 
-def learned_form(H, p, err, syn, syn_np, args, plain_rate, plain_ler) -> dict:
-    """The learned-strength form on the same shots (same steps/warmup)."""
+def learned_form(H, p, err, syn, syn_np, args, plain_rate, plain_ler, legs=None, damp=None) -> dict:
+    """The learned-strength form on the same shots (same steps/warmup).  ``legs``
+    and ``damp`` override the learned-leg count and the last learned leg's
+    damping stored with the weights."""
     import numpy as np
     import torch
 
@@ -267,8 +270,11 @@ def learned_form(H, p, err, syn, syn_np, args, plain_rate, plain_ler) -> dict:
     if not wpath.exists():
         return {"skipped": f"no trained weights at {wpath.relative_to(ROOT)}"}
     with np.load(wpath, allow_pickle=False) as f:
-        legs = int(f["learned_legs"]) if "learned_legs" in f.files else 1
-        damp = float(f["learned_last_damp"]) if "learned_last_damp" in f.files else 0.0
+        stored_legs = int(f["learned_legs"]) if "learned_legs" in f.files else 1
+        stored_damp = float(f["learned_last_damp"]) if "learned_last_damp" in f.files else 0.0
+    legs = stored_legs if legs is None else legs
+    damp = stored_damp if damp is None else damp
+    stored = (legs, damp) == (stored_legs, stored_damp)
     net = learned.GammaMLP.load(wpath)
     dec = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0], learned_gamma=net,
                          learned_legs=legs, learned_last_damp=damp, **RELAY)
@@ -287,7 +293,7 @@ def learned_form(H, p, err, syn, syn_np, args, plain_rate, plain_ler) -> dict:
     rate = m["rate"]
     seq_ms = sum(seq["kernel_ms"]) / len(seq["kernel_ms"])
     roof = roofline(int(out["iterations"].long().sum()), bytes_per_iteration(dec.C, dec.V, dec.E), m["kernel_ms"],
-                    m["elapsed"], args.steps, seq_ms, args.depth, _traffic("learned"),
+                    m["elapsed"], args.steps, seq_ms, args.depth, _traffic("learned") if stored else None,
                     "relay_decode_kernel<1024, 15, true> (learned strengths)")
     big = ler_eval(dec, H, p)
     return {"shots_per_s": round(rate, 1), "kernel_ms": round(sum(m["kernel_ms"]) / len(m["kernel_ms"]), 4),
@@ -418,6 +424,8 @@ def main() -> int:
                     help="re-queue threshold of the timed steps' plan (0: off; the library default is 16)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-strength form")
+    ap.add_argument("--no-damped", action="store_true",
+                    help="skip the learned variant with a fourth, half-shifted learned leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-latency runs")
     ap.add_argument("--no-batched", action="store_true", help="skip the three-code graph launch")
     ap.add_argument("--serving-seconds", type=float, default=10.0,
@@ -547,6 +555,13 @@ def main() -> int:
             result["ms_per_step"] = round(args.shots / lr["shots_per_s"] * 1e3, 3)
             result["roofline"] = lr.pop("roofline")
             result["plain"]["roofline"] = roof_plain
+        if not args.no_damped and "skipped" not in lr:
+            # Not the default: a fourth learned leg at half shift trades a few
+            # failures per 1e5 shots for fewer iterations.
+            dv = learned_form(H, p, err, syn, syn_np, args, shots_per_s, big["ler"], legs=4, damp=0.5)
+            dv.pop("roofline", None)
+            dv["default"] = False
+            result["learned_damped_fourth_leg"] = dv
     if not args.no_batched:
         result["batched"] = batched_graph()
     if not args.no_serving:
