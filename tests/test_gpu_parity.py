@@ -246,6 +246,22 @@ def test_batch_subset_and_single_shot_invariance(regular_case):
         assert torch.equal(full[k][int(rows[0])], one[k][0]), k
 
 
+def test_empty_and_all_zero_syndrome_batches(regular_case):
+    """An empty batch returns empty results without a launch; an all-zero
+    syndrome batch decodes to the zero error at weight 0, as the oracle does."""
+    H, priors, S = regular_case
+    dec = _decoder(H, priors, pre_iter=10, num_sets=20, set_max_iter=20, stop_nconv=2, seed=7)
+    out = _run(dec, S[:0])
+    assert tuple(out["errors"].shape) == (0, dec.V) and out["errors"].dtype == torch.uint8
+    for k in ("weights", "iterations", "found_count", "valid_mask", "unconverged_indices"):
+        assert out[k].shape[0] == 0, k
+    zero = np.zeros((5, dec.C), np.uint8)
+    out = _run(dec, zero)
+    assert not out["errors"].any() and (out["weights"] == 0).all()
+    assert out["valid_mask"].all()
+    _compare(out, _oracle_for(dec, zero))
+
+
 def test_never_converged_counts_every_iteration():
     """A syndrome no error pattern explains never converges and reports
     pre_iter + num_sets * set_max_iter (tests/test_decoder_fixes.py:367-374)."""
