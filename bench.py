@@ -2,7 +2,7 @@
 plain and with learned strengths, plus serving latency.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--shots 4096] [--depth 3] [--requeue-below 0]
-                    [--no-cpu] [--no-learned] [--no-damped] [--no-serving] [--no-batched]
+                    [--no-cpu] [--no-learned] [--no-damped] [--no-serving] [--no-batched] [--no-bb72]
 
 A step decodes the whole 4096-shot batch (syndromes already resident in HBM)
 into its own output buffers.  The timed steps run with ``--depth`` launches in
@@ -21,7 +21,9 @@ lies in the plain form's 95% interval.  A learned variant with a fourth
 learned leg at half shift (not the default) is reported beside them.  Also the decode kernel's roofline
 figures (algorithmic edge-message bytes over the per-launch kernel time
 measured with HIP events on each launch's stream, the aggregate rate, and the
-same for launches one at a time), one graph launch over three codes, the
+same for launches one at a time), the [[72,12,6]] code at code capacity
+(shots/s, LDS rate, per-shot latency, logical error rate), one graph launch
+over three codes, the
 continuous-batching server's p50/p99 latency at five Poisson arrival rates,
 and the CPU oracle timed on the same shots on this host.
 """
@@ -308,6 +310,63 @@ def learned_form(H, p, err, syn, syn_np, args, plain_rate, plain_ler, legs=None,
             "weights": str(wpath.relative_to(ROOT)), "roofline": roof}
 
 
+def code_capacity_bb72(args) -> dict:
+    """The [[72,12,6]] bivariate bicycle code at code capacity: 1024 shots at
+    priors 0.03 (errors from seed 0), the same relay parameters.  Its per-shot
+    state is a few KB, so HBM is not the bound: besides shots/s this reports
+    the LDS bytes per shot-iteration the design moves (20 B per edge slot over
+    both passes, 20 B per variable, 16 B per check; no padding counted) as a
+    rate, the per-shot decode latency from the device clock (one launch, no
+    re-queue), and the logical error rate over the code's 12 logical X
+    operators on 16,384 fresh shots (failure: no solution, or an observable
+    flipped; pelagos_benchmark.py:1666-1669)."""
+    import numpy as np
+    import torch
+
+    from pelagos_amd import RelayBPDecoder, codes
+    from pelagos_amd.observables import ObservableDecoderRunner
+
+    H = codes.bb_72_12_6()
+    L = codes.logical_operators(H.toarray(), codes.bb_72_12_6_hz())
+    p = np.full(H.shape[1], 0.03)
+    C, V, E = int(H.shape[0]), int(H.shape[1]), int(H.nnz)
+    err = codes.sample_errors(p, 1024, seed=0)
+    syn = torch.from_numpy(np.ascontiguousarray(codes.syndromes_of(H, err))).cuda()
+
+    def make():
+        return RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0], **RELAY)
+
+    dec, dec_t, dec_l = make(), make(), make()
+    dec_t.requeue_below = args.requeue_below
+    ref = dec.decode(syn)
+    m = measure(dec_t, syn, args.steps, args.warmup, args.depth)
+    seq = measure(dec, syn, max(3, args.steps // 4), 1, 1)
+    out = m["out"]
+    shot_iters = int(out["iterations"].long().sum())
+    kern = sum(m["kernel_ms"]) / len(m["kernel_ms"])
+    seq_ms = sum(seq["kernel_ms"]) / len(seq["kernel_ms"])
+    lds_b = 20 * E + 20 * V + 16 * C
+    dec_l.requeue_below = 0
+    dec_l.record_shot_clock = True
+    dec_l.decode(syn)
+    torch.cuda.synchronize()
+    clk = dec_l.shot_clock.cpu().numpy()
+    us = (clk[:, 1] - clk[:, 0]) / 100.0  # the device wall clock runs at 100 MHz
+    ler = ObservableDecoderRunner(dec, L).logical_error_rate(codes.sample_errors(p, 16384, seed=1))
+    return {"code": "[[72,12,6]] BB, H_X 36x72, priors 0.03", "shots": 1024,
+            "shots_per_s": round(m["rate"], 1), "kernel_ms": round(kern, 4),
+            "sequential_shots_per_s": round(seq["rate"], 1), "sequential_kernel_ms": round(seq_ms, 4),
+            "mean_iterations": round(shot_iters / 1024, 3),
+            "lds_bytes_per_iteration": lds_b,
+            "lds_gb_per_s": round(lds_b * shot_iters / (kern * 1e-3) / 1e9, 1),
+            "hbm_algorithmic_gb_per_s": round(bytes_per_iteration(C, V, E) * shot_iters / (kern * 1e-3) / 1e9, 1),
+            "shot_latency_us": {"p50": round(float(np.percentile(us, 50)), 2),
+                                "p99": round(float(np.percentile(us, 99)), 2), "max": round(float(us.max()), 2)},
+            "ler": {k: ler[k] for k in ("shots", "failures", "observable_errors", "unconverged",
+                                         "logical_error_rate", "wilson95", "mean_iterations")},
+            "matches_decode": same_outputs(m["slots"] + seq["slots"], ref)}
+
+
 THREE_CODES = {
     # name: (shots, description)
     "bb72": (8192, "[[72,12,6]] BB code, uniform priors"),
@@ -428,6 +487,7 @@ def main() -> int:
                     help="skip the learned variant with a fourth, half-shifted learned leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-latency runs")
     ap.add_argument("--no-batched", action="store_true", help="skip the three-code graph launch")
+    ap.add_argument("--no-bb72", action="store_true", help="skip the [[72,12,6]] code-capacity runs")
     ap.add_argument("--serving-seconds", type=float, default=10.0,
                     help="length of each serving run (one per arrival rate)")
     ap.add_argument("--block", type=int, default=0)
@@ -562,6 +622,8 @@ def main() -> int:
             dv.pop("roofline", None)
             dv["default"] = False
             result["learned_damped_fourth_leg"] = dv
+    if not args.no_bb72:
+        result["code_capacity_bb72"] = code_capacity_bb72(args)
     if not args.no_batched:
         result["batched"] = batched_graph()
     if not args.no_serving:
