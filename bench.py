@@ -176,6 +176,27 @@ def roofline(iters_per_launch: int, b_iter: int, kern_ms, elapsed: float, steps:
             "isolated_achieved": round(isolated, 1), "isolated_frac": round(isolated / HBM_PEAK_GBS, 4)}
 
 
+def copy_bandwidth(mib: int = 1024, reps: int = 10) -> float:
+    """Achievable HBM bandwidth on this GPU: a device-to-device copy of ``mib``
+    MiB, read plus written bytes over its HIP-event time (GB/s)."""
+    import torch
+
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    b.synchronize()
+    gbs = 2.0 * n * reps / (a.elapsed_time(b) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return round(gbs, 1)
+
+
 def _traffic(form: str):
     tfile = ROOT / "profiles" / "relay_decode_banded936x8785_traffic.json"
     try:
@@ -622,6 +643,11 @@ def main() -> int:
             dv.pop("roofline", None)
             dv["default"] = False
             result["learned_damped_fourth_leg"] = dv
+    copy_gbs = copy_bandwidth()
+    for r in (result.get("roofline"), result["plain"].get("roofline"), result.get("learned", {}).get("roofline")):
+        if r:
+            r["measured_copy_gbs"] = copy_gbs
+            r["frac_of_measured_copy"] = round(r["achieved"] / copy_gbs, 4)
     if not args.no_bb72:
         result["code_capacity_bb72"] = code_capacity_bb72(args)
     if not args.no_batched:
