@@ -388,6 +388,61 @@ def code_capacity_bb72(args) -> dict:
             "matches_decode": same_outputs(m["slots"] + seq["slots"], ref)}
 
 
+def ler_study(batches: int) -> int:
+    """Paired logical error rates of the plain form, the shipped learned form
+    and the damped fourth leg (shift x 0.5) on the same ``batches`` x 4096
+    fresh shots of the 936 x 8785 workload: errors drawn on the GPU from the
+    priors (batch k seeded 5000 + k), syndromes and observable flips computed
+    exactly in fp64.  A shot fails when it did not converge or an observable
+    flipped.  Prints one JSON line: per form the failures, their Wilson 95%
+    interval, the mean iterations, and how many failures it shares with the
+    plain form."""
+    import numpy as np
+    import torch
+
+    from pelagos_amd import RelayBPDecoder, codes, learned
+
+    H, p = codes.banded_circuit_code(**WORKLOAD)
+    B, V = 4096, H.shape[1]
+    dev = torch.device("cuda:0")
+    Hd = torch.tensor(H.toarray(), dtype=torch.float64, device=dev)
+    Od = torch.tensor(codes.synthetic_observables(V, seed=0), dtype=torch.float64, device=dev)
+    pt = torch.tensor(p, dtype=torch.float32, device=dev)
+    wpath = learned.default_weights_path()
+    with np.load(wpath, allow_pickle=False) as f:
+        legs = int(f["learned_legs"]) if "learned_legs" in f.files else 1
+        damp = float(f["learned_last_damp"]) if "learned_last_damp" in f.files else 0.0
+    net = learned.GammaMLP.load(wpath)
+    forms = [("plain", {}), ("learned", dict(learned_gamma=net, learned_legs=legs, learned_last_damp=damp)),
+             ("learned_damped_fourth_leg", dict(learned_gamma=net, learned_legs=4, learned_last_damp=0.5))]
+    res, fails = {}, {}
+    for name, kw in forms:
+        dec = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0], **RELAY, **kw)
+        g = torch.Generator(device=dev)
+        bad_all, iters = [], 0
+        for k in range(batches):
+            g.manual_seed(5000 + k)
+            e = (torch.rand(B, V, generator=g, device=dev) < pt).to(torch.uint8)
+            syn = torch.remainder(e.double() @ Hd.T, 2).to(torch.uint8).contiguous()
+            if k == 0 and name == "plain" and not np.array_equal(syn.cpu().numpy(),
+                                                                 codes.syndromes_of(H, e.cpu().numpy())):
+                raise RuntimeError("device syndromes differ from H e mod 2")
+            out = dec.decode(syn)
+            conv = out["found_count"] >= dec.stop_nconv
+            flipped = torch.remainder((out["errors"] ^ e).double() @ Od.T, 2).any(1)
+            bad_all += (torch.nonzero(~conv | flipped).flatten() + k * B).tolist()
+            iters += int(out["iterations"].long().sum())
+        n = batches * B
+        fails[name] = set(bad_all)
+        res[name] = {"shots": n, "failures": len(bad_all), "ler": [round(x, 8) for x in wilson(len(bad_all), n)],
+                     "mean_iterations": round(iters / n, 3)}
+        print(f"[bench] ler study {name}: {len(bad_all)} / {n}", file=sys.stderr, flush=True)
+    for name in res:
+        res[name]["shared_with_plain"] = len(fails[name] & fails["plain"])
+    print(json.dumps({"ler_study": res, "learned_legs": legs, "learned_last_damp": damp}), flush=True)
+    return 0
+
+
 THREE_CODES = {
     # name: (shots, description)
     "bb72": (8192, "[[72,12,6]] BB code, uniform priors"),
@@ -509,6 +564,9 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-latency runs")
     ap.add_argument("--no-batched", action="store_true", help="skip the three-code graph launch")
     ap.add_argument("--no-bb72", action="store_true", help="skip the [[72,12,6]] code-capacity runs")
+    ap.add_argument("--ler-study", type=int, default=0, metavar="BATCHES",
+                    help="only compare the error rates of the plain, learned and damped forms on the same "
+                         "BATCHES x 4096 fresh shots")
     ap.add_argument("--serving-seconds", type=float, default=10.0,
                     help="length of each serving run (one per arrival rate)")
     ap.add_argument("--block", type=int, default=0)
@@ -520,6 +578,8 @@ def main() -> int:
                          "of a 1-step run of this benchmark, then of --copy-probe) into the HBM-traffic "
                          "record read by the roofline object")
     args = ap.parse_args()
+    if args.ler_study:
+        return ler_study(args.ler_study)
     if args.traffic_from:
         return traffic_record(*args.traffic_from)
     if args.copy_probe:
