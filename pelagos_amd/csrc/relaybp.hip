@@ -31,8 +31,9 @@
 //     workgroup barriers.  No cross-lane reductions are needed.
 //   * Rows and variables are renumbered by decreasing degree and stored as
 //     64-wide ELL slices (one slice per wave), so the lanes of a wave do equal
-//     work (row slices are then reordered to balance the SIMDs) and element k of every row of a slice is one coalesced 256-byte
-//     read.  The index entries are the LDS byte addresses of the state they
+//     work (row slices are then reordered to balance the SIMDs, and positions
+//     within a slice chosen to spread LDS gathers over the banks) and element
+//     k of every row of a slice is one coalesced 256-byte read.  The index entries are the LDS byte addresses of the state they
 //     select, so a gather needs no address arithmetic.  Per-variable inputs
 //     and outputs (priors, gammas, error bits) stay in the caller's order and
 //     are mapped through the permutation.  Slots past a row's or a variable's
@@ -91,7 +92,7 @@ struct GraphDev {
     int32_t C, V, E;
     uint32_t sbb, lsb;          // canonical LDS byte offsets of SB and LS
     uint32_t cs_shift;          // log2 of the byte stride between the four CS planes
-    // check side, by row position (slices of rows sorted by decreasing degree)
+    // check side, by row position (slices of rows of similar degree)
     const int32_t *row_orig;    // [C]  original check index
     const int32_t *row_deg;     // [C]
     const int4 *rslice;         // [C/64] {first ELL element, first code word, width (whole chunks), 0}
@@ -1723,6 +1724,103 @@ void bank_spread_vars(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C,
     }
 }
 
+// Lanes of the rows within each 16-lane group.  The variable pass gathers, for
+// the 64 variables of a slice at one slot, each one's check message and code
+// word; both sit at an LDS bank fixed by the row's lane (lane mod 32 over 32
+// banks, lane mod 64 over 64), so two variables of one gather whose rows share
+// a bank collide unless they read the same word.  A row may take any lane of
+// its 16-lane group without changing a check-pass gather group (the rows of
+// one 16-lane group), so the variable positions above stay as good.  Each row
+// in turn, widest first, takes the free lane whose banks the fewest rows it is
+// gathered with already hold; then pairs of a group swap lanes while a swap
+// lowers that count.
+void bank_spread_rows(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, int32_t V,
+                      const std::vector<int32_t> &vposv, std::vector<int32_t> &rorig) {
+    std::vector<std::vector<int32_t>> vrows(V);  // by variable position: its rows in slot order
+    for (int32_t i = 0; i < C; ++i)
+        for (int32_t e = chk_ptr[i]; e < chk_ptr[i + 1]; ++e) vrows[vposv[chk_var[e]]].push_back(i);
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> occ(C);  // row -> (row gathered with it, times)
+    {
+        std::vector<std::vector<int32_t>> cnt(C);
+        std::vector<int32_t> rows;
+        for (int32_t v0 = 0; v0 < V; v0 += kSlice) {
+            const int32_t v1 = std::min(V, v0 + kSlice);
+            size_t w = 0;
+            for (int32_t v = v0; v < v1; ++v) w = std::max(w, vrows[v].size());
+            for (size_t u = 0; u < w; ++u) {
+                rows.clear();
+                for (int32_t v = v0; v < v1; ++v)
+                    if (u < vrows[v].size()) rows.push_back(vrows[v][u]);
+                for (int32_t a : rows)
+                    for (int32_t b : rows)
+                        if (a != b) cnt[a].push_back(b);
+            }
+        }
+        for (int32_t r = 0; r < C; ++r) {
+            std::sort(cnt[r].begin(), cnt[r].end());
+            for (size_t i = 0; i < cnt[r].size();) {
+                size_t j = i;
+                while (j < cnt[r].size() && cnt[r][j] == cnt[r][i]) ++j;
+                occ[r].emplace_back(cnt[r][i], (int32_t)(j - i));
+                i = j;
+            }
+        }
+    }
+    std::vector<int32_t> pos(C, -1);
+    auto cost = [&](int32_t r, int32_t p) {
+        int64_t c = 0;
+        for (const auto &o : occ[r]) {
+            const int32_t q = pos[o.first];
+            if (q < 0) continue;
+            if ((q & 31) == (p & 31)) c += o.second;
+            if ((q & 63) == (p & 63)) c += o.second;
+        }
+        return c;
+    };
+    auto degree = [&](int32_t r) { return chk_ptr[r + 1] - chk_ptr[r]; };
+    for (int32_t g0 = 0; g0 < C; g0 += 16) {
+        const int32_t n = std::min(C - g0, 16);
+        std::vector<int32_t> rows(rorig.begin() + g0, rorig.begin() + g0 + n);
+        std::stable_sort(rows.begin(), rows.end(), [&](int32_t x, int32_t y) { return degree(x) > degree(y); });
+        std::vector<bool> used(n, false);
+        for (int32_t r : rows) {
+            int32_t best = -1;
+            int64_t bc = 0;
+            for (int32_t l = 0; l < n; ++l) {
+                if (used[l]) continue;
+                const int64_t c = cost(r, g0 + l);
+                if (best < 0 || c < bc) best = l, bc = c;
+            }
+            used[best] = true;
+            pos[r] = g0 + best;
+            rorig[g0 + best] = r;
+        }
+    }
+    for (int sweep = 0; sweep < 4; ++sweep) {
+        bool moved = false;
+        for (int32_t g0 = 0; g0 < C; g0 += 16) {
+            const int32_t n = std::min(C - g0, 16);
+            for (int32_t i = 0; i < n; ++i)
+                for (int32_t j = i + 1; j < n; ++j) {
+                    const int32_t a = rorig[g0 + i], b = rorig[g0 + j];
+                    const int64_t before = cost(a, g0 + i) + cost(b, g0 + j);
+                    pos[a] = g0 + j;
+                    pos[b] = g0 + i;
+                    const int64_t after = cost(a, g0 + j) + cost(b, g0 + i);
+                    if (after < before) {
+                        rorig[g0 + i] = b;
+                        rorig[g0 + j] = a;
+                        moved = true;
+                    } else {
+                        pos[a] = g0 + i;
+                        pos[b] = g0 + j;
+                    }
+                }
+        }
+        if (!moved) break;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1771,12 +1869,18 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
         vposv[vperm[v]] = v;
         vdeg[v] = vdeg0[vperm[v]];
     }
+    bank_spread_rows(chk_ptr, chk_var, C, V, vposv, rorig);
+    for (int32_t p = 0; p < C; ++p) {
+        rpos[rorig[p]] = p;
+        rdeg[p] = chk_ptr[rorig[p] + 1] - chk_ptr[rorig[p]];
+    }
     // Row slices: width padded to whole chunks; edge-code words of 16 edges.
     const int32_t nrs = (C + kSlice - 1) / kSlice;
     std::vector<int4> rs(nrs);
     int64_t nidx = 0, nword = 0;
     for (int32_t s = 0; s < nrs; ++s) {
-        const int32_t w = rdeg[s * kSlice];                         // widest row comes first
+        int32_t w = 0;
+        for (int32_t p = s * kSlice; p < std::min(C, (s + 1) * kSlice); ++p) w = std::max(w, rdeg[p]);
         const int32_t wpad = (w + kChunk - 1) / kChunk * kChunk;
         rs[s] = make_int4((int32_t)nidx, (int32_t)nword, wpad, 0);
         nidx += (int64_t)wpad * kSlice;
