@@ -383,8 +383,9 @@ __device__ __forceinline__ void row_info_init(const KArgs &a, const uint8_t *__r
 template <bool FIRST, int PLACE>
 __device__ __forceinline__ uint32_t check_chunk(const KArgs &a, const ShotMem<PLACE> &m, const uint32_t (&e)[kChunk],
                                                 int k0, uint32_t wcur, uint32_t qrow, uint32_t bits, float &m1,
-                                                float &m2, uint32_t &argk, uint64_t &hpar) {
+                                                float &m2, uint32_t &arg_lo, uint32_t &arg_base, uint64_t &hpar) {
     float x[kChunk], y[kChunk], nuo[kChunk];
+    bool any_lt = false;
 #pragma unroll
     for (int u = 0; u < kChunk; ++u) {
         if (FIRST) {
@@ -416,9 +417,11 @@ __device__ __forceinline__ uint32_t check_chunk(const KArgs &a, const ShotMem<PL
         const float am = fabsf(mu);
         const bool lt = am < m1;
         m2 = __builtin_amdgcn_fmed3f(m1, am, m2);
-        argk = lt ? (uint32_t)(k0 + u) : argk;
+        arg_lo = lt ? (uint32_t)u : arg_lo;  // an inline constant: no copy of k0 + u per slot
+        any_lt |= lt;
         m1 = lt ? am : m1;
     }
+    arg_base = any_lt ? (uint32_t)k0 : arg_base;  // the chunk of the last new minimum
     return bits;
 }
 
@@ -452,7 +455,7 @@ __device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m
     const uint32_t qrow = 4u * (uint32_t)pos;  // this row's messages CS[0..3][pos]
     const uint32_t *__restrict__ ent = a.g.row_ent + rs.x;
     float m1 = INFINITY, m2 = INFINITY;
-    uint32_t argk = 0;
+    uint32_t arg_lo = 0, arg_base = 0;  // argmin = arg_base + arg_lo
     uint64_t hpar = 0;  // lane mask: parity of hard decisions so far
     uint32_t neg = 0;
     uint32_t ea[kChunk], eb[kChunk];
@@ -469,14 +472,14 @@ __device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m
 #pragma unroll
             for (int u = 0; u < kChunk; ++u) eb[u] = load_ell(c, lane_off, u);
         }
-        uint32_t bits = check_chunk<FIRST, PLACE>(a, m, ea, k0, wold >> 16, qrow, 0u, m1, m2, argk, hpar);
+        uint32_t bits = check_chunk<FIRST, PLACE>(a, m, ea, k0, wold >> 16, qrow, 0u, m1, m2, arg_lo, arg_base, hpar);
         if (two) {
             if (k0 + 2 * kChunk < rs.z) {
                 const uint32_t *__restrict__ c = ent + (int64_t)(k0 + 2 * kChunk) * kSlice;
 #pragma unroll
                 for (int u = 0; u < kChunk; ++u) ea[u] = load_ell(c, lane_off, u);
             }
-            bits = check_chunk<FIRST, PLACE>(a, m, eb, k0 + kChunk, wold, qrow, bits, m1, m2, argk, hpar);
+            bits = check_chunk<FIRST, PLACE>(a, m, eb, k0 + kChunk, wold, qrow, bits, m1, m2, arg_lo, arg_base, hpar);
         } else {
             bits <<= 16;  // a word's first chunk holds its high half
         }
@@ -486,6 +489,7 @@ __device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m
     }
     if (info & 2u) {
         // Mark the edge holding the minimum (after the sign words above).
+        const uint32_t argk = arg_base + arg_lo;
         const uint32_t aw = wbase + (argk >> 4) * kSlice;
         m.sb_set(aw, m.sb_get(aw) | (1u << (30u - 2u * (argk & 15u))));
         float o1 = m1, o2 = m2;
