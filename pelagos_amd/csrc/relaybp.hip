@@ -144,6 +144,9 @@ struct KArgs {
     uint32_t rtab, vtab;       // LDS byte addresses of the row and variable slice descriptor tables
     uint32_t rowinfo;          // LDS byte address of the shot's row flags (u8 by row position: bit 0 the
                                // syndrome, bit 1 degree > 0, bit 2 degree == 1)
+    uint32_t best_lds;         // LDS byte address of the shot's output bits (one per variable position,
+                               // 64 positions per 8-byte word): its best solution so far, or the
+                               // hard decision the unconverged correction falls back to
     // learned memory strengths (plans made with PEL_PLAN_LEARNED)
     const float *mlp;          // network parameters, or null (then features are only recorded)
     int32_t learned_legs;      // relay legs 0 .. learned_legs-1 get per-variable strengths
@@ -962,6 +965,40 @@ __device__ __forceinline__ void record_strengths(const KArgs &a, int b, int r, c
 
 constexpr int kBatchU = 4;  // positions per thread handled together by the per-shot loops
 
+// The shot's output bits from its current state: bit v = (post_v <= 0), or 0
+// everywhere when no iteration has run yet (the reference's hard_dec starts at
+// zero).  A wave packs the 64 positions of one output word with a ballot.
+// Readers of the words are separated from this by a workgroup barrier.
+template <int BLOCK, int PLACE>
+__device__ __forceinline__ void best_from_state(const KArgs &a, const ShotMem<PLACE> &m, bool any_iter) {
+    const int V = a.g.V;
+    for (int vb = (int)(threadIdx.x & ~63u); vb < V; vb += BLOCK) {
+        const float2 ls = m.ls_pos(min(vb + (int)(threadIdx.x & 63u), V));  // position V: the sentinel, +inf
+        const uint64_t bits = __ballot(any_iter && ls.x + ls.y <= 0.f);
+        if ((opaque_tid() & 63u) == 0) *lds_at<uint64_t>(a.best_lds + 8u * (uint32_t)(vb >> 6)) = bits;
+    }
+}
+
+// The shot's error row, in the caller's order, from its output bits: the only
+// write of the row, made once by the workgroup that completes the shot.
+template <int BLOCK>
+__device__ __forceinline__ void write_errors(const KArgs &a, int b) {
+    const int V = a.g.V;
+    uint8_t *err = a.errors + (size_t)b * V;
+    __syncthreads();  // every wave's output words are in LDS
+    for (int j0 = threadIdx.x; j0 < V; j0 += kBatchU * BLOCK) {
+        int v[kBatchU];
+        uint32_t w[kBatchU];
+#pragma unroll
+        for (int u = 0; u < kBatchU; ++u) v[u] = a.g.vpos[min(j0 + u * BLOCK, V - 1)];
+#pragma unroll
+        for (int u = 0; u < kBatchU; ++u) w[u] = *lds_at<uint32_t>(a.best_lds + 4u * (uint32_t)(v[u] >> 5));
+#pragma unroll
+        for (int u = 0; u < kBatchU; ++u)
+            if (j0 + u * BLOCK < V) err[j0 + u * BLOCK] = (uint8_t)((w[u] >> (v[u] & 31)) & 1u);
+    }
+}
+
 // f(j, state) for every caller-order index j this thread owns, with the state
 // of its position; kBatchU at a time, so that their index loads and state
 // reads are in flight together.
@@ -983,10 +1020,11 @@ __device__ __forceinline__ void for_each_caller_index(const KArgs &a, const Shot
 
 // check_and_select_kernel (kernels.py:657-741) for a shot whose hard decision
 // satisfies every check: weight = sum of wj over bits with post <= 0 (double
-// accumulation, rounded once), strict improvement or first solution wins.
+// accumulation, rounded once), strict improvement or first solution wins.  The
+// winner goes to the shot's output bits (best_errors, kernels.py:722-734).
 template <int BLOCK, int PLACE>
 __device__ __forceinline__ void select_solution(const KArgs &a, const ShotMem<PLACE> &m, const float *__restrict__ wj,
-                                uint8_t *err, ShotState &st, double *s_red) {
+                                ShotState &st, double *s_red) {
     const int V = a.g.V;
     double acc = 0.0;
     // Position order: contiguous state reads.  The double sum of n fp32
@@ -1016,7 +1054,7 @@ __device__ __forceinline__ void select_solution(const KArgs &a, const ShotMem<PL
     const float w = (float)block_sum<BLOCK>(acc, s_red);
     if (st.best_w >= 1e20f || w < st.best_w) {
         st.best_w = w;
-        for_each_caller_index<BLOCK, PLACE>(a, m, [&](int j, float2 ls) { err[j] = (ls.x + ls.y <= 0.f) ? 1 : 0; });
+        best_from_state<BLOCK, PLACE>(a, m, true);
     }
     st.found += 1;
 }
@@ -1027,7 +1065,7 @@ __device__ __forceinline__ void select_solution(const KArgs &a, const ShotMem<PL
 // strengths; only legs whose end state feeds the network count them).
 template <int BLOCK, int PLACE, bool FLIPS = false>
 __device__ __forceinline__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m, int T, const GammaSrc &gs,
-                        const float *__restrict__ wj, const uint8_t *__restrict__ syn, uint8_t *err,
+                        const float *__restrict__ wj, const uint8_t *__restrict__ syn,
                         ShotState &st, double *s_red, int *s_flag) {
     const int C = a.g.C, V = a.g.V;
     for (int t = 0; t < T; ++t) {
@@ -1055,7 +1093,7 @@ __device__ __forceinline__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m,
             st.first_ok = 1;
         }
         if (t > 0 && !bad) {  // iteration t-1 converged
-            select_solution<BLOCK, PLACE>(a, m, wj, err, st, s_red);
+            select_solution<BLOCK, PLACE>(a, m, wj, st, s_red);
             return true;
         }
         for (int v = threadIdx.x; v < V; v += BLOCK) var_update<PLACE, FLIPS>(a, m, v, gs);
@@ -1068,32 +1106,23 @@ __device__ __forceinline__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m,
         for (int i = threadIdx.x; i < C; i += BLOCK) bad |= parity_row<PLACE>(a, m, i, syn);
         bad = block_or<BLOCK>(bad, s_flag);
         if (!bad) {
-            select_solution<BLOCK, PLACE>(a, m, wj, err, st, s_red);
+            select_solution<BLOCK, PLACE>(a, m, wj, st, s_red);
             return true;
         }
     }
     return false;
 }
 
-template <int BLOCK, int PLACE>
-__device__ __forceinline__ void write_hard(const KArgs &a, const ShotMem<PLACE> &m, uint8_t *err, bool any_iter) {
-    if (any_iter)
-        for_each_caller_index<BLOCK, PLACE>(a, m, [&](int j, float2 ls) { err[j] = (ls.x + ls.y <= 0.f) ? 1 : 0; });
-    else
-        for (int j = threadIdx.x; j < a.g.V; j += BLOCK) err[j] = 0;
-}
-
-// Per-shot slices of the batch arrays.
+// Per-shot slices of the batch inputs.
 struct ShotIO {
     const uint8_t *syn;
     const float *l0;
     const float *wj;
-    uint8_t *err;
 };
 
 __device__ __forceinline__ ShotIO shot_io(const KArgs &a, int b) {
     const size_t pofs = a.prior_batched ? (size_t)b * a.g.V : 0;
-    return ShotIO{a.syn + (size_t)b * a.g.C, a.prior + pofs, a.wj + pofs, a.errors + (size_t)b * a.g.V};
+    return ShotIO{a.syn + (size_t)b * a.g.C, a.prior + pofs, a.wj + pofs};
 }
 
 // Stage 1 of a shot: state reset (decoder.py:3430-3464) and the pre-iter leg
@@ -1121,7 +1150,6 @@ __device__ __forceinline__ bool first_leg(const KArgs &a, int b, ShotMem<PLACE> 
             m.ls_set(v[u], make_float2(p0[u], 0.f));
             if (PLACE & (1 << kL0)) m.l0_set(v[u], lj[u]);
             if (LEARN) *lds_at<uint8_t>(a.flips_lds + (uint32_t)v[u]) = 0;
-            io.err[j0 + u * BLOCK] = 0;
         }
     }
     row_info_init<BLOCK>(a, io.syn);
@@ -1137,12 +1165,12 @@ __device__ __forceinline__ bool first_leg(const KArgs &a, int b, ShotMem<PLACE> 
     if (st.found < a.stop_nconv) {
         // the network reads the flips of this leg
         if (LEARN && a.learned_legs > 0)
-            st.last_ok = run_leg<BLOCK, PLACE, true>(a, m, a.pre_iter, g0, io.wj, io.syn, io.err, st, s_red, s_flag);
+            st.last_ok = run_leg<BLOCK, PLACE, true>(a, m, a.pre_iter, g0, io.wj, io.syn, st, s_red, s_flag);
         else
-            st.last_ok = run_leg<BLOCK, PLACE, false>(a, m, a.pre_iter, g0, io.wj, io.syn, io.err, st, s_red, s_flag);
+            st.last_ok = run_leg<BLOCK, PLACE, false>(a, m, a.pre_iter, g0, io.wj, io.syn, st, s_red, s_flag);
     }
-    // first_leg_hard_dec (decoder.py:3597-3598): kept in err while no solution exists.
-    if (!a.last_leg && st.found == 0) write_hard<BLOCK, PLACE>(a, m, io.err, st.any_iter);
+    // first_leg_hard_dec (decoder.py:3597-3598): the output bits while no solution exists.
+    if (!a.last_leg && st.found == 0) best_from_state<BLOCK, PLACE>(a, m, st.any_iter != 0);
     return !a.skip_relay_legs && a.num_sets > 0 && st.found < a.stop_nconv;
 }
 
@@ -1197,17 +1225,16 @@ __device__ __forceinline__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE>
                 count = trace && leg + 1 < a.learned_legs;
             }
             if (count)
-                st.last_ok = run_leg<BLOCK, PLACE, true>(a, m, a.set_max_iter, gl, io.wj, io.syn, io.err, st, s_red,
-                                                         s_flag);
+                st.last_ok = run_leg<BLOCK, PLACE, true>(a, m, a.set_max_iter, gl, io.wj, io.syn, st, s_red, s_flag);
             else
-                st.last_ok = run_leg<BLOCK, PLACE, false>(a, m, a.set_max_iter, gl, io.wj, io.syn, io.err, st, s_red,
-                                                          s_flag);
+                st.last_ok = run_leg<BLOCK, PLACE, false>(a, m, a.set_max_iter, gl, io.wj, io.syn, st, s_red, s_flag);
             if (leg == 0 && a.pre_iter == 0 && !a.last_leg && st.found == 0)
-                write_hard<BLOCK, PLACE>(a, m, io.err, st.any_iter);  // decoder.py:3809-3810
+                best_from_state<BLOCK, PLACE>(a, m, st.any_iter != 0);  // decoder.py:3809-3810
         }
     }
     // last_leg correction: the live hard decision for shots with no solution.
-    if (a.last_leg && st.found == 0) write_hard<BLOCK, PLACE>(a, m, io.err, st.any_iter);
+    if (a.last_leg && st.found == 0) best_from_state<BLOCK, PLACE>(a, m, st.any_iter != 0);
+    write_errors<BLOCK>(a, b);
     if (a.last_post) {
         float *lp = a.last_post + (size_t)b * V;
         for_each_caller_index<BLOCK, PLACE>(a, m, [&](int j, float2 ls) { lp[j] = ls.x + ls.y; });
@@ -1218,11 +1245,14 @@ __device__ __forceinline__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE>
 // the messages are about to be reset (kernels.py:935-975), so the posterior and
 // the leg bookkeeping are the whole state: the posterior goes to rq_post, the
 // scalars to rq_state, and the shot is published in the next rq_slot.  The
-// resuming workgroup may run on another XCD, so the stores must be visible
-// device-wide before the leader publishes: the barrier orders every thread's
-// stores before the leader's agent-scope release fence, which is cumulative,
-// so one fence (one L2 write-back) covers the whole workgroup.  A fence in
-// every thread would write back and invalidate the XCD's L2 once per wave.
+// parking workgroup writes nothing else of the shot (its error row and
+// counters are written only by the workgroup that completes it; the first
+// solution is the sign of the parked posterior).  The resuming workgroup may
+// run on another XCD, so the stores must be visible device-wide before the
+// leader publishes: each wave waits until its own stores have reached the L2,
+// the barrier orders that before the leader's agent-scope release fence, and
+// that one fence (one L2 write-back) then covers the whole workgroup.  A fence
+// in every thread would write back and invalidate the XCD's L2 once per wave.
 template <int BLOCK, int PLACE, bool LEARN>
 __device__ __forceinline__ void requeue_shot(const KArgs &a, int b, const ShotMem<PLACE> &m, const ShotState &st) {
     const int V = a.g.V;
@@ -1232,6 +1262,7 @@ __device__ __forceinline__ void requeue_shot(const KArgs &a, int b, const ShotMe
         post[v] = ls.x + ls.y;
         if (LEARN && a.mlp) a.rq_gamma[(size_t)b * V + v] = *lds_at<int8_t>(a.gamma_lds + (uint32_t)v);
     }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (opaque_tid() == 0) {
         __threadfence();
@@ -1242,7 +1273,10 @@ __device__ __forceinline__ void requeue_shot(const KArgs &a, int b, const ShotMe
     }
 }
 
-// Resume a re-queued shot: posterior back as lam_mixed = post, S = 0.
+// Resume a re-queued shot: posterior back as lam_mixed = post, S = 0, and its
+// output bits from the posterior's signs (the first leg's solution, its only one
+// so far).  The leader has acquired the parked state (wait_requeued) before the
+// workgroup barrier that precedes every read of it here.
 template <int BLOCK, int PLACE, bool LEARN>
 __device__ __forceinline__ void resume_shot(const KArgs &a, int b, ShotMem<PLACE> &m) {
     const int V = a.g.V;
@@ -1273,6 +1307,7 @@ __device__ __forceinline__ void resume_shot(const KArgs &a, int b, ShotMem<PLACE
     row_info_init<BLOCK>(a, io.syn);
     m.l0g = io.l0;
     __syncthreads();
+    best_from_state<BLOCK, PLACE>(a, m, true);
 }
 
 // Leader only: the shot behind re-queue ticket t2, or B when no shot will ever
@@ -1586,9 +1621,11 @@ void place_arrays(const StateSizes &st, int place, int block, bool learn, uint32
         }
     }
     // scratch: s_red doubles, s_flag ints, s_next[2]; then the slice descriptor
-    // tables: rows {ent, word, width, 0}, variables {ent, width}
+    // tables: rows {ent, word, width, 0}, variables {ent, width}; the row flags
+    // (u8 per row); the output bits (8 bytes per 64 variable positions)
     *scratch = (uint32_t)*lds;
-    *lds += align_up(12LL * (block / 64) + 8, 16) + 16LL * st.nrs + align_up(8LL * st.nvs, 16) + align_up(st.C, 16);
+    *lds += align_up(12LL * (block / 64) + 8, 16) + 16LL * st.nrs + align_up(8LL * st.nvs, 16) + align_up(st.C, 16) +
+            align_up(8LL * st.nvs, 16);
     LearnLds l{0, 0, 0};
     if (learn) {  // flip counts u8 [V+1], violated checks u8 [C+1], strength shifts i8 [V]
         l.flips = (uint32_t)*lds;
@@ -2212,6 +2249,7 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     a.rtab = row_table_addr(scratch, plan->block);
     a.vtab = a.rtab + (uint32_t)(16LL * st.nrs);
     a.rowinfo = a.vtab + (uint32_t)align_up(8LL * st.nvs, 16);
+    a.best_lds = a.rowinfo + (uint32_t)align_up(st.C, 16);
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int key_blocks = (int)((b->B + kOrderBlock / 64 - 1) / (kOrderBlock / 64));
     hipLaunchKernelGGL(syndrome_key_kernel, dim3(key_blocks), dim3(kOrderBlock), 0, s, b->syndromes, b->B, g->C,

@@ -24,9 +24,6 @@ def _codes():
     return {"bb72": (codes.bb_72_12_6(), np.full(72, 0.03)), "banded4": (Hs, ps), "banded6": (Hm, pm)}
 
 
-@pytest.mark.xfail(strict=False, reason="one mismatch seen once: the first replay's [[72,12,6]] error rows "
-                   "differed from an eager decode of the same syndromes, while the same library had passed this "
-                   "test before; cause not yet found, so the test reports without stopping the suite")
 def test_graph_replays_match_eager_decodes():
     from pelagos_amd import RelayBPDecoder
     from pelagos_amd.batching import MultiCodeBatch
