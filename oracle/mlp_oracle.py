@@ -7,12 +7,14 @@ decode kernel.  The reference has no learned component; this restates the
 network defined in include/pelagos_relaybp.h:
 
     h1 = relu(W1 x + b1), h2 = relu(W2 h1 + b2), z = w3 . h2 + b3,
-    gamma = -0.24 + 0.9 * sigmoid(z),
+    shift = (hi - lo) * (sigmoid(z) - 1/2)   (added to a learned leg's uniform
+                                             draw, then clamped to [lo, hi]),
     loss  = mean_i w_i * (softplus(z_i) - y_i z_i).
 
 Everything runs in float64 so the GPU's fp32 sums can be compared against it
 with a stated tolerance.  Its gradient is pinned by central finite
-differences in tests/test_learned_gamma.py.
+differences in tests/test_learned_gamma.py, and ``shift_q8`` (the kernel's
+bf16 inference) by ``forward`` on a network whose every rounding is exact.
 """
 
 from __future__ import annotations
@@ -23,7 +25,6 @@ import numpy as np
 
 F, H = 8, 32
 P = H * F + H + H * H + H + H + 1
-GAMMA_LO, GAMMA_SPAN = -0.24, 0.9
 
 
 def unpack(p: np.ndarray) -> Dict[str, np.ndarray]:
@@ -49,9 +50,10 @@ def forward(p: np.ndarray, x: np.ndarray) -> Tuple[np.ndarray, Dict[str, np.ndar
     return z, dict(x=x, a1=a1, h1=h1, a2=a2, h2=h2)
 
 
-def gamma(p: np.ndarray, x: np.ndarray) -> np.ndarray:
+def shift(p: np.ndarray, x: np.ndarray, lo: float = -0.24, hi: float = 0.66) -> np.ndarray:
+    """(hi - lo) (sigmoid(z) - 1/2) in float64."""
     z, _ = forward(p, x)
-    return GAMMA_LO + GAMMA_SPAN / (1.0 + np.exp(-z))
+    return (hi - lo) * (1.0 / (1.0 + np.exp(-z)) - 0.5)
 
 
 def loss(p: np.ndarray, x: np.ndarray, y: np.ndarray, w: np.ndarray) -> float:

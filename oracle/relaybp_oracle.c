@@ -36,7 +36,8 @@
  *     rounded once to float (exact, hence order-free, for these inputs).
  *
  * Parity status: pinned by tests/test_oracle_pins.py (closed forms, tree-code
- * exactness, brute-force ML, schedule invariants, planted-bug mutants).
+ * exactness, brute-force ML, schedule invariants, relay-leg schedule closed
+ * forms, planted-bug mutants 1-9).
  */
 #include <math.h>
 #include <stdint.h>
@@ -224,7 +225,13 @@ static int run_leg(const orc_graph *g, const orc_params *p, const uint8_t *syn,
             for (int32_t j = 0; j < g->V; ++j)
                 if (L->post[j] <= 0.0f) acc += (double)wj[j];
             float w = (float)acc;
+#if ORC_PLANT == 7
+            if (st->best_w >= 1e20f) {                  /* planted: the first solution is never replaced */
+#elif ORC_PLANT == 9
+            if (st->best_w >= 1e20f || w <= st->best_w) { /* planted: a tie replaces the best */
+#else
             if (st->best_w >= 1e20f || w < st->best_w) {
+#endif
                 st->best_w = w;
                 for (int32_t j = 0; j < g->V; ++j) L->best[j] = L->post[j] <= 0.0f;
             }
@@ -266,9 +273,18 @@ static void decode_shot(const orc_graph *g, const orc_params *p, const orc_batch
     if (!p->skip_relay_legs) {
         for (int32_t leg = 0; leg < p->num_sets; ++leg) { /* decoder.py:3672 */
             if (st.found >= S) break;                     /* lane stays frozen */
+            int32_t row = leg;
+#if ORC_PLANT == 8
+            row = (leg + 1) % p->num_sets;                /* planted: leg l reads the next leg's row */
+#endif
             L->gamma = bt->gamma_legs +
-                       (bt->gamma_per_row ? ((size_t)leg * bt->B + b) * V : (size_t)leg * V);
+                       (bt->gamma_per_row ? ((size_t)row * bt->B + b) * V : (size_t)row * V);
+#if ORC_PLANT != 5
             init_messages(g, lam0, L);                    /* decoder.py:3729-3739 */
+#endif                                                    /* (planted 5: messages carried over) */
+#if ORC_PLANT == 6
+            for (int32_t j = 0; j < V; ++j) L->post[j] = lam0[j];  /* planted: posterior reset */
+#endif
             run_leg(g, p, syn, lam0, wj_buf, p->set_max_iter, L, &st);
             if (leg == 0 && p->pre_iter == 0)             /* decoder.py:3809-3810 */
                 memcpy(L->first_hd, L->hard, (size_t)V);

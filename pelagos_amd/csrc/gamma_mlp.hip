@@ -1,15 +1,15 @@
 // gamma_mlp.hip -- training kernels of the learned memory-strength proposal.
 //
-// The network maps 8 per-variable features (taken at the end of a leg) to the
-// memory strength of that variable in the next relay leg:
+// The network maps 8 per-variable features (taken at the end of a leg) to a
+// shift of that variable's memory strength in the next relay legs:
 //     h1 = relu(W1 x + b1)      W1 [32][8]
 //     h2 = relu(W2 h1 + b2)     W2 [32][32]
 //     z  = w3 . h2 + b3         w3 [32]
-//     gamma = -0.24 + 0.9 * sigmoid(z)     (strictly inside (-0.24, 0.66))
-// Parameters are one fp32 vector in that order (kMlpParams floats).  Training
-// fits sigmoid(z) to "this variable's hard decision is already correct" with
-// a weighted logistic loss, so confident-correct variables keep strong memory
-// and suspect ones get disordered (negative) memory.
+//     shift = (hi - lo) (sigmoid(z) - 1/2), gamma = clamp(u + shift, lo, hi)
+// with u the leg's own uniform draw on [lo, hi].  Parameters are one fp32
+// vector in that order (kMlpParams floats).  Training fits sigmoid(z) to soft
+// targets in [0, 1] with a weighted logistic loss (the targets and weights
+// come from reward-weighted regression over candidate shifts, in learned.py).
 //
 // Kernels: a fused forward/backward per sample whose parameter gradients are
 // summed in a fixed order (per-block partial sums, then a fixed-order sum over

@@ -28,6 +28,7 @@ from .utils import (
     UNCONVERGED_CORRECTIONS,
     bitpack_errors,
     compute_log_prior_ratios,
+    explicit_leg_rows,
     sample_gamma_per_variable,
     sample_gamma_uniform,
     validate_csr_matrix,
@@ -369,7 +370,7 @@ class RelayBPDecoder:
             table = torch.zeros(1, V, dtype=self.dtype, device=dev)
         elif self._explicit_gammas is not None:
             K = int(self._explicit_gammas.shape[0])
-            idx = torch.tensor([(l + 1) % K for l in range(R)], device=dev)
+            idx = torch.tensor(explicit_leg_rows(K, R), device=dev)
             table = self._explicit_gammas.index_select(0, idx).contiguous()
         else:
             rows = B if per_row else 1

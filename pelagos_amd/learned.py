@@ -6,12 +6,18 @@ memory strength of every variable at the start of each of the first
 in.  The network does not replace a leg's random draw, whose disorder is what
 lets successive legs explore differently (a fixed strength per variable
 repeats a failed leg forever); it shifts each variable's draw by
-(hi - lo) (sigmoid(z) - 1/2).  It is trained on traces of the plain decoder
-(uniform-draw legs) to predict, from the features, the mean draw of the legs
-that converged, so it moves each variable's strengths towards the values that
-let legs converge.  Training runs on the GPU through the library's own
-forward/backward and Adam kernels (``pel_mlp_grad``, ``pel_mlp_adam``); at
-decode time the network runs inside the decode kernel on the matrix cores.
+(hi - lo) (sigmoid(z) - 1/2).  It is trained by reward-weighted regression
+(``rwr_training_set``): the training shots are decoded once per candidate
+constant shift, and each shot whose pre-iter leg ended at a valid solution
+contributes its variables' features with the candidate as target, weighted by
+how short its learned legs were when it still decoded to the true error.  The
+fitted network so proposes, per state, the shift under which the relay legs
+converge fastest without changing the decoding.  (``training_set``, the
+earlier mean-draw method, regresses onto the mean uniform draw of legs that
+converged; it is kept for comparison.)  Training runs on the GPU through the
+library's own forward/backward and Adam kernels (``pel_mlp_grad``,
+``pel_mlp_adam``); at decode time the network runs inside the decode kernel on
+the matrix cores.
 
 Uniform-draw legs remain the fallback (every leg after the learned ones), and
 the plain decoder's acceptance rules judge every shot: a solution counts only
@@ -35,7 +41,6 @@ from . import _native
 N_FEATURES = 8
 HIDDEN = 32
 N_PARAMS = _native.PEL_MLP_PARAMS
-GAMMA_LO, GAMMA_SPAN = -0.24, 0.9
 FEATURE_NAMES = ("prior_llr/8", "posterior/16", "flips/8", "mean_incoming/8", "min_incoming/8",
                  "column_weight/8", "violated_check_fraction", "leg/4")
 WEIGHTS_DIR = Path(__file__).resolve().parent / "data"
@@ -78,8 +83,11 @@ class GammaMLP:
                                                     z.data_ptr(), self._stream()), "pel_mlp_forward", mlp=True)
         return z
 
-    def gamma(self, x: torch.Tensor) -> torch.Tensor:
-        return GAMMA_LO + GAMMA_SPAN * torch.sigmoid(self.logits(x))
+    def shift(self, x: torch.Tensor, lo: float = -0.24, hi: float = 0.66) -> torch.Tensor:
+        """The shift (hi - lo) (sigmoid(z) - 1/2) a learned leg adds to each
+        variable's uniform draw (in fp32; the decode kernel's inference rounds
+        operands to bf16 and the shift to a multiple of (hi - lo) / 254)."""
+        return (hi - lo) * (torch.sigmoid(self.logits(x)) - 0.5)
 
     def grad(self, x: torch.Tensor, y: torch.Tensor, w: torch.Tensor):
         """(gradient of the mean weighted logistic loss, loss) from pel_mlp_grad."""

@@ -85,6 +85,15 @@ def sample_gamma_per_variable(rows: int, V: int, gamma_intervals: torch.Tensor, 
     return low + (high - low) * uniform
 
 
+def explicit_leg_rows(K: int, R: int) -> list:
+    """Row of an explicit [K, V] gamma table that relay leg l = 0..R-1 uses:
+    (l + 1) % K, the Rust decoder's rotation the reference reproduces
+    (decoder.py:3685-3688)."""
+    if K <= 0:
+        raise ValueError("explicit_gammas must have at least one row")
+    return [(leg + 1) % K for leg in range(R)]
+
+
 def bitpack_errors(errors: torch.Tensor, bits_per_word: int = 32) -> torch.Tensor:
     """[B, V] 0/1 -> [B, ceil(V/32)] int32, bit i of word w = variable 32*w + i
     (utils.py:32-60)."""

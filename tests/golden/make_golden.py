@@ -26,6 +26,7 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parents[2]
 sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
 
 OUT = Path(__file__).resolve().parent
 RELAY = dict(pre_iter=80, num_sets=100, set_max_iter=60, stop_nconv=5, gamma0=0.65)
@@ -60,10 +61,10 @@ def check_update_vectors() -> dict:
 def chain_ml() -> dict:
     """Repetition (chain) code of 9 bits, uniform prior 0.1: for every syndrome,
     the weight of the most likely error pattern, by enumeration of all 512."""
-    from pelagos_amd import codes
+    import toy_codes
 
     n = 9
-    H = codes.repetition_code(n).toarray().astype(np.int64)
+    H = toy_codes.repetition_code(n).toarray().astype(np.int64)
     pats = np.array(list(itertools.product([0, 1], repeat=n)), np.int64)
     syn_of = (pats @ H.T) % 2
     llr = np.float32(np.log(0.9 / 0.1))

@@ -15,6 +15,7 @@ import pytest
 import torch
 
 import oracle
+import toy_codes
 from pelagos_amd import RelayBPDecoder, codes, utils
 
 pytestmark = pytest.mark.gpu
@@ -123,7 +124,7 @@ def test_requeue_threshold_does_not_change_results(requeue_below):
 def regular_case():
     """(3,6)-regular code, non-uniform priors, 128 syndromes of increasing difficulty
     (tests/test_decoder_fixes.py:89-97)."""
-    H = codes.regular_code(3)
+    H = toy_codes.regular_code(3)
     V = H.shape[1]
     priors = np.random.default_rng(4).uniform(0.01, 0.03, V)
     scale = np.repeat([0.5, 1.0, 1.5, 3.0], 32)[:, None]
@@ -200,10 +201,10 @@ def test_toy_codes_match_oracle():
     from scipy.sparse import csr_matrix
 
     cases = [
-        (codes.triangle_code(), 0.01),
-        (codes.high_degree_code(), 0.01),
+        (toy_codes.triangle_code(), 0.01),
+        (toy_codes.high_degree_code(), 0.01),
         (csr_matrix(dense), 0.02),
-        (codes.repetition_code(9), 0.1),
+        (toy_codes.repetition_code(9), 0.1),
     ]
     for H, pval in cases:
         V = H.shape[1]

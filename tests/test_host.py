@@ -13,6 +13,7 @@ import pytest
 import torch
 from scipy.sparse import csr_matrix
 
+import toy_codes
 from pelagos_amd import _native, codes, utils
 from pelagos_amd.decoder import alpha_schedule
 from pelagos_amd.graph import CSRGraph
@@ -85,7 +86,7 @@ def test_validate_csr_matrix():
         utils.validate_csr_matrix(csr_matrix((0, 3)))
     with pytest.raises(ValueError, match="zero non-zero"):
         utils.validate_csr_matrix(csr_matrix((2, 3)))
-    assert utils.validate_csr_matrix(codes.triangle_code().tocoo())
+    assert utils.validate_csr_matrix(toy_codes.triangle_code().tocoo())
 
 
 # --- bit packing (tests/test_cpu_units.py:93-99) ----------------------------------------------------
@@ -130,6 +131,17 @@ def test_per_variable_gamma_respects_intervals():
     assert (g >= iv[:, 0]).all() and (g <= iv[:, 1]).all()
     with pytest.raises(ValueError, match="gamma_intervals"):
         utils.sample_gamma_per_variable(1, V + 1, iv, "cpu", seed=3)
+
+
+def test_explicit_gamma_rows_rotate_by_one():
+    """The reference's known answer (tests/test_decoder_fixes.py:400-429): relay
+    leg L starts with explicit_gammas[(L + 1) % K]; for K = 3 over 7 legs that is
+    rows 1, 2, 0, 1, 2, 0, 1, and a single row serves every leg."""
+    assert utils.explicit_leg_rows(3, 7) == [1, 2, 0, 1, 2, 0, 1]
+    assert utils.explicit_leg_rows(1, 4) == [0, 0, 0, 0]
+    assert utils.explicit_leg_rows(5, 0) == []
+    with pytest.raises(ValueError):
+        utils.explicit_leg_rows(0, 2)
 
 
 def test_alpha_schedule():

@@ -57,6 +57,47 @@ def test_oracle_loss_closed_form_at_zero_network():
     assert not g[:-1].any()
 
 
+def _bf16_exact_network(seed):
+    """Features in multiples of 1/4 on [-1, 1], W1 and W2 entries in {-1, 0, 1}/4
+    and /8, dyadic biases: every first-layer activation has at most 8
+    significant bits, so the kernel's bf16 roundings (features, weights, first
+    layer) are all exact and its inference must equal the float64 network."""
+    rng = np.random.default_rng(seed)
+    W1 = rng.integers(-1, 2, (32, 8)) / 4.0
+    b1 = rng.integers(-4, 5, 32) / 8.0
+    W2 = rng.integers(-1, 2, (32, 32)) / 8.0
+    b2 = rng.integers(-4, 5, 32) / 16.0
+    w3 = rng.normal(0, 0.6, 32).astype(np.float32)
+    p = np.concatenate([W1.ravel(), b1, W2.ravel(), b2, w3, [0.1]]).astype(np.float32)
+    x = (rng.integers(-4, 5, (4000, 8)) / 4.0).astype(np.float32)
+    return p, x
+
+
+def test_oracle_bf16_inference_matches_the_network_when_rounding_is_exact():
+    """shift_q8 (the decode kernel's bf16 inference, rounded to int8 steps of
+    (hi - lo) / 254) against forward() in float64: on a network and features
+    whose bf16 roundings are exact, the steps agree wherever 254 (sigmoid(z) -
+    1/2) is not within 1e-4 of a rounding tie.  A transposed W2, swapped
+    biases or a dropped relu change the logits of this network."""
+    lo, hi = -0.24, 0.66
+    for seed in (0, 1, 2):
+        p, x = _bf16_exact_network(seed)
+        assert np.array_equal(mo.bf16(p[:32 * 8]), p[:32 * 8]) and np.array_equal(mo.bf16(x), x)
+        z, c = mo.forward(p.astype(np.float64), x)
+        assert np.array_equal(mo.bf16(c["h1"].astype(np.float32)), c["h1"].astype(np.float32))
+        t = 254.0 * (1.0 / (1.0 + np.exp(-z)) - 0.5)
+        steps_ref = np.rint(t)
+        steps = np.rint(mo.shift_q8(p, x, lo, hi) / np.float32((hi - lo) / 254.0))
+        clear = np.abs(np.abs(t - np.floor(t)) - 0.5) > 1e-4
+        assert clear.mean() > 0.99 and np.abs(steps_ref).max() > 20
+        np.testing.assert_array_equal(steps[clear], steps_ref[clear])
+        assert np.abs(steps - steps_ref).max() <= 1
+        # a plausible slip: W2 transposed gives other steps on this network
+        q = p.copy()
+        q[32 * 8 + 32:32 * 8 + 32 + 1024] = p[32 * 8 + 32:32 * 8 + 32 + 1024].reshape(32, 32).T.ravel()
+        assert (np.rint(mo.shift_q8(q, x, lo, hi) / np.float32((hi - lo) / 254.0)) != steps).mean() > 0.1
+
+
 # --- GPU: training kernels ------------------------------------------------------------------
 
 

@@ -15,9 +15,16 @@ of the oracle is pinned by what the mathematics fixes:
   least the brute-force ML weight (tests/test_triton_against_rust.py:279-309);
 * schedule invariants of the reference tests (iterations of a never-converged
   shot, valid_mask, batch independence, first_leg vs num_sets=0, last_leg);
-* planted mistakes: four mutant builds of the oracle (syndrome sign dropped,
+* the relay-leg schedule (decoder.py:3655-3739, kernels.py:706-741): the
+  posterior carried from leg to leg and each leg's own gamma row (a closed
+  form across legs), the message reset at every leg start (a gamma-0 relay leg
+  replays the plain pre-iter leg bit for bit), and the best-solution rule (a
+  lighter later solution replaces the first, an equal-weight one does not);
+* planted mistakes: nine mutant builds of the oracle (syndrome sign dropped,
   own edge not excluded, memory weights swapped, extrinsic subtraction
-  dropped) each fail at least one of these checks.
+  dropped, messages not reset at a relay leg, posterior reset at a relay leg,
+  first solution never replaced, leg l reading row l+1, ties replacing the
+  best) each fail at least one of these checks.
 """
 
 from __future__ import annotations
@@ -29,11 +36,12 @@ import pytest
 from scipy.sparse import csr_matrix
 
 import oracle
+import toy_codes
 from pelagos_amd import codes
 from pelagos_amd.graph import CSRGraph
 from pelagos_amd.utils import compute_log_prior_ratios
 
-MUTANTS = {n: oracle._DIR / "_build" / f"liboracle_plant{n}.so" for n in (1, 2, 3, 4)}
+MUTANTS = {n: oracle._DIR / "_build" / f"liboracle_plant{n}.so" for n in range(1, 10)}
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -122,7 +130,7 @@ def check_first_iteration_closed_form(lib_path=None):
     """Uniform prior p, no memory, one iteration, checks of degree >= 2:
     every nu into variable j is (-1)^{s_i} lam0, so post_j = lam0 * (1 + sum_i (-1)^{s_i})
     summed in ascending check order from 0."""
-    H = codes.regular_code(1, V=60, C=30).toarray()
+    H = toy_codes.regular_code(1, V=60, C=30).toarray()
     lam0 = compute_log_prior_ratios(np.array([0.02]))[0]
     rng = np.random.default_rng(2)
     syn = rng.integers(0, 2, (16, 30)).astype(np.uint8)
@@ -186,7 +194,7 @@ def check_tree_codes_are_ml(lib_path=None):
     """Chain codes (tree Tanner graphs): plain min-sum returns the ML pattern."""
     rng = np.random.default_rng(5)
     for n in (5, 8, 11):
-        H = codes.repetition_code(n).toarray()
+        H = toy_codes.repetition_code(n).toarray()
         p = rng.uniform(0.02, 0.3, n)
         for _ in range(12):
             e = (rng.random(n) < p).astype(np.uint8)
@@ -201,7 +209,7 @@ def check_tiny_codes_valid_and_not_below_ml(lib_path=None):
     rng = np.random.default_rng(6)
     for trial in range(6):
         n, m = 12, 6
-        H = codes.random_ldpc(m, n, 2, seed=trial).toarray()
+        H = toy_codes.random_ldpc(m, n, 2, seed=trial).toarray()
         p = rng.uniform(0.02, 0.15, n)
         E = (rng.random((24, n)) < p).astype(np.uint8)
         S = (E @ H.T) % 2
@@ -219,7 +227,7 @@ def check_tiny_codes_valid_and_not_below_ml(lib_path=None):
 
 
 def check_schedule_invariants(lib_path=None):
-    H = codes.regular_code(3, V=120, C=60).toarray()
+    H = toy_codes.regular_code(3, V=120, C=60).toarray()
     p = np.full(120, 0.03)
     rng = np.random.default_rng(7)
     E = (rng.random((40, 120)) < p * 3).astype(np.uint8)
@@ -249,6 +257,118 @@ def check_schedule_invariants(lib_path=None):
     assert np.array_equal(full["errors"][never], o0["errors"][never])
 
 
+def _mix(g, lam0, post):
+    """lam_mixed = (1-g)*lam0 + g*post with the kernel's two roundings (kernels.py:1309)."""
+    g = np.float32(g)
+    return np.float32(np.float32(np.float32(np.float32(1.0) - g) * np.float32(lam0)) + np.float32(g * np.float32(post)))
+
+
+def check_relay_legs_carry_the_posterior_with_their_own_rows(lib_path=None):
+    """Check {v0, v1} with syndrome 1 and an empty check with syndrome 1 (never
+    satisfied, so every leg runs).  v1 has degree 1 and a zero strength in
+    every leg, so its message is its prior l1 and v0 receives the constant
+    c = -l1: v0 follows post <- (1-g)*l0 + g*post + c, whose fixed point
+    depends on g, so the legs' order matters.  The pre-iter leg uses init_gamma
+    (0.3 for v0), then relay leg l uses row l of the table, each leg starting
+    from the posterior the previous one left (decoder.py:3672-3700).  A reset
+    posterior or a shifted row changes the final value."""
+    H = np.zeros((2, 2), np.uint8)
+    H[0, :] = 1
+    gcl = CSRGraph.from_matrix(csr_matrix(H))
+    lam = np.array([2.0, 0.75], np.float32)
+    rows = np.array([[0.5, 0.0], [-0.2, 0.0], [0.1, 0.0], [0.62, 0.0]], np.float32)
+    post0 = np.array([[-4.0, 0.75]], np.float32)
+    T0, Tr = 3, 2
+    out = oracle.decode(gcl.chk_ptr, gcl.edge_var, 2, np.array([[1, 1]], np.uint8), lam, rows,
+                        pre_iter=T0, num_sets=rows.shape[0], set_max_iter=Tr, stop_nconv=1, gamma0=0.65,
+                        init_gamma=np.array([[0.3, 0.0]], np.float32), init_posterior=post0,
+                        want_posterior=True, lib_path=lib_path)
+    c = np.float32(np.float32(0.0) + np.float32(-0.75))
+    p = np.float32(post0[0, 0])
+    for _ in range(T0):
+        p = np.float32(_mix(0.3, lam[0], p) + c)
+    for leg in range(rows.shape[0]):
+        for _ in range(Tr):
+            p = np.float32(_mix(rows[leg, 0], lam[0], p) + c)
+    assert out["found_count"][0] == 0 and out["iterations"][0] == T0 + rows.shape[0] * Tr
+    assert out["last_posterior"][0, 0] == p
+
+
+def _never_converging_case():
+    """A loopy (3,6)-regular code plus one empty check whose syndrome bit is
+    1, so no error pattern explains the syndrome and BP runs every
+    iteration."""
+    H0 = toy_codes.regular_code(3, V=48, C=24).toarray()
+    H = np.vstack([H0, np.zeros((1, H0.shape[1]), np.uint8)])
+    rng = np.random.default_rng(12)
+    e = (rng.random((8, H.shape[1])) < 0.15).astype(np.uint8)
+    S = (e @ H.T) % 2
+    S[:, -1] = 1
+    return H, np.full(H.shape[1], 0.04), S
+
+
+def check_relay_leg_resets_the_messages(lib_path=None):
+    """With a gamma-0 row a relay leg mixes nothing of the posterior in, so,
+    because every leg starts from messages reset to the priors
+    (init_messages_with_priors_EB_kernel, kernels.py:903-1000; decoder.py:
+    3729-3739), it replays the plain pre-iter leg bit for bit.  Continuing the
+    pre-iter leg's messages instead gives the 2T-iteration trajectory, which
+    differs."""
+    H, p, S = _never_converging_case()
+    T = 9
+    zero_row = np.zeros((1, H.shape[1]), np.float32)
+    a = _decode(H, p, S, lib_path, pre_iter=T, want_posterior=True)
+    b = _decode(H, p, S, lib_path, pre_iter=T, num_sets=1, set_max_iter=T, gammas=zero_row, want_posterior=True)
+    c = _decode(H, p, S, lib_path, pre_iter=2 * T, want_posterior=True)
+    assert (a["found_count"] == 0).all() and (b["iterations"] == 2 * T).all()
+    assert not np.array_equal(a["last_posterior"], c["last_posterior"]), "trajectory must move"
+    np.testing.assert_array_equal(b["last_posterior"], a["last_posterior"])
+
+
+def check_lighter_solution_replaces_the_first_and_ties_do_not(lib_path=None):
+    """Two checks {v0,v1}, {v2,v3} with syndrome (1,1) and LLRs (1,2,2,1):
+    four solutions of weights {0,2}=3, {0,3}=2, {1,2}=4, {1,3}=3.  The
+    pre-iter leg (gamma0 = 1, so the first iteration's mix is the initial
+    posterior) is driven to {0,2}.  A relay leg with a zero row then converges
+    at once to the lighter {0,3}, which replaces it; a relay leg whose row
+    (-0.2, -0.5, 0, 0) mixes in the carried posterior converges to {1,3}, whose
+    equal weight keeps {0,2} (first solution, then strictly lower weight:
+    kernels.py:706-741)."""
+    H = np.array([[1, 1, 0, 0], [0, 0, 1, 1]], np.uint8)
+    gcl = CSRGraph.from_matrix(csr_matrix(H))
+    lam = np.array([1.0, 2.0, 2.0, 1.0], np.float32)
+    syn = np.array([[1, 1]], np.uint8)
+    post0 = np.array([[-10.0, 10.0, -10.0, 10.0]], np.float32)
+    kw = dict(pre_iter=5, num_sets=1, set_max_iter=5, stop_nconv=2, gamma0=1.0, init_posterior=post0,
+              lib_path=lib_path)
+    first = oracle.decode(gcl.chk_ptr, gcl.edge_var, 4, syn, lam, np.zeros((0, 4), np.float32),
+                          **dict(kw, num_sets=0, stop_nconv=1))
+    assert np.array_equal(first["errors"][0], [1, 0, 1, 0]) and first["weights"][0] == 3.0
+    lighter = oracle.decode(gcl.chk_ptr, gcl.edge_var, 4, syn, lam, np.zeros((1, 4), np.float32), **kw)
+    assert lighter["found_count"][0] == 2 and lighter["iterations"][0] == 2
+    assert np.array_equal(lighter["errors"][0], [1, 0, 0, 1]) and lighter["weights"][0] == 2.0
+    tie = oracle.decode(gcl.chk_ptr, gcl.edge_var, 4, syn, lam, np.array([[-0.2, -0.5, 0.0, 0.0]], np.float32),
+                        want_posterior=True, **kw)
+    assert tie["found_count"][0] == 2 and tie["iterations"][0] == 2
+    assert np.array_equal((tie["last_posterior"][0] <= 0).astype(np.uint8), [0, 1, 0, 1])  # found {1,3}
+    assert np.array_equal(tie["errors"][0], [1, 0, 1, 0]) and tie["weights"][0] == 3.0
+
+
+def check_pre_iter_zero_falls_back_to_relay_leg_one(lib_path=None):
+    """pre_iter = 0: a shot without a solution returns the hard decision after
+    relay leg 1, which is what a one-leg last_leg decode returns
+    (tests/test_unconverged_correction.py:124-139)."""
+    H, p, S = _never_converging_case()
+    rows = np.random.default_rng(3).uniform(-0.24, 0.66, (3, H.shape[1])).astype(np.float32)
+    full = _decode(H, p, S, lib_path, pre_iter=0, num_sets=3, set_max_iter=7, gammas=rows,
+                   gamma0=0.65, plain=False)
+    leg1 = _decode(H, p, S, lib_path, pre_iter=0, num_sets=1, set_max_iter=7, gammas=rows[:1],
+                   gamma0=0.65, plain=False, last_leg=True)
+    assert (full["found_count"] == 0).all()
+    np.testing.assert_array_equal(full["errors"], leg1["errors"])
+    assert full["errors"].any()
+
+
 ALL_CHECKS = [
     check_row_brute,
     check_first_iteration_closed_form,
@@ -257,6 +377,10 @@ ALL_CHECKS = [
     check_tree_codes_are_ml,
     check_tiny_codes_valid_and_not_below_ml,
     check_schedule_invariants,
+    check_relay_legs_carry_the_posterior_with_their_own_rows,
+    check_relay_leg_resets_the_messages,
+    check_lighter_solution_replaces_the_first_and_ties_do_not,
+    check_pre_iter_zero_falls_back_to_relay_leg_one,
 ]
 
 
