@@ -37,7 +37,7 @@
  *
  * Parity status: pinned by tests/test_oracle_pins.py (closed forms, tree-code
  * exactness, brute-force ML, schedule invariants, relay-leg schedule closed
- * forms, planted-bug mutants 1-9).
+ * forms, planted-bug mutants 1-10).
  */
 #include <math.h>
 #include <stdint.h>
@@ -130,7 +130,11 @@ static void check_update(const orc_graph *g, const orc_params *p, const uint8_t 
         if (r1 <= r0) continue;
         int deg_is_one = (r1 - r0) == 1;
         int neg = 0, cnt1 = 0;
-        float min1 = 1e30f, min2 = 1e30f;
+        float min1 = 1e30f, min2 = 1e30f;         /* the reference's start and padding value: magnitudes
+                                                     are clipped at 1e30 (kernels.py:1120-1126, 1146) */
+#if ORC_PLANT == 10
+        min1 = min2 = INFINITY;                   /* planted: no clip */
+#endif
         for (int32_t e = r0; e < r1; ++e) {
             float m = L->mu[e];
             float a = fabsf(m);

@@ -9,13 +9,13 @@
   blocks of 72 checks; every column has weight uniform in {3..6}, its checks
   drawn without replacement from two consecutive blocks; priors log-uniform in
   [1e-4, 1e-2].  936 x 8785 with 13 blocks, 1872 x 17570 with 26 blocks.
-* the reference tests' toy builders (tests/test_decoder_fixes.py:73-97,
-  215-229; tests/test_triton_against_rust.py:322-328, 578-581, 663-705).
+* ``three_code_workload`` -- the three codes decoded together by the batched
+  and serving workloads, at a given mean prior.
 """
 
 from __future__ import annotations
 
-from typing import Tuple
+from typing import Dict, Tuple
 
 import numpy as np
 from scipy.sparse import csr_matrix
@@ -158,3 +158,20 @@ def synthetic_observables(V: int, k: int = 12, weight: int = 64, seed: int = 0) 
     for r in range(k):
         O[r, rng.choice(V, min(weight, V), replace=False)] = 1
     return O
+
+
+# The three codes of the batched and serving workloads, by name: the [[72,12,6]]
+# BB code, the 936 x 8785 banded matrix (13 round-blocks) and a 1872 x 17570
+# banded matrix of the same construction over 26 round-blocks.
+THREE_CODES = ("bb72", "banded936", "banded1872")
+
+
+def three_code_workload(prior_scale: float) -> Dict[str, Tuple[csr_matrix, np.ndarray]]:
+    """name -> (H, priors) of the three codes with their priors scaled to mean
+    ``prior_scale``: uniform for the BB code, the banded matrices' log-uniform
+    priors rescaled (capped at 0.45)."""
+    out = {"bb72": (bb_72_12_6(), np.full(72, prior_scale, dtype=np.float32))}
+    for name, nb, V in (("banded936", 13, 8785), ("banded1872", 26, 17570)):
+        H, p = banded_circuit_code(n_blocks=nb, V=V, seed=0)
+        out[name] = (H, np.minimum(p * (prior_scale / float(p.mean())), 0.45).astype(np.float32))
+    return out

@@ -78,6 +78,7 @@ constexpr int kSlice = 64;         // rows / variables per ELL slice (one wave)
 constexpr int kChunk = 8;          // check-row edges per chunk (two chunks per edge-code word)
 constexpr int kVChunk = 4;         // variable edges per chunk (column weights are small)
 constexpr int kEdgesPerWord = 16;  // 2-bit edge codes
+constexpr float kMinStart = 1e30f; // start of a check's running minima: the reference's clip (kernels.py:1120-1126)
 
 // Queue counters at the start of the workspace: first-leg tickets handed out,
 // re-queue tickets handed out, shots re-queued, shots that left their first leg.
@@ -445,7 +446,10 @@ __device__ __forceinline__ uint32_t check_chunk(const KArgs &a, const ShotMem<PL
 // The running reduction keeps the reference's tie semantics (kernels.py:
 // 1160-1178): the argmin edge is the first to attain the minimum and m2 equals
 // m1 when two edges share it; med3(m1, a, m2) is the new second minimum for
-// every order of a against m1 <= m2.
+// every order of a against m1 <= m2.  Both start at 1e30 as the reference's
+// do (kernels.py:1120-1126, padding masked to 1e30 at 1146), so an outgoing
+// magnitude is min(1e30, min over the other edges): messages stay clipped
+// at 1e30 when posteriors grow past it under strong memory.
 template <bool FIRST, int PLACE>
 __device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m, int pos, float alpha,
                                          const uint8_t *__restrict__ syn) {
@@ -457,7 +461,7 @@ __device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m
     const uint32_t lane_off = 4u * (uint32_t)lane;
     const uint32_t qrow = 4u * (uint32_t)pos;  // this row's messages CS[0..3][pos]
     const uint32_t *__restrict__ ent = a.g.row_ent + rs.x;
-    float m1 = INFINITY, m2 = INFINITY;
+    float m1 = kMinStart, m2 = kMinStart;
     uint32_t arg_lo = 0, arg_base = 0;  // argmin = arg_base + arg_lo
     uint64_t hpar = 0;  // lane mask: parity of hard decisions so far
     uint32_t neg = 0;

@@ -61,20 +61,33 @@ class _Slot:
     footprint: int = 0
 
 
+class ServerBusy(RuntimeError):
+    """The server's queue holds its maximum number of shots: the request was
+    not admitted (the caller may retry later)."""
+
+
 class DecodeServer:
     """decoders: code name -> RelayBPDecoder (one device).  max_batch: shots per
     launch.  slots: concurrent launches (at most the device's hardware queues
     are useful).  hbm_budget: bytes of device memory in-flight work may hold
-    (workspace, outputs and per-request result copies)."""
+    (workspace, outputs and per-request result copies).  max_queued_shots:
+    shots the queue may hold while they wait for a launch; a request that
+    would exceed it is refused with ServerBusy, so a saturated server answers
+    at once instead of letting its queue (and every later request's latency)
+    grow without bound."""
 
     def __init__(self, decoders: Dict[str, RelayBPDecoder], max_batch: int = 2048, slots: int = 4,
-                 hbm_budget: int = 64 << 30):
+                 hbm_budget: int = 64 << 30, max_queued_shots: int = 16384):
         if not decoders:
             raise ValueError("no decoders")
         self.decoders = decoders
         self.device = next(iter(decoders.values())).device
         self.max_batch = int(max_batch)
         self.hbm_budget = int(hbm_budget)
+        self.max_queued_shots = int(max_queued_shots)
+        if self.max_queued_shots < self.max_batch:
+            raise ValueError("max_queued_shots must hold at least one full batch")
+        self.queued_shots = 0
         vmax = max(d.V for d in decoders.values())
         ws_need = {}
         for name, d in decoders.items():
@@ -113,10 +126,15 @@ class DecodeServer:
             raise ValueError(f"{code}: syndromes must be [b, {dec.C}]")
         if syndromes.shape[0] > self.max_batch:
             raise ValueError(f"request of {syndromes.shape[0]} shots exceeds max_batch={self.max_batch}")
+        b = int(syndromes.shape[0])
+        if self.queued_shots + b > self.max_queued_shots:
+            raise ServerBusy(f"{self.queued_shots} shots queued; a request of {b} exceeds "
+                             f"max_queued_shots={self.max_queued_shots}")
         r = Request(self._next_id, code, syndromes.to(self.device, torch.uint8).contiguous(),
                     self.now() if arrival is None else arrival)
         self._next_id += 1
         self.queues[code].append(r)
+        self.queued_shots += b
         return r
 
     # -- server side -------------------------------------------------------------------------------
@@ -128,6 +146,7 @@ class DecodeServer:
             r = q.popleft()
             reqs.append(r)
             shots += r.syndromes.shape[0]
+        self.queued_shots -= shots
         # footprint: workspace and buffers of the slot plus the result copies
         foot = self.slot_bytes + shots * (dec.V + 12)
         slot.busy, slot.reqs, slot.footprint = True, reqs, foot
@@ -198,28 +217,57 @@ def poisson_trace(rate_shots_per_s: float, duration_s: float, codes: List[str], 
         out.append((t, codes[rng.integers(len(codes))], int(rng.integers(1, max_shots + 1))))
 
 
-def run_trace(server: DecodeServer, trace: List[tuple], syndrome_pool: Dict[str, torch.Tensor]) -> dict:
+def run_trace(server: DecodeServer, trace: List[tuple], syndrome_pool: Dict[str, torch.Tensor],
+              policy: str = "reject") -> dict:
     """Replay a trace in real time: submit each request when its arrival time
     comes (its syndromes are slices of a device-resident pool), keep polling,
-    and report latency percentiles and throughput."""
+    and report latency percentiles and throughput.  A request the server
+    refuses (ServerBusy) is dropped and counted under ``policy="reject"``, or
+    kept by the client and submitted again at each poll under
+    ``policy="defer"`` (its latency still counts from its arrival)."""
+    if policy not in ("reject", "defer"):
+        raise ValueError("policy must be 'reject' or 'defer'")
     server.finished = []
     server.t0 = time.perf_counter()
     offs = {k: 0 for k in syndrome_pool}
+    backlog: collections.deque = collections.deque()
+    rejected = rejected_shots = deferred = 0
     i = 0
-    while i < len(trace) or server.pending():
+    while i < len(trace) or backlog or server.pending():
         now = server.now()
+        while backlog:
+            t, code, syn = backlog[0]
+            try:
+                server.submit(code, syn, arrival=t)
+            except ServerBusy:
+                break
+            backlog.popleft()
         while i < len(trace) and trace[i][0] <= now:
             t, code, b = trace[i]
             pool = syndrome_pool[code]
             o = offs[code] % max(1, pool.shape[0] - b)
             offs[code] += b
-            server.submit(code, pool[o:o + b], arrival=t)
             i += 1
+            if backlog:
+                backlog.append((t, code, pool[o:o + b]))
+                deferred += 1
+                continue
+            try:
+                server.submit(code, pool[o:o + b], arrival=t)
+            except ServerBusy:
+                if policy == "reject":
+                    rejected += 1
+                    rejected_shots += b
+                else:
+                    backlog.append((t, code, pool[o:o + b]))
+                    deferred += 1
         server.poll()
     end = server.now()
-    lat = np.array([r.done - r.arrival for r in server.finished]) * 1e3
+    lat = np.array([r.done - r.arrival for r in server.finished] or [0.0]) * 1e3
     shots = sum(r.syndromes.shape[0] for r in server.finished)
     dur = max(end, trace[-1][0] if trace else end)
     return dict(requests=len(server.finished), shots=int(shots), seconds=round(dur, 3),
                 shots_per_s=round(shots / dur, 1), p50_ms=round(float(np.percentile(lat, 50)), 3),
-                p99_ms=round(float(np.percentile(lat, 99)), 3), max_ms=round(float(lat.max()), 3))
+                p99_ms=round(float(np.percentile(lat, 99)), 3), max_ms=round(float(lat.max()), 3),
+                policy=policy, rejected_requests=rejected, rejected_shots=int(rejected_shots),
+                deferred_requests=deferred)

@@ -191,6 +191,25 @@ def test_decode_overrides_match_oracle(regular_case):
     _compare(out2, ref2)
 
 
+def test_check_magnitudes_clip_at_1e30_like_the_oracle(regular_case):
+    """Posteriors beyond 1e30 (an initial posterior of +-1.5e30..5e30 on a third
+    of the variables, kept by gamma0 = 1): from the second iteration on the
+    incoming |mu| exceed 1e30 and the outgoing magnitudes are clipped there
+    (kernels.py:1120-1126), as in the oracle."""
+    H, priors, S = regular_case
+    B, V = S.shape[0], H.shape[1]
+    rng = np.random.default_rng(13)
+    lam = utils.compute_log_prior_ratios(priors)
+    ip = np.broadcast_to(lam, (B, V)).astype(np.float32).copy()
+    big = rng.random((B, V)) < 0.33
+    ip[big] = (np.where(rng.random(big.sum()) < 0.5, -1.0, 1.0) * rng.uniform(1.5e30, 5e30, big.sum())).astype(np.float32)
+    dec = _decoder(H, priors, pre_iter=12, num_sets=4, set_max_iter=10, stop_nconv=2, gamma0=1.0, seed=7)
+    out = dec.decode(torch.from_numpy(S).to("cuda:0"), init_posterior=torch.from_numpy(ip).to("cuda:0"),
+                     return_last_posterior=True)
+    torch.cuda.synchronize()
+    _compare(out, _oracle_for(dec, S, init_posterior=ip))
+
+
 def test_toy_codes_match_oracle():
     """Triangle, high-degree, a 100-wide check, degree-1 and empty checks."""
     dense = np.zeros((5, 120), np.uint8)

@@ -5,8 +5,9 @@ offline and the reference stores no golden vectors (every expected value in
 its tests is computed live from relay_bp or from an invariant).  So each part
 of the oracle is pinned by what the mathematics fixes:
 
-* the check update against a brute-force ``prod sign * min over the others``
-  on exhaustive small rows, ties and zeros included (kernels.py:1128-1215);
+* the check update against a brute-force ``prod sign * min(1e30, min over
+  the others)`` on exhaustive small rows, ties, zeros and magnitudes beyond
+  1e30 included (kernels.py:1120-1215);
 * the first iteration against its closed form (uniform priors: every message
   is +-lam0, the posterior is lam0 plus a signed count);
 * tree Tanner graphs (chains): min-sum is exact, so the decoded pattern equals
@@ -20,11 +21,12 @@ of the oracle is pinned by what the mathematics fixes:
   form across legs), the message reset at every leg start (a gamma-0 relay leg
   replays the plain pre-iter leg bit for bit), and the best-solution rule (a
   lighter later solution replaces the first, an equal-weight one does not);
-* planted mistakes: nine mutant builds of the oracle (syndrome sign dropped,
+* planted mistakes: ten mutant builds of the oracle (syndrome sign dropped,
   own edge not excluded, memory weights swapped, extrinsic subtraction
   dropped, messages not reset at a relay leg, posterior reset at a relay leg,
   first solution never replaced, leg l reading row l+1, ties replacing the
-  best) each fail at least one of these checks.
+  best, check magnitudes not clipped at 1e30) each fail at least one of these
+  checks.
 """
 
 from __future__ import annotations
@@ -41,7 +43,7 @@ from pelagos_amd import codes
 from pelagos_amd.graph import CSRGraph
 from pelagos_amd.utils import compute_log_prior_ratios
 
-MUTANTS = {n: oracle._DIR / "_build" / f"liboracle_plant{n}.so" for n in range(1, 10)}
+MUTANTS = {n: oracle._DIR / "_build" / f"liboracle_plant{n}.so" for n in range(1, 11)}
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -52,8 +54,12 @@ def _built():
 # --- brute-force references (independent of the oracle) --------------------------------------
 
 
+CLIP = np.float32(1e30)
+
+
 def brute_check(mu, syn, alpha=None, beta=None):
-    """nu_e = (-1)^s * prod_{k!=e} sgn(mu_k) * min_{k!=e} |mu_k|, sgn(0) = +1."""
+    """nu_e = (-1)^s * prod_{k!=e} sgn(mu_k) * min(1e30, min_{k!=e} |mu_k|), sgn(0) = +1
+    (the reference's running minima start at 1e30, kernels.py:1120-1126)."""
     mu = np.asarray(mu, np.float32)
     d = mu.shape[0]
     out = np.zeros(d, np.float32)
@@ -65,7 +71,7 @@ def brute_check(mu, syn, alpha=None, beta=None):
         sign = -1.0 if syn else 1.0
         for k in others:
             sign *= 1.0 if mu[k] >= 0 else -1.0
-        mag = np.float32(min(abs(mu[k]) for k in others))
+        mag = np.float32(min([CLIP] + [abs(mu[k]) for k in others]))
         if alpha is not None:
             mag = np.float32(np.float32(alpha) * mag)
         if beta is not None:
@@ -108,6 +114,12 @@ def check_row_brute(lib_path=None):
     vals = np.array([-2.5, -1.0, -0.0, 0.0, 0.5, 1.0, 2.5], np.float32)
     for d in (1, 2, 3, 4):
         for mu in itertools.product(vals, repeat=d):
+            for syn in (0, 1):
+                got = oracle.check_update_row(np.array(mu, np.float32), syn, lib_path=lib_path)
+                np.testing.assert_array_equal(got, brute_check(mu, syn))
+    big = np.array([-3e30, 2e30, 1e30, -1e30, 5e37, np.inf, -np.inf], np.float32)
+    for d in (2, 3, 4):
+        for mu in itertools.product(np.concatenate([big, vals[[0, 4]]]), repeat=d):
             for syn in (0, 1):
                 got = oracle.check_update_row(np.array(mu, np.float32), syn, lib_path=lib_path)
                 np.testing.assert_array_equal(got, brute_check(mu, syn))

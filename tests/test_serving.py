@@ -21,6 +21,65 @@ def test_poisson_trace_rate_and_sizes():
     assert {c for _, c, _ in tr} == {"a", "b", "c"}
 
 
+class _ScriptedServer:
+    """Stands in for DecodeServer on the CPU: a queue of at most ``cap`` shots
+    that completes one request per poll."""
+
+    def __init__(self, cap):
+        import collections
+        import time
+
+        self.cap, self.q, self.finished, self.queued_shots = cap, collections.deque(), [], 0
+        self.t0 = time.perf_counter()
+
+    def now(self):
+        import time
+
+        return time.perf_counter() - self.t0
+
+    def submit(self, code, syn, arrival=None):
+        from types import SimpleNamespace
+
+        from pelagos_amd.serving import ServerBusy
+
+        b = int(syn.shape[0])
+        if self.queued_shots + b > self.cap:
+            raise ServerBusy("full")
+        r = SimpleNamespace(code=code, syndromes=syn, arrival=arrival, done=-1.0)
+        self.q.append(r)
+        self.queued_shots += b
+        return r
+
+    def poll(self):
+        if self.q:
+            r = self.q.popleft()
+            self.queued_shots -= int(r.syndromes.shape[0])
+            r.done = self.now()
+            self.finished.append(r)
+
+    def pending(self):
+        return len(self.q)
+
+
+@pytest.mark.parametrize("policy", ["reject", "defer"])
+def test_admission_policy_accounts_for_every_request(policy):
+    """A burst larger than the queue: under "reject" the refused requests are
+    counted and dropped, under "defer" every request is served eventually, in
+    arrival order, and none is refused."""
+    from pelagos_amd.serving import run_trace
+
+    trace = [(0.0, "a", 10)] * 20
+    pools = {"a": torch.zeros(64, 3, dtype=torch.uint8)}
+    srv = _ScriptedServer(cap=35)
+    st = run_trace(srv, trace, pools, policy=policy)
+    if policy == "reject":
+        assert st["requests"] + st["rejected_requests"] == 20 and st["rejected_requests"] > 0
+        assert st["rejected_shots"] == 10 * st["rejected_requests"] and st["deferred_requests"] == 0
+    else:
+        assert st["requests"] == 20 and st["rejected_requests"] == 0 and st["deferred_requests"] > 0
+        assert [r.arrival for r in srv.finished] == sorted(r.arrival for r in srv.finished)
+
+
 def _decoders():
     from pelagos_amd import RelayBPDecoder
 
