@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define PEL_ABI_VERSION 4
+#define PEL_ABI_VERSION 5
 
 /* status codes */
 #define PEL_OK 0
@@ -169,6 +169,27 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override,
  * the kernels it launches (kernels.py:446-1350). */
 int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, const pel_plan *plan,
                void *stream);
+
+/* Workspace bytes pel_decode_noalloc holds per GPU (32 MiB). */
+#define PEL_NOALLOC_MAX_BYTES (32LL << 20)
+
+/* pel_decode without a caller workspace and without any allocation: the same
+ * launches, with the workspace in a library-owned device array of
+ * PEL_NOALLOC_MAX_BYTES on each GPU (held until the process ends).  b->workspace
+ * must be NULL; every other buffer is the caller's as in pel_decode.  A decode
+ * whose workspace (the queue, order and re-queue arrays for B shots, plus
+ * plan->spill_stride x plan->grid) exceeds PEL_NOALLOC_MAX_BYTES is refused with
+ * PEL_EINVAL before anything is launched; a plan with requeue_below = 0 needs
+ * no parked posteriors (about 7.4 MB for 4096 shots of the 936 x 8785 code).
+ * The kernels reset every counter they rely on, so calls need no clearing in
+ * between.  All calls on one GPU share its array: calls whose launches
+ * overlap on one GPU (other streams, other threads) race and must be ordered
+ * by the caller.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). */
+int pel_decode_noalloc(const pel_graph *g, const pel_params *p, const pel_batch *b, const pel_plan *plan,
+                       void *stream);
 
 /* ---- learned memory strengths (no counterpart in the reference) ----------
  * Training entry points for the MLP above.  x [n][8] features, y [n] soft

@@ -13,7 +13,8 @@ from typing import Optional
 
 LIB_PATH = Path(__file__).resolve().parent / "libpelagos_relaybp.so"
 
-PEL_ABI_VERSION = 4
+PEL_ABI_VERSION = 5
+PEL_NOALLOC_MAX_BYTES = 32 << 20
 PEL_PLAN_LEARNED = 1
 PEL_MLP_PARAMS = 32 * 8 + 32 + 32 * 32 + 32 + 32 + 1
 PEL_OK = 0
@@ -101,6 +102,8 @@ SIGNATURES = {
                                        ctypes.POINTER(PelPlan)]),
     "pel_decode": (ctypes.c_int, [_vp, ctypes.POINTER(PelParams), ctypes.POINTER(PelBatch),
                                   ctypes.POINTER(PelPlan), _vp]),
+    "pel_decode_noalloc": (ctypes.c_int, [_vp, ctypes.POINTER(PelParams), ctypes.POINTER(PelBatch),
+                                          ctypes.POINTER(PelPlan), _vp]),
     "pel_mlp_param_count": (ctypes.c_int, []),
     "pel_mlp_last_error": (ctypes.c_char_p, []),
     "pel_mlp_grad_workspace_bytes": (ctypes.c_int64, []),

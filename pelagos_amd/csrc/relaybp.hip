@@ -52,6 +52,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -1512,6 +1513,36 @@ __global__ void permute_vars_kernel(const float *src, float *dst, const int32_t 
 
 typedef void (*KernelFn)(KArgs);
 
+}  // namespace
+
+// The workspace of pel_decode_noalloc: one copy of this array on every GPU the
+// library runs on, held until the process ends.  Host code never uses the
+// array's name as an address (in host code it is only a stand-in for the
+// symbol): scratch_on_device looks up each device's own address once, with
+// that device current, and every launch takes that address.
+static __device__ __align__(256) uint8_t scratch[PEL_NOALLOC_MAX_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t *g_scratch_ptr[kMaxDevices];
+
+// The address of `scratch` on device dev (the current device).
+int scratch_on_device(int dev, uint8_t **out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(PEL_EINVAL, "pel_decode_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(PEL_EDEVICE, std::string("pel_decode_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<uint8_t *>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return PEL_OK;
+}
+
 // Placements: 15 and 7 keep every state array but the priors (7) in LDS in the
 // canonical layout; 3 and 1 keep the variable state (and 3 also the check
 // state) in LDS; 0 keeps nothing there.
@@ -2108,9 +2139,9 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32
             return fail(PEL_EINVAL, "pel_plan_decode: requested placement exceeds the LDS budget");
         place = placement_override;
     }
-    uint32_t offs[kNumArrays], scratch = 0;
+    uint32_t offs[kNumArrays], scr_off = 0;
     int64_t lds = 0, spill = 0;
-    place_arrays(st, place, block, learn, offs, &scratch, &lds, &spill);
+    place_arrays(st, place, block, learn, offs, &scr_off, &lds, &spill);
     KernelFn fn = pick_kernel(block, place, learn);
     {
         // LDS is addressed absolutely (the dynamic region must start at address 0),
@@ -2144,51 +2175,72 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32
     return PEL_OK;
 }
 
-int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, const pel_plan *plan, void *stream) {
-    if (!g || !p || !b || !plan) return fail(PEL_EINVAL, "pel_decode: NULL argument");
-    if (b->B < 0) return fail(PEL_EINVAL, "pel_decode: negative batch");
+}  // extern "C"
+
+namespace {
+
+// pel_decode and pel_decode_noalloc: the same launches, with the workspace from
+// the batch (pel_decode) or the library's per-GPU scratch array (noalloc).
+int decode_impl(const char *who, const pel_graph *g, const pel_params *p, const pel_batch *b, const pel_plan *plan,
+                bool noalloc, void *stream) {
+    const std::string w_ = who;
+    if (!g || !p || !b || !plan) return fail(PEL_EINVAL, w_ + ": NULL argument");
+    if (b->B < 0) return fail(PEL_EINVAL, w_ + ": negative batch");
     if (b->B == 0) return PEL_OK;
     if (p->pre_iter < 0 || p->num_sets < 0 || p->set_max_iter < 0)
-        return fail(PEL_EINVAL, "pel_decode: iteration counts must be >= 0");
-    if (p->stop_nconv < 1) return fail(PEL_EINVAL, "pel_decode: stop_nconv must be >= 1");
+        return fail(PEL_EINVAL, w_ + ": iteration counts must be >= 0");
+    if (p->stop_nconv < 1) return fail(PEL_EINVAL, w_ + ": stop_nconv must be >= 1");
     if (!b->syndromes || !b->prior_llr || !b->wj || !b->errors || !b->weights || !b->iterations ||
-        !b->found_count || !b->workspace)
-        return fail(PEL_EINVAL, "pel_decode: NULL buffer");
+        !b->found_count || (!noalloc && !b->workspace))
+        return fail(PEL_EINVAL, w_ + ": NULL buffer");
+    if (noalloc && b->workspace)
+        return fail(PEL_EINVAL, w_ + ": takes no workspace (it uses the library's own); pass NULL");
     if (p->num_sets > 0 && !p->skip_relay_legs && !b->gamma_legs)
-        return fail(PEL_EINVAL, "pel_decode: gamma_legs is NULL with num_sets > 0");
+        return fail(PEL_EINVAL, w_ + ": gamma_legs is NULL with num_sets > 0");
     if (b->gamma_per_row && b->gamma_by_position)
-        return fail(PEL_EINVAL, "pel_decode: per-shot gamma rows are taken in the caller's order");
-    if (p->use_alpha && !p->alpha_t) return fail(PEL_EINVAL, "pel_decode: alpha_t is NULL with use_alpha");
-    if (b->learned_legs < 0) return fail(PEL_EINVAL, "pel_decode: learned_legs must be >= 0");
+        return fail(PEL_EINVAL, w_ + ": per-shot gamma rows are taken in the caller's order");
+    if (p->use_alpha && !p->alpha_t) return fail(PEL_EINVAL, w_ + ": alpha_t is NULL with use_alpha");
+    if (b->learned_legs < 0) return fail(PEL_EINVAL, w_ + ": learned_legs must be >= 0");
     if ((b->learned_legs > 0 || b->mlp || b->feat_out || b->hard_out || b->gamma_out) && !plan->learned)
-        return fail(PEL_EINVAL, "pel_decode: learned strengths need a plan made with PEL_PLAN_LEARNED");
+        return fail(PEL_EINVAL, w_ + ": learned strengths need a plan made with PEL_PLAN_LEARNED");
     if ((b->mlp || b->feat_out || b->hard_out || b->gamma_out) && b->learned_legs == 0)
-        return fail(PEL_EINVAL, "pel_decode: learned strengths or records with learned_legs == 0");
-    if (b->gamma_out && !b->mlp) return fail(PEL_EINVAL, "pel_decode: gamma_out needs network parameters");
+        return fail(PEL_EINVAL, w_ + ": learned strengths or records with learned_legs == 0");
+    if (b->gamma_out && !b->mlp) return fail(PEL_EINVAL, w_ + ": gamma_out needs network parameters");
     if (b->mlp && !(b->learned_lo <= b->learned_hi))
-        return fail(PEL_EINVAL, "pel_decode: learned_lo must not exceed learned_hi");
+        return fail(PEL_EINVAL, w_ + ": learned_lo must not exceed learned_hi");
     if (!(b->learned_last_damp >= 0.f && b->learned_last_damp <= 1.f))
-        return fail(PEL_EINVAL, "pel_decode: learned_last_damp must lie in [0, 1]");
+        return fail(PEL_EINVAL, w_ + ": learned_last_damp must lie in [0, 1]");
     if (b->mlp && (p->skip_relay_legs || p->num_sets < b->learned_legs))
-        return fail(PEL_EINVAL, "pel_decode: learned legs need that many relay legs");
+        return fail(PEL_EINVAL, w_ + ": learned legs need that many relay legs");
     int dev = 0;
     PEL_HIP(hipGetDevice(&dev));
-    if (dev != g->device) return fail(PEL_EINVAL, "pel_decode: current device differs from the graph's");
+    if (dev != g->device) return fail(PEL_EINVAL, w_ + ": current device differs from the graph's");
     bool known = false;
     for (int pl : kPlacements) known |= (pl == plan->placement);
     const StateSizes st = g->state();
     if (plan->block != 64 && plan->block != 128 && plan->block != 256 && plan->block != 512 && plan->block != 1024)
-        return fail(PEL_EINVAL, "pel_decode: plan block must be 64, 128, 256, 512 or 1024");
-    uint32_t offs[kNumArrays], scratch = 0;
+        return fail(PEL_EINVAL, w_ + ": plan block must be 64, 128, 256, 512 or 1024");
+    uint32_t offs[kNumArrays], scr_off = 0;
     int64_t lds = 0, spill = 0;
     const bool learn = plan->learned != 0;
     LearnLds ll;
-    place_arrays(st, plan->placement, plan->block, learn, offs, &scratch, &lds, &spill, &ll);
-    if (plan->requeue_below < 0) return fail(PEL_EINVAL, "pel_decode: requeue_below must be >= 0");
+    place_arrays(st, plan->placement, plan->block, learn, offs, &scr_off, &lds, &spill, &ll);
+    if (plan->requeue_below < 0) return fail(PEL_EINVAL, w_ + ": requeue_below must be >= 0");
     const WsLayout w = ws_layout(g, b->B, plan->requeue_below > 0, learn);
-    if (!known || (learn && (plan->placement & 7) != 7) || lds != plan->lds_bytes || align_up(spill, 256) != plan->spill_stride || plan->grid <= 0 ||
-        plan->workspace_bytes < w.total_fixed + plan->spill_stride * plan->grid)
-        return fail(PEL_EINVAL, "pel_decode: plan does not match this graph and batch size");
+    const int64_t need = w.total_fixed + plan->spill_stride * plan->grid;
+    if (!known || (learn && (plan->placement & 7) != 7) || lds != plan->lds_bytes ||
+        align_up(spill, 256) != plan->spill_stride || plan->grid <= 0 || (!noalloc && plan->workspace_bytes < need))
+        return fail(PEL_EINVAL, w_ + ": plan does not match this graph and batch size");
+    uint8_t *ws = static_cast<uint8_t *>(b->workspace);
+    if (noalloc) {
+        if (need > (int64_t)PEL_NOALLOC_MAX_BYTES)
+            return fail(PEL_EINVAL, w_ + ": this decode needs " + std::to_string(need) +
+                                        " bytes of workspace, more than the " + std::to_string(PEL_NOALLOC_MAX_BYTES) +
+                                        " the library holds per GPU (use fewer shots, a plan with requeue_below = 0, "
+                                        "or pel_decode with a workspace)");
+        const int rc = scratch_on_device(dev, &ws);
+        if (rc != PEL_OK) return rc;
+    }
     KArgs a;
     memset(&a, 0, sizeof a);
     a.g = g->d;
@@ -2209,7 +2261,6 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     a.prior = b->prior_llr;
     a.wj = b->wj;
     a.prior_batched = b->prior_batched;
-    uint8_t *ws = static_cast<uint8_t *>(b->workspace);
     if (!b->prior_batched) {
         a.wj_pos = reinterpret_cast<const float *>(ws + w.by_pos);
         a.prior_pos = reinterpret_cast<const float *>(ws + w.by_pos + align_up(4LL * g->V, 256));
@@ -2249,8 +2300,8 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     a.spill = ws + w.spill;
     a.spill_stride = plan->spill_stride;
     for (int k = 0; k < kNumArrays; ++k) a.off[k] = offs[k];
-    a.scratch_off = scratch;
-    a.rtab = row_table_addr(scratch, plan->block);
+    a.scratch_off = scr_off;
+    a.rtab = row_table_addr(scr_off, plan->block);
     a.vtab = a.rtab + (uint32_t)(16LL * st.nrs);
     a.rowinfo = a.vtab + (uint32_t)align_up(8LL * st.nvs, 16);
     a.best_lds = a.rowinfo + (uint32_t)align_up(st.C, 16);
@@ -2281,6 +2332,19 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
     PEL_HIP(hipGetLastError());
     if (b->ev_end) PEL_HIP(hipEventRecord(static_cast<hipEvent_t>(b->ev_end), s));
     return PEL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, const pel_plan *plan, void *stream) {
+    return decode_impl("pel_decode", g, p, b, plan, false, stream);
+}
+
+int pel_decode_noalloc(const pel_graph *g, const pel_params *p, const pel_batch *b, const pel_plan *plan,
+                       void *stream) {
+    return decode_impl("pel_decode_noalloc", g, p, b, plan, true, stream);
 }
 
 }  // extern "C"

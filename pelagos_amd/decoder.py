@@ -545,13 +545,16 @@ class RelayBPDecoder:
         return int(plan.workspace_bytes)
 
     def decode_async(self, syndromes: torch.Tensor, errors: torch.Tensor, weights: torch.Tensor,
-                     iterations: torch.Tensor, found_count: torch.Tensor, workspace: torch.Tensor,
+                     iterations: torch.Tensor, found_count: torch.Tensor, workspace: Optional[torch.Tensor],
                      stream: torch.cuda.Stream) -> None:
         """Decode device-resident ``syndromes [B, C]`` into caller-owned buffers
         (errors [B, V] uint8, weights [B] float32, iterations / found_count [B]
         int32) with a caller-owned workspace, ordered on ``stream``, without
         allocating or waiting: the launch a serving loop or a captured graph
-        makes.  Call prepare(B) once first (broadcast gamma draws only)."""
+        makes.  Call prepare(B) once first (broadcast gamma draws only).  With
+        ``workspace=None`` the launch uses the library's own per-GPU workspace
+        (``pel_decode_noalloc``, at most PEL_NOALLOC_MAX_BYTES; such calls
+        must not overlap on one GPU)."""
         B = int(syndromes.shape[0])
         # the buffers are handed to the kernel as raw pointers: check what it assumes
         for name, t, shape, dtype in (("syndromes", syndromes, (B, self.C), torch.uint8),
@@ -563,7 +566,8 @@ class RelayBPDecoder:
                 raise ValueError(f"decode_async: {name} must be a contiguous {dtype} tensor of shape {shape} on "
                                  f"{self.device}, got {t.dtype} {tuple(t.shape)} on {t.device}"
                                  f"{'' if t.is_contiguous() else ' (not contiguous)'}")
-        if workspace.dtype != torch.uint8 or not workspace.is_contiguous() or workspace.device != self.device:
+        if workspace is not None and (workspace.dtype != torch.uint8 or not workspace.is_contiguous()
+                                      or workspace.device != self.device):
             raise ValueError("decode_async: workspace must be a contiguous uint8 tensor on the decoder's device")
         if B == 0:
             return
@@ -574,7 +578,8 @@ class RelayBPDecoder:
             learn = dict(mlp=self.learned_params, legs=self.learned_legs)
         self._launch(syndromes, self.log_prior_ratios_per_gpu[0].reshape(self.V), self.wj, False,
                      self.gamma_table(B), None, None, errors, weights, iterations, found_count, None,
-                     self.pre_iter, False, B, learn, workspace=workspace, stream=stream)
+                     self.pre_iter, False, B, learn, workspace=workspace, stream=stream,
+                     noalloc=workspace is None)
 
     def collect_traces(self, syndromes: torch.Tensor, legs: int, with_network: bool = False) -> Dict[str, torch.Tensor]:
         """Decode as ``decode`` does and record, at the start of relay legs
@@ -597,9 +602,12 @@ class RelayBPDecoder:
         return out
 
     def _launch(self, syn, prior, wj, prior_batched, gtab, ig, ip, errors, weights, iters, found,
-                last_post, pre_iter, skip_relay_legs, B, learn=None, workspace=None, stream=None) -> None:
+                last_post, pre_iter, skip_relay_legs, B, learn=None, workspace=None, stream=None,
+                noalloc=False) -> None:
         plan = self.plan(B, learn is not None)
-        if workspace is not None:
+        if noalloc:
+            ws = None
+        elif workspace is not None:
             if workspace.numel() < plan.workspace_bytes:
                 raise ValueError(f"workspace holds {workspace.numel()} bytes, the plan needs {plan.workspace_bytes}")
             ws = workspace
@@ -643,7 +651,7 @@ class RelayBPDecoder:
             iterations=iters.data_ptr(),
             found_count=found.data_ptr(),
             last_posterior=_ptr(last_post),
-            workspace=ws.data_ptr(),
+            workspace=None if ws is None else ws.data_ptr(),
             shot_clock=_ptr(self._shot_clock_for(B)),
         )
         if learn is not None:
@@ -666,11 +674,9 @@ class RelayBPDecoder:
             e1.record(cur)
             batch.ev_begin = e0.cuda_event
             batch.ev_end = e1.cuda_event
-        _native.check(
-            _native.lib().pel_decode(self._handle.h, ctypes.byref(params), ctypes.byref(batch),
-                                     ctypes.byref(plan), ctypes.c_void_p(cur.cuda_stream)),
-            "pel_decode",
-        )
+        fn = _native.lib().pel_decode_noalloc if noalloc else _native.lib().pel_decode
+        _native.check(fn(self._handle.h, ctypes.byref(params), ctypes.byref(batch), ctypes.byref(plan),
+                         ctypes.c_void_p(cur.cuda_stream)), "pel_decode_noalloc" if noalloc else "pel_decode")
         if ev is not None:
             ev.append((e0, e1))
 
