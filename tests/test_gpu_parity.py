@@ -158,6 +158,18 @@ def test_regular_code_options_match_oracle(regular_case, case):
     _compare(out, ref)
 
 
+@pytest.mark.parametrize("perf", ["default", "rowwise", "throughput", None])
+@pytest.mark.parametrize("case", ["relay_default", "last_leg", "per_row"])
+def test_both_perf_modes_match_oracle(regular_case, case, perf):
+    """The reference runs its parity tests under perf="default" and
+    "throughput" (tests/test_decoder_fixes.py:325-341); both modes compute the
+    same per-shot results, and so do their aliases here."""
+    H, priors, S = regular_case
+    dec = _decoder(H, priors, seed=7, perf=perf, **OPTION_CASES[case])
+    assert dec.perf == ("throughput" if perf in ("throughput", None) else "default")
+    _compare(_run(dec, S), _oracle_for(dec, S))
+
+
 def test_explicit_gammas_and_by_error_match_oracle(regular_case):
     H, priors, S = regular_case
     V = H.shape[1]
