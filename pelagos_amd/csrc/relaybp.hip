@@ -182,6 +182,8 @@ typedef float vec4f __attribute__((ext_vector_type(4)));
 // access at a baked address; otherwise it goes through per-array pointers.
 template <int PLACE>
 struct ShotMem {
+    uint32_t lb;            // LDS base of the shot's group: 0 when a workgroup runs the shot, the wave's
+                            // region when each wave of a workgroup runs its own shot
     uint8_t *ls, *cs, *sb;  // array bases (LDS or workspace slot)
     vec4f *first;           // fast placements: the workgroup's copy of LDS [0, first_bytes) (CS and SB)
     int32_t first_words;    // after a shot's first check pass, in 16-byte words
@@ -191,9 +193,15 @@ struct ShotMem {
 
     static constexpr bool kFast = IsFast<PLACE>::value;
 
+    // LDS at the group-relative byte address addr.
+    template <typename T>
+    __device__ __forceinline__ __attribute__((address_space(3))) T *lds(uint32_t addr) const {
+        return lds_at<T>(addr + lb);
+    }
+
     __device__ __forceinline__ float2 ls_ent(uint32_t ent) const {  // ent = lsb + 8 * position
         if (kFast) {
-            const vec2f r = *lds_at<vec2f>(ent);
+            const vec2f r = *lds<vec2f>(ent);
             return make_float2(r.x, r.y);
         }
         return *reinterpret_cast<const float2 *>(ls + (ent - lsb));
@@ -201,17 +209,17 @@ struct ShotMem {
     // Prior of the variable whose LS entry is ent (placement bit kL0, which
     // only the fast placement 15 has): LS and L0 are parallel arrays.
     __device__ __forceinline__ float l0_ent(uint32_t ent) const {
-        return *lds_at<float>((ent >> 1) + (l0b - (lsb >> 1)));
+        return *lds<float>((ent >> 1) + (l0b - (lsb >> 1)));
     }
     __device__ __forceinline__ void ls_set(int v, float2 x) const {
-        if (kFast) *lds_at<vec2f>(lsb + 8u * (uint32_t)v) = vec2f{x.x, x.y};
+        if (kFast) *lds<vec2f>(lsb + 8u * (uint32_t)v) = vec2f{x.x, x.y};
         else *reinterpret_cast<float2 *>(ls + 8u * (uint32_t)v) = x;
     }
-    __device__ __forceinline__ float l0_pos(int v) const { return *lds_at<float>(l0b + 4u * (uint32_t)v); }
-    __device__ __forceinline__ void l0_set(int v, float x) const { *lds_at<float>(l0b + 4u * (uint32_t)v) = x; }
+    __device__ __forceinline__ float l0_pos(int v) const { return *lds<float>(l0b + 4u * (uint32_t)v); }
+    __device__ __forceinline__ void l0_set(int v, float x) const { *lds<float>(l0b + 4u * (uint32_t)v) = x; }
     __device__ __forceinline__ float2 ls_pos(int v) const {
         if (kFast) {
-            const vec2f r = *lds_at<vec2f>(lsb + 8u * (uint32_t)v);
+            const vec2f r = *lds<vec2f>(lsb + 8u * (uint32_t)v);
             return make_float2(r.x, r.y);
         }
         return *reinterpret_cast<const float2 *>(ls + 8u * (uint32_t)v);
@@ -219,10 +227,10 @@ struct ShotMem {
     __device__ __forceinline__ void cs_set(int r, float4 x) const {
         const uint32_t o = 4u * (uint32_t)r, st = 1u << cs_shift;
         if (kFast) {
-            *lds_at<float>(o) = x.x;
-            *lds_at<float>(o + st) = x.y;
-            *lds_at<float>(o + 2u * st) = x.z;
-            *lds_at<float>(o + 3u * st) = x.w;
+            *lds<float>(o) = x.x;
+            *lds<float>(o + st) = x.y;
+            *lds<float>(o + 2u * st) = x.z;
+            *lds<float>(o + 3u * st) = x.w;
         } else {
             *reinterpret_cast<float *>(cs + o) = x.x;
             *reinterpret_cast<float *>(cs + o + st) = x.y;
@@ -231,19 +239,19 @@ struct ShotMem {
         }
     }
     __device__ __forceinline__ float cs_msg(uint32_t e) const {  // e = 4 * row position + (code << cs_shift)
-        if (kFast) return *lds_at<float>(e);
+        if (kFast) return *lds<float>(e);
         return *reinterpret_cast<const float *>(cs + e);
     }
     __device__ __forceinline__ uint32_t sb_get(uint32_t w) const {
-        if (kFast) return *lds_at<uint32_t>(sbb + 4u * w);
+        if (kFast) return *lds<uint32_t>(sbb + 4u * w);
         return *reinterpret_cast<const uint32_t *>(sb + 4u * w);
     }
     __device__ __forceinline__ void sb_set(uint32_t w, uint32_t x) const {
-        if (kFast) *lds_at<uint32_t>(sbb + 4u * w) = x;
+        if (kFast) *lds<uint32_t>(sbb + 4u * w) = x;
         else *reinterpret_cast<uint32_t *>(sb + 4u * w) = x;
     }
     __device__ __forceinline__ uint32_t sb_ent(uint32_t e) const {  // e = ((sbb + 4 word) << 5) | bit
-        if (kFast) return *lds_at<uint32_t>(e >> 5);
+        if (kFast) return *lds<uint32_t>(e >> 5);
         return *reinterpret_cast<const uint32_t *>(sb + ((e >> 5) - sbb));
     }
 };
@@ -317,10 +325,39 @@ __device__ __forceinline__ T load_ell(const T *__restrict__ chunk, uint32_t lane
                                         (size_t)(u * kSlice * (int)sizeof(T)));
 }
 
+// A shot runs on a group of BLOCK threads: the whole workgroup (BLOCK is the
+// workgroup size) or, with BLOCK == 64, one wave of a workgroup whose waves run
+// a shot each (the shot-per-wave mode of small codes).  Shot-level code takes
+// its thread index, barriers and leader from these.
+template <int BLOCK>
+__device__ __forceinline__ int gtid() {
+    return (int)(threadIdx.x & (unsigned)(BLOCK - 1));
+}
+// Group barrier.  A one-wave group needs no s_barrier: the workgroup-scope
+// fences still order its LDS and global accesses across its lanes (the waits
+// __syncthreads has), without stopping the workgroup's other waves.
+template <int BLOCK>
+__device__ __forceinline__ void gsync() {
+    if (BLOCK == 64) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+        __syncthreads();
+    }
+}
+template <int BLOCK>
+__device__ __forceinline__ bool gleader() {
+    return (opaque_tid() & (unsigned)(BLOCK - 1)) == 0u;
+}
+
+// Sum over the group (double, the same butterfly order in every wave, then
+// the waves in order, from +0).
 template <int BLOCK>
 __device__ __forceinline__ double block_sum(double v, double *s_red) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (BLOCK == 64) return uniform_f64(0.0 + v);
     const int wid = threadIdx.x >> 6;
     if ((opaque_tid() & 63u) == 0) s_red[wid] = v;
     __syncthreads();
@@ -338,6 +375,7 @@ __device__ __forceinline__ double block_sum(double v, double *s_red) {
 template <int BLOCK>
 __device__ __forceinline__ int block_or(int pred, int *s_flag) {
     const bool any = __ballot(pred) != 0ull;
+    if (BLOCK == 64) return any ? 1 : 0;
     if ((opaque_tid() & 63u) == 0) s_flag[threadIdx.x >> 6] = any ? 1 : 0;
     __syncthreads();
     int r = 0;
@@ -346,13 +384,15 @@ __device__ __forceinline__ int block_or(int pred, int *s_flag) {
     return uniform_i32(r);
 }
 
-// Slice descriptors, read from the workgroup's LDS copy (one broadcast read).
-__device__ __forceinline__ int4 row_slice(const KArgs &a, int pos) {
-    const vec4f r = *lds_at<vec4f>(a.rtab + 16u * (uint32_t)(pos >> 6));
+// Slice descriptors, read from the group's LDS copy (one broadcast read).
+template <int PLACE>
+__device__ __forceinline__ int4 row_slice(const KArgs &a, const ShotMem<PLACE> &m, int pos) {
+    const vec4f r = *m.template lds<vec4f>(a.rtab + 16u * (uint32_t)(pos >> 6));
     return uniform_i4(make_int4(__float_as_int(r.x), __float_as_int(r.y), __float_as_int(r.z), __float_as_int(r.w)));
 }
-__device__ __forceinline__ int2 var_slice(const KArgs &a, int v) {
-    const vec2f r = *lds_at<vec2f>(a.vtab + 8u * (uint32_t)(v >> 6));
+template <int PLACE>
+__device__ __forceinline__ int2 var_slice(const KArgs &a, const ShotMem<PLACE> &m, int v) {
+    const vec2f r = *m.template lds<vec2f>(a.vtab + 8u * (uint32_t)(v >> 6));
     return make_int2(uniform_i32(__float_as_int(r.x)), uniform_i32(__float_as_int(r.y)));
 }
 
@@ -360,11 +400,11 @@ __device__ __forceinline__ int2 var_slice(const KArgs &a, int v) {
 // shot (syndrome bit, degree > 0, degree == 1 by row position) so that a check
 // pass does not start with dependent global loads of the syndrome and the row
 // degree, whose latency every wave would wait out.
-template <int BLOCK>
-__device__ __forceinline__ void row_info_init(const KArgs &a, const uint8_t *__restrict__ syn) {
-    for (int pos = threadIdx.x; pos < a.g.C; pos += BLOCK) {
+template <int BLOCK, int PLACE>
+__device__ __forceinline__ void row_info_init(const KArgs &a, const ShotMem<PLACE> &m, const uint8_t *__restrict__ syn) {
+    for (int pos = gtid<BLOCK>(); pos < a.g.C; pos += BLOCK) {
         const int deg = a.g.row_deg[pos];
-        *lds_at<uint8_t>(a.rowinfo + (uint32_t)pos) =
+        *m.template lds<uint8_t>(a.rowinfo + (uint32_t)pos) =
             (uint8_t)((syn[a.g.row_orig[pos]] & 1u) | (deg > 0 ? 2u : 0u) | (deg == 1 ? 4u : 0u));
     }
 }
@@ -454,9 +494,9 @@ __device__ __forceinline__ uint32_t check_chunk(const KArgs &a, const ShotMem<PL
 template <bool FIRST, int PLACE>
 __device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m, int pos, float alpha,
                                          const uint8_t *__restrict__ syn) {
-    const int4 rs = row_slice(a, pos);  // {ent, word, width, 0}
+    const int4 rs = row_slice(a, m, pos);  // {ent, word, width, 0}
     const int lane = pos & 63;
-    const uint32_t info = *lds_at<uint8_t>(a.rowinfo + (uint32_t)pos);
+    const uint32_t info = *m.template lds<uint8_t>(a.rowinfo + (uint32_t)pos);
     const uint32_t s_i = info & 1u;
     const uint32_t wbase = (uint32_t)(rs.y + lane);
     const uint32_t lane_off = 4u * (uint32_t)lane;
@@ -524,8 +564,8 @@ __device__ __forceinline__ int check_row(const KArgs &a, const ShotMem<PLACE> &m
 template <int PLACE>
 __device__ __forceinline__ int parity_row(const KArgs &a, const ShotMem<PLACE> &m, int pos,
                                           const uint8_t *__restrict__ syn) {
-    const int4 rs = row_slice(a, pos);
-    bool hpar = (*lds_at<uint8_t>(a.rowinfo + (uint32_t)pos) & 1u) != 0u;
+    const int4 rs = row_slice(a, m, pos);
+    bool hpar = (*m.template lds<uint8_t>(a.rowinfo + (uint32_t)pos) & 1u) != 0u;
     const uint32_t lane_off = 4u * (uint32_t)(pos & 63);
     for (int k0 = 0; k0 < rs.z; k0 += kChunk) {
         const uint32_t *__restrict__ chunk = a.g.row_ent + rs.x + (int64_t)k0 * kSlice;
@@ -566,7 +606,7 @@ __device__ __forceinline__ void var_messages(const ShotMem<PLACE> &m, const uint
 // messages are added one at a time in ascending edge id.
 template <int PLACE, bool FLIPS = false>
 __device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> &m, int v, const GammaSrc &gs) {
-    const int2 vs = var_slice(a, v);  // {ent, width}
+    const int2 vs = var_slice(a, m, v);  // {ent, width}
     const float g = gs(v);
     const float l0 = (PLACE & (1 << kL0)) ? m.l0_pos(v) : m.l0g[a.g.vperm[v]];
     const float2 ls = m.ls_pos(v);
@@ -599,7 +639,7 @@ __device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> 
     m.ls_set(v, make_float2(lm, s));
     if (FLIPS && ((post_prev <= 0.f) != (lm + s <= 0.f))) {
         // hard-decision flips during the leg (a feature of the learned strengths)
-        __attribute__((address_space(3))) uint8_t *c = lds_at<uint8_t>(a.flips_lds + (uint32_t)v);
+        __attribute__((address_space(3))) uint8_t *c = m.template lds<uint8_t>(a.flips_lds + (uint32_t)v);
         *c = (uint8_t)min((int)*c + 1, 255);
     }
 }
@@ -824,7 +864,7 @@ struct VarFeatures {
         for (int u = 0; u < W; ++u) {
             const uint32_t w = m.sb_ent(e[u].y);
             msg[u] = m.cs_msg(e[u].x + (__builtin_amdgcn_ubfe(w, e[u].y, 2u) << m.cs_shift));
-            un[u] = KNOWN_VALID ? 0 : (int)*lds_at<uint8_t>(a.unsat_lds + (e[u].x >> 2));
+            un[u] = KNOWN_VALID ? 0 : (int)*m.template lds<uint8_t>(a.unsat_lds + (e[u].x >> 2));
         }
 #pragma unroll
         for (int u = 0; u < W; ++u) {  // selects, not branches: no load is sunk into a branch
@@ -837,7 +877,7 @@ struct VarFeatures {
         ls = m.ls_pos(vc);
         deg = a.g.var_deg[vc];
         l0 = (PLACE & (1 << kL0)) ? m.l0_pos(vc) : m.l0g[a.g.vperm[vc]];
-        vs = var_slice(a, vc);
+        vs = var_slice(a, m, vc);
         mn = INFINITY;
         uns = 0;
         switch (vs.y) {
@@ -865,11 +905,11 @@ __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const Shot
     constexpr bool known_valid = KNOWN_VALID;
     const int C = a.g.C, V = a.g.V, L = a.learned_legs;
     if (!known_valid) {
-        for (int i = threadIdx.x; i <= C; i += BLOCK)
-            *lds_at<uint8_t>(a.unsat_lds + (uint32_t)i) = (uint8_t)(i < C ? parity_row<PLACE>(a, m, i, syn) : 0);
-        __syncthreads();
+        for (int i = gtid<BLOCK>(); i <= C; i += BLOCK)
+            *m.template lds<uint8_t>(a.unsat_lds + (uint32_t)i) = (uint8_t)(i < C ? parity_row<PLACE>(a, m, i, syn) : 0);
+        gsync<BLOCK>();
     }
-    const int lane = threadIdx.x & 63, q = lane >> 4;
+    const int lane = gtid<BLOCK>() & 63, q = lane >> 4;
     const bool net = a.mlp && apply;
     // A wave takes kFeatBatch of its slices at a time (v0 is uniform, so every
     // lane of a wave runs the same rounds; the MFMAs need all 64 lanes), and
@@ -877,7 +917,7 @@ __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const Shot
     // across the gathers would spill).  Lanes past V, and batch slots past the
     // last slice, compute a copy of variable V - 1 (in the last slice) and store
     // nothing.
-    for (int vb = threadIdx.x & ~63; vb < V; vb += kFeatBatch * BLOCK) {
+    for (int vb = gtid<BLOCK>() & ~63; vb < V; vb += kFeatBatch * BLOCK) {
         VarFeatures<PLACE, KNOWN_VALID> vf[kFeatBatch];
 #pragma unroll
         for (int j = 0; j < kFeatBatch; ++j) vf[j].start(a, m, min(vb + j * BLOCK + lane, V - 1));
@@ -892,7 +932,7 @@ __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const Shot
             const float post = x.ls.x + x.ls.y;
             f[0] = x.l0 * 0.125f;
             f[1] = clamp4(post * 0.0625f);
-            f[2] = (float)*lds_at<uint8_t>(a.flips_lds + (uint32_t)vc) * 0.125f;
+            f[2] = (float)*m.template lds<uint8_t>(a.flips_lds + (uint32_t)vc) * 0.125f;
             f[3] = clamp4(x.ls.y * inv_deg * 0.125f);
             f[4] = x.deg > 0 ? clamp4(x.mn * 0.125f) : 0.f;
             f[5] = (float)x.deg * 0.125f;
@@ -941,16 +981,16 @@ __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const Shot
                 // sigmoid, which moves a shift by one step only at a rounding tie)
                 const float sig = __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-z * 1.4426950408889634f));
                 const float steps = __builtin_rintf(254.f * (sig - 0.5f));
-                if (v < V) *lds_at<int8_t>(a.gamma_lds + (uint32_t)v) = (int8_t)(int)steps;
+                if (v < V) *m.template lds<int8_t>(a.gamma_lds + (uint32_t)v) = (int8_t)(int)steps;
             }
         }
 #pragma unroll
         for (int j = 0; j < kFeatBatch; ++j) {
             const int v = vb + j * BLOCK + lane;
-            if (v < V) *lds_at<uint8_t>(a.flips_lds + (uint32_t)v) = 0;
+            if (v < V) *m.template lds<uint8_t>(a.flips_lds + (uint32_t)v) = 0;
         }
     }
-    __syncthreads();
+    gsync<BLOCK>();
 }
 
 template <int BLOCK, int PLACE>
@@ -965,7 +1005,7 @@ __device__ __forceinline__ void leg_features(const KArgs &a, int b, const ShotMe
 template <int BLOCK>
 __device__ __forceinline__ void record_strengths(const KArgs &a, int b, int r, const GammaSrc &gs) {
     const int V = a.g.V, L = a.learned_legs;
-    for (int v = threadIdx.x; v < V; v += BLOCK) a.gamma_out[((size_t)b * L + r) * V + a.g.vperm[v]] = gs(v);
+    for (int v = gtid<BLOCK>(); v < V; v += BLOCK) a.gamma_out[((size_t)b * L + r) * V + a.g.vperm[v]] = gs(v);
 }
 
 constexpr int kBatchU = 4;  // positions per thread handled together by the per-shot loops
@@ -977,27 +1017,27 @@ constexpr int kBatchU = 4;  // positions per thread handled together by the per-
 template <int BLOCK, int PLACE>
 __device__ __forceinline__ void best_from_state(const KArgs &a, const ShotMem<PLACE> &m, bool any_iter) {
     const int V = a.g.V;
-    for (int vb = (int)(threadIdx.x & ~63u); vb < V; vb += BLOCK) {
-        const float2 ls = m.ls_pos(min(vb + (int)(threadIdx.x & 63u), V));  // position V: the sentinel, +inf
+    for (int vb = (int)(gtid<BLOCK>() & ~63u); vb < V; vb += BLOCK) {
+        const float2 ls = m.ls_pos(min(vb + (int)(gtid<BLOCK>() & 63u), V));  // position V: the sentinel, +inf
         const uint64_t bits = __ballot(any_iter && ls.x + ls.y <= 0.f);
-        if ((opaque_tid() & 63u) == 0) *lds_at<uint64_t>(a.best_lds + 8u * (uint32_t)(vb >> 6)) = bits;
+        if ((opaque_tid() & 63u) == 0) *m.template lds<uint64_t>(a.best_lds + 8u * (uint32_t)(vb >> 6)) = bits;
     }
 }
 
 // The shot's error row, in the caller's order, from its output bits: the only
 // write of the row, made once by the workgroup that completes the shot.
-template <int BLOCK>
-__device__ __forceinline__ void write_errors(const KArgs &a, int b) {
+template <int BLOCK, int PLACE>
+__device__ __forceinline__ void write_errors(const KArgs &a, const ShotMem<PLACE> &m, int b) {
     const int V = a.g.V;
     uint8_t *err = a.errors + (size_t)b * V;
-    __syncthreads();  // every wave's output words are in LDS
-    for (int j0 = threadIdx.x; j0 < V; j0 += kBatchU * BLOCK) {
+    gsync<BLOCK>();  // every wave's output words are in LDS
+    for (int j0 = gtid<BLOCK>(); j0 < V; j0 += kBatchU * BLOCK) {
         int v[kBatchU];
         uint32_t w[kBatchU];
 #pragma unroll
         for (int u = 0; u < kBatchU; ++u) v[u] = a.g.vpos[min(j0 + u * BLOCK, V - 1)];
 #pragma unroll
-        for (int u = 0; u < kBatchU; ++u) w[u] = *lds_at<uint32_t>(a.best_lds + 4u * (uint32_t)(v[u] >> 5));
+        for (int u = 0; u < kBatchU; ++u) w[u] = *m.template lds<uint32_t>(a.best_lds + 4u * (uint32_t)(v[u] >> 5));
 #pragma unroll
         for (int u = 0; u < kBatchU; ++u)
             if (j0 + u * BLOCK < V) err[j0 + u * BLOCK] = (uint8_t)((w[u] >> (v[u] & 31)) & 1u);
@@ -1010,7 +1050,7 @@ __device__ __forceinline__ void write_errors(const KArgs &a, int b) {
 template <int BLOCK, int PLACE, typename F>
 __device__ __forceinline__ void for_each_caller_index(const KArgs &a, const ShotMem<PLACE> &m, F f) {
     const int V = a.g.V;
-    for (int j0 = threadIdx.x; j0 < V; j0 += kBatchU * BLOCK) {
+    for (int j0 = gtid<BLOCK>(); j0 < V; j0 += kBatchU * BLOCK) {
         int v[kBatchU];
         float2 ls[kBatchU];
 #pragma unroll
@@ -1038,7 +1078,7 @@ __device__ __forceinline__ void select_solution(const KArgs &a, const ShotMem<PL
     if (a.wj_pos) {
         // weights in position order too: kBatchU contiguous loads per thread in
         // flight together, and a select rather than a branch
-        for (int v0 = threadIdx.x; v0 < V; v0 += kBatchU * BLOCK) {
+        for (int v0 = gtid<BLOCK>(); v0 < V; v0 += kBatchU * BLOCK) {
             float2 ls[kBatchU];
             float w[kBatchU];
 #pragma unroll
@@ -1051,7 +1091,7 @@ __device__ __forceinline__ void select_solution(const KArgs &a, const ShotMem<PL
         }
     } else {
         // per-shot weights in the caller's order: gathered only for the few set bits
-        for (int v = threadIdx.x; v < V; v += BLOCK) {
+        for (int v = gtid<BLOCK>(); v < V; v += BLOCK) {
             const float2 ls = m.ls_pos(v);
             if (ls.x + ls.y <= 0.f) acc += (double)wj[a.g.vperm[v]];
         }
@@ -1084,31 +1124,31 @@ __device__ __forceinline__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m,
         // back.  Each thread restores exactly the words it saved.
         const bool restore = t == 0 && ShotMem<PLACE>::kFast && st.first_ok;
         if (restore) {
-            for (int i = threadIdx.x; i < m.first_words; i += BLOCK) *lds_at<vec4f>(16u * (uint32_t)i) = m.first[i];
+            for (int i = gtid<BLOCK>(); i < m.first_words; i += BLOCK) *m.template lds<vec4f>(16u * (uint32_t)i) = m.first[i];
         } else if (t == 0) {
-            for (int i = threadIdx.x; i < C; i += BLOCK) check_row<true, PLACE>(a, m, i, alpha, syn);
+            for (int i = gtid<BLOCK>(); i < C; i += BLOCK) check_row<true, PLACE>(a, m, i, alpha, syn);
         } else {
-            for (int i = threadIdx.x; i < C; i += BLOCK) bad |= check_row<false, PLACE>(a, m, i, alpha, syn);
+            for (int i = gtid<BLOCK>(); i < C; i += BLOCK) bad |= check_row<false, PLACE>(a, m, i, alpha, syn);
         }
         bad = block_or<BLOCK>(bad, s_flag);
         if (t == 0 && ShotMem<PLACE>::kFast && !st.first_ok) {
             // read before the barrier that ends this iteration; the next check
             // pass rewrites CS and SB only after it
-            for (int i = threadIdx.x; i < m.first_words; i += BLOCK) m.first[i] = *lds_at<vec4f>(16u * (uint32_t)i);
+            for (int i = gtid<BLOCK>(); i < m.first_words; i += BLOCK) m.first[i] = *m.template lds<vec4f>(16u * (uint32_t)i);
             st.first_ok = 1;
         }
         if (t > 0 && !bad) {  // iteration t-1 converged
             select_solution<BLOCK, PLACE>(a, m, wj, st, s_red);
             return true;
         }
-        for (int v = threadIdx.x; v < V; v += BLOCK) var_update<PLACE, FLIPS>(a, m, v, gs);
-        __syncthreads();
+        for (int v = gtid<BLOCK>(); v < V; v += BLOCK) var_update<PLACE, FLIPS>(a, m, v, gs);
+        gsync<BLOCK>();
         st.iters += 1;
         st.any_iter = 1;
     }
     if (T > 0) {
         int bad = 0;
-        for (int i = threadIdx.x; i < C; i += BLOCK) bad |= parity_row<PLACE>(a, m, i, syn);
+        for (int i = gtid<BLOCK>(); i < C; i += BLOCK) bad |= parity_row<PLACE>(a, m, i, syn);
         bad = block_or<BLOCK>(bad, s_flag);
         if (!bad) {
             select_solution<BLOCK, PLACE>(a, m, wj, st, s_red);
@@ -1139,7 +1179,7 @@ __device__ __forceinline__ bool first_leg(const KArgs &a, int b, ShotMem<PLACE> 
                           int *s_flag) {
     const int V = a.g.V;
     const ShotIO io = shot_io(a, b);
-    for (int j0 = threadIdx.x; j0 < V; j0 += kBatchU * BLOCK) {  // kBatchU per thread at a time
+    for (int j0 = gtid<BLOCK>(); j0 < V; j0 += kBatchU * BLOCK) {  // kBatchU per thread at a time
         int v[kBatchU];
         float lj[kBatchU], p0[kBatchU];
 #pragma unroll
@@ -1154,12 +1194,12 @@ __device__ __forceinline__ bool first_leg(const KArgs &a, int b, ShotMem<PLACE> 
             if (j0 + u * BLOCK >= V) break;
             m.ls_set(v[u], make_float2(p0[u], 0.f));
             if (PLACE & (1 << kL0)) m.l0_set(v[u], lj[u]);
-            if (LEARN) *lds_at<uint8_t>(a.flips_lds + (uint32_t)v[u]) = 0;
+            if (LEARN) *m.template lds<uint8_t>(a.flips_lds + (uint32_t)v[u]) = 0;
         }
     }
-    row_info_init<BLOCK>(a, io.syn);
+    row_info_init<BLOCK, PLACE>(a, m, io.syn);
     m.l0g = io.l0;
-    __syncthreads();
+    gsync<BLOCK>();
 
     st = ShotState{INFINITY, 0, 0, 0, 0, 0, 0};
     GammaSrc g0{nullptr, nullptr, a.plain ? 0.f : a.gamma0};
@@ -1221,7 +1261,7 @@ __device__ __forceinline__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE>
                     leg_features<BLOCK, PLACE>(a, b, m, leg, io.syn, st.last_ok != 0, false);
                 }
                 if (st.shifted && st.last_ok) {
-                    gl.lds = a.gamma_lds;
+                    gl.lds = a.gamma_lds + m.lb;
                     gl.lo = a.learned_lo;
                     gl.hi = a.learned_hi;
                     gl.scale = leg == a.learned_legs - 1 ? a.last_scale : 1.f;
@@ -1239,7 +1279,7 @@ __device__ __forceinline__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE>
     }
     // last_leg correction: the live hard decision for shots with no solution.
     if (a.last_leg && st.found == 0) best_from_state<BLOCK, PLACE>(a, m, st.any_iter != 0);
-    write_errors<BLOCK>(a, b);
+    write_errors<BLOCK, PLACE>(a, m, b);
     if (a.last_post) {
         float *lp = a.last_post + (size_t)b * V;
         for_each_caller_index<BLOCK, PLACE>(a, m, [&](int j, float2 ls) { lp[j] = ls.x + ls.y; });
@@ -1262,14 +1302,14 @@ template <int BLOCK, int PLACE, bool LEARN>
 __device__ __forceinline__ void requeue_shot(const KArgs &a, int b, const ShotMem<PLACE> &m, const ShotState &st) {
     const int V = a.g.V;
     float *post = a.rq_post + (size_t)b * V;
-    for (int v = threadIdx.x; v < V; v += BLOCK) {
+    for (int v = gtid<BLOCK>(); v < V; v += BLOCK) {
         const float2 ls = m.ls_pos(v);
         post[v] = ls.x + ls.y;
-        if (LEARN && a.mlp) a.rq_gamma[(size_t)b * V + v] = *lds_at<int8_t>(a.gamma_lds + (uint32_t)v);
+        if (LEARN && a.mlp) a.rq_gamma[(size_t)b * V + v] = *m.template lds<int8_t>(a.gamma_lds + (uint32_t)v);
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
-    if (opaque_tid() == 0) {
+    gsync<BLOCK>();
+    if (gleader<BLOCK>()) {
         __threadfence();
         a.rq_state[b] = make_int4(__float_as_int(st.best_w), st.found, st.iters, st.any_iter);
         const int slot = atomicAdd(&a.ctrl[kCtrlPushed], 1);
@@ -1287,7 +1327,7 @@ __device__ __forceinline__ void resume_shot(const KArgs &a, int b, ShotMem<PLACE
     const int V = a.g.V;
     const ShotIO io = shot_io(a, b);
     const float *post = a.rq_post + (size_t)b * V;
-    for (int v0 = threadIdx.x; v0 < V; v0 += kBatchU * BLOCK) {  // kBatchU per thread at a time
+    for (int v0 = gtid<BLOCK>(); v0 < V; v0 += kBatchU * BLOCK) {  // kBatchU per thread at a time
         float p[kBatchU], l[kBatchU];
         int8_t gq[kBatchU];
 #pragma unroll
@@ -1304,14 +1344,14 @@ __device__ __forceinline__ void resume_shot(const KArgs &a, int b, ShotMem<PLACE
             m.ls_set(v, make_float2(p[u], 0.f));
             if (PLACE & (1 << kL0)) m.l0_set(v, l[u]);
             if (LEARN) {
-                *lds_at<uint8_t>(a.flips_lds + (uint32_t)v) = 0;  // leg 0's flips start from zero
-                if (a.mlp) *lds_at<int8_t>(a.gamma_lds + (uint32_t)v) = gq[u];
+                *m.template lds<uint8_t>(a.flips_lds + (uint32_t)v) = 0;  // leg 0's flips start from zero
+                if (a.mlp) *m.template lds<int8_t>(a.gamma_lds + (uint32_t)v) = gq[u];
             }
         }
     }
-    row_info_init<BLOCK>(a, io.syn);
+    row_info_init<BLOCK, PLACE>(a, m, io.syn);
     m.l0g = io.l0;
-    __syncthreads();
+    gsync<BLOCK>();
     best_from_state<BLOCK, PLACE>(a, m, true);
 }
 
@@ -1344,6 +1384,7 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
     int *s_next = s_flag + BLOCK / 64;
     uint8_t *slot = a.spill + (int64_t)blockIdx.x * a.spill_stride;
     ShotMem<PLACE> m;
+    m.lb = 0;  // the workgroup runs one shot at a time
     m.ls = ((PLACE & (1 << kLS)) ? smem : slot) + a.off[kLS];
     m.cs = ((PLACE & (1 << kCS)) ? smem : slot) + a.off[kCS];
     m.sb = ((PLACE & (1 << kSB)) ? smem : slot) + a.off[kSB];
