@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define PEL_ABI_VERSION 5
+#define PEL_ABI_VERSION 6
 
 /* status codes */
 #define PEL_OK 0
@@ -190,6 +190,46 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
  * forbid hipMalloc). */
 int pel_decode_noalloc(const pel_graph *g, const pel_params *p, const pel_batch *b, const pel_plan *plan,
                        void *stream);
+
+/* ---- several batches, of one code or several, in one persistent launch ----
+ * A job is one batch of one graph: its own pel_params and pel_batch (buffers
+ * and a workspace of pel_jobs_plan.workspace_bytes[i] bytes).  One launch
+ * decodes up to PEL_MAX_JOBS jobs, taking them in order: a small code's shots
+ * run one per wave (sixteen shots per workgroup at a time), a large code's one
+ * per workgroup, and a workgroup moves to the next job as soon as the current
+ * one has no shot left, so the slow last shots of one job overlap the next
+ * jobs' work.  Each job's results equal a pel_decode of that batch (no
+ * re-queue is used: the later jobs fill the tail).
+ * No single counterpart in the reference, which decodes one batch of one code
+ * per call; it replaces the difficult-case re-queue and the per-problem launch
+ * configuration around the decode loop (decoder.py:1634-1683, 2890-2937,
+ * 905-969) for callers with several batches in hand. */
+#define PEL_MAX_JOBS 8
+
+typedef struct {
+    int32_t n_jobs;
+    int32_t block;                         /* threads per workgroup */
+    int32_t grid;                          /* persistent workgroups */
+    int32_t lds_bytes;                     /* dynamic LDS per workgroup */
+    int32_t group[PEL_MAX_JOBS];           /* threads per shot of job i: 64 (a shot per wave) or block */
+    int32_t placement[PEL_MAX_JOBS];       /* as pel_plan.placement, for one shot group */
+    int32_t learned[PEL_MAX_JOBS];         /* planned with PEL_PLAN_LEARNED */
+    int32_t job_lds[PEL_MAX_JOBS];         /* LDS bytes of one shot group of job i */
+    int64_t spill_stride[PEL_MAX_JOBS];    /* workspace bytes per shot group of job i */
+    int64_t workspace_bytes[PEL_MAX_JOBS]; /* workspace job i's pel_batch must provide */
+} pel_jobs_plan;
+
+/* Plan a launch over n jobs: graphs[i] with shots[i] shots and plan flags
+ * flags[i] (PEL_PLAN_LEARNED, or 0; flags may be NULL). */
+int pel_plan_jobs(const pel_graph *const *graphs, const int32_t *shots, const int32_t *flags, int32_t n,
+                  pel_jobs_plan *out);
+
+/* Decode n jobs (graphs[i], params[i], batches[i]) with one launch on
+ * `stream`.  Job i's batch must have the shot count and workspace its plan was
+ * made for; batches[0].ev_begin / ev_end, if set, are recorded around the
+ * launch. */
+int pel_decode_jobs(const pel_graph *const *graphs, const pel_params *params, const pel_batch *batches, int32_t n,
+                    const pel_jobs_plan *plan, void *stream);
 
 /* ---- learned memory strengths (no counterpart in the reference) ----------
  * Training entry points for the MLP above.  x [n][8] features, y [n] soft

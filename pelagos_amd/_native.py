@@ -13,7 +13,8 @@ from typing import Optional
 
 LIB_PATH = Path(__file__).resolve().parent / "libpelagos_relaybp.so"
 
-PEL_ABI_VERSION = 5
+PEL_ABI_VERSION = 6
+PEL_MAX_JOBS = 8
 PEL_NOALLOC_MAX_BYTES = 32 << 20
 PEL_PLAN_LEARNED = 1
 PEL_MLP_PARAMS = 32 * 8 + 32 + 32 * 32 + 32 + 32 + 1
@@ -87,6 +88,21 @@ class PelPlan(ctypes.Structure):
     ]
 
 
+class PelJobsPlan(ctypes.Structure):
+    _fields_ = [
+        ("n_jobs", ctypes.c_int32),
+        ("block", ctypes.c_int32),
+        ("grid", ctypes.c_int32),
+        ("lds_bytes", ctypes.c_int32),
+        ("group", ctypes.c_int32 * 8),
+        ("placement", ctypes.c_int32 * 8),
+        ("learned", ctypes.c_int32 * 8),
+        ("job_lds", ctypes.c_int32 * 8),
+        ("spill_stride", ctypes.c_int64 * 8),
+        ("workspace_bytes", ctypes.c_int64 * 8),
+    ]
+
+
 # Every symbol the header declares, with its ctypes signature.
 SIGNATURES = {
     "pel_abi_version": (ctypes.c_int, []),
@@ -104,6 +120,8 @@ SIGNATURES = {
                                   ctypes.POINTER(PelPlan), _vp]),
     "pel_decode_noalloc": (ctypes.c_int, [_vp, ctypes.POINTER(PelParams), ctypes.POINTER(PelBatch),
                                           ctypes.POINTER(PelPlan), _vp]),
+    "pel_plan_jobs": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.POINTER(PelJobsPlan)]),
+    "pel_decode_jobs": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.POINTER(PelJobsPlan), _vp]),
     "pel_mlp_param_count": (ctypes.c_int, []),
     "pel_mlp_last_error": (ctypes.c_char_p, []),
     "pel_mlp_grad_workspace_bytes": (ctypes.c_int64, []),

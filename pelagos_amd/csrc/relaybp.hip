@@ -1374,6 +1374,48 @@ __device__ __forceinline__ int wait_requeued(const KArgs &a, int t2) {
     }
 }
 
+// A shot group's working memory: LDS at the group's base lb, the rest in the
+// group's workspace slot.
+template <int PLACE>
+__device__ __forceinline__ ShotMem<PLACE> shot_mem(const KArgs &a, uint8_t *smem, uint32_t lb, uint8_t *slot) {
+    ShotMem<PLACE> m;
+    m.lb = lb;
+    m.ls = ((PLACE & (1 << kLS)) ? smem + lb : slot) + a.off[kLS];
+    m.cs = ((PLACE & (1 << kCS)) ? smem + lb : slot) + a.off[kCS];
+    m.sb = ((PLACE & (1 << kSB)) ? smem + lb : slot) + a.off[kSB];
+    m.l0g = nullptr;
+    m.lsb = a.g.lsb;
+    m.sbb = a.g.sbb;
+    m.l0b = a.off[kL0];
+    m.cs_shift = a.g.cs_shift;
+    m.first = reinterpret_cast<vec4f *>(slot);
+    m.first_words = (int32_t)(a.g.lsb / 16u);  // CS at 0, SB at sbb, LS at lsb
+    return m;
+}
+
+// The group's copy of the slice descriptor tables, and the sentinels that
+// padding slots point at (no pass ever writes them).  Every later read is
+// separated from these writes by the group barrier at the end of a shot's
+// state reset.
+template <int BLOCK, int PLACE>
+__device__ __forceinline__ void group_setup(const KArgs &a, const ShotMem<PLACE> &m) {
+    const int nrs = (a.g.C + kSlice - 1) / kSlice, nvs = (a.g.V + kSlice - 1) / kSlice;
+    for (int i = gtid<BLOCK>(); i < nrs; i += BLOCK) {
+        const int4 r = a.g.rslice[i];
+        *m.template lds<vec4f>(a.rtab + 16u * (uint32_t)i) =
+            vec4f{__int_as_float(r.x), __int_as_float(r.y), __int_as_float(r.z), __int_as_float(r.w)};
+    }
+    for (int i = gtid<BLOCK>(); i < nvs; i += BLOCK) {
+        const int4 r = a.g.vslice[i];
+        *m.template lds<vec2f>(a.vtab + 8u * (uint32_t)i) = vec2f{__int_as_float(r.x), __int_as_float(r.y)};
+    }
+    if (gleader<BLOCK>()) {
+        m.ls_set(a.g.V, make_float2(INFINITY, 0.f));
+        if (PLACE & (1 << kL0)) m.l0_set(a.g.V, INFINITY);
+        m.cs_set(a.g.C, make_float4(0.f, 0.f, 0.f, 0.f));
+    }
+}
+
 template <int BLOCK, int PLACE, bool LEARN>
 __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
     // All LDS is dynamic (state arrays, then the reduction scratch), so the
@@ -1383,37 +1425,8 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
     int *s_flag = reinterpret_cast<int *>(smem + a.scratch_off + 8u * (BLOCK / 64));
     int *s_next = s_flag + BLOCK / 64;
     uint8_t *slot = a.spill + (int64_t)blockIdx.x * a.spill_stride;
-    ShotMem<PLACE> m;
-    m.lb = 0;  // the workgroup runs one shot at a time
-    m.ls = ((PLACE & (1 << kLS)) ? smem : slot) + a.off[kLS];
-    m.cs = ((PLACE & (1 << kCS)) ? smem : slot) + a.off[kCS];
-    m.sb = ((PLACE & (1 << kSB)) ? smem : slot) + a.off[kSB];
-    m.l0g = nullptr;
-    m.lsb = a.g.lsb;
-    m.sbb = a.g.sbb;
-    m.l0b = a.off[kL0];
-    m.cs_shift = a.g.cs_shift;
-    m.first = reinterpret_cast<vec4f *>(slot);
-    m.first_words = (int32_t)(a.g.lsb / 16u);  // CS at 0, SB at sbb, LS at lsb
-    // This workgroup's copy of the slice descriptor tables.
-    {
-        const int nrs = (a.g.C + kSlice - 1) / kSlice, nvs = (a.g.V + kSlice - 1) / kSlice;
-        for (int i = threadIdx.x; i < nrs; i += BLOCK) {
-            const int4 r = a.g.rslice[i];
-            *lds_at<vec4f>(a.rtab + 16u * (uint32_t)i) =
-                vec4f{__int_as_float(r.x), __int_as_float(r.y), __int_as_float(r.z), __int_as_float(r.w)};
-        }
-        for (int i = threadIdx.x; i < nvs; i += BLOCK) {
-            const int4 r = a.g.vslice[i];
-            *lds_at<vec2f>(a.vtab + 8u * (uint32_t)i) = vec2f{__int_as_float(r.x), __int_as_float(r.y)};
-        }
-    }
-    // Sentinels that padding slots point at; no pass ever writes them.
-    if (opaque_tid() == 0) {
-        m.ls_set(a.g.V, make_float2(INFINITY, 0.f));
-        if (PLACE & (1 << kL0)) m.l0_set(a.g.V, INFINITY);
-        m.cs_set(a.g.C, make_float4(0.f, 0.f, 0.f, 0.f));
-    }
+    ShotMem<PLACE> m = shot_mem<PLACE>(a, smem, 0, slot);  // the workgroup runs one shot at a time
+    group_setup<BLOCK, PLACE>(a, m);
     // The leader writes the finished shot's scalars and takes the next ticket in
     // one branch: first-leg tickets in queue order, then re-queued shots.
     int prev = -1;
@@ -1472,6 +1485,144 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
         relay_legs<BLOCK, PLACE, LEARN>(a, b, m, st, more, s_red, s_flag, stage == 2);
         prev = b;
     }
+}
+
+// ---- one persistent launch over several jobs ----------------------------------------------------
+// A job is one batch of one code (its own graph, parameters, buffers and
+// workspace).  The launch's workgroups (kEngineBlock threads, one per CU)
+// take the jobs in order.  A large code's shots run on the whole workgroup, one
+// first-leg ticket at a time, in the job's syndrome-weight order (as in
+// relay_decode_kernel without re-queue); a small code's shots run one per wave,
+// each wave taking the job's next shot until none is left.  A workgroup moves
+// on when the current job has no shot left, so one job's last, slow shots
+// overlap the next jobs' work instead of leaving CUs idle.
+constexpr int kEngineBlock = 1024;
+constexpr int kEngineWaves = kEngineBlock / 64;
+constexpr int kMaxJobs = PEL_MAX_JOBS;
+
+// A job's shot routine: its placement, plus these flags.
+enum { kKindWave = 16, kKindLearn = 32 };
+
+struct EngineArgs {
+    int32_t n_jobs;
+    uint32_t ctl_lds;           // LDS byte address of the workgroup's two control words
+    int32_t kind[kMaxJobs];     // placement | kKindWave (a shot per wave) | kKindLearn
+    uint32_t stride[kMaxJobs];  // LDS bytes per wave of a shot-per-wave job
+    KArgs a[kMaxJobs];
+};
+static_assert(sizeof(EngineArgs) <= 4096, "the jobs travel as kernel arguments");
+
+// One shot of a job on a group (the workgroup, or BLOCK = 64: one wave at LDS
+// base lb): its whole schedule, then the leader writes its scalars.
+template <int BLOCK, int PLACE, bool LEARN>
+__device__ __forceinline__ void engine_shot(const KArgs &a, int b, uint8_t *smem, uint32_t lb, uint8_t *slot) {
+    ShotMem<PLACE> m = shot_mem<PLACE>(a, smem, lb, slot);
+    double *s_red = reinterpret_cast<double *>(smem + lb + a.scratch_off);
+    int *s_flag = reinterpret_cast<int *>(smem + lb + a.scratch_off + 8u * (BLOCK / 64));
+    group_setup<BLOCK, PLACE>(a, m);
+    ShotState st{0.f, 0, 0, 0, 0, 0, 0};
+    const bool more = first_leg<BLOCK, PLACE, LEARN>(a, b, m, st, s_red, s_flag);
+    relay_legs<BLOCK, PLACE, LEARN>(a, b, m, st, more, s_red, s_flag);
+    if (gleader<BLOCK>()) {
+        a.weights[b] = st.best_w;
+        a.iters[b] = st.iters;
+        a.found[b] = st.found;
+    }
+}
+
+// A shot-per-wave job: every wave of the workgroup takes the job's next shot
+// (lane 0 takes the ticket) until the job has none left.
+template <int PLACE>
+__device__ __forceinline__ void engine_wave_job(const KArgs &a, uint32_t stride, uint8_t *smem) {
+    const uint32_t wave = threadIdx.x >> 6;
+    uint8_t *slot = a.spill + ((int64_t)blockIdx.x * kEngineWaves + wave) * a.spill_stride;
+    for (;;) {
+        int t = 0;
+        if ((opaque_tid() & 63u) == 0) t = atomicAdd(&a.ctrl[kCtrlFirst], 1);
+        t = uniform_i32(t);  // lane 0's ticket (the first active lane)
+        if (t >= a.B) break;
+        engine_shot<64, PLACE, false>(a, a.order[t], smem, wave * stride, slot);
+    }
+}
+
+// Job kinds as bits of an engine instantiation's KINDS mask: the launch uses
+// the smallest instantiation whose routines cover its jobs, so a launch of
+// plain jobs does not carry the learned routines' register pressure.
+constexpr int kind_bit(int kind) {
+    return kind == 15 ? 1 : kind == 7 ? 2 : kind == 3 ? 4 : kind == 1 ? 8 : kind == 0 ? 16
+         : kind == (15 | kKindLearn) ? 32 : kind == (7 | kKindLearn) ? 64 : kind == (15 | kKindWave) ? 128
+         : kind == (7 | kKindWave) ? 256 : 0;
+}
+constexpr int kKindsPlainWg = 1 | 2 | 4 | 8 | 16;
+constexpr int kKindsWave = 128 | 256;
+constexpr int kKindsLearn = 32 | 64;
+constexpr int kKindsAll = kKindsPlainWg | kKindsWave | kKindsLearn;
+// the instantiations, smallest first
+
+template <int KINDS>
+__global__ void __launch_bounds__(kEngineBlock) engine_kernel(EngineArgs) {
+    // The jobs are read in place from the kernel-argument segment (the only
+    // argument sits at its start): naming the by-value argument with a runtime
+    // job index made the compiler copy all of it to private memory.
+    const EngineArgs &e = *(const EngineArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+    extern __shared__ __align__(16) uint8_t smem[];
+    int *ctl = reinterpret_cast<int *>(smem + e.ctl_lds);
+    int j = 0;
+    for (;;) {
+        // The leader finds the first job from j on with a shot left, and takes a
+        // whole-workgroup job's ticket (a shot-per-wave job's waves take theirs).
+        if (opaque_tid() == 0) {
+            int jj = j, t = 0;
+            for (; jj < e.n_jobs; ++jj) {
+                const KArgs &a = e.a[jj];
+                if (a.B <= 0) continue;  // an empty job has no workspace
+                if (e.kind[jj] & kKindWave) {
+                    if (__hip_atomic_load(&a.ctrl[kCtrlFirst], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < a.B) break;
+                } else {
+                    t = atomicAdd(&a.ctrl[kCtrlFirst], 1);
+                    if (t < a.B) break;
+                }
+            }
+            ctl[0] = jj;
+            ctl[1] = t;
+        }
+        __syncthreads();
+        const int jj = uniform_i32(ctl[0]);
+        const int t = uniform_i32(ctl[1]);
+        __syncthreads();
+        if (jj >= e.n_jobs) break;
+        j = jj;
+        const KArgs &a = e.a[j];
+        uint8_t *slot = a.spill + (int64_t)blockIdx.x * a.spill_stride;
+        const int kind = uniform_i32(e.kind[j]);
+        const int b = kind & kKindWave ? 0 : a.order[t];
+#define PEL_ENGINE_CASE(K, CALL) \
+    else if ((KINDS & kind_bit(K)) && kind == (K)) CALL;
+        if (false) {
+        }
+        PEL_ENGINE_CASE(15, (engine_shot<kEngineBlock, 15, false>(a, b, smem, 0, slot)))
+        PEL_ENGINE_CASE(7, (engine_shot<kEngineBlock, 7, false>(a, b, smem, 0, slot)))
+        PEL_ENGINE_CASE(3, (engine_shot<kEngineBlock, 3, false>(a, b, smem, 0, slot)))
+        PEL_ENGINE_CASE(1, (engine_shot<kEngineBlock, 1, false>(a, b, smem, 0, slot)))
+        PEL_ENGINE_CASE(0, (engine_shot<kEngineBlock, 0, false>(a, b, smem, 0, slot)))
+        PEL_ENGINE_CASE(15 | kKindLearn, (engine_shot<kEngineBlock, 15, true>(a, b, smem, 0, slot)))
+        PEL_ENGINE_CASE(7 | kKindLearn, (engine_shot<kEngineBlock, 7, true>(a, b, smem, 0, slot)))
+        PEL_ENGINE_CASE(15 | kKindWave, (engine_wave_job<15>(a, e.stride[j], smem)))
+        PEL_ENGINE_CASE(7 | kKindWave, (engine_wave_job<7>(a, e.stride[j], smem)))
+#undef PEL_ENGINE_CASE
+    }
+}
+
+typedef void (*EngineFn)(EngineArgs);
+
+// The smallest engine instantiation whose routines cover the job kinds in mask.
+EngineFn pick_engine(int mask) {
+    if ((mask & ~1) == 0) return engine_kernel<1>;
+    if ((mask & ~32) == 0) return engine_kernel<32>;
+    if ((mask & ~(1 | 128)) == 0) return engine_kernel<1 | 128>;
+    if ((mask & ~(1 | 8 | 32 | 128)) == 0) return engine_kernel<1 | 8 | 32 | 128>;  // the three-code mix
+    if ((mask & ~(kKindsPlainWg | kKindsWave)) == 0) return engine_kernel<kKindsPlainWg | kKindsWave>;
+    return engine_kernel<kKindsAll>;
 }
 
 // Work-queue order of the first legs: shots by decreasing syndrome weight
@@ -2220,17 +2371,15 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32
 
 namespace {
 
-// pel_decode and pel_decode_noalloc: the same launches, with the workspace from
-// the batch (pel_decode) or the library's per-GPU scratch array (noalloc).
-int decode_impl(const char *who, const pel_graph *g, const pel_params *p, const pel_batch *b, const pel_plan *plan,
-                bool noalloc, void *stream) {
-    const std::string w_ = who;
-    if (!g || !p || !b || !plan) return fail(PEL_EINVAL, w_ + ": NULL argument");
+// Checks of one batch that do not depend on the launch layout.
+int check_batch(const std::string &w_, const pel_graph *g, const pel_params *p, const pel_batch *b, bool learned_plan,
+                bool noalloc) {
+    if (!g || !p || !b) return fail(PEL_EINVAL, w_ + ": NULL argument");
     if (b->B < 0) return fail(PEL_EINVAL, w_ + ": negative batch");
-    if (b->B == 0) return PEL_OK;
     if (p->pre_iter < 0 || p->num_sets < 0 || p->set_max_iter < 0)
         return fail(PEL_EINVAL, w_ + ": iteration counts must be >= 0");
     if (p->stop_nconv < 1) return fail(PEL_EINVAL, w_ + ": stop_nconv must be >= 1");
+    if (b->B == 0) return PEL_OK;
     if (!b->syndromes || !b->prior_llr || !b->wj || !b->errors || !b->weights || !b->iterations ||
         !b->found_count || (!noalloc && !b->workspace))
         return fail(PEL_EINVAL, w_ + ": NULL buffer");
@@ -2242,7 +2391,7 @@ int decode_impl(const char *who, const pel_graph *g, const pel_params *p, const 
         return fail(PEL_EINVAL, w_ + ": per-shot gamma rows are taken in the caller's order");
     if (p->use_alpha && !p->alpha_t) return fail(PEL_EINVAL, w_ + ": alpha_t is NULL with use_alpha");
     if (b->learned_legs < 0) return fail(PEL_EINVAL, w_ + ": learned_legs must be >= 0");
-    if ((b->learned_legs > 0 || b->mlp || b->feat_out || b->hard_out || b->gamma_out) && !plan->learned)
+    if ((b->learned_legs > 0 || b->mlp || b->feat_out || b->hard_out || b->gamma_out) && !learned_plan)
         return fail(PEL_EINVAL, w_ + ": learned strengths need a plan made with PEL_PLAN_LEARNED");
     if ((b->mlp || b->feat_out || b->hard_out || b->gamma_out) && b->learned_legs == 0)
         return fail(PEL_EINVAL, w_ + ": learned strengths or records with learned_legs == 0");
@@ -2256,33 +2405,28 @@ int decode_impl(const char *who, const pel_graph *g, const pel_params *p, const 
     int dev = 0;
     PEL_HIP(hipGetDevice(&dev));
     if (dev != g->device) return fail(PEL_EINVAL, w_ + ": current device differs from the graph's");
-    bool known = false;
-    for (int pl : kPlacements) known |= (pl == plan->placement);
-    const StateSizes st = g->state();
-    if (plan->block != 64 && plan->block != 128 && plan->block != 256 && plan->block != 512 && plan->block != 1024)
-        return fail(PEL_EINVAL, w_ + ": plan block must be 64, 128, 256, 512 or 1024");
-    uint32_t offs[kNumArrays], scr_off = 0;
-    int64_t lds = 0, spill = 0;
-    const bool learn = plan->learned != 0;
+    return PEL_OK;
+}
+
+// A shot group's layout: the arrays' offsets, the reduction scratch and the
+// learned-strength arrays in LDS, the LDS and workspace-slot bytes.
+struct GroupLayout {
+    uint32_t offs[kNumArrays], scr_off;
+    int64_t lds, spill;
     LearnLds ll;
-    place_arrays(st, plan->placement, plan->block, learn, offs, &scr_off, &lds, &spill, &ll);
-    if (plan->requeue_below < 0) return fail(PEL_EINVAL, w_ + ": requeue_below must be >= 0");
-    const WsLayout w = ws_layout(g, b->B, plan->requeue_below > 0, learn);
-    const int64_t need = w.total_fixed + plan->spill_stride * plan->grid;
-    if (!known || (learn && (plan->placement & 7) != 7) || lds != plan->lds_bytes ||
-        align_up(spill, 256) != plan->spill_stride || plan->grid <= 0 || (!noalloc && plan->workspace_bytes < need))
-        return fail(PEL_EINVAL, w_ + ": plan does not match this graph and batch size");
-    uint8_t *ws = static_cast<uint8_t *>(b->workspace);
-    if (noalloc) {
-        if (need > (int64_t)PEL_NOALLOC_MAX_BYTES)
-            return fail(PEL_EINVAL, w_ + ": this decode needs " + std::to_string(need) +
-                                        " bytes of workspace, more than the " + std::to_string(PEL_NOALLOC_MAX_BYTES) +
-                                        " the library holds per GPU (use fewer shots, a plan with requeue_below = 0, "
-                                        "or pel_decode with a workspace)");
-        const int rc = scratch_on_device(dev, &ws);
-        if (rc != PEL_OK) return rc;
-    }
-    KArgs a;
+};
+
+GroupLayout group_layout(const pel_graph *g, int place, int group, bool learn) {
+    GroupLayout L;
+    place_arrays(g->state(), place, group, learn, L.offs, &L.scr_off, &L.lds, &L.spill, &L.ll);
+    return L;
+}
+
+// The kernel arguments of one batch with workspace ws laid out as w.
+void fill_args(KArgs &a, const pel_graph *g, const pel_params *p, const pel_batch *b, uint8_t *ws,
+               const WsLayout &w, bool learn, int32_t requeue_below, int group, const GroupLayout &L,
+               int64_t spill_stride) {
+    const StateSizes st = g->state();
     memset(&a, 0, sizeof a);
     a.g = g->d;
     a.B = b->B;
@@ -2318,40 +2462,46 @@ int decode_impl(const char *who, const pel_graph *g, const pel_params *p, const 
     a.last_post = b->last_posterior;
     a.shot_clock = b->shot_clock;
     a.ctrl = reinterpret_cast<int32_t *>(ws);
-    int32_t *order = reinterpret_cast<int32_t *>(ws + w.order);
-    int32_t *key = reinterpret_cast<int32_t *>(ws + w.key);
-    a.order = order;
-    a.requeue_below = plan->requeue_below;
+    a.order = reinterpret_cast<int32_t *>(ws + w.order);
+    a.requeue_below = requeue_below;
     a.rq_slot = reinterpret_cast<int32_t *>(ws + w.rq_slot);
     a.rq_state = reinterpret_cast<int4 *>(ws + w.rq_state);
-    a.rq_post = plan->requeue_below > 0 ? reinterpret_cast<float *>(ws + w.rq_post) : nullptr;
-    a.rq_gamma = plan->requeue_below > 0 && learn ? reinterpret_cast<int8_t *>(ws + w.rq_gamma) : nullptr;
+    a.rq_post = requeue_below > 0 ? reinterpret_cast<float *>(ws + w.rq_post) : nullptr;
+    a.rq_gamma = requeue_below > 0 && learn ? reinterpret_cast<int8_t *>(ws + w.rq_gamma) : nullptr;
     a.mlp_frag = learn ? reinterpret_cast<const uint32_t *>(ws + w.mlp_frag) : nullptr;
     a.mlp = b->mlp;
     a.learned_legs = learn ? b->learned_legs : 0;
     a.feat_out = b->feat_out;
     a.hard_out = b->hard_out;
     a.gamma_out = b->gamma_out;
-    a.flips_lds = ll.flips;
-    a.unsat_lds = ll.unsat;
-    a.gamma_lds = ll.gamma;
+    a.flips_lds = L.ll.flips;
+    a.unsat_lds = L.ll.unsat;
+    a.gamma_lds = L.ll.gamma;
     a.learned_lo = b->learned_lo;
     a.learned_hi = b->learned_hi;
     a.last_scale = 1.f - b->learned_last_damp;
     a.spill = ws + w.spill;
-    a.spill_stride = plan->spill_stride;
-    for (int k = 0; k < kNumArrays; ++k) a.off[k] = offs[k];
-    a.scratch_off = scr_off;
-    a.rtab = row_table_addr(scr_off, plan->block);
+    a.spill_stride = spill_stride;
+    for (int k = 0; k < kNumArrays; ++k) a.off[k] = L.offs[k];
+    a.scratch_off = L.scr_off;
+    a.rtab = row_table_addr(L.scr_off, group);
     a.vtab = a.rtab + (uint32_t)(16LL * st.nrs);
     a.rowinfo = a.vtab + (uint32_t)align_up(8LL * st.nvs, 16);
     a.best_lds = a.rowinfo + (uint32_t)align_up(st.C, 16);
-    hipStream_t s = static_cast<hipStream_t>(stream);
+}
+
+// The per-decode launches before the decode kernel: syndrome weights and the
+// first-leg queue order (which also reset the queue counters), the shared
+// weights and priors in position order, and the network's operand layout.
+int launch_prep(const pel_graph *g, const pel_batch *b, const KArgs &a, uint8_t *ws, const WsLayout &w,
+                hipStream_t s) {
+    int32_t *key = reinterpret_cast<int32_t *>(ws + w.key);
     const int key_blocks = (int)((b->B + kOrderBlock / 64 - 1) / (kOrderBlock / 64));
     hipLaunchKernelGGL(syndrome_key_kernel, dim3(key_blocks), dim3(kOrderBlock), 0, s, b->syndromes, b->B, g->C,
                        key, a.rq_slot);
     PEL_HIP(hipGetLastError());
-    hipLaunchKernelGGL(order_shots_kernel, dim3(1), dim3(kOrderBlock), 0, s, key, b->B, order, a.ctrl);
+    hipLaunchKernelGGL(order_shots_kernel, dim3(1), dim3(kOrderBlock), 0, s, key, b->B, const_cast<int32_t *>(a.order),
+                       a.ctrl);
     PEL_HIP(hipGetLastError());
     if (!b->prior_batched) {  // the shared weights and priors in position order, once per decode
         const unsigned pblocks = (unsigned)std::min<int64_t>((g->V + 255) / 256, 1024);
@@ -2367,11 +2517,198 @@ int decode_impl(const char *who, const pel_graph *g, const pel_params *p, const 
                            reinterpret_cast<uint32_t *>(ws + w.mlp_frag));
         PEL_HIP(hipGetLastError());
     }
+    return PEL_OK;
+}
+
+// pel_decode and pel_decode_noalloc: the same launches, with the workspace from
+// the batch (pel_decode) or the library's per-GPU scratch array (noalloc).
+int decode_impl(const char *who, const pel_graph *g, const pel_params *p, const pel_batch *b, const pel_plan *plan,
+                bool noalloc, void *stream) {
+    const std::string w_ = who;
+    if (!plan) return fail(PEL_EINVAL, w_ + ": NULL argument");
+    int rc = check_batch(w_, g, p, b, plan->learned != 0, noalloc);
+    if (rc != PEL_OK || b->B == 0) return rc;
+    bool known = false;
+    for (int pl : kPlacements) known |= (pl == plan->placement);
+    if (plan->block != 64 && plan->block != 128 && plan->block != 256 && plan->block != 512 && plan->block != 1024)
+        return fail(PEL_EINVAL, w_ + ": plan block must be 64, 128, 256, 512 or 1024");
+    const bool learn = plan->learned != 0;
+    const GroupLayout L = group_layout(g, plan->placement, plan->block, learn);
+    if (plan->requeue_below < 0) return fail(PEL_EINVAL, w_ + ": requeue_below must be >= 0");
+    const WsLayout w = ws_layout(g, b->B, plan->requeue_below > 0, learn);
+    const int64_t need = w.total_fixed + plan->spill_stride * plan->grid;
+    if (!known || (learn && (plan->placement & 7) != 7) || L.lds != plan->lds_bytes ||
+        align_up(L.spill, 256) != plan->spill_stride || plan->grid <= 0 || (!noalloc && plan->workspace_bytes < need))
+        return fail(PEL_EINVAL, w_ + ": plan does not match this graph and batch size");
+    uint8_t *ws = static_cast<uint8_t *>(b->workspace);
+    if (noalloc) {
+        if (need > (int64_t)PEL_NOALLOC_MAX_BYTES)
+            return fail(PEL_EINVAL, w_ + ": this decode needs " + std::to_string(need) +
+                                        " bytes of workspace, more than the " + std::to_string(PEL_NOALLOC_MAX_BYTES) +
+                                        " the library holds per GPU (use fewer shots, a plan with requeue_below = 0, "
+                                        "or pel_decode with a workspace)");
+        int dev = 0;
+        PEL_HIP(hipGetDevice(&dev));
+        rc = scratch_on_device(dev, &ws);
+        if (rc != PEL_OK) return rc;
+    }
+    KArgs a;
+    fill_args(a, g, p, b, ws, w, learn, plan->requeue_below, plan->block, L, plan->spill_stride);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    rc = launch_prep(g, b, a, ws, w, s);
+    if (rc != PEL_OK) return rc;
     if (b->ev_begin) PEL_HIP(hipEventRecord(static_cast<hipEvent_t>(b->ev_begin), s));
     KernelFn fn = pick_kernel(plan->block, plan->placement, learn);
     hipLaunchKernelGGL(fn, dim3(plan->grid), dim3(plan->block), (size_t)plan->lds_bytes, s, a);
     PEL_HIP(hipGetLastError());
     if (b->ev_end) PEL_HIP(hipEventRecord(static_cast<hipEvent_t>(b->ev_end), s));
+    return PEL_OK;
+}
+
+// The role of one job in an engine launch: a shot per wave (a small code whose
+// single-shot plan is one wave, in a fast placement that fits sixteen times in
+// the LDS budget), else a shot per workgroup in the largest placement that
+// fits (a fast one for learned strengths).  Returns the LDS bytes the job needs
+// (per workgroup), or -1 when it does not fit.
+int64_t engine_role(const pel_graph *g, bool learn, int64_t budget, int *group, int *place, GroupLayout *L) {
+    const int want = std::max(g->C, g->V / 8);
+    if (!learn && want <= 64) {
+        for (int p : {15, 7}) {
+            const GroupLayout l = group_layout(g, p, 64, false);
+            if (kEngineWaves * align_up(l.lds, 16) <= budget) {
+                *group = 64;
+                *place = p;
+                *L = l;
+                return kEngineWaves * align_up(l.lds, 16);
+            }
+        }
+    }
+    int best = -1;
+    GroupLayout lb{};
+    for (int p : kPlacements) {
+        if (learn && (p & 7) != 7) continue;
+        const GroupLayout l = group_layout(g, p, kEngineBlock, learn);
+        if (l.lds <= budget) {
+            best = p;
+            lb = l;
+        }
+    }
+    if (best < 0) return -1;
+    *group = kEngineBlock;
+    *place = best;
+    *L = lb;
+    return lb.lds;
+}
+
+int plan_jobs_impl(const pel_graph *const *graphs, const int32_t *shots, const int32_t *flags, int32_t n,
+                   pel_jobs_plan *out) {
+    if (!graphs || !shots || !out) return fail(PEL_EINVAL, "pel_plan_jobs: NULL argument");
+    if (n < 1 || n > kMaxJobs) return fail(PEL_EINVAL, "pel_plan_jobs: between 1 and PEL_MAX_JOBS jobs");
+    int dev = 0, max_lds = 0, n_cu = 0;
+    PEL_HIP(hipGetDevice(&dev));
+    PEL_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    PEL_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    memset(out, 0, sizeof *out);
+    const int64_t budget = (int64_t)max_lds - 16;  // the control words sit after every job's LDS
+    int64_t lds = 0, units = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const pel_graph *g = graphs[i];
+        if (!g) return fail(PEL_EINVAL, "pel_plan_jobs: NULL graph");
+        if (g->device != dev) return fail(PEL_EINVAL, "pel_plan_jobs: a graph is on another device");
+        if (shots[i] < 0) return fail(PEL_EINVAL, "pel_plan_jobs: negative batch");
+        const int32_t fl = flags ? flags[i] : 0;
+        if (fl & ~PEL_PLAN_LEARNED) return fail(PEL_EINVAL, "pel_plan_jobs: unknown flags");
+        const bool learn = (fl & PEL_PLAN_LEARNED) != 0;
+        int group = 0, place = -1;
+        GroupLayout L;
+        const int64_t need = engine_role(g, learn, budget, &group, &place, &L);
+        if (need < 0) return fail(PEL_EINVAL, "pel_plan_jobs: a job's state does not fit the LDS budget");
+        out->group[i] = group;
+        out->placement[i] = place;
+        out->learned[i] = learn ? 1 : 0;
+        out->job_lds[i] = (int32_t)L.lds;
+        out->spill_stride[i] = align_up(L.spill, 256);
+        lds = std::max(lds, need);
+        units += group == 64 ? (shots[i] + kEngineWaves - 1) / kEngineWaves : shots[i];
+    }
+    lds = align_up(lds, 16) + 16;
+    int mask = 0;
+    for (int32_t i = 0; i < n; ++i)
+        mask |= kind_bit(out->placement[i] | (out->group[i] == 64 ? kKindWave : 0) | (out->learned[i] ? kKindLearn : 0));
+    const void *fn = reinterpret_cast<const void *>(pick_engine(mask));
+    hipFuncAttributes fa;
+    PEL_HIP(hipFuncGetAttributes(&fa, fn));
+    if (fa.sharedSizeBytes != 0) return fail(PEL_EDEVICE, "pel_plan_jobs: the engine kernel has static LDS");
+    if (lds > 65536) {
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipGetLastError();
+    }
+    int per_cu = 0;
+    PEL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kEngineBlock, (size_t)lds));
+    if (per_cu <= 0) return fail(PEL_EDEVICE, "pel_plan_jobs: the engine kernel cannot be resident");
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * n_cu, units));
+    out->n_jobs = n;
+    out->block = kEngineBlock;
+    out->grid = (int32_t)grid;
+    out->lds_bytes = (int32_t)lds;
+    for (int32_t i = 0; i < n; ++i) {
+        const bool learn = out->learned[i] != 0;
+        const int64_t slots = grid * (out->group[i] == 64 ? kEngineWaves : 1);
+        out->workspace_bytes[i] = ws_layout(graphs[i], shots[i], false, learn).total_fixed + out->spill_stride[i] * slots;
+    }
+    return PEL_OK;
+}
+
+int decode_jobs_impl(const pel_graph *const *graphs, const pel_params *params, const pel_batch *batches, int32_t n,
+                     const pel_jobs_plan *plan, void *stream) {
+    const std::string w_ = "pel_decode_jobs";
+    if (!graphs || !params || !batches || !plan) return fail(PEL_EINVAL, w_ + ": NULL argument");
+    if (n < 1 || n > kMaxJobs || plan->n_jobs != n || plan->block != kEngineBlock || plan->grid <= 0)
+        return fail(PEL_EINVAL, w_ + ": plan does not match these jobs");
+    EngineArgs e;
+    memset(&e, 0, sizeof e);
+    e.n_jobs = n;
+    e.ctl_lds = (uint32_t)(plan->lds_bytes - 16);
+    int64_t lds_need = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const pel_graph *g = graphs[i];
+        const pel_batch *b = &batches[i];
+        const bool learn = plan->learned[i] != 0;
+        int rc = check_batch(w_, g, &params[i], b, learn, false);
+        if (rc != PEL_OK) return rc;
+        const int group = plan->group[i], place = plan->placement[i];
+        bool known = false;
+        for (int pl : kPlacements) known |= (pl == place);
+        if (!known || (group != 64 && group != kEngineBlock) || (group == 64 && (learn || (place & 7) != 7)) ||
+            (learn && (place & 7) != 7))
+            return fail(PEL_EINVAL, w_ + ": plan does not match these jobs");
+        const GroupLayout L = group_layout(g, place, group, learn);
+        const WsLayout w = ws_layout(g, b->B, false, learn);
+        const int64_t slots = (int64_t)plan->grid * (group == 64 ? kEngineWaves : 1);
+        if (L.lds != plan->job_lds[i] || align_up(L.spill, 256) != plan->spill_stride[i] ||
+            plan->workspace_bytes[i] < w.total_fixed + plan->spill_stride[i] * slots)
+            return fail(PEL_EINVAL, w_ + ": plan does not match these jobs");
+        lds_need = std::max(lds_need, group == 64 ? kEngineWaves * align_up(L.lds, 16) : L.lds);
+        fill_args(e.a[i], g, &params[i], b, static_cast<uint8_t *>(b->workspace), w, learn, 0, group, L,
+                  plan->spill_stride[i]);
+        e.a[i].shot_clock = nullptr;
+        e.kind[i] = place | (group == 64 ? kKindWave : 0) | (learn ? kKindLearn : 0);
+        e.stride[i] = (uint32_t)align_up(L.lds, 16);
+    }
+    if (lds_need + 16 > plan->lds_bytes) return fail(PEL_EINVAL, w_ + ": plan does not match these jobs");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    for (int32_t i = 0; i < n; ++i) {
+        if (batches[i].B == 0) continue;
+        const WsLayout w = ws_layout(graphs[i], batches[i].B, false, plan->learned[i] != 0);
+        const int rc = launch_prep(graphs[i], &batches[i], e.a[i], static_cast<uint8_t *>(batches[i].workspace), w, s);
+        if (rc != PEL_OK) return rc;
+    }
+    if (batches[0].ev_begin) PEL_HIP(hipEventRecord(static_cast<hipEvent_t>(batches[0].ev_begin), s));
+    int mask = 0;
+    for (int32_t i = 0; i < n; ++i) mask |= kind_bit(e.kind[i]);
+    hipLaunchKernelGGL(pick_engine(mask), dim3(plan->grid), dim3(kEngineBlock), (size_t)plan->lds_bytes, s, e);
+    PEL_HIP(hipGetLastError());
+    if (batches[0].ev_end) PEL_HIP(hipEventRecord(static_cast<hipEvent_t>(batches[0].ev_end), s));
     return PEL_OK;
 }
 
@@ -2386,6 +2723,16 @@ int pel_decode(const pel_graph *g, const pel_params *p, const pel_batch *b, cons
 int pel_decode_noalloc(const pel_graph *g, const pel_params *p, const pel_batch *b, const pel_plan *plan,
                        void *stream) {
     return decode_impl("pel_decode_noalloc", g, p, b, plan, true, stream);
+}
+
+int pel_plan_jobs(const pel_graph *const *graphs, const int32_t *shots, const int32_t *flags, int32_t n,
+                  pel_jobs_plan *out) {
+    return plan_jobs_impl(graphs, shots, flags, n, out);
+}
+
+int pel_decode_jobs(const pel_graph *const *graphs, const pel_params *params, const pel_batch *batches, int32_t n,
+                    const pel_jobs_plan *plan, void *stream) {
+    return decode_jobs_impl(graphs, params, batches, n, plan, stream);
 }
 
 }  // extern "C"

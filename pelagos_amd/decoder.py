@@ -613,6 +613,28 @@ class RelayBPDecoder:
             ws = workspace
         else:
             ws = self._workspace_for(plan.workspace_bytes)
+        params, batch = self._args(syn, prior, wj, prior_batched, gtab, ig, ip, errors, weights, iters, found,
+                                   last_post, pre_iter, skip_relay_legs, B, learn, ws)
+        cur = stream if stream is not None else torch.cuda.current_stream(self.device)
+        ev = self.kernel_events
+        if ev is not None:
+            # The library records the pair right around the decode kernel on this
+            # stream; recording them once here creates the underlying HIP events.
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(cur)
+            e1.record(cur)
+            batch.ev_begin = e0.cuda_event
+            batch.ev_end = e1.cuda_event
+        fn = _native.lib().pel_decode_noalloc if noalloc else _native.lib().pel_decode
+        _native.check(fn(self._handle.h, ctypes.byref(params), ctypes.byref(batch), ctypes.byref(plan),
+                         ctypes.c_void_p(cur.cuda_stream)), "pel_decode_noalloc" if noalloc else "pel_decode")
+        if ev is not None:
+            ev.append((e0, e1))
+
+    def _args(self, syn, prior, wj, prior_batched, gtab, ig, ip, errors, weights, iters, found, last_post,
+              pre_iter, skip_relay_legs, B, learn, ws):
+        """The (pel_params, pel_batch) pair of one decode."""
         per_row = gtab.dim() == 3
         # the kernel reads rows 0 .. num_sets-1 (and, per row, B shots) of the table
         if (self.num_sets > 0 and gtab.shape[0] < self.num_sets) or (per_row and gtab.shape[1] != B) \
@@ -652,7 +674,7 @@ class RelayBPDecoder:
             found_count=found.data_ptr(),
             last_posterior=_ptr(last_post),
             workspace=None if ws is None else ws.data_ptr(),
-            shot_clock=_ptr(self._shot_clock_for(B)),
+            shot_clock=_ptr(self._shot_clock_for(B)) if B > 0 else None,
         )
         if learn is not None:
             batch.mlp = _ptr(learn.get("mlp"))
@@ -663,22 +685,20 @@ class RelayBPDecoder:
             batch.learned_lo = float(self.gamma_dist_interval[0])
             batch.learned_hi = float(self.gamma_dist_interval[1])
             batch.learned_last_damp = float(learn.get("last_damp", self.learned_last_damp))
-        cur = stream if stream is not None else torch.cuda.current_stream(self.device)
-        ev = self.kernel_events
-        if ev is not None:
-            # The library records the pair right around the decode kernel on this
-            # stream; recording them once here creates the underlying HIP events.
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(cur)
-            e1.record(cur)
-            batch.ev_begin = e0.cuda_event
-            batch.ev_end = e1.cuda_event
-        fn = _native.lib().pel_decode_noalloc if noalloc else _native.lib().pel_decode
-        _native.check(fn(self._handle.h, ctypes.byref(params), ctypes.byref(batch), ctypes.byref(plan),
-                         ctypes.c_void_p(cur.cuda_stream)), "pel_decode_noalloc" if noalloc else "pel_decode")
-        if ev is not None:
-            ev.append((e0, e1))
+        return params, batch
+
+    def job_args(self, syndromes: torch.Tensor, errors: torch.Tensor, weights: torch.Tensor,
+                 iterations: torch.Tensor, found_count: torch.Tensor, workspace: torch.Tensor):
+        """(pel_params, pel_batch) of one job of a multi-job launch (``pel_decode_jobs``):
+        a decode of device-resident ``syndromes`` into caller-owned buffers, as
+        ``decode_async`` makes it.  Call prepare(B) first."""
+        B = int(syndromes.shape[0])
+        learn = None
+        if self.learned_params is not None and self.learned_legs > 0:
+            learn = dict(mlp=self.learned_params, legs=self.learned_legs)
+        return self._args(syndromes, self.log_prior_ratios_per_gpu[0].reshape(self.V), self.wj, False,
+                          self.gamma_table(B), None, None, errors, weights, iterations, found_count, None,
+                          self.pre_iter, False, B, learn, workspace)
 
     def _assemble(self, errors, weights, iters, found, last_post) -> Dict[str, torch.Tensor]:
         unconverged = torch.nonzero(found < self.stop_nconv, as_tuple=False).flatten().to(torch.int32)

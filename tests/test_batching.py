@@ -1,8 +1,9 @@
-"""Several codes in one captured graph launch, and the launch-shape autotuner.
+"""Several codes and batches in one persistent launch (pel_decode_jobs), also
+captured in a graph, and the launch-shape autotuner.
 
-The graph is replayed several times with fresh syndromes and ordinary eager
-decodes in between, and every replay is checked against the eager decode of
-the same syndromes (bit for bit)."""
+Every job's results are checked against an ordinary decode() of the same
+syndromes, bit for bit; the graph is replayed several times with fresh
+syndromes and eager decodes in between."""
 
 from __future__ import annotations
 
@@ -97,3 +98,64 @@ def test_decode_async_matches_decode_and_checks_its_buffers():
             dec.decode_async(raw, out["errors"], out["weights"], out["iterations"], out["found_count"], ws, s)
     with pytest.raises(ValueError, match="iterations"):
         dec.decode_async(syn, out["errors"], out["weights"], out["iterations"].long(), out["found_count"], ws, s)
+
+
+def _check_jobs(jl, syn, decs):
+    for name, dec in decs.items():
+        if jl.shots[name] == 0:
+            continue
+        ref = dec.decode(syn[name])
+        torch.cuda.synchronize()
+        for key in ("errors", "weights", "iterations", "found_count"):
+            assert torch.equal(jl.out[name][key], ref[key]), (name, key)
+
+
+def test_one_launch_over_mixed_jobs_equals_separate_decodes():
+    """One pel_decode_jobs launch over the [[72,12,6]] code (a shot per wave),
+    two banded codes (a shot per workgroup), a learned-strength job and an
+    empty job: every job's results equal its own decode(), on two launches
+    with fresh syndromes."""
+    from pelagos_amd import RelayBPDecoder, learned
+    from pelagos_amd.batching import JobLaunch
+
+    cs = _codes()
+    H4, p4 = cs["banded4"]
+    decs = {k: RelayBPDecoder(H, p, **RELAY) for k, (H, p) in cs.items()}
+    decs["learned4"] = RelayBPDecoder(H4, p4, learned_gamma=learned.GammaMLP(seed=2), learned_legs=2, **RELAY)
+    decs["empty"] = RelayBPDecoder(H4, p4, **RELAY)
+    shots = {"bb72": 300, "banded4": 50, "empty": 0, "learned4": 40, "banded6": 20}
+    jl = JobLaunch([(k, decs[k], shots[k]) for k in ("bb72", "banded4", "empty", "learned4", "banded6")])
+    roles = jl.roles()
+    assert roles["bb72"]["group"] == 64 and roles["banded4"]["group"] == 1024 and roles["learned4"]["learned"]
+    rng = np.random.default_rng(3)
+    for rep in range(2):
+        syn = {}
+        for k, dec in decs.items():
+            H, p = (H4, p4) if k in ("learned4", "empty") else cs[k]
+            err = codes.sample_errors(p, shots[k], seed=int(rng.integers(1 << 30)), scale=1.5)
+            syn[k] = torch.from_numpy(codes.syndromes_of(H, err)).cuda().contiguous()
+            jl.syndromes[k].copy_(syn[k])
+        print(f"[gpu] job launch rep {rep}", flush=True)
+        jl.launch()
+        torch.cuda.synchronize()
+        _check_jobs(jl, syn, decs)
+
+
+def test_one_launch_over_batches_of_one_code():
+    """Five batches of the 936 x 8785 code in one launch (the later batches fill
+    the earlier ones' tails): each equals its own decode()."""
+    from pelagos_amd import RelayBPDecoder
+    from pelagos_amd.batching import JobLaunch
+
+    H, p = codes.banded_circuit_code(seed=0)
+    dec = RelayBPDecoder(H, p, **dict(RELAY, pre_iter=80, num_sets=100, set_max_iter=60, stop_nconv=5, seed=0))
+    jl = JobLaunch([(f"b{i}", dec, 96) for i in range(5)])
+    syn = {}
+    for i in range(5):
+        err = codes.sample_errors(p, 96, seed=60 + i)
+        syn[f"b{i}"] = torch.from_numpy(codes.syndromes_of(H, err)).cuda().contiguous()
+        jl.syndromes[f"b{i}"].copy_(syn[f"b{i}"])
+    print("[gpu] five batches in one launch", flush=True)
+    jl.launch()
+    torch.cuda.synchronize()
+    _check_jobs(jl, syn, {f"b{i}": dec for i in range(5)})
