@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define PEL_ABI_VERSION 6
+#define PEL_ABI_VERSION 7
 
 /* status codes */
 #define PEL_OK 0
@@ -204,10 +204,12 @@ int pel_decode_noalloc(const pel_graph *g, const pel_params *p, const pel_batch 
  * per call; it replaces the difficult-case re-queue and the per-problem launch
  * configuration around the decode loop (decoder.py:1634-1683, 2890-2937,
  * 905-969) for callers with several batches in hand. */
-#define PEL_MAX_JOBS 8
+#define PEL_MAX_JOBS 64
 
 typedef struct {
     int32_t n_jobs;
+    int32_t pad;
+    int64_t table_bytes;                   /* device job table pel_decode_jobs needs */
     int32_t block;                         /* threads per workgroup */
     int32_t grid;                          /* persistent workgroups */
     int32_t lds_bytes;                     /* dynamic LDS per workgroup */
@@ -226,10 +228,12 @@ int pel_plan_jobs(const pel_graph *const *graphs, const int32_t *shots, const in
 
 /* Decode n jobs (graphs[i], params[i], batches[i]) with one launch on
  * `stream`.  Job i's batch must have the shot count and workspace its plan was
- * made for; batches[0].ev_begin / ev_end, if set, are recorded around the
- * launch. */
+ * made for; `table` is device memory of plan->table_bytes for the jobs'
+ * descriptors (written on the stream before the launch, so it may be reused
+ * by the next call on the same stream); batches[0].ev_begin / ev_end, if set,
+ * are recorded around the decode launch. */
 int pel_decode_jobs(const pel_graph *const *graphs, const pel_params *params, const pel_batch *batches, int32_t n,
-                    const pel_jobs_plan *plan, void *stream);
+                    const pel_jobs_plan *plan, void *table, void *stream);
 
 /* ---- learned memory strengths (no counterpart in the reference) ----------
  * Training entry points for the MLP above.  x [n][8] features, y [n] soft

@@ -13,8 +13,8 @@ from typing import Optional
 
 LIB_PATH = Path(__file__).resolve().parent / "libpelagos_relaybp.so"
 
-PEL_ABI_VERSION = 6
-PEL_MAX_JOBS = 8
+PEL_ABI_VERSION = 7
+PEL_MAX_JOBS = 64
 PEL_NOALLOC_MAX_BYTES = 32 << 20
 PEL_PLAN_LEARNED = 1
 PEL_MLP_PARAMS = 32 * 8 + 32 + 32 * 32 + 32 + 32 + 1
@@ -91,15 +91,17 @@ class PelPlan(ctypes.Structure):
 class PelJobsPlan(ctypes.Structure):
     _fields_ = [
         ("n_jobs", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("table_bytes", ctypes.c_int64),
         ("block", ctypes.c_int32),
         ("grid", ctypes.c_int32),
         ("lds_bytes", ctypes.c_int32),
-        ("group", ctypes.c_int32 * 8),
-        ("placement", ctypes.c_int32 * 8),
-        ("learned", ctypes.c_int32 * 8),
-        ("job_lds", ctypes.c_int32 * 8),
-        ("spill_stride", ctypes.c_int64 * 8),
-        ("workspace_bytes", ctypes.c_int64 * 8),
+        ("group", ctypes.c_int32 * PEL_MAX_JOBS),
+        ("placement", ctypes.c_int32 * PEL_MAX_JOBS),
+        ("learned", ctypes.c_int32 * PEL_MAX_JOBS),
+        ("job_lds", ctypes.c_int32 * PEL_MAX_JOBS),
+        ("spill_stride", ctypes.c_int64 * PEL_MAX_JOBS),
+        ("workspace_bytes", ctypes.c_int64 * PEL_MAX_JOBS),
     ]
 
 
@@ -121,7 +123,7 @@ SIGNATURES = {
     "pel_decode_noalloc": (ctypes.c_int, [_vp, ctypes.POINTER(PelParams), ctypes.POINTER(PelBatch),
                                           ctypes.POINTER(PelPlan), _vp]),
     "pel_plan_jobs": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.POINTER(PelJobsPlan)]),
-    "pel_decode_jobs": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.POINTER(PelJobsPlan), _vp]),
+    "pel_decode_jobs": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.POINTER(PelJobsPlan), _vp, _vp]),
     "pel_mlp_param_count": (ctypes.c_int, []),
     "pel_mlp_last_error": (ctypes.c_char_p, []),
     "pel_mlp_grad_workspace_bytes": (ctypes.c_int64, []),

@@ -82,6 +82,7 @@ class JobLaunch:
             batches.append(bat)
         self._params = (_native.PelParams * n)(*params)
         self._batches = (_native.PelBatch * n)(*batches)
+        self._table = torch.empty(int(self.plan.table_bytes), dtype=torch.uint8, device=dev)
         self.graph: Optional[torch.cuda.CUDAGraph] = None
         self.kernel_events: Optional[list] = None
 
@@ -107,6 +108,7 @@ class JobLaunch:
             with torch.cuda.device(self.device):
                 _native.check(_native.lib().pel_decode_jobs(self._graphs, self._params, self._batches,
                                                             len(self.names), ctypes.byref(self.plan),
+                                                            ctypes.c_void_p(self._table.data_ptr()),
                                                             ctypes.c_void_p(cur.cuda_stream)), "pel_decode_jobs")
         finally:
             b0.ev_begin = b0.ev_end = None

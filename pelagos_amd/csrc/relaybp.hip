@@ -1503,14 +1503,40 @@ constexpr int kMaxJobs = PEL_MAX_JOBS;
 // A job's shot routine: its placement, plus these flags.
 enum { kKindWave = 16, kKindLearn = 32 };
 
+// One job's descriptor in the launch's job table (device memory, written by
+// jobs_setup_kernel before the engine runs and only read by it).
+struct JobDesc {
+    KArgs a;
+    int32_t kind;     // placement | kKindWave (a shot per wave) | kKindLearn
+    uint32_t stride;  // LDS bytes per wave of a shot-per-wave job
+};
+
 struct EngineArgs {
     int32_t n_jobs;
-    uint32_t ctl_lds;           // LDS byte address of the workgroup's two control words
-    int32_t kind[kMaxJobs];     // placement | kKindWave (a shot per wave) | kKindLearn
-    uint32_t stride[kMaxJobs];  // LDS bytes per wave of a shot-per-wave job
-    KArgs a[kMaxJobs];
+    uint32_t ctl_lds;  // LDS byte address of the workgroup's two control words
 };
-static_assert(sizeof(EngineArgs) <= 4096, "the jobs travel as kernel arguments");
+
+// Descriptors for the job table travel as kernel arguments, kSetupJobs per
+// setup launch.
+constexpr int kSetupJobs = 7;
+struct SetupArgs {
+    JobDesc *dst;
+    int32_t count;
+    int32_t pad;
+    JobDesc src[kSetupJobs];
+};
+static_assert(sizeof(SetupArgs) <= 4096, "kernel arguments are at most 4 KiB");
+static_assert(sizeof(JobDesc) % 4 == 0, "descriptors are copied as words");
+
+// Copies its descriptors into the job table word by word, reading them in
+// place from the kernel-argument segment.
+__global__ void __launch_bounds__(256) jobs_setup_kernel(SetupArgs) {
+    const SetupArgs &sa = *(const SetupArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(&sa.src[0]);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(sa.dst);
+    const int words = sa.count * (int)(sizeof(JobDesc) / 4);
+    for (int i = threadIdx.x; i < words; i += 256) dst[i] = src[i];
+}
 
 // One shot of a job on a group (the workgroup, or BLOCK = 64: one wave at LDS
 // base lb): its whole schedule, then the leader writes its scalars.
@@ -1560,11 +1586,7 @@ constexpr int kKindsAll = kKindsPlainWg | kKindsWave | kKindsLearn;
 // the instantiations, smallest first
 
 template <int KINDS>
-__global__ void __launch_bounds__(kEngineBlock) engine_kernel(EngineArgs) {
-    // The jobs are read in place from the kernel-argument segment (the only
-    // argument sits at its start): naming the by-value argument with a runtime
-    // job index made the compiler copy all of it to private memory.
-    const EngineArgs &e = *(const EngineArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+__global__ void __launch_bounds__(kEngineBlock) engine_kernel(EngineArgs e, const JobDesc *__restrict__ jobs) {
     extern __shared__ __align__(16) uint8_t smem[];
     int *ctl = reinterpret_cast<int *>(smem + e.ctl_lds);
     int j = 0;
@@ -1574,9 +1596,9 @@ __global__ void __launch_bounds__(kEngineBlock) engine_kernel(EngineArgs) {
         if (opaque_tid() == 0) {
             int jj = j, t = 0;
             for (; jj < e.n_jobs; ++jj) {
-                const KArgs &a = e.a[jj];
+                const KArgs &a = jobs[jj].a;
                 if (a.B <= 0) continue;  // an empty job has no workspace
-                if (e.kind[jj] & kKindWave) {
+                if (jobs[jj].kind & kKindWave) {
                     if (__hip_atomic_load(&a.ctrl[kCtrlFirst], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < a.B) break;
                 } else {
                     t = atomicAdd(&a.ctrl[kCtrlFirst], 1);
@@ -1592,9 +1614,9 @@ __global__ void __launch_bounds__(kEngineBlock) engine_kernel(EngineArgs) {
         __syncthreads();
         if (jj >= e.n_jobs) break;
         j = jj;
-        const KArgs &a = e.a[j];
+        const KArgs &a = jobs[j].a;
         uint8_t *slot = a.spill + (int64_t)blockIdx.x * a.spill_stride;
-        const int kind = uniform_i32(e.kind[j]);
+        const int kind = uniform_i32(jobs[j].kind);
         const int b = kind & kKindWave ? 0 : a.order[t];
 #define PEL_ENGINE_CASE(K, CALL) \
     else if ((KINDS & kind_bit(K)) && kind == (K)) CALL;
@@ -1607,13 +1629,13 @@ __global__ void __launch_bounds__(kEngineBlock) engine_kernel(EngineArgs) {
         PEL_ENGINE_CASE(0, (engine_shot<kEngineBlock, 0, false>(a, b, smem, 0, slot)))
         PEL_ENGINE_CASE(15 | kKindLearn, (engine_shot<kEngineBlock, 15, true>(a, b, smem, 0, slot)))
         PEL_ENGINE_CASE(7 | kKindLearn, (engine_shot<kEngineBlock, 7, true>(a, b, smem, 0, slot)))
-        PEL_ENGINE_CASE(15 | kKindWave, (engine_wave_job<15>(a, e.stride[j], smem)))
-        PEL_ENGINE_CASE(7 | kKindWave, (engine_wave_job<7>(a, e.stride[j], smem)))
+        PEL_ENGINE_CASE(15 | kKindWave, (engine_wave_job<15>(a, jobs[j].stride, smem)))
+        PEL_ENGINE_CASE(7 | kKindWave, (engine_wave_job<7>(a, jobs[j].stride, smem)))
 #undef PEL_ENGINE_CASE
     }
 }
 
-typedef void (*EngineFn)(EngineArgs);
+typedef void (*EngineFn)(EngineArgs, const JobDesc *);
 
 // The smallest engine instantiation whose routines cover the job kinds in mask.
 EngineFn pick_engine(int mask) {
@@ -2648,6 +2670,7 @@ int plan_jobs_impl(const pel_graph *const *graphs, const int32_t *shots, const i
     if (per_cu <= 0) return fail(PEL_EDEVICE, "pel_plan_jobs: the engine kernel cannot be resident");
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * n_cu, units));
     out->n_jobs = n;
+    out->table_bytes = (int64_t)sizeof(JobDesc) * n;
     out->block = kEngineBlock;
     out->grid = (int32_t)grid;
     out->lds_bytes = (int32_t)lds;
@@ -2660,15 +2683,18 @@ int plan_jobs_impl(const pel_graph *const *graphs, const int32_t *shots, const i
 }
 
 int decode_jobs_impl(const pel_graph *const *graphs, const pel_params *params, const pel_batch *batches, int32_t n,
-                     const pel_jobs_plan *plan, void *stream) {
+                     const pel_jobs_plan *plan, void *table, void *stream) {
     const std::string w_ = "pel_decode_jobs";
-    if (!graphs || !params || !batches || !plan) return fail(PEL_EINVAL, w_ + ": NULL argument");
-    if (n < 1 || n > kMaxJobs || plan->n_jobs != n || plan->block != kEngineBlock || plan->grid <= 0)
+    if (!graphs || !params || !batches || !plan || !table) return fail(PEL_EINVAL, w_ + ": NULL argument");
+    if (n < 1 || n > kMaxJobs || plan->n_jobs != n || plan->block != kEngineBlock || plan->grid <= 0 ||
+        plan->table_bytes < (int64_t)sizeof(JobDesc) * n)
         return fail(PEL_EINVAL, w_ + ": plan does not match these jobs");
     EngineArgs e;
     memset(&e, 0, sizeof e);
     e.n_jobs = n;
     e.ctl_lds = (uint32_t)(plan->lds_bytes - 16);
+    std::vector<JobDesc> desc((size_t)n);
+    memset(desc.data(), 0, sizeof(JobDesc) * desc.size());
     int64_t lds_need = 0;
     for (int32_t i = 0; i < n; ++i) {
         const pel_graph *g = graphs[i];
@@ -2689,24 +2715,35 @@ int decode_jobs_impl(const pel_graph *const *graphs, const pel_params *params, c
             plan->workspace_bytes[i] < w.total_fixed + plan->spill_stride[i] * slots)
             return fail(PEL_EINVAL, w_ + ": plan does not match these jobs");
         lds_need = std::max(lds_need, group == 64 ? kEngineWaves * align_up(L.lds, 16) : L.lds);
-        fill_args(e.a[i], g, &params[i], b, static_cast<uint8_t *>(b->workspace), w, learn, 0, group, L,
+        fill_args(desc[i].a, g, &params[i], b, static_cast<uint8_t *>(b->workspace), w, learn, 0, group, L,
                   plan->spill_stride[i]);
-        e.a[i].shot_clock = nullptr;
-        e.kind[i] = place | (group == 64 ? kKindWave : 0) | (learn ? kKindLearn : 0);
-        e.stride[i] = (uint32_t)align_up(L.lds, 16);
+        desc[i].a.shot_clock = nullptr;
+        desc[i].kind = place | (group == 64 ? kKindWave : 0) | (learn ? kKindLearn : 0);
+        desc[i].stride = (uint32_t)align_up(L.lds, 16);
     }
     if (lds_need + 16 > plan->lds_bytes) return fail(PEL_EINVAL, w_ + ": plan does not match these jobs");
     hipStream_t s = static_cast<hipStream_t>(stream);
     for (int32_t i = 0; i < n; ++i) {
         if (batches[i].B == 0) continue;
         const WsLayout w = ws_layout(graphs[i], batches[i].B, false, plan->learned[i] != 0);
-        const int rc = launch_prep(graphs[i], &batches[i], e.a[i], static_cast<uint8_t *>(batches[i].workspace), w, s);
+        const int rc = launch_prep(graphs[i], &batches[i], desc[i].a, static_cast<uint8_t *>(batches[i].workspace), w, s);
         if (rc != PEL_OK) return rc;
+    }
+    JobDesc *tab = static_cast<JobDesc *>(table);
+    for (int32_t i0 = 0; i0 < n; i0 += kSetupJobs) {  // the job table, kSetupJobs descriptors per launch
+        SetupArgs sa;
+        memset(&sa, 0, sizeof sa);
+        sa.dst = tab + i0;
+        sa.count = std::min<int32_t>(kSetupJobs, n - i0);
+        for (int32_t k = 0; k < sa.count; ++k) sa.src[k] = desc[(size_t)(i0 + k)];
+        hipLaunchKernelGGL(jobs_setup_kernel, dim3(1), dim3(256), 0, s, sa);
+        PEL_HIP(hipGetLastError());
     }
     if (batches[0].ev_begin) PEL_HIP(hipEventRecord(static_cast<hipEvent_t>(batches[0].ev_begin), s));
     int mask = 0;
-    for (int32_t i = 0; i < n; ++i) mask |= kind_bit(e.kind[i]);
-    hipLaunchKernelGGL(pick_engine(mask), dim3(plan->grid), dim3(kEngineBlock), (size_t)plan->lds_bytes, s, e);
+    for (int32_t i = 0; i < n; ++i) mask |= kind_bit(desc[i].kind);
+    hipLaunchKernelGGL(pick_engine(mask), dim3(plan->grid), dim3(kEngineBlock), (size_t)plan->lds_bytes, s, e,
+                       static_cast<const JobDesc *>(tab));
     PEL_HIP(hipGetLastError());
     if (batches[0].ev_end) PEL_HIP(hipEventRecord(static_cast<hipEvent_t>(batches[0].ev_end), s));
     return PEL_OK;
@@ -2731,8 +2768,8 @@ int pel_plan_jobs(const pel_graph *const *graphs, const int32_t *shots, const in
 }
 
 int pel_decode_jobs(const pel_graph *const *graphs, const pel_params *params, const pel_batch *batches, int32_t n,
-                    const pel_jobs_plan *plan, void *stream) {
-    return decode_jobs_impl(graphs, params, batches, n, plan, stream);
+                    const pel_jobs_plan *plan, void *table, void *stream) {
+    return decode_jobs_impl(graphs, params, batches, n, plan, table, stream);
 }
 
 }  // extern "C"
