@@ -40,6 +40,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 LDS_PEAK_B_PER_CLK_CU = 128
+N_XCD = 8  # MI355X: 8 XCDs of 32 CUs
 WORKLOAD = dict(n_blocks=13, checks_per_block=72, V=8785, seed=0)
 RELAY = dict(pre_iter=80, num_sets=100, set_max_iter=60, gamma0=0.65,
              gamma_dist_interval=(-0.24, 0.66), stop_nconv=5)
@@ -146,6 +147,11 @@ def cpu_baseline(graph, syn_np, lam, gtab, target_s: float = 10.0, single_sample
     }, ref
 
 
+# The engine instantiation of each form, by its demangled or mangled name.
+ENGINE_FORMS = (("plain", ("engine_kernel<1>", "engine_kernelILi1E")),
+                ("learned", ("engine_kernel<32>", "engine_kernelILi32E")))
+
+
 def _pmc_rows(folder: str):
     import csv
     import glob
@@ -177,13 +183,13 @@ def traffic_record(fetch_dir: str, write_dir: str, cfetch_dir: str, cwrite_dir: 
     cp_w = list(_per_dispatch(_pmc_rows(cwrite_dir), pred_copy, "WRITE_SIZE").values())
     f_scale = copy_bytes / (sum(cp_f) / len(cp_f) * 1024)
     w_scale = copy_bytes / (sum(cp_w) / len(cp_w) * 1024)
-    rec = {"kernel": "engine_kernel", "workload": "banded 936x8785, launches of 8 batches of 4096 shots",
+    rec = {"kernel": "engine_kernel", "workload": "banded 936x8785, launches of 50 batches of 4096 shots (bench.py --jobs 50)",
            "copy_probe_bytes": copy_bytes, "copy_fetch_kib_raw": sum(cp_f) / len(cp_f),
            "copy_write_kib_raw": sum(cp_w) / len(cp_w), "fetch_scale": round(f_scale, 4),
            "write_scale": round(w_scale, 4)}
     frows, wrows = _pmc_rows(fetch_dir), _pmc_rows(write_dir)
-    for form, tag in (("plain", "engine_kernelILi1E"), ("learned", "engine_kernelILi32E")):
-        pred = (lambda k, tag=tag: tag in k)
+    for form, tags in ENGINE_FORMS:
+        pred = (lambda k, tags=tags: any(t in k for t in tags))
         f = list(_per_dispatch(frows, pred, "FETCH_SIZE").values())
         w = list(_per_dispatch(wrows, pred, "WRITE_SIZE").values())
         if not f or not w:
@@ -203,10 +209,10 @@ def counters_record(dirs) -> int:
     for d in dirs:
         rows += _pmc_rows(d)
     names = sorted({r["Counter_Name"] for r in rows})
-    rec = {"kernel": "engine_kernel", "workload": "banded 936x8785, launches of 8 batches of 4096 shots",
+    rec = {"kernel": "engine_kernel", "workload": "banded 936x8785, launches of 50 batches of 4096 shots (bench.py --jobs 50)",
            "counters": names}
-    for form, tag in (("plain", "engine_kernelILi1E"), ("learned", "engine_kernelILi32E")):
-        pred = (lambda k, tag=tag: tag in k)
+    for form, tags in ENGINE_FORMS:
+        pred = (lambda k, tags=tags: any(t in k for t in tags))
         vals = {}
         for c in names:
             v = list(_per_dispatch(rows, pred, c).values())
@@ -319,8 +325,9 @@ def roofline(form: str, m: dict, b_iter: int, lds_iter: int, n_cu: int, copy_gbs
     launch measured by FETCH_SIZE / WRITE_SIZE (profiles/), with the rate and
     fraction of peak it represents.  The kernel's real limiter is instruction
     issue inside the CU: valu_issue_frac = VALU wave-instructions x 4 cycles
-    over (CUs x 4 SIMDs x GRBM_GUI_ACTIVE cycles) of the same launch, and the
-    LDS rate per clock per CU of the design's LDS bytes over those cycles."""
+    over (CUs x 4 SIMDs x launch cycles), the launch cycles being
+    GRBM_GUI_ACTIVE over the XCDs (it sums their busy cycles), and the LDS
+    rate per clock per CU of the design's LDS bytes over those cycles."""
     kern_ms = sum(m["kernel_ms"]) / max(len(m["kernel_ms"]), 1)
     iters_launch = m["batch_iterations"] * m["jobs_per_launch"]
     algo = iters_launch * b_iter
@@ -343,7 +350,7 @@ def roofline(form: str, m: dict, b_iter: int, lds_iter: int, n_cu: int, copy_gbs
         out["traffic_source"] = str(TRAFFIC_FILE.relative_to(ROOT))
     ct = (_json_or_none(COUNTERS_FILE) or {}).get(form)
     if ct and "SQ_INSTS_VALU" in ct and "GRBM_GUI_ACTIVE" in ct:
-        cyc = ct["GRBM_GUI_ACTIVE"]
+        cyc = ct["GRBM_GUI_ACTIVE"] / N_XCD  # the counter sums the busy cycles of every XCD
         out["valu_issue_frac"] = round(ct["SQ_INSTS_VALU"] * 4 / (n_cu * 4 * cyc), 4)
         out["clock_ghz"] = round(cyc / (kern_ms * 1e-3) / 1e9, 3)
         out["lds_bytes_per_clk_per_cu"] = round(lds_iter * iters_launch / (n_cu * cyc), 2)
