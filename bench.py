@@ -560,7 +560,7 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-latency runs")
     ap.add_argument("--no-batched", action="store_true", help="skip the three-code launch")
     ap.add_argument("--no-bb72", action="store_true", help="skip the [[72,12,6]] code-capacity runs")
-    ap.add_argument("--ler-batches", type=int, default=16,
+    ap.add_argument("--ler-batches", type=int, default=64,
                     help="4096-shot batches of the paired error-rate comparison")
     ap.add_argument("--serving-seconds", type=float, default=10.0,
                     help="length of each serving run (one per arrival rate)")

@@ -14,21 +14,12 @@ launch can be captured in a HIP graph: its work is kernels only (the queue
 counters are cleared by a kernel, not a memset).
 
 ``MultiCodeBatch`` is the same over a dict of codes (one job per code).
-
-``autotune`` times the launch shapes ``pel_plan_decode`` allows for one code
-and batch size (workgroup size, re-queue threshold) on the caller's
-syndromes and keeps the fastest in a JSON cache keyed by the graph's sparsity
-hash, the batch size and the device name, like the reference's per-problem
-autotune cache (autotune.py:130-133, 240-259).
 """
 
 from __future__ import annotations
 
 import ctypes
-import json
-import os
-from pathlib import Path
-from typing import Dict, Iterable, List, Optional, Sequence, Tuple
+from typing import Dict, Optional, Sequence, Tuple
 
 import torch
 
@@ -138,71 +129,3 @@ class MultiCodeBatch(JobLaunch):
 
     def __init__(self, decoders: Dict[str, RelayBPDecoder], shots: Dict[str, int]):
         super().__init__([(name, dec, shots[name]) for name, dec in decoders.items()])
-
-
-def _cache_path() -> Path:
-    base = os.environ.get("PELAGOS_AMD_TUNE_CACHE")
-    if base:
-        return Path(base)
-    return Path(os.path.expanduser("~")) / ".cache" / "pelagos_amd_tune.json"
-
-
-def _key(dec: RelayBPDecoder, B: int) -> str:
-    name = torch.cuda.get_device_name(dec.device)
-    return f"{dec.graph.sparsity_hash}|B={B}|{name}"
-
-
-def autotune(dec: RelayBPDecoder, syndromes: torch.Tensor, blocks: Iterable[int] = (256, 512, 1024),
-             requeue: Iterable[int] = (0, 16, 32), reps: int = 3, cache: Optional[Path] = None,
-             force: bool = False) -> Tuple[int, int, float]:
-    """Pick (block, requeue_below) for this decoder and batch size by timing
-    each allowed shape (kernel time from HIP events, best of ``reps``); apply
-    it to ``dec`` and return (block, requeue_below, milliseconds)."""
-    B = int(syndromes.shape[0])
-    path = cache or _cache_path()
-    key = _key(dec, B)
-    table = {}
-    if path.exists():
-        try:
-            table = json.loads(path.read_text())
-        except (OSError, ValueError):
-            table = {}
-    if key in table and not force:
-        blk, rq, ms = table[key]
-        _apply(dec, blk, rq)
-        return blk, rq, ms
-    best = None
-    for blk in blocks:
-        for rq in requeue:
-            try:
-                _apply(dec, blk, rq)
-                dec.decode(syndromes)  # plan + warm-up
-            except (ValueError, RuntimeError):
-                continue  # shape not allowed for this graph (LDS, block size)
-            times = []
-            for _ in range(reps):
-                ev = []
-                dec.kernel_events = ev
-                dec.decode(syndromes)
-                torch.cuda.synchronize(dec.device)
-                dec.kernel_events = None
-                times.append(ev[0][0].elapsed_time(ev[0][1]))
-            ms = min(times)
-            if best is None or ms < best[2]:
-                best = (blk, rq, ms)
-    if best is None:
-        raise RuntimeError("no launch shape could run this decode")
-    _apply(dec, best[0], best[1])
-    table[key] = list(best)
-    try:
-        path.parent.mkdir(parents=True, exist_ok=True)
-        path.write_text(json.dumps(table, indent=1))
-    except OSError:
-        pass
-    return best
-
-
-def _apply(dec: RelayBPDecoder, block: int, requeue_below: int) -> None:
-    dec._block_size = int(block)
-    dec.requeue_below = int(requeue_below)
-    dec._plans.clear()

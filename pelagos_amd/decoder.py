@@ -15,6 +15,10 @@ utils.py:128-225); all decoding arithmetic runs in the HIP library.
 from __future__ import annotations
 
 import ctypes
+import json
+import os
+import threading
+from pathlib import Path
 from typing import Dict, List, Optional, Tuple, Union
 
 import numpy as np
@@ -109,6 +113,36 @@ def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
     return None if t is None else t.data_ptr()
 
 
+_TUNE_LOCK = threading.Lock()
+
+
+def tune_cache_path() -> Path:
+    """The launch-shape cache: PELAGOS_AMD_TUNE_CACHE, else
+    ~/.cache/pelagos_amd_tune.json."""
+    base = os.environ.get("PELAGOS_AMD_TUNE_CACHE")
+    return Path(base) if base else Path(os.path.expanduser("~")) / ".cache" / "pelagos_amd_tune.json"
+
+
+def _load_tune_cache() -> dict:
+    try:
+        return json.loads(tune_cache_path().read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def _save_tune_cache(table: dict) -> None:
+    """Write atomically (a temporary file renamed over the cache), so that
+    concurrent decoders never read a half-written file."""
+    path = tune_cache_path()
+    try:
+        path.parent.mkdir(parents=True, exist_ok=True)
+        tmp = path.with_suffix(f".{os.getpid()}.tmp")
+        tmp.write_text(json.dumps(table, indent=1))
+        os.replace(tmp, path)
+    except OSError:
+        pass
+
+
 class RelayBPDecoder:
     """Relay-BP-S min-sum decoder with disordered memory strengths on one MI355X.
 
@@ -124,6 +158,13 @@ class RelayBPDecoder:
     kernel; the remaining legs use the uniform draws as before.
     ``learned_last_damp`` in [0, 1] scales the network's shift on leg L-1 by
     1 - learned_last_damp (a milder last learned leg).
+    ``autotune`` (addition): on the first decode of a batch size the launch
+    shapes the library allows (workgroup size, re-queue threshold) are timed on
+    the caller's syndromes and the fastest is kept, in memory and in a JSON
+    cache keyed by the problem (``tune_cache_path()``), as the reference tunes
+    its kernels on first sight of a problem key (decoder.py:905-969,
+    autotune.py:240-259).  None: on unless the environment sets
+    PELAGOS_AMD_AUTOTUNE=0.  Results never depend on the launch shape.
     """
 
     def __init__(
@@ -161,6 +202,7 @@ class RelayBPDecoder:
         learned_gamma=None,
         learned_legs: int = 0,
         learned_last_damp: float = 0.0,
+        autotune: Optional[bool] = None,
     ):
         # Validation in the reference's order and wording (decoder.py:362-512).
         if unconverged_correction not in UNCONVERGED_CORRECTIONS:
@@ -287,6 +329,11 @@ class RelayBPDecoder:
         self._placement = -1 if state_placement is None else int(state_placement)
         self._handle = _GraphHandle(self.graph, self.device)
         self._plans: Dict[int, _native.PelPlan] = {}
+        if autotune is None:
+            autotune = os.environ.get("PELAGOS_AMD_AUTOTUNE", "1") != "0"
+        self.autotune = bool(autotune)
+        self._tuned: Dict[Tuple[int, bool], Tuple[int, int]] = {}  # (B, learned) -> (block, requeue_below)
+        self.tune_runs = 0  # first-sight tunings this decoder timed (cache misses)
         self._workspace: Optional[torch.Tensor] = None
         self._gamma_cache: Dict[Tuple, torch.Tensor] = {}
         self._gamma_perm_cache: Dict[Tuple, Tuple[torch.Tensor, torch.Tensor]] = {}
@@ -418,7 +465,11 @@ class RelayBPDecoder:
     def plan(self, B: int, learned: bool = False) -> _native.PelPlan:
         p = self._plans.get((B, learned))
         if p is None:
-            p = self._handle.plan(B, self._block_size, self._placement, learned)
+            tuned = self._tuned.get((B, learned))
+            block = self._block_size or (tuned[0] if tuned else 0)
+            p = self._handle.plan(B, block, self._placement, learned)
+            if self.requeue_below is None and tuned is not None and (tuned[1] == 0 or p.requeue_below > 0):
+                p.requeue_below = int(tuned[1])
             if self.requeue_below is not None:
                 if self.requeue_below < 0:
                     raise ValueError("requeue_below must be >= 0")
@@ -427,6 +478,69 @@ class RelayBPDecoder:
                 p.requeue_below = int(self.requeue_below)
             self._plans[(B, learned)] = p
         return p
+
+    # ------------------------------------------------------------- autotuning
+    def _tune_key(self, B: int, learned: bool) -> str:
+        name = torch.cuda.get_device_name(self.device)
+        return (f"{self.graph.sparsity_hash}|B={B}|learned={int(learned)}|placement={self._placement}"
+                f"|{name}|abi={_native.PEL_ABI_VERSION}")
+
+    def _candidates(self, B: int, learned: bool) -> List[Tuple[int, int]]:
+        """Launch shapes to time: the block sizes around the library's own choice
+        and the re-queue thresholds 0, 16 and 32."""
+        base = self._handle.plan(B, 0, self._placement, learned).block
+        blocks = sorted({b for b in (base // 4, base // 2, base) if 64 <= b <= 1024})
+        if self._block_size:
+            blocks = [self._block_size]
+        requeue = [int(self.requeue_below)] if self.requeue_below is not None else [0, 16, 32]
+        return [(b, r) for b in blocks for r in requeue]
+
+    def _tune(self, B: int, learned: bool, run) -> None:
+        """First sight of (B, learned): the cached choice, or time every
+        candidate with ``run()`` (one decode of the caller's syndromes), keep
+        the fastest and save it."""
+        key = self._tune_key(B, learned)
+        self._tuned[(B, learned)] = (self._block_size, 0)  # placeholder while timing
+        with _TUNE_LOCK:
+            table = _load_tune_cache()
+        hit = table.get(key)
+        if hit is not None:
+            self._tuned[(B, learned)] = (int(hit["block"]), int(hit["requeue_below"]))
+            self._plans.pop((B, learned), None)
+            return
+        best = None
+        for blk, rq in self._candidates(B, learned):
+            self._tuned[(B, learned)] = (blk, rq)
+            self._plans.pop((B, learned), None)
+            try:
+                if self.plan(B, learned).requeue_below != rq:
+                    continue  # this plan has no workspace for re-queued shots
+                run()  # warm-up
+                ms = []
+                for _ in range(3):
+                    ev = []
+                    self.kernel_events = ev
+                    run()
+                    self.kernel_events = None
+                    torch.cuda.synchronize(self.device)
+                    ms.append(ev[0][0].elapsed_time(ev[0][1]))
+            except (ValueError, RuntimeError):
+                self.kernel_events = None
+                continue  # a shape the graph does not allow
+            t = min(ms)
+            if best is None or t < best[2]:
+                best = (blk, rq, t)
+        self.tune_runs += 1
+        if best is None:
+            del self._tuned[(B, learned)]
+            self._plans.pop((B, learned), None)
+            return
+        self._tuned[(B, learned)] = (best[0], best[1])
+        self._plans.pop((B, learned), None)
+        with _TUNE_LOCK:
+            table = _load_tune_cache()
+            table[key] = {"block": best[0], "requeue_below": best[1], "ms": round(best[2], 4)}
+            _save_tune_cache(table)
 
     def _shot_clock_for(self, B: int) -> Optional[torch.Tensor]:
         if not self.record_shot_clock:
@@ -526,8 +640,13 @@ class RelayBPDecoder:
                     self.learned_gammas = torch.zeros(B, self.learned_legs, V, dtype=self.dtype, device=dev)
                     learn["gamma_out"] = self.learned_gammas
             if B > 0:
-                self._launch(syn, prior, wj, prior_llr is not None, gtab, ig, ip, errors, weights,
-                             iters, found, last_post, pre_iter, skip_relay_legs, B, learn)
+                args = (syn, prior, wj, prior_llr is not None, gtab, ig, ip, errors, weights,
+                        iters, found, last_post, pre_iter, skip_relay_legs, B, learn)
+                lkey = (B, learn is not None)
+                if self.autotune and lkey not in self._tuned and self.kernel_events is None \
+                        and not self.record_shot_clock:
+                    self._tune(B, learn is not None, lambda: self._launch(*args))
+                self._launch(*args)
             return self._assemble(errors, weights, iters, found, last_post)
 
     # ------------------------------------------------------------ serving path

@@ -1,9 +1,13 @@
+import os
 import sys
 from pathlib import Path
 
 import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
+# Decoders tune their launch shape on first sight unless told otherwise; the
+# suite builds many small decoders, so only the test of tuning turns it on.
+os.environ.setdefault("PELAGOS_AMD_AUTOTUNE", "0")
 if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))
 

@@ -1,5 +1,5 @@
 """Several codes and batches in one persistent launch (pel_decode_jobs), also
-captured in a graph, and the launch-shape autotuner.
+captured in a graph, and the first-sight launch-shape tuning of decode().
 
 Every job's results are checked against an ordinary decode() of the same
 syndromes, bit for bit; the graph is replayed several times with fresh
@@ -52,22 +52,32 @@ def test_graph_replays_match_eager_decodes():
                 assert torch.equal(mb.out[k][key], ref[key]), (rep, k, key)
 
 
-def test_autotune_picks_a_shape_and_caches_it(tmp_path):
+def test_decode_tunes_its_launch_shape_on_first_sight(tmp_path, monkeypatch):
+    """autotune=True: the first decode of a batch size times the candidate
+    launch shapes, keeps the fastest and writes it to the cache; the results
+    equal an untuned decoder's, and a second decoder of the same problem takes
+    the shape from the cache without timing anything."""
     from pelagos_amd import RelayBPDecoder
-    from pelagos_amd.batching import autotune
+    from pelagos_amd.decoder import tune_cache_path
 
+    monkeypatch.setenv("PELAGOS_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
     H, p = codes.banded_circuit_code(n_blocks=4, V=2700, seed=2)
-    dec = RelayBPDecoder(H, p, **RELAY)
     syn = torch.from_numpy(codes.syndromes_of(H, codes.sample_errors(p, 256, seed=1, scale=1.5))).cuda()
-    base = dec.decode(syn)
-    cache = tmp_path / "tune.json"
+    base = RelayBPDecoder(H, p, autotune=False, **RELAY).decode(syn)
+    dec = RelayBPDecoder(H, p, autotune=True, **RELAY)
     print("[gpu] autotune", flush=True)
-    blk, rq, ms = autotune(dec, syn, blocks=(256, 1024), requeue=(0, 16), reps=1, cache=cache)
-    assert blk in (256, 1024) and rq in (0, 16) and ms > 0 and cache.exists()
     out = dec.decode(syn)
+    assert dec.tune_runs == 1 and tune_cache_path().exists()
+    blk, rq = dec._tuned[(256, False)]
+    assert blk in (256, 512, 1024) and rq in (0, 16, 32)
+    assert dec.plan(256).block == blk and dec.plan(256).requeue_below == rq
     for key in ("errors", "weights", "iterations", "found_count"):
         assert torch.equal(out[key], base[key]), key
-    assert autotune(dec, syn, blocks=(256, 1024), requeue=(0, 16), reps=1, cache=cache) == (blk, rq, ms)
+    again = RelayBPDecoder(H, p, autotune=True, **RELAY)
+    out2 = again.decode(syn)
+    assert again.tune_runs == 0 and again._tuned[(256, False)] == (blk, rq)
+    for key in ("errors", "weights", "iterations", "found_count"):
+        assert torch.equal(out2[key], base[key]), key
 
 
 def test_decode_async_matches_decode_and_checks_its_buffers():
