@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define PEL_ABI_VERSION 7
+#define PEL_ABI_VERSION 8
 
 /* status codes */
 #define PEL_OK 0
@@ -58,6 +58,13 @@ typedef struct {
     const float *alpha_t;    /* device [max(T0,Tr)] alpha per in-leg iteration, or NULL */
     int32_t last_leg;        /* unconverged_correction == "last_leg" */
     int32_t skip_relay_legs; /* decode(skip_relay_legs=True) */
+    int32_t leg_step;        /* 0: the whole schedule.  k >= 1 (plans made with PEL_PLAN_LEGWISE): only leg
+                                k - 1 (0 the pre-iter leg, l the relay leg l - 1) for the shots still short
+                                of stop_nconv solutions, resuming what the call with k - 1 on the same
+                                buffers and workspace parked; every shot's outputs are as if the decode
+                                ended after that leg.  Calls k = 1, 2, ... are the reference's
+                                leg-synchronous schedule, whose fast_max_unconverged_pct exit checks the
+                                found counts between legs (decoder.py:2890-2937, 3655-3835). */
 } pel_params;
 
 /* One batch of shots.  Shapes: B shots, C checks, V variables, R = num_sets. */
@@ -118,10 +125,12 @@ typedef struct {
                                 the workspace); 0 = never.  Lowering it to 0 after planning is
                                 allowed; raising it needs a plan made with it. */
     int32_t learned;         /* planned with PEL_PLAN_LEARNED */
+    int32_t legwise;         /* planned with PEL_PLAN_LEGWISE */
 } pel_plan;
 
 /* pel_plan_decode flags */
 #define PEL_PLAN_LEARNED 1   /* reserve LDS for learned memory strengths (fast placements only) */
+#define PEL_PLAN_LEGWISE 2   /* workspace for leg-stepped decodes (pel_params.leg_step) */
 
 /* ABI version (PEL_ABI_VERSION). */
 int pel_abi_version(void);

@@ -13,10 +13,11 @@ from typing import Optional
 
 LIB_PATH = Path(__file__).resolve().parent / "libpelagos_relaybp.so"
 
-PEL_ABI_VERSION = 7
+PEL_ABI_VERSION = 8
 PEL_MAX_JOBS = 64
 PEL_NOALLOC_MAX_BYTES = 32 << 20
 PEL_PLAN_LEARNED = 1
+PEL_PLAN_LEGWISE = 2
 PEL_MLP_PARAMS = 32 * 8 + 32 + 32 * 32 + 32 + 32 + 1
 PEL_OK = 0
 PEL_EINVAL = -1
@@ -40,6 +41,7 @@ class PelParams(ctypes.Structure):
         ("alpha_t", _vp),
         ("last_leg", ctypes.c_int32),
         ("skip_relay_legs", ctypes.c_int32),
+        ("leg_step", ctypes.c_int32),
     ]
 
 
@@ -85,6 +87,7 @@ class PelPlan(ctypes.Structure):
         ("workspace_bytes", ctypes.c_int64),
         ("requeue_below", ctypes.c_int32),
         ("learned", ctypes.c_int32),
+        ("legwise", ctypes.c_int32),
     ]
 
 

@@ -136,7 +136,12 @@ struct KArgs {
     const int32_t *order;  // [B] shot taken at each first-leg queue position
     int32_t requeue_below; // re-queue shots whose first leg converged in fewer iterations (0: never)
     int32_t *rq_slot;      // [B] re-queued shots in publication order (-1: not yet published)
-    int4 *rq_state;        // [B] {best weight bits, found, iterations, any iteration} of a re-queued shot
+    int4 *rq_state;        // [B][2] of a parked shot: {best weight bits, found, iterations, any iteration},
+                           // {last leg converged, strength shifts computed, 0, 0}
+    uint64_t *rq_bits;     // [B][ceil(V/64)] a parked shot's output bits (leg-stepped decodes)
+    int32_t leg_step;      // -1: the whole schedule; k >= 0: only leg k (0 the pre-iter leg, k relay leg
+                           // k-1) for shots short of stop_nconv solutions, every shot's outputs as if the
+                           // decode ended after it, its state parked for the next leg
     float *rq_post;        // [B,V] posterior of a re-queued shot, by variable position
     int64_t *shot_clock;   // [B,3] start/end wall clock and workgroup, or null
     uint8_t *spill;
@@ -1236,11 +1241,11 @@ __device__ __forceinline__ GammaSrc leg_gamma(const KArgs &a, int b, int leg) {
 // the unconverged correction and the optional last posterior.
 template <int BLOCK, int PLACE, bool LEARN>
 __device__ __forceinline__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &st, bool run, double *s_red,
-                           int *s_flag, bool leg0_ready = false) {
+                           int *s_flag, bool leg0_ready = false, int leg_begin = 0, int leg_end = 1 << 30) {
     const int V = a.g.V;
     const ShotIO io = shot_io(a, b);
     if (run) {
-        for (int leg = 0; leg < a.num_sets && st.found < a.stop_nconv; ++leg) {
+        for (int leg = leg_begin; leg < min(a.num_sets, leg_end) && st.found < a.stop_nconv; ++leg) {
             GammaSrc gl = leg_gamma(a, b, leg);
             bool count = false;
             if (LEARN && leg < a.learned_legs) {
@@ -1311,19 +1316,22 @@ __device__ __forceinline__ void requeue_shot(const KArgs &a, int b, const ShotMe
     gsync<BLOCK>();
     if (gleader<BLOCK>()) {
         __threadfence();
-        a.rq_state[b] = make_int4(__float_as_int(st.best_w), st.found, st.iters, st.any_iter);
+        a.rq_state[2 * b] = make_int4(__float_as_int(st.best_w), st.found, st.iters, st.any_iter);
+        a.rq_state[2 * b + 1] = make_int4(st.last_ok, st.shifted, 0, 0);
         const int slot = atomicAdd(&a.ctrl[kCtrlPushed], 1);
         __hip_atomic_store(&a.rq_slot[slot], b, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_fetch_add(&a.ctrl[kCtrlLeft], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
-// Resume a re-queued shot: posterior back as lam_mixed = post, S = 0, and its
-// output bits from the posterior's signs (the first leg's solution, its only one
-// so far).  The leader has acquired the parked state (wait_requeued) before the
-// workgroup barrier that precedes every read of it here.
+// Resume a parked shot: posterior back as lam_mixed = post, S = 0, and its
+// output bits: from the posterior's signs for a re-queued shot (the first
+// leg's solution, its only one so far), else the parked bits.  A re-queued
+// shot's parked state was acquired by the leader (wait_requeued) before the
+// workgroup barrier that precedes every read of it here; a leg-stepped
+// decode's was written by an earlier launch.
 template <int BLOCK, int PLACE, bool LEARN>
-__device__ __forceinline__ void resume_shot(const KArgs &a, int b, ShotMem<PLACE> &m) {
+__device__ __forceinline__ void resume_shot(const KArgs &a, int b, ShotMem<PLACE> &m, bool bits_from_post = true) {
     const int V = a.g.V;
     const ShotIO io = shot_io(a, b);
     const float *post = a.rq_post + (size_t)b * V;
@@ -1352,7 +1360,61 @@ __device__ __forceinline__ void resume_shot(const KArgs &a, int b, ShotMem<PLACE
     row_info_init<BLOCK, PLACE>(a, m, io.syn);
     m.l0g = io.l0;
     gsync<BLOCK>();
-    best_from_state<BLOCK, PLACE>(a, m, true);
+    if (bits_from_post) {
+        best_from_state<BLOCK, PLACE>(a, m, true);
+    } else {
+        const int nw = (V + 63) >> 6;
+        for (int w = gtid<BLOCK>(); w < nw; w += BLOCK)
+            *m.template lds<uint64_t>(a.best_lds + 8u * (uint32_t)w) = a.rq_bits[(size_t)b * nw + w];
+    }
+}
+
+// Leg-stepped decodes: the whole state of a shot after the leg a launch ran
+// (posterior, strength shifts, output bits, scalars), for the next launch.
+template <int BLOCK, int PLACE, bool LEARN>
+__device__ __forceinline__ void park_shot(const KArgs &a, int b, const ShotMem<PLACE> &m, const ShotState &st) {
+    const int V = a.g.V;
+    float *post = a.rq_post + (size_t)b * V;
+    gsync<BLOCK>();  // every wave's output words are in LDS
+    for (int v = gtid<BLOCK>(); v < V; v += BLOCK) {
+        const float2 ls = m.ls_pos(v);
+        post[v] = ls.x + ls.y;
+        if (LEARN && a.mlp) a.rq_gamma[(size_t)b * V + v] = *m.template lds<int8_t>(a.gamma_lds + (uint32_t)v);
+    }
+    const int nw = (V + 63) >> 6;
+    for (int w = gtid<BLOCK>(); w < nw; w += BLOCK)
+        a.rq_bits[(size_t)b * nw + w] = *m.template lds<uint64_t>(a.best_lds + 8u * (uint32_t)w);
+    if (gleader<BLOCK>()) {
+        a.rq_state[2 * b] = make_int4(__float_as_int(st.best_w), st.found, st.iters, st.any_iter);
+        a.rq_state[2 * b + 1] = make_int4(st.last_ok, st.shifted, 0, 0);
+    }
+}
+
+// One launch of a leg-stepped decode for shot b: leg a.leg_step only, for a
+// shot still short of stop_nconv solutions (the outputs of the others are
+// final), then the outputs as if the decode ended here and the parked state.
+// The learned network runs, as in a whole decode, on the state after the
+// pre-iter leg; its shifts are parked with the rest.
+template <int BLOCK, int PLACE, bool LEARN>
+__device__ __forceinline__ void legwise_shot(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &st, double *s_red,
+                                             int *s_flag) {
+    if (a.leg_step == 0) {
+        const bool more = first_leg<BLOCK, PLACE, LEARN>(a, b, m, st, s_red, s_flag);
+        if (LEARN && more && a.learned_legs > 0) {
+            st.shifted = a.mlp != nullptr && st.last_ok;
+            if (st.shifted) leg_features<BLOCK, PLACE>(a, b, m, 0, a.syn + (size_t)b * a.g.C, true, true);
+        }
+        relay_legs<BLOCK, PLACE, LEARN>(a, b, m, st, false, s_red, s_flag);
+        park_shot<BLOCK, PLACE, LEARN>(a, b, m, st);
+        return;
+    }
+    const int4 s0 = uniform_i4(a.rq_state[2 * b]);
+    const int4 s1 = uniform_i4(a.rq_state[2 * b + 1]);
+    st = ShotState{__int_as_float(s0.x), s0.y, s0.z, s0.w, s1.x, s1.y, 0};
+    if (st.found >= a.stop_nconv) return;  // done in an earlier launch: its outputs are written
+    resume_shot<BLOCK, PLACE, LEARN>(a, b, m, false);
+    relay_legs<BLOCK, PLACE, LEARN>(a, b, m, st, true, s_red, s_flag, true, a.leg_step - 1, a.leg_step);
+    park_shot<BLOCK, PLACE, LEARN>(a, b, m, st);
 }
 
 // Leader only: the shot behind re-queue ticket t2, or B when no shot will ever
@@ -1460,6 +1522,11 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
         const int stage = uniform_i32(s_next[1]);
         __syncthreads();
         if (b >= a.B) break;
+        if (a.leg_step >= 0) {
+            legwise_shot<BLOCK, PLACE, LEARN>(a, b, m, st, s_red, s_flag);
+            prev = b;
+            continue;
+        }
         bool more = true;
         if (stage == 1) {
             more = first_leg<BLOCK, PLACE, LEARN>(a, b, m, st, s_red, s_flag);
@@ -1477,7 +1544,7 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
             }
         } else {
             resume_shot<BLOCK, PLACE, LEARN>(a, b, m);
-            const int4 rs = uniform_i4(a.rq_state[b]);
+            const int4 rs = uniform_i4(a.rq_state[2 * b]);
             // parked shots are those whose first leg converged; the network ran before
             // they were parked
             st = ShotState{__int_as_float(rs.x), rs.y, rs.z, rs.w, 1, a.mlp != nullptr, 0};
@@ -1819,23 +1886,26 @@ struct pel_graph {
 namespace {
 
 // Workspace layout: [0,256) queue counters | order [B] | key [B] | rq_slot [B]
-// | rq_state [B] | rq_post [B,V] (only when shots are re-queued) | rq_gamma
-// [B,V] int8 (re-queue with learned strengths) | mlp_frag (learned plans) |
+// | rq_state [B][2] | rq_post [B,V] (only when shots are parked: re-queue or
+// leg-stepped decodes) | rq_gamma [B,V] int8 (parked with learned strengths)
+// | rq_bits [B][ceil(V/64)] u64 (leg-stepped) | mlp_frag (learned plans) |
 // spill slots
 struct WsLayout {
-    int64_t order, key, rq_slot, rq_state, rq_post, rq_gamma, mlp_frag, by_pos, spill, total_fixed;
+    int64_t order, key, rq_slot, rq_state, rq_post, rq_gamma, rq_bits, mlp_frag, by_pos, spill, total_fixed;
 };
 
-WsLayout ws_layout(const pel_graph *g, int64_t B, bool requeue, bool learn) {
+WsLayout ws_layout(const pel_graph *g, int64_t B, bool requeue, bool learn, bool legwise = false) {
     const int64_t nb = std::max<int64_t>(B, 1);
+    const bool park = requeue || legwise;
     WsLayout w;
     w.order = 256;
     w.key = align_up(w.order + 4 * nb, 256);
     w.rq_slot = align_up(w.key + 4 * nb, 256);
     w.rq_state = align_up(w.rq_slot + 4 * nb, 256);
-    w.rq_post = align_up(w.rq_state + 16 * nb, 256);
-    w.rq_gamma = align_up(w.rq_post + (requeue ? 4 * nb * g->V : 0), 256);
-    w.mlp_frag = align_up(w.rq_gamma + (requeue && learn ? nb * g->V : 0), 256);
+    w.rq_post = align_up(w.rq_state + 32 * nb, 256);
+    w.rq_gamma = align_up(w.rq_post + (park ? 4 * nb * g->V : 0), 256);
+    w.rq_bits = align_up(w.rq_gamma + (park && learn ? nb * g->V : 0), 256);
+    w.mlp_frag = align_up(w.rq_bits + (legwise ? 8 * nb * ((g->V + 63) / 64) : 0), 256);
     w.by_pos = align_up(w.mlp_frag + (learn ? 4LL * kMlpStride * 64 : 0), 256);  // wj, priors by position
     w.spill = align_up(w.by_pos + 2 * align_up(4LL * g->V, 256), 256);
     w.total_fixed = w.spill;
@@ -2331,8 +2401,9 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32
     }
     if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
         return fail(PEL_EINVAL, "pel_plan_decode: block must be 64, 128, 256, 512 or 1024");
-    if (flags & ~PEL_PLAN_LEARNED) return fail(PEL_EINVAL, "pel_plan_decode: unknown flags");
+    if (flags & ~(PEL_PLAN_LEARNED | PEL_PLAN_LEGWISE)) return fail(PEL_EINVAL, "pel_plan_decode: unknown flags");
     const bool learn = (flags & PEL_PLAN_LEARNED) != 0;
+    const bool legwise = (flags & PEL_PLAN_LEGWISE) != 0;
     const StateSizes st = g->state();
     const int64_t budget = (int64_t)max_lds;
     int place = -1;
@@ -2379,7 +2450,8 @@ int pel_plan_decode(const pel_graph *g, int32_t B, int32_t block_override, int32
     grid = std::min<int64_t>(grid, std::max<int32_t>(B, 1));
     out->requeue_below = kDefaultRequeueBelow;
     out->learned = learn ? 1 : 0;
-    const WsLayout w = ws_layout(g, B, out->requeue_below > 0, learn);
+    out->legwise = legwise ? 1 : 0;
+    const WsLayout w = ws_layout(g, B, out->requeue_below > 0, learn, legwise);
     out->block = block;
     out->grid = (int32_t)grid;
     out->placement = place;
@@ -2447,7 +2519,7 @@ GroupLayout group_layout(const pel_graph *g, int place, int group, bool learn) {
 // The kernel arguments of one batch with workspace ws laid out as w.
 void fill_args(KArgs &a, const pel_graph *g, const pel_params *p, const pel_batch *b, uint8_t *ws,
                const WsLayout &w, bool learn, int32_t requeue_below, int group, const GroupLayout &L,
-               int64_t spill_stride) {
+               int64_t spill_stride, bool legwise = false) {
     const StateSizes st = g->state();
     memset(&a, 0, sizeof a);
     a.g = g->d;
@@ -2485,11 +2557,14 @@ void fill_args(KArgs &a, const pel_graph *g, const pel_params *p, const pel_batc
     a.shot_clock = b->shot_clock;
     a.ctrl = reinterpret_cast<int32_t *>(ws);
     a.order = reinterpret_cast<int32_t *>(ws + w.order);
-    a.requeue_below = requeue_below;
+    a.leg_step = legwise ? p->leg_step - 1 : -1;
+    a.requeue_below = a.leg_step >= 0 ? 0 : requeue_below;
+    const bool park = a.requeue_below > 0 || a.leg_step >= 0;
     a.rq_slot = reinterpret_cast<int32_t *>(ws + w.rq_slot);
     a.rq_state = reinterpret_cast<int4 *>(ws + w.rq_state);
-    a.rq_post = requeue_below > 0 ? reinterpret_cast<float *>(ws + w.rq_post) : nullptr;
-    a.rq_gamma = requeue_below > 0 && learn ? reinterpret_cast<int8_t *>(ws + w.rq_gamma) : nullptr;
+    a.rq_post = park ? reinterpret_cast<float *>(ws + w.rq_post) : nullptr;
+    a.rq_gamma = park && learn ? reinterpret_cast<int8_t *>(ws + w.rq_gamma) : nullptr;
+    a.rq_bits = a.leg_step >= 0 ? reinterpret_cast<uint64_t *>(ws + w.rq_bits) : nullptr;
     a.mlp_frag = learn ? reinterpret_cast<const uint32_t *>(ws + w.mlp_frag) : nullptr;
     a.mlp = b->mlp;
     a.learned_legs = learn ? b->learned_legs : 0;
@@ -2557,7 +2632,13 @@ int decode_impl(const char *who, const pel_graph *g, const pel_params *p, const 
     const bool learn = plan->learned != 0;
     const GroupLayout L = group_layout(g, plan->placement, plan->block, learn);
     if (plan->requeue_below < 0) return fail(PEL_EINVAL, w_ + ": requeue_below must be >= 0");
-    const WsLayout w = ws_layout(g, b->B, plan->requeue_below > 0, learn);
+    if (p->leg_step < 0 || p->leg_step > p->num_sets + 1)
+        return fail(PEL_EINVAL, w_ + ": leg_step must lie in 0 .. num_sets + 1");
+    if (p->leg_step > 0 && !plan->legwise)
+        return fail(PEL_EINVAL, w_ + ": leg-stepped decodes need a plan made with PEL_PLAN_LEGWISE");
+    if (p->leg_step > 0 && (b->feat_out || b->hard_out || b->shot_clock || p->skip_relay_legs))
+        return fail(PEL_EINVAL, w_ + ": leg-stepped decodes record no traces or clocks and run the relay legs");
+    const WsLayout w = ws_layout(g, b->B, plan->requeue_below > 0, learn, plan->legwise != 0);
     const int64_t need = w.total_fixed + plan->spill_stride * plan->grid;
     if (!known || (learn && (plan->placement & 7) != 7) || L.lds != plan->lds_bytes ||
         align_up(L.spill, 256) != plan->spill_stride || plan->grid <= 0 || (!noalloc && plan->workspace_bytes < need))
@@ -2575,7 +2656,7 @@ int decode_impl(const char *who, const pel_graph *g, const pel_params *p, const 
         if (rc != PEL_OK) return rc;
     }
     KArgs a;
-    fill_args(a, g, p, b, ws, w, learn, plan->requeue_below, plan->block, L, plan->spill_stride);
+    fill_args(a, g, p, b, ws, w, learn, plan->requeue_below, plan->block, L, plan->spill_stride, p->leg_step > 0);
     hipStream_t s = static_cast<hipStream_t>(stream);
     rc = launch_prep(g, b, a, ws, w, s);
     if (rc != PEL_OK) return rc;

@@ -90,9 +90,10 @@ class _GraphHandle:
             )
         self.h = h
 
-    def plan(self, B: int, block: int = 0, placement: int = -1, learned: bool = False) -> _native.PelPlan:
+    def plan(self, B: int, block: int = 0, placement: int = -1, learned: bool = False,
+             legwise: bool = False) -> _native.PelPlan:
         p = _native.PelPlan()
-        flags = _native.PEL_PLAN_LEARNED if learned else 0
+        flags = (_native.PEL_PLAN_LEARNED if learned else 0) | (_native.PEL_PLAN_LEGWISE if legwise else 0)
         with torch.cuda.device(self.device):
             _native.check(_native.lib().pel_plan_decode(self.h, int(B), int(block), int(placement), flags,
                                                         ctypes.byref(p)),
@@ -462,7 +463,13 @@ class RelayBPDecoder:
             self._alpha_cache[n] = t
         return t
 
-    def plan(self, B: int, learned: bool = False) -> _native.PelPlan:
+    def plan(self, B: int, learned: bool = False, legwise: bool = False) -> _native.PelPlan:
+        if legwise:
+            p = self._plans.get((B, learned, True))
+            if p is None:
+                p = self._handle.plan(B, self._block_size, self._placement, learned, legwise=True)
+                self._plans[(B, learned, True)] = p
+            return p
         p = self._plans.get((B, learned))
         if p is None:
             tuned = self._tuned.get((B, learned))
@@ -602,12 +609,7 @@ class RelayBPDecoder:
             raise ValueError(f"init_gamma shape {tuple(init_gamma.shape)} != ({B}, {V})")
         if prior_llr is not None and tuple(prior_llr.shape) != (B, V):
             raise ValueError(f"prior_llr shape {tuple(prior_llr.shape)} != ({B}, {V})")
-        if self.fast_max_unconverged_pct is not None:
-            return self._decode_legwise(
-                syndromes, init_posterior=init_posterior, init_gamma=init_gamma,
-                skip_relay_legs=skip_relay_legs, pre_iter_override=pre_iter_override,
-                return_last_posterior=return_last_posterior, prior_llr=prior_llr,
-            )
+        legwise = self.fast_max_unconverged_pct is not None and not skip_relay_legs and _records is None
         dev = self.device
         pre_iter = self.pre_iter if pre_iter_override is None else int(pre_iter_override)
         if pre_iter < 0:
@@ -639,7 +641,10 @@ class RelayBPDecoder:
                 if self.record_learned_gammas:
                     self.learned_gammas = torch.zeros(B, self.learned_legs, V, dtype=self.dtype, device=dev)
                     learn["gamma_out"] = self.learned_gammas
-            if B > 0:
+            if B > 0 and legwise:
+                self._decode_legwise(syn, prior, wj, prior_llr is not None, gtab, ig, ip, errors, weights,
+                                     iters, found, last_post, pre_iter, B, learn)
+            elif B > 0:
                 args = (syn, prior, wj, prior_llr is not None, gtab, ig, ip, errors, weights,
                         iters, found, last_post, pre_iter, skip_relay_legs, B, learn)
                 lkey = (B, learn is not None)
@@ -722,8 +727,8 @@ class RelayBPDecoder:
 
     def _launch(self, syn, prior, wj, prior_batched, gtab, ig, ip, errors, weights, iters, found,
                 last_post, pre_iter, skip_relay_legs, B, learn=None, workspace=None, stream=None,
-                noalloc=False) -> None:
-        plan = self.plan(B, learn is not None)
+                noalloc=False, leg_step=0) -> None:
+        plan = self.plan(B, learn is not None, legwise=leg_step > 0)
         if noalloc:
             ws = None
         elif workspace is not None:
@@ -734,6 +739,7 @@ class RelayBPDecoder:
             ws = self._workspace_for(plan.workspace_bytes)
         params, batch = self._args(syn, prior, wj, prior_batched, gtab, ig, ip, errors, weights, iters, found,
                                    last_post, pre_iter, skip_relay_legs, B, learn, ws)
+        params.leg_step = int(leg_step)
         cur = stream if stream is not None else torch.cuda.current_stream(self.device)
         ev = self.kernel_events
         if ev is not None:
@@ -833,10 +839,26 @@ class RelayBPDecoder:
             out["last_posterior"] = last_post
         return out
 
-    def _decode_legwise(self, syndromes, **kw):
-        raise NotImplementedError(
-            "fast_max_unconverged_pct needs the leg-synchronous schedule, which is not built yet"
-        )
+    def _decode_legwise(self, syn, prior, wj, prior_batched, gtab, ig, ip, errors, weights, iters, found,
+                        last_post, pre_iter, B, learn) -> None:
+        """The reference's leg-synchronous schedule with its fast_max_unconverged_pct
+        exit (decoder.py:2890-2937, 3655-3835): one launch per leg (the pre-iter
+        leg, then relay legs 0, 1, ...), each running that leg for the shots still
+        short of stop_nconv solutions from the state the previous launch parked and
+        leaving every shot's outputs as if the decode ended there.  After each leg
+        the found counts are read (one host synchronisation per leg, as the
+        reference's): the decode stops when every shot has stop_nconv solutions,
+        or when the fraction without them is below fast_max_unconverged_pct (and
+        below 1)."""
+        pct = float(self.fast_max_unconverged_pct)
+        for k in range(self.num_sets + 1):
+            self._launch(syn, prior, wj, prior_batched, gtab, ig, ip, errors, weights, iters, found, last_post,
+                         pre_iter, False, B, learn, leg_step=k + 1)
+            if k == self.num_sets:
+                break
+            unconverged = int((found < self.stop_nconv).sum()) / B
+            if unconverged == 0.0 or (unconverged < pct and unconverged < 1.0):
+                break
 
     def get_stats(self) -> Dict:
         return {

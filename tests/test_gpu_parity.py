@@ -310,3 +310,50 @@ def test_never_converged_counts_every_iteration():
     assert out["found_count"].cpu().tolist() == [0, 0]
     assert torch.isinf(out["weights"]).all()
     _compare(out, _oracle_for(dec, syn))
+
+
+# --- fast_max_unconverged_pct: the leg-synchronous early exit ---------------------------------
+
+
+def _unconverged(out, S):
+    return float((out["found_count"] < S).float().mean())
+
+
+@pytest.mark.parametrize("correction", ["first_leg", "last_leg"])
+def test_early_exit_after_relay_leg_one_equals_a_one_leg_decode(regular_case, correction):
+    """With the threshold between the unconverged fractions after the pre-iter
+    leg and after relay leg 1, the decode exits after relay leg 1: its results
+    equal a decode with num_sets = 1 (and the oracle's), bit for bit
+    (tests/test_unconverged_correction.py:105-121, 177-194)."""
+    H, priors, S = regular_case
+    kw = dict(pre_iter=2, set_max_iter=15, stop_nconv=1, seed=7, unconverged_correction=correction)
+    dev = "cuda:0"
+    syn = torch.from_numpy(S).to(dev)
+    leg0 = _decoder(H, priors, num_sets=0, **kw).decode(syn)
+    one = _decoder(H, priors, num_sets=1, **kw)
+    leg1 = one.decode(syn, return_last_posterior=True)
+    u0, u1 = _unconverged(leg0, 1), _unconverged(leg1, 1)
+    assert u1 < u0 < 1.0, (u0, u1)
+    dec = _decoder(H, priors, num_sets=6, fast_max_unconverged_pct=(u0 + u1) / 2, **kw)
+    out = dec.decode(syn, return_last_posterior=True)
+    torch.cuda.synchronize()
+    assert len(out["unconverged_indices"]) > 0
+    for k in ("errors", "weights", "iterations", "found_count", "last_posterior", "valid_mask"):
+        assert torch.equal(out[k], leg1[k]), k
+    _compare(out, _oracle_for(one, S))
+
+
+def test_early_exit_threshold_zero_runs_the_whole_schedule(regular_case):
+    """A threshold of 0 never exits early: the leg-synchronous decode equals
+    the whole-schedule decode, on the plain and the learned form."""
+    from pelagos_amd import learned
+
+    H, priors, S = regular_case
+    syn = torch.from_numpy(S).to("cuda:0")
+    for extra in ({}, dict(learned_gamma=learned.GammaMLP(seed=3), learned_legs=2)):
+        kw = dict(pre_iter=8, num_sets=6, set_max_iter=8, stop_nconv=3, seed=7, **extra)
+        whole = _decoder(H, priors, **kw).decode(syn, return_last_posterior=True)
+        stepped = _decoder(H, priors, fast_max_unconverged_pct=0.0, **kw).decode(syn, return_last_posterior=True)
+        torch.cuda.synchronize()
+        for k in ("errors", "weights", "iterations", "found_count", "last_posterior"):
+            assert torch.equal(stepped[k], whole[k]), (bool(extra), k)
