@@ -511,11 +511,14 @@ def batched_three_codes(scales=(0.001, 0.003, 0.01)) -> dict:
     return {"codes": {k: {"shots": v} for k, v in shots.items()}, "roles": roles, "launches": res}
 
 
-def serving_runs(seconds: float, prior_scale: float, rates=(2_000, 10_000, 50_000, 100_000, 200_000)) -> dict:
-    """The continuous-batching server on the three codes (priors scaled to mean
-    prior_scale, requests of 1-256 shots, code chosen uniformly, Poisson
-    arrivals, learned form on the 936 x 8785 code), with a bounded queue
-    that refuses work beyond it."""
+def serving_runs(seconds: float, prior_scale: float,
+                 rates=(2_000, 10_000, 50_000, 100_000, 200_000, 400_000)) -> dict:
+    """The server on the three codes (priors scaled to mean prior_scale,
+    requests of 1-256 shots, code chosen uniformly, Poisson arrivals, learned
+    form on the 936 x 8785 code): one persistent serving launch that admits
+    each request's shots as they arrive, with a bounded queue that refuses work
+    beyond it.  ``saturation_shots_per_s`` is the highest served rate among the
+    offered rates at which under 1 % of the offered shots were refused."""
     import numpy as np
     import torch
 
@@ -534,18 +537,27 @@ def serving_runs(seconds: float, prior_scale: float, rates=(2_000, 10_000, 50_00
         decs[k] = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0], **RELAY, **kw)
     pools = {k: torch.from_numpy(codes.syndromes_of(H, codes.sample_errors(p, 4096, seed=4))).cuda()
              for k, (H, p) in cs.items()}
-    server = DecodeServer(decs, max_batch=2048, slots=4, hbm_budget=64 << 30, max_queued_shots=16384)
+    torch.cuda.synchronize()
     out = []
-    for i, rate in enumerate(rates):
-        trace = poisson_trace(rate, seconds, list(cs), seed=10 + i)
-        st = run_trace(server, trace, pools, policy="reject")
-        st["rate"] = rate
-        out.append(st)
-        print(f"[bench] serving p={prior_scale} {rate}/s: p50 {st['p50_ms']} ms p99 {st['p99_ms']} ms "
-              f"{st['shots_per_s']} shots/s, refused {st['rejected_requests']}", file=sys.stderr, flush=True)
+    with DecodeServer(decs, max_batch=2048, max_queued_shots=16384, idle_ms=2.0) as server:
+        roles = server.roles()
+        for i, rate in enumerate(rates):
+            trace = poisson_trace(rate, seconds, list(cs), seed=10 + i)
+            l0 = server.launches
+            st = run_trace(server, trace, pools, policy="reject")
+            st["rate"] = rate
+            st["offered_shots"] = int(sum(b for _, _, b in trace))
+            st["engine_launches"] = server.launches - l0
+            out.append(st)
+            print(f"[bench] serving p={prior_scale} {rate}/s: p50 {st['p50_ms']} ms p99 {st['p99_ms']} ms "
+                  f"{st['shots_per_s']} shots/s, refused {st['rejected_requests']}, "
+                  f"launches {st['engine_launches']}", file=sys.stderr, flush=True)
+    ok = [st["shots_per_s"] for st in out if st["rejected_shots"] < 0.01 * max(1, st["offered_shots"])]
     return {"codes": list(cs), "prior_scale": prior_scale, "seconds_per_rate": seconds,
             "request_shots": "1-256 uniform", "learned_form_on": "banded936" if wpath.exists() else None,
-            "hbm_budget_gb": 64, "max_queued_shots": 16384, "policy": "reject", "rates": out}
+            "engine": "one persistent serving launch, shots admitted as requests arrive (pel_serve_launch)",
+            "roles": roles, "max_queued_shots": 16384, "policy": "reject",
+            "saturation_shots_per_s": max(ok) if ok else None, "rates": out}
 
 
 def main() -> int:

@@ -24,13 +24,14 @@
 extern "C" {
 #endif
 
-#define PEL_ABI_VERSION 8
+#define PEL_ABI_VERSION 9
 
 /* status codes */
 #define PEL_OK 0
 #define PEL_EINVAL (-1)    /* bad argument: maps to the reference's ValueError */
 #define PEL_EDEVICE (-2)   /* HIP runtime error: maps to RuntimeError */
 #define PEL_ENOMEM (-3)
+#define PEL_EBUSY (-4)     /* a serving queue has no free entry (pel_serve_publish) */
 
 typedef struct pel_graph pel_graph;
 
@@ -243,6 +244,80 @@ int pel_plan_jobs(const pel_graph *const *graphs, const int32_t *shots, const in
  * are recorded around the decode launch. */
 int pel_decode_jobs(const pel_graph *const *graphs, const pel_params *params, const pel_batch *batches, int32_t n,
                     const pel_jobs_plan *plan, void *table, void *stream);
+
+/* ---- serving: a persistent launch that takes shots as they are published --
+ * The host publishes units of work into a queue in host memory that the
+ * device reads: one shot of a shot-per-workgroup job, or up to
+ * PEL_SERVE_WAVE_SHOTS shots (one request's) of a shot-per-wave job.  The
+ * serving engine's workgroups claim units in publication order while the
+ * launch runs, so shots published after the launch started join it at once,
+ * and a slow shot holds one workgroup, not the shots published after it.  Each
+ * unit names its own syndrome rows and result buffers (device memory); once
+ * the engine has written a unit's results it sets the unit's done flag in host
+ * memory.  A workgroup that finds no unit for idle_us microseconds, or finds
+ * no unit left while the stop flag is set, leaves, so every launch ends by
+ * itself; the caller
+ * launches the engine again when it publishes work and the last launch has
+ * ended (units published meanwhile wait in the queue).  Each shot's results
+ * equal a pel_decode of that shot alone.
+ * No single counterpart in the reference: it replaces the persistent worker
+ * pool with ordered reassembly (process.py:114-254) and the per-batch decode
+ * call (decoder.py:2317-2449) for callers whose shots arrive over time. */
+#define PEL_SERVE_WAVE_SHOTS 64
+
+typedef struct {
+    int32_t job;              /* job index of the serving plan */
+    int32_t count;            /* shots: 1 for a shot-per-workgroup job, 1..PEL_SERVE_WAVE_SHOTS for a shot-per-wave
+                                 job (a unit that breaks this is marked done without being decoded) */
+    int64_t syn_row_stride;   /* bytes from one shot's syndrome row to the next (C when contiguous) */
+    const uint8_t *syndromes; /* device [count, C], 0/1 bytes, strided as given */
+    uint8_t *errors;          /* device [count, V] */
+    float *weights;           /* device [count] */
+    int32_t *iterations;      /* device [count] */
+    int32_t *found_count;     /* device [count] */
+    int64_t syn_col_stride;   /* bytes from one check's syndrome bit to the next (1 when contiguous) */
+} pel_serve_unit;
+
+typedef struct pel_serve_queue pel_serve_queue;
+
+/* A queue of `capacity` units: the ring, the published count and the stop flag
+ * in host memory the device reads, the done flags in host memory the device
+ * writes, the claim counter in device memory (on the current device). */
+int pel_serve_queue_create(int32_t capacity, pel_serve_queue **out);
+int pel_serve_queue_destroy(pel_serve_queue *q);
+
+/* Host address of the done flags: flag[k % capacity] >= k + 1 once unit k
+ * (counted from 0 in publication order) is done (an entry is reused only after
+ * its unit is done, so a later unit's number there also means done).  Also the
+ * capacity and the units published so far. */
+int pel_serve_queue_info(const pel_serve_queue *q, const int64_t **done, int32_t *capacity, int64_t *published);
+
+/* Publish n units; *first (may be NULL) receives the first one's number.
+ * PEL_EBUSY, publishing nothing, if an entry they need still holds a unit
+ * that is not done.  The units' buffers must stay valid and their syndromes
+ * written (the producing work complete) until their done flags are set. */
+int pel_serve_publish(pel_serve_queue *q, const pel_serve_unit *units, int32_t n, int64_t *first);
+
+/* Units of [first, first + n) that are done (a negative status on bad arguments). */
+int64_t pel_serve_done_count(const pel_serve_queue *q, int64_t first, int64_t n);
+
+/* Set (1) or clear (0) the stop flag: with it set, the engine's workgroups
+ * finish every unit published so far and leave without waiting for more. */
+int pel_serve_set_stop(pel_serve_queue *q, int32_t stop);
+
+/* Plan a serving engine over n jobs (graphs[i], flags[i] as in pel_plan_jobs).
+ * Each job's pel_batch holds staging rows, not shots: B =
+ * grid x (16 for a shot-per-wave job, else 1) syndrome and result rows the
+ * engine stages units through, with the workspace of workspace_bytes[i]. */
+int pel_plan_serve(const pel_graph *const *graphs, const int32_t *flags, int32_t n, pel_jobs_plan *out);
+
+/* Launch the serving engine on `stream` over the jobs of a pel_plan_serve plan;
+ * `table` as in pel_decode_jobs.  It returns at once; the launch ends when its
+ * workgroups have found no unit for idle_us microseconds, or none left with
+ * the stop flag set.  The queue, the jobs' buffers and the table must outlive
+ * the launch. */
+int pel_serve_launch(const pel_graph *const *graphs, const pel_params *params, const pel_batch *batches, int32_t n,
+                     const pel_jobs_plan *plan, void *table, pel_serve_queue *q, int64_t idle_us, void *stream);
 
 /* ---- learned memory strengths (no counterpart in the reference) ----------
  * Training entry points for the MLP above.  x [n][8] features, y [n] soft

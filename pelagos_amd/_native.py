@@ -13,7 +13,7 @@ from typing import Optional
 
 LIB_PATH = Path(__file__).resolve().parent / "libpelagos_relaybp.so"
 
-PEL_ABI_VERSION = 8
+PEL_ABI_VERSION = 9
 PEL_MAX_JOBS = 64
 PEL_NOALLOC_MAX_BYTES = 32 << 20
 PEL_PLAN_LEARNED = 1
@@ -23,6 +23,8 @@ PEL_OK = 0
 PEL_EINVAL = -1
 PEL_EDEVICE = -2
 PEL_ENOMEM = -3
+PEL_EBUSY = -4
+PEL_SERVE_WAVE_SHOTS = 64
 
 _vp = ctypes.c_void_p
 
@@ -108,6 +110,20 @@ class PelJobsPlan(ctypes.Structure):
     ]
 
 
+class PelServeUnit(ctypes.Structure):
+    _fields_ = [
+        ("job", ctypes.c_int32),
+        ("count", ctypes.c_int32),
+        ("syn_row_stride", ctypes.c_int64),
+        ("syndromes", _vp),
+        ("errors", _vp),
+        ("weights", _vp),
+        ("iterations", _vp),
+        ("found_count", _vp),
+        ("syn_col_stride", ctypes.c_int64),
+    ]
+
+
 # Every symbol the header declares, with its ctypes signature.
 SIGNATURES = {
     "pel_abi_version": (ctypes.c_int, []),
@@ -127,6 +143,17 @@ SIGNATURES = {
                                           ctypes.POINTER(PelPlan), _vp]),
     "pel_plan_jobs": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.POINTER(PelJobsPlan)]),
     "pel_decode_jobs": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.POINTER(PelJobsPlan), _vp, _vp]),
+    "pel_serve_queue_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(_vp)]),
+    "pel_serve_queue_destroy": (ctypes.c_int, [_vp]),
+    "pel_serve_queue_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.POINTER(ctypes.c_int64)),
+                                            ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]),
+    "pel_serve_publish": (ctypes.c_int, [_vp, ctypes.POINTER(PelServeUnit), ctypes.c_int32,
+                                         ctypes.POINTER(ctypes.c_int64)]),
+    "pel_serve_set_stop": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "pel_serve_done_count": (ctypes.c_int64, [_vp, ctypes.c_int64, ctypes.c_int64]),
+    "pel_plan_serve": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.POINTER(PelJobsPlan)]),
+    "pel_serve_launch": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.POINTER(PelJobsPlan), _vp, _vp,
+                                        ctypes.c_int64, _vp]),
     "pel_mlp_param_count": (ctypes.c_int, []),
     "pel_mlp_last_error": (ctypes.c_char_p, []),
     "pel_mlp_grad_workspace_bytes": (ctypes.c_int64, []),

@@ -52,6 +52,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -1714,6 +1715,254 @@ EngineFn pick_engine(int mask) {
     return engine_kernel<kKindsAll>;
 }
 
+// ---- serving: a persistent launch that takes units as they are published -----------------------
+// The host writes units into a ring in host memory, then raises the published
+// count (and fences, so the stores leave the CPU).  A workgroup's leader claims
+// the next unit with a compare-and-swap on the claim word (device memory),
+// below a published count it read from memory, and copies the unit into LDS.
+// The unit's shots are staged through the job's rows of this workgroup (a
+// shot-per-workgroup job) or of each wave (a shot-per-wave job, whose waves
+// take the unit's shots one at a time), decoded by the job's routine, and
+// their results copied to the unit's buffers.  Then every wave waits for its
+// stores to complete, the workgroup synchronises, and the leader sets the
+// unit's done flag.
+//
+// The claim word (device memory) holds the units claimed so far, the number of
+// workgroups holding a unit, and a closed bit.  A waiting leader backs off
+// (bounded sleeps); once it has waited idle_ticks, no workgroup holds a unit
+// and no unit is left, it sets the closed bit in the same compare-and-swap
+// that claims compete on, and every workgroup leaves at its next look.  So a
+// launch either has all its workgroups taking units or is ending, never a few
+// stragglers holding slow shots while new units wait for them: a launch stays
+// open while any shot runs.  A leader that sees the stop flag with no unit
+// left leaves at once.  A unit in hand is always finished.
+struct ServeArgs {
+    const int64_t *published;     // host memory: units published so far
+    const int32_t *stop;          // host memory: non-zero makes waiting workgroups leave
+    const pel_serve_unit *units;  // host memory: the ring, capacity entries
+    int64_t *done;                // host memory: done[k % capacity] = k + 1 once unit k is complete
+    uint64_t *claim;              // device memory: the claim word
+    int64_t idle_ticks;           // wall-clock ticks (100 MHz) a workgroup waits before it may close
+    int32_t capacity;
+    int32_t n_jobs;
+    uint32_t ctl_lds;             // LDS byte address of the workgroup's control block
+};
+constexpr int kServeBusyShift = 40;
+constexpr uint64_t kServeCount = (1ull << kServeBusyShift) - 1;  // units claimed
+constexpr uint64_t kServeBusy = 1ull << kServeBusyShift;         // one workgroup holding a unit
+constexpr uint64_t kServeClosed = 1ull << 63;
+
+// Reopens the claim word for a new launch (the last one ended closed, with no
+// unit held).
+__global__ void __launch_bounds__(64) serve_open_kernel(uint64_t *claim) {
+    if (threadIdx.x == 0) __hip_atomic_fetch_and(claim, ~kServeClosed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// The workgroup's control block in LDS, 8-byte words: the unit number (-1:
+// leave), the unit (eight words), the leader's last read of the published
+// count and the time it last finished a unit, and the waves' shot counter.
+// Everything a workgroup needs after a shot is read back from here, so no
+// register holds it across the shot.
+constexpr int kServeCtlBytes = 128;
+enum { kCtlUnit = 0, kCtlWords = 1, kCtlSeen = 9, kCtlIdle = 10, kCtlNext = 11 };
+static_assert(sizeof(pel_serve_unit) == 64, "a unit is eight 8-byte words");
+static_assert(8 * (kCtlNext + 1) <= kServeCtlBytes, "control block");
+
+template <typename T>
+__device__ __forceinline__ T *uniform_ptr(int64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+    return reinterpret_cast<T *>((uint64_t)lo | ((uint64_t)hi << 32));
+}
+
+__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
+    return ((int64_t)uniform_i32((int)(v >> 32)) << 32) | (uint32_t)uniform_i32((int)v);
+}
+
+// The unit in the control block, as uniform values.
+__device__ __forceinline__ pel_serve_unit ctl_unit(const int64_t *ctl) {
+    pel_serve_unit u;
+    u.job = uniform_i32((int)ctl[kCtlWords]);
+    u.count = uniform_i32((int)(ctl[kCtlWords] >> 32));
+    u.syn_row_stride = uniform_i64(ctl[kCtlWords + 1]);
+    u.syndromes = uniform_ptr<const uint8_t>(ctl[kCtlWords + 2]);
+    u.errors = uniform_ptr<uint8_t>(ctl[kCtlWords + 3]);
+    u.weights = uniform_ptr<float>(ctl[kCtlWords + 4]);
+    u.iterations = uniform_ptr<int32_t>(ctl[kCtlWords + 5]);
+    u.found_count = uniform_ptr<int32_t>(ctl[kCtlWords + 6]);
+    u.syn_col_stride = uniform_i64(ctl[kCtlWords + 7]);
+    return u;
+}
+
+// n bytes from src to dst by the threads of a BLOCK-thread group.
+template <int BLOCK>
+__device__ __forceinline__ void copy_bytes(uint8_t *__restrict__ dst, const uint8_t *__restrict__ src, int n) {
+    for (int i = gtid<BLOCK>(); i < n; i += BLOCK) dst[i] = src[i];
+}
+
+// Memory the engine shares with the host and with other kernels (the units'
+// syndromes and results) is read and written with system-scope accesses, which
+// go to memory instead of the caches: a unit's syndromes may have been written
+// after this launch read the same addresses, and its results must be in memory
+// before its done flag.  No acquire fence (an L2 invalidate of the whole XCD,
+// which would also evict the graph indices every other shot on the XCD reads
+// each iteration) is needed for either.
+template <typename T>
+__device__ __forceinline__ T sys_load(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+template <typename T>
+__device__ __forceinline__ void sys_store(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Shot i's syndrome row of the unit (strided as the unit says) into the
+// staging row dst of C bytes.
+template <int BLOCK>
+__device__ __forceinline__ void serve_stage(uint8_t *__restrict__ dst, const pel_serve_unit &u, int i, int C) {
+    const uint8_t *src = u.syndromes + (int64_t)i * u.syn_row_stride;
+    for (int c = gtid<BLOCK>(); c < C; c += BLOCK) dst[c] = sys_load(src + (int64_t)c * u.syn_col_stride);
+}
+
+// Shot i of the unit from staging row b: its error row and scalars.
+template <int BLOCK>
+__device__ __forceinline__ void serve_results(const KArgs &a, int b, const pel_serve_unit &u, int i) {
+    const int V = a.g.V;
+    const uint8_t *src = a.errors + (size_t)b * V;
+    uint8_t *dst = u.errors + (size_t)i * V;
+    for (int j = gtid<BLOCK>(); j < V; j += BLOCK) sys_store(dst + j, src[j]);
+    if (gleader<BLOCK>()) {
+        sys_store(reinterpret_cast<uint32_t *>(u.weights) + i, __float_as_uint(a.weights[b]));
+        sys_store(u.iterations + i, a.iters[b]);
+        sys_store(u.found_count + i, a.found[b]);
+    }
+}
+
+// The unit's one shot on the whole workgroup through its staging row blockIdx.x.
+template <int PLACE, bool LEARN>
+__device__ __forceinline__ void serve_group_unit(const KArgs &a, uint8_t *smem, const int64_t *ctl) {
+    const int b = (int)blockIdx.x;
+    const size_t C = (size_t)a.g.C;
+    serve_stage<kEngineBlock>(const_cast<uint8_t *>(a.syn) + (size_t)b * C, ctl_unit(ctl), 0, (int)C);
+    __syncthreads();
+    engine_shot<kEngineBlock, PLACE, LEARN>(a, b, smem, 0, a.spill + (int64_t)b * a.spill_stride);
+    __syncthreads();  // the shot's error row and scalars, for every thread's copy
+    serve_results<kEngineBlock>(a, b, ctl_unit(ctl), 0);
+}
+
+// The unit's shots on the waves of the workgroup, each wave taking the next
+// one (counter in LDS) through its own staging row.
+template <int PLACE>
+__device__ __forceinline__ void serve_wave_unit(const KArgs &a, uint32_t stride, uint8_t *smem, int64_t *ctl) {
+    const uint32_t wave = threadIdx.x >> 6;
+    const int b = (int)(blockIdx.x * kEngineWaves + wave);
+    int *next = reinterpret_cast<int *>(ctl + kCtlNext);
+    for (;;) {
+        int i = 0;
+        if ((opaque_tid() & 63u) == 0) i = atomicAdd(next, 1);
+        i = uniform_i32(i);
+        const pel_serve_unit u = ctl_unit(ctl);
+        if (i >= u.count) break;
+        const size_t C = (size_t)a.g.C;
+        serve_stage<64>(const_cast<uint8_t *>(a.syn) + (size_t)b * C, u, i, (int)C);
+        gsync<64>();
+        engine_shot<64, PLACE, false>(a, b, smem, wave * stride, a.spill + (int64_t)b * a.spill_stride);
+        gsync<64>();
+        serve_results<64>(a, b, ctl_unit(ctl), i);
+    }
+}
+
+// The leader's wait for the next unit: its number, or -1 to leave.
+__device__ __forceinline__ int64_t serve_claim(const ServeArgs &s, int64_t *ctl) {
+    int64_t seen = ctl[kCtlSeen];
+    const int64_t idle_from = ctl[kCtlIdle];
+    int64_t k = -1;
+    int nap = 1;
+    for (;;) {
+        uint64_t w = __hip_atomic_load(s.claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (w & kServeClosed) break;
+        const int64_t c = (int64_t)(w & kServeCount);
+        if (c >= seen) seen = sys_load(s.published);
+        if (c < seen) {
+            if (__hip_atomic_compare_exchange_strong(s.claim, &w, w + 1 + kServeBusy, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                k = c;
+                break;
+            }
+            continue;
+        }
+        if (__hip_atomic_load(s.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) break;
+        if ((int64_t)wall_clock64() - idle_from > s.idle_ticks && (w & ~kServeClosed) >> kServeBusyShift == 0) {
+            if (__hip_atomic_compare_exchange_strong(s.claim, &w, w | kServeClosed, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                break;
+            continue;
+        }
+        for (int z = 0; z < nap; ++z) __builtin_amdgcn_s_sleep(127);  // about 3.4 us each
+        nap = nap < 8 ? 2 * nap : 8;
+    }
+    ctl[kCtlSeen] = seen;
+    ctl[kCtlUnit] = k;
+    if (k >= 0) {
+        const int64_t *src = reinterpret_cast<const int64_t *>(s.units + k % s.capacity);
+        for (int w = 0; w < 8; ++w) ctl[kCtlWords + w] = sys_load(src + w);
+        ctl[kCtlNext] = 0;
+    }
+    return k;
+}
+
+template <int KINDS>
+__global__ void __launch_bounds__(kEngineBlock) serve_kernel(ServeArgs s, const JobDesc *__restrict__ jobs) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    int64_t *ctl = reinterpret_cast<int64_t *>(smem + s.ctl_lds);
+    if (opaque_tid() == 0) {
+        ctl[kCtlSeen] = 0;
+        ctl[kCtlIdle] = wall_clock64();
+    }
+    for (;;) {
+        if (opaque_tid() == 0) (void)serve_claim(s, ctl);
+        __syncthreads();
+        if (uniform_i64(ctl[kCtlUnit]) < 0) break;
+        const pel_serve_unit u = ctl_unit(ctl);
+        // a malformed unit (the host checks them) is marked done without work
+        const bool ok = u.job >= 0 && u.job < s.n_jobs && u.count >= 1 && u.count <= PEL_SERVE_WAVE_SHOTS &&
+                        u.syndromes && u.errors && u.weights && u.iterations && u.found_count;
+        const int kind = ok ? uniform_i32(jobs[u.job].kind) : -1;
+        if (ok && ((kind & kKindWave) || u.count == 1)) {
+            const JobDesc &jd = jobs[u.job];
+            const KArgs &a = jd.a;
+#define PEL_SERVE_CASE(K, CALL) \
+    else if ((KINDS & kind_bit(K)) && kind == (K)) CALL;
+            if (false) {
+            }
+            PEL_SERVE_CASE(15, (serve_group_unit<15, false>(a, smem, ctl)))
+            PEL_SERVE_CASE(7, (serve_group_unit<7, false>(a, smem, ctl)))
+            PEL_SERVE_CASE(3, (serve_group_unit<3, false>(a, smem, ctl)))
+            PEL_SERVE_CASE(1, (serve_group_unit<1, false>(a, smem, ctl)))
+            PEL_SERVE_CASE(0, (serve_group_unit<0, false>(a, smem, ctl)))
+            PEL_SERVE_CASE(15 | kKindLearn, (serve_group_unit<15, true>(a, smem, ctl)))
+            PEL_SERVE_CASE(7 | kKindLearn, (serve_group_unit<7, true>(a, smem, ctl)))
+            PEL_SERVE_CASE(15 | kKindWave, (serve_wave_unit<15>(a, jd.stride, smem, ctl)))
+            PEL_SERVE_CASE(7 | kKindWave, (serve_wave_unit<7>(a, jd.stride, smem, ctl)))
+#undef PEL_SERVE_CASE
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wave's result stores are in memory
+        __syncthreads();
+        if (opaque_tid() == 0) {
+            const int64_t k = ctl[kCtlUnit];
+            sys_store(s.done + k % s.capacity, k + 1);
+            __hip_atomic_fetch_sub(s.claim, kServeBusy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ctl[kCtlIdle] = wall_clock64();
+        }
+    }
+}
+
+typedef void (*ServeFn)(ServeArgs, const JobDesc *);
+
+ServeFn pick_serve(int mask) {
+    if ((mask & ~(1 | 8 | 32 | 128)) == 0) return serve_kernel<1 | 8 | 32 | 128>;  // the three-code mix
+    return serve_kernel<kKindsAll>;
+}
+
 // Work-queue order of the first legs: shots by decreasing syndrome weight
 // (heavier syndromes tend to need more iterations, so they start first).
 // syndrome_key_kernel: one wave per shot computes the bucket key (weights of
@@ -2590,16 +2839,19 @@ void fill_args(KArgs &a, const pel_graph *g, const pel_params *p, const pel_batc
 // The per-decode launches before the decode kernel: syndrome weights and the
 // first-leg queue order (which also reset the queue counters), the shared
 // weights and priors in position order, and the network's operand layout.
+// (order = false: the serving engine, which takes no shot from the queue.)
 int launch_prep(const pel_graph *g, const pel_batch *b, const KArgs &a, uint8_t *ws, const WsLayout &w,
-                hipStream_t s) {
-    int32_t *key = reinterpret_cast<int32_t *>(ws + w.key);
-    const int key_blocks = (int)((b->B + kOrderBlock / 64 - 1) / (kOrderBlock / 64));
-    hipLaunchKernelGGL(syndrome_key_kernel, dim3(key_blocks), dim3(kOrderBlock), 0, s, b->syndromes, b->B, g->C,
-                       key, a.rq_slot);
-    PEL_HIP(hipGetLastError());
-    hipLaunchKernelGGL(order_shots_kernel, dim3(1), dim3(kOrderBlock), 0, s, key, b->B, const_cast<int32_t *>(a.order),
-                       a.ctrl);
-    PEL_HIP(hipGetLastError());
+                hipStream_t s, bool order = true) {
+    if (order) {
+        int32_t *key = reinterpret_cast<int32_t *>(ws + w.key);
+        const int key_blocks = (int)((b->B + kOrderBlock / 64 - 1) / (kOrderBlock / 64));
+        hipLaunchKernelGGL(syndrome_key_kernel, dim3(key_blocks), dim3(kOrderBlock), 0, s, b->syndromes, b->B, g->C,
+                           key, a.rq_slot);
+        PEL_HIP(hipGetLastError());
+        hipLaunchKernelGGL(order_shots_kernel, dim3(1), dim3(kOrderBlock), 0, s, key, b->B,
+                           const_cast<int32_t *>(a.order), a.ctrl);
+        PEL_HIP(hipGetLastError());
+    }
     if (!b->prior_batched) {  // the shared weights and priors in position order, once per decode
         const unsigned pblocks = (unsigned)std::min<int64_t>((g->V + 255) / 256, 1024);
         hipLaunchKernelGGL(permute_vars_kernel, dim3(pblocks), dim3(256), 0, s, b->wj, const_cast<float *>(a.wj_pos),
@@ -2703,53 +2955,59 @@ int64_t engine_role(const pel_graph *g, bool learn, int64_t budget, int *group, 
     return lb.lds;
 }
 
-int plan_jobs_impl(const pel_graph *const *graphs, const int32_t *shots, const int32_t *flags, int32_t n,
-                   pel_jobs_plan *out) {
-    if (!graphs || !shots || !out) return fail(PEL_EINVAL, "pel_plan_jobs: NULL argument");
-    if (n < 1 || n > kMaxJobs) return fail(PEL_EINVAL, "pel_plan_jobs: between 1 and PEL_MAX_JOBS jobs");
+// Plans an engine launch over n jobs: pel_decode_jobs over shots[i] shots
+// (serve = false), or the serving engine (serve = true: every workgroup that
+// can be resident, and each job's batch holds its staging rows).
+int plan_engine(const std::string &w_, const pel_graph *const *graphs, const int32_t *shots, const int32_t *flags,
+                int32_t n, pel_jobs_plan *out, bool serve) {
+    if (!graphs || (!serve && !shots) || !out) return fail(PEL_EINVAL, w_ + ": NULL argument");
+    if (n < 1 || n > kMaxJobs) return fail(PEL_EINVAL, w_ + ": between 1 and PEL_MAX_JOBS jobs");
     int dev = 0, max_lds = 0, n_cu = 0;
     PEL_HIP(hipGetDevice(&dev));
     PEL_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     PEL_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
     memset(out, 0, sizeof *out);
-    const int64_t budget = (int64_t)max_lds - 16;  // the control words sit after every job's LDS
+    const int64_t ctl = serve ? kServeCtlBytes : 16;
+    const int64_t budget = (int64_t)max_lds - ctl;  // the control words sit after every job's LDS
     int64_t lds = 0, units = 0;
     for (int32_t i = 0; i < n; ++i) {
         const pel_graph *g = graphs[i];
-        if (!g) return fail(PEL_EINVAL, "pel_plan_jobs: NULL graph");
-        if (g->device != dev) return fail(PEL_EINVAL, "pel_plan_jobs: a graph is on another device");
-        if (shots[i] < 0) return fail(PEL_EINVAL, "pel_plan_jobs: negative batch");
+        if (!g) return fail(PEL_EINVAL, w_ + ": NULL graph");
+        if (g->device != dev) return fail(PEL_EINVAL, w_ + ": a graph is on another device");
+        if (!serve && shots[i] < 0) return fail(PEL_EINVAL, w_ + ": negative batch");
         const int32_t fl = flags ? flags[i] : 0;
-        if (fl & ~PEL_PLAN_LEARNED) return fail(PEL_EINVAL, "pel_plan_jobs: unknown flags");
+        if (fl & ~PEL_PLAN_LEARNED) return fail(PEL_EINVAL, w_ + ": unknown flags");
         const bool learn = (fl & PEL_PLAN_LEARNED) != 0;
         int group = 0, place = -1;
         GroupLayout L;
         const int64_t need = engine_role(g, learn, budget, &group, &place, &L);
-        if (need < 0) return fail(PEL_EINVAL, "pel_plan_jobs: a job's state does not fit the LDS budget");
+        if (need < 0) return fail(PEL_EINVAL, w_ + ": a job's state does not fit the LDS budget");
         out->group[i] = group;
         out->placement[i] = place;
         out->learned[i] = learn ? 1 : 0;
         out->job_lds[i] = (int32_t)L.lds;
         out->spill_stride[i] = align_up(L.spill, 256);
         lds = std::max(lds, need);
-        units += group == 64 ? (shots[i] + kEngineWaves - 1) / kEngineWaves : shots[i];
+        if (!serve) units += group == 64 ? (shots[i] + kEngineWaves - 1) / kEngineWaves : shots[i];
     }
-    lds = align_up(lds, 16) + 16;
+    lds = align_up(lds, 16) + ctl;
     int mask = 0;
     for (int32_t i = 0; i < n; ++i)
         mask |= kind_bit(out->placement[i] | (out->group[i] == 64 ? kKindWave : 0) | (out->learned[i] ? kKindLearn : 0));
-    const void *fn = reinterpret_cast<const void *>(pick_engine(mask));
+    const void *fn = serve ? reinterpret_cast<const void *>(pick_serve(mask))
+                           : reinterpret_cast<const void *>(pick_engine(mask));
     hipFuncAttributes fa;
     PEL_HIP(hipFuncGetAttributes(&fa, fn));
-    if (fa.sharedSizeBytes != 0) return fail(PEL_EDEVICE, "pel_plan_jobs: the engine kernel has static LDS");
+    if (fa.sharedSizeBytes != 0) return fail(PEL_EDEVICE, w_ + ": the engine kernel has static LDS");
     if (lds > 65536) {
         (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         (void)hipGetLastError();
     }
     int per_cu = 0;
     PEL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kEngineBlock, (size_t)lds));
-    if (per_cu <= 0) return fail(PEL_EDEVICE, "pel_plan_jobs: the engine kernel cannot be resident");
-    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * n_cu, units));
+    if (per_cu <= 0) return fail(PEL_EDEVICE, w_ + ": the engine kernel cannot be resident");
+    const int64_t resident = (int64_t)per_cu * n_cu;
+    const int64_t grid = serve ? resident : std::max<int64_t>(1, std::min<int64_t>(resident, units));
     out->n_jobs = n;
     out->table_bytes = (int64_t)sizeof(JobDesc) * n;
     out->block = kEngineBlock;
@@ -2758,25 +3016,32 @@ int plan_jobs_impl(const pel_graph *const *graphs, const int32_t *shots, const i
     for (int32_t i = 0; i < n; ++i) {
         const bool learn = out->learned[i] != 0;
         const int64_t slots = grid * (out->group[i] == 64 ? kEngineWaves : 1);
-        out->workspace_bytes[i] = ws_layout(graphs[i], shots[i], false, learn).total_fixed + out->spill_stride[i] * slots;
+        const int32_t B = serve ? (int32_t)slots : shots[i];
+        out->workspace_bytes[i] = ws_layout(graphs[i], B, false, learn).total_fixed + out->spill_stride[i] * slots;
     }
     return PEL_OK;
 }
 
-int decode_jobs_impl(const pel_graph *const *graphs, const pel_params *params, const pel_batch *batches, int32_t n,
-                     const pel_jobs_plan *plan, void *table, void *stream) {
-    const std::string w_ = "pel_decode_jobs";
-    if (!graphs || !params || !batches || !plan || !table) return fail(PEL_EINVAL, w_ + ": NULL argument");
+// Whether a batch has per-shot inputs or records, which a serving job (whose
+// rows are staging rows, not shots) cannot take.
+bool p_has_per_shot(const pel_params *p, const pel_batch *b) {
+    return b->prior_batched || b->gamma_per_row || b->init_gamma || b->init_posterior || b->last_posterior ||
+           b->shot_clock || b->feat_out || b->hard_out || b->gamma_out || p->leg_step != 0;
+}
+
+// The job table's descriptors of n jobs under an engine plan (serve: the
+// serving engine, whose batches are staging rows); *mask the job kinds.
+int job_descs(const std::string &w_, const pel_graph *const *graphs, const pel_params *params,
+              const pel_batch *batches, int32_t n, const pel_jobs_plan *plan, bool serve, std::vector<JobDesc> &desc,
+              int *mask) {
+    if (!graphs || !params || !batches || !plan) return fail(PEL_EINVAL, w_ + ": NULL argument");
     if (n < 1 || n > kMaxJobs || plan->n_jobs != n || plan->block != kEngineBlock || plan->grid <= 0 ||
         plan->table_bytes < (int64_t)sizeof(JobDesc) * n)
         return fail(PEL_EINVAL, w_ + ": plan does not match these jobs");
-    EngineArgs e;
-    memset(&e, 0, sizeof e);
-    e.n_jobs = n;
-    e.ctl_lds = (uint32_t)(plan->lds_bytes - 16);
-    std::vector<JobDesc> desc((size_t)n);
+    desc.assign((size_t)n, JobDesc{});
     memset(desc.data(), 0, sizeof(JobDesc) * desc.size());
     int64_t lds_need = 0;
+    *mask = 0;
     for (int32_t i = 0; i < n; ++i) {
         const pel_graph *g = graphs[i];
         const pel_batch *b = &batches[i];
@@ -2795,23 +3060,26 @@ int decode_jobs_impl(const pel_graph *const *graphs, const pel_params *params, c
         if (L.lds != plan->job_lds[i] || align_up(L.spill, 256) != plan->spill_stride[i] ||
             plan->workspace_bytes[i] < w.total_fixed + plan->spill_stride[i] * slots)
             return fail(PEL_EINVAL, w_ + ": plan does not match these jobs");
+        if (serve && (b->B != slots || p_has_per_shot(&params[i], b)))
+            return fail(PEL_EINVAL, w_ + ": a serving job's batch is grid x (16 or 1) staging rows with shared "
+                                         "priors and gamma rows, no records and no leg steps");
         lds_need = std::max(lds_need, group == 64 ? kEngineWaves * align_up(L.lds, 16) : L.lds);
         fill_args(desc[i].a, g, &params[i], b, static_cast<uint8_t *>(b->workspace), w, learn, 0, group, L,
                   plan->spill_stride[i]);
         desc[i].a.shot_clock = nullptr;
         desc[i].kind = place | (group == 64 ? kKindWave : 0) | (learn ? kKindLearn : 0);
         desc[i].stride = (uint32_t)align_up(L.lds, 16);
+        *mask |= kind_bit(desc[i].kind);
     }
-    if (lds_need + 16 > plan->lds_bytes) return fail(PEL_EINVAL, w_ + ": plan does not match these jobs");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    for (int32_t i = 0; i < n; ++i) {
-        if (batches[i].B == 0) continue;
-        const WsLayout w = ws_layout(graphs[i], batches[i].B, false, plan->learned[i] != 0);
-        const int rc = launch_prep(graphs[i], &batches[i], desc[i].a, static_cast<uint8_t *>(batches[i].workspace), w, s);
-        if (rc != PEL_OK) return rc;
-    }
-    JobDesc *tab = static_cast<JobDesc *>(table);
-    for (int32_t i0 = 0; i0 < n; i0 += kSetupJobs) {  // the job table, kSetupJobs descriptors per launch
+    if (lds_need + (serve ? kServeCtlBytes : 16) > plan->lds_bytes)
+        return fail(PEL_EINVAL, w_ + ": plan does not match these jobs");
+    return PEL_OK;
+}
+
+// The job table, kSetupJobs descriptors per setup launch on stream s.
+int write_job_table(const std::vector<JobDesc> &desc, JobDesc *tab, hipStream_t s) {
+    const int32_t n = (int32_t)desc.size();
+    for (int32_t i0 = 0; i0 < n; i0 += kSetupJobs) {
         SetupArgs sa;
         memset(&sa, 0, sizeof sa);
         sa.dst = tab + i0;
@@ -2820,13 +3088,97 @@ int decode_jobs_impl(const pel_graph *const *graphs, const pel_params *params, c
         hipLaunchKernelGGL(jobs_setup_kernel, dim3(1), dim3(256), 0, s, sa);
         PEL_HIP(hipGetLastError());
     }
-    if (batches[0].ev_begin) PEL_HIP(hipEventRecord(static_cast<hipEvent_t>(batches[0].ev_begin), s));
+    return PEL_OK;
+}
+
+int decode_jobs_impl(const pel_graph *const *graphs, const pel_params *params, const pel_batch *batches, int32_t n,
+                     const pel_jobs_plan *plan, void *table, void *stream) {
+    const std::string w_ = "pel_decode_jobs";
+    if (!table) return fail(PEL_EINVAL, w_ + ": NULL argument");
+    std::vector<JobDesc> desc;
     int mask = 0;
-    for (int32_t i = 0; i < n; ++i) mask |= kind_bit(desc[i].kind);
+    int rc = job_descs(w_, graphs, params, batches, n, plan, false, desc, &mask);
+    if (rc != PEL_OK) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    for (int32_t i = 0; i < n; ++i) {
+        if (batches[i].B == 0) continue;
+        const WsLayout w = ws_layout(graphs[i], batches[i].B, false, plan->learned[i] != 0);
+        rc = launch_prep(graphs[i], &batches[i], desc[i].a, static_cast<uint8_t *>(batches[i].workspace), w, s);
+        if (rc != PEL_OK) return rc;
+    }
+    JobDesc *tab = static_cast<JobDesc *>(table);
+    rc = write_job_table(desc, tab, s);
+    if (rc != PEL_OK) return rc;
+    EngineArgs e;
+    memset(&e, 0, sizeof e);
+    e.n_jobs = n;
+    e.ctl_lds = (uint32_t)(plan->lds_bytes - 16);
+    if (batches[0].ev_begin) PEL_HIP(hipEventRecord(static_cast<hipEvent_t>(batches[0].ev_begin), s));
     hipLaunchKernelGGL(pick_engine(mask), dim3(plan->grid), dim3(kEngineBlock), (size_t)plan->lds_bytes, s, e,
                        static_cast<const JobDesc *>(tab));
     PEL_HIP(hipGetLastError());
     if (batches[0].ev_end) PEL_HIP(hipEventRecord(static_cast<hipEvent_t>(batches[0].ev_end), s));
+    return PEL_OK;
+}
+
+}  // namespace
+
+// The serving queue: one block of host memory the device reads and writes
+// (the published count and stop flag, the unit ring, the done flags) and the
+// claim counter in device memory.
+struct pel_serve_queue {
+    int device = 0;
+    int32_t capacity = 0;
+    uint8_t *host = nullptr;     // host address of the block
+    uint8_t *mapped = nullptr;   // its device address
+    uint64_t *claim = nullptr;   // device memory: the claim word
+    int64_t next = 0;            // units published
+    std::mutex mu;
+    static constexpr int64_t kUnitsOff = 64;
+    int64_t *published() const { return reinterpret_cast<int64_t *>(host); }
+    int32_t *stop() const { return reinterpret_cast<int32_t *>(host + 8); }
+    pel_serve_unit *units() const { return reinterpret_cast<pel_serve_unit *>(host + kUnitsOff); }
+    int64_t *done() const { return reinterpret_cast<int64_t *>(host + kUnitsOff + 64LL * capacity); }
+    int64_t bytes() const { return kUnitsOff + 64LL * capacity + 8LL * capacity; }
+};
+
+namespace {
+
+int serve_launch_impl(const pel_graph *const *graphs, const pel_params *params, const pel_batch *batches, int32_t n,
+                      const pel_jobs_plan *plan, void *table, pel_serve_queue *q, int64_t idle_us, void *stream) {
+    const std::string w_ = "pel_serve_launch";
+    if (!table || !q) return fail(PEL_EINVAL, w_ + ": NULL argument");
+    if (idle_us < 0 || idle_us > 60LL * 1000 * 1000) return fail(PEL_EINVAL, w_ + ": idle_us must lie in 0 .. 6e7");
+    std::vector<JobDesc> desc;
+    int mask = 0;
+    int rc = job_descs(w_, graphs, params, batches, n, plan, true, desc, &mask);
+    if (rc != PEL_OK) return rc;
+    if (graphs[0]->device != q->device) return fail(PEL_EINVAL, w_ + ": the queue is on another device");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    for (int32_t i = 0; i < n; ++i) {
+        const WsLayout w = ws_layout(graphs[i], batches[i].B, false, plan->learned[i] != 0);
+        rc = launch_prep(graphs[i], &batches[i], desc[i].a, static_cast<uint8_t *>(batches[i].workspace), w, s, false);
+        if (rc != PEL_OK) return rc;
+    }
+    JobDesc *tab = static_cast<JobDesc *>(table);
+    rc = write_job_table(desc, tab, s);
+    if (rc != PEL_OK) return rc;
+    hipLaunchKernelGGL(serve_open_kernel, dim3(1), dim3(64), 0, s, q->claim);
+    PEL_HIP(hipGetLastError());
+    ServeArgs sa;
+    memset(&sa, 0, sizeof sa);
+    sa.published = reinterpret_cast<const int64_t *>(q->mapped);
+    sa.stop = reinterpret_cast<const int32_t *>(q->mapped + 8);
+    sa.units = reinterpret_cast<const pel_serve_unit *>(q->mapped + pel_serve_queue::kUnitsOff);
+    sa.done = reinterpret_cast<int64_t *>(q->mapped + pel_serve_queue::kUnitsOff + 64LL * q->capacity);
+    sa.claim = q->claim;
+    sa.idle_ticks = idle_us * 100;  // the wall clock runs at 100 MHz
+    sa.capacity = q->capacity;
+    sa.n_jobs = n;
+    sa.ctl_lds = (uint32_t)(plan->lds_bytes - kServeCtlBytes);
+    hipLaunchKernelGGL(pick_serve(mask), dim3(plan->grid), dim3(kEngineBlock), (size_t)plan->lds_bytes, s, sa,
+                       static_cast<const JobDesc *>(tab));
+    PEL_HIP(hipGetLastError());
     return PEL_OK;
 }
 
@@ -2845,12 +3197,110 @@ int pel_decode_noalloc(const pel_graph *g, const pel_params *p, const pel_batch 
 
 int pel_plan_jobs(const pel_graph *const *graphs, const int32_t *shots, const int32_t *flags, int32_t n,
                   pel_jobs_plan *out) {
-    return plan_jobs_impl(graphs, shots, flags, n, out);
+    return plan_engine("pel_plan_jobs", graphs, shots, flags, n, out, false);
 }
 
 int pel_decode_jobs(const pel_graph *const *graphs, const pel_params *params, const pel_batch *batches, int32_t n,
                     const pel_jobs_plan *plan, void *table, void *stream) {
     return decode_jobs_impl(graphs, params, batches, n, plan, table, stream);
+}
+
+int pel_plan_serve(const pel_graph *const *graphs, const int32_t *flags, int32_t n, pel_jobs_plan *out) {
+    return plan_engine("pel_plan_serve", graphs, nullptr, flags, n, out, true);
+}
+
+int pel_serve_launch(const pel_graph *const *graphs, const pel_params *params, const pel_batch *batches, int32_t n,
+                     const pel_jobs_plan *plan, void *table, pel_serve_queue *q, int64_t idle_us, void *stream) {
+    return serve_launch_impl(graphs, params, batches, n, plan, table, q, idle_us, stream);
+}
+
+int pel_serve_queue_create(int32_t capacity, pel_serve_queue **out) {
+    if (!out) return fail(PEL_EINVAL, "pel_serve_queue_create: NULL argument");
+    *out = nullptr;
+    if (capacity < 1 || capacity > (1 << 24)) return fail(PEL_EINVAL, "pel_serve_queue_create: capacity must lie in 1 .. 2^24");
+    pel_serve_queue *q = new pel_serve_queue();
+    q->capacity = capacity;
+    PEL_HIP(hipGetDevice(&q->device));
+    void *h = nullptr;
+    hipError_t e = hipHostMalloc(&h, (size_t)q->bytes(), hipHostMallocCoherent | hipHostMallocMapped);
+    if (e != hipSuccess) {
+        delete q;
+        return fail(PEL_ENOMEM, std::string("pel_serve_queue_create: ") + hipGetErrorString(e));
+    }
+    q->host = static_cast<uint8_t *>(h);
+    memset(q->host, 0, (size_t)q->bytes());
+    void *d = nullptr;
+    e = hipHostGetDevicePointer(&d, h, 0);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&q->claim), 8);
+    if (e == hipSuccess) e = hipMemset(q->claim, 0, 8);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        if (q->claim) (void)hipFree(q->claim);
+        (void)hipHostFree(h);
+        delete q;
+        return fail(PEL_EDEVICE, std::string("pel_serve_queue_create: ") + hipGetErrorString(e));
+    }
+    q->mapped = static_cast<uint8_t *>(d);
+    *out = q;
+    return PEL_OK;
+}
+
+int pel_serve_queue_destroy(pel_serve_queue *q) {
+    if (!q) return PEL_OK;
+    hipError_t e1 = hipFree(q->claim);
+    hipError_t e2 = hipHostFree(q->host);
+    delete q;
+    if (e1 != hipSuccess || e2 != hipSuccess)
+        return fail(PEL_EDEVICE, std::string("pel_serve_queue_destroy: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+    return PEL_OK;
+}
+
+int pel_serve_queue_info(const pel_serve_queue *q, const int64_t **done, int32_t *capacity, int64_t *published) {
+    if (!q) return fail(PEL_EINVAL, "pel_serve_queue_info: NULL queue");
+    if (done) *done = q->done();
+    if (capacity) *capacity = q->capacity;
+    if (published) *published = __atomic_load_n(q->published(), __ATOMIC_ACQUIRE);
+    return PEL_OK;
+}
+
+int pel_serve_publish(pel_serve_queue *q, const pel_serve_unit *units, int32_t n, int64_t *first) {
+    if (!q || (!units && n > 0)) return fail(PEL_EINVAL, "pel_serve_publish: NULL argument");
+    if (n < 0) return fail(PEL_EINVAL, "pel_serve_publish: negative count");
+    for (int32_t i = 0; i < n; ++i) {
+        const pel_serve_unit &u = units[i];
+        if (u.job < 0 || u.count < 1 || u.count > PEL_SERVE_WAVE_SHOTS || !u.syndromes || !u.errors || !u.weights ||
+            !u.iterations || !u.found_count)
+            return fail(PEL_EINVAL, "pel_serve_publish: a unit has a bad job, count or NULL buffer");
+    }
+    std::lock_guard<std::mutex> lock(q->mu);
+    const int64_t k0 = q->next;
+    if (n > q->capacity) return fail(PEL_EBUSY, "pel_serve_publish: more units than the queue holds");
+    for (int32_t i = 0; i < n; ++i) {  // entry k % capacity is free once unit k - capacity is done
+        const int64_t k = k0 + i;
+        if (k >= q->capacity && __atomic_load_n(q->done() + k % q->capacity, __ATOMIC_ACQUIRE) < k - q->capacity + 1)
+            return fail(PEL_EBUSY, "pel_serve_publish: the queue is full");
+    }
+    for (int32_t i = 0; i < n; ++i) q->units()[(k0 + i) % q->capacity] = units[i];
+    __atomic_store_n(q->published(), k0 + n, __ATOMIC_RELEASE);
+    std::atomic_thread_fence(std::memory_order_seq_cst);  // drains the store buffers: the device sees it now
+    q->next = k0 + n;
+    if (first) *first = k0;
+    return PEL_OK;
+}
+
+int pel_serve_set_stop(pel_serve_queue *q, int32_t stop) {
+    if (!q) return fail(PEL_EINVAL, "pel_serve_set_stop: NULL queue");
+    __atomic_store_n(q->stop(), stop ? 1 : 0, __ATOMIC_RELEASE);
+    std::atomic_thread_fence(std::memory_order_seq_cst);
+    return PEL_OK;
+}
+
+int64_t pel_serve_done_count(const pel_serve_queue *q, int64_t first, int64_t n) {
+    if (!q || first < 0 || n < 0) return fail(PEL_EINVAL, "pel_serve_done_count: bad argument");
+    int64_t done = 0;
+    for (int64_t k = first; k < first + n; ++k)  // a later unit in the entry implies this one was done
+        done += __atomic_load_n(q->done() + k % q->capacity, __ATOMIC_ACQUIRE) >= k + 1;
+    return done;
 }
 
 }  // extern "C"

@@ -1,18 +1,26 @@
-"""Continuous-batching decode server on one GPU.
+"""Continuous-batching decode server on one GPU, with shot-level admission.
 
 Requests (syndrome batches of one of several codes) arrive over time.  The
-server keeps every code's decoder, graph layout and launch plans resident on
-the device, and a small pool of *slots*: a stream, a workspace and output
-buffers sized for the largest batch.  Whenever a slot is free and the HBM
-budget allows, all requests waiting for one code (oldest code first) are
-merged into one persistent launch on that slot's stream.  Launches on
-different slots overlap on the device: a launch's persistent workgroups free
-their CUs as its shots finish, and the next launch's workgroups take them.
-Each finished launch's per-request results are copied out on the slot's
-stream and the slot is recycled.  Latency is measured per request from its
-arrival to the moment the host sees its results complete.
+server keeps every code's decoder, graph layout, gamma rows and a serving
+engine (``pel_plan_serve`` / ``pel_serve_launch``) resident on the device: one
+persistent launch whose workgroups take units of work from a queue in host
+memory while it runs.  ``submit`` turns a request into units at once (one shot
+each for a code decoded a shot per workgroup; up to 64 shots for a small code
+decoded a shot per wave) and publishes them, so a request's shots start on
+whichever workgroups free up next, alongside the shots of earlier requests
+still running; a slow shot holds one workgroup, not the requests behind it.
+The engine writes each shot's results straight into the request's own output
+tensors and raises a done flag per unit in host memory; ``poll`` marks a
+request finished when all of its units are done.  The engine's workgroups
+leave after ``idle_ms`` without work, and the server launches the engine again
+when work arrives and the last launch has ended.
 
-Each shot's result does not depend on the other shots of its launch (the
+Backpressure: at most ``max_queued_shots`` shots may be submitted and not yet
+finished; a request that would exceed that is refused with ServerBusy at once,
+so a saturated server answers immediately instead of letting its queue (and
+every later request's latency) grow without bound.
+
+Each shot's result does not depend on the other shots decoded with it (the
 decoder's results are batch-independent), so a request decodes the same alone
 and in the mix; tests/test_serving.py checks that bit for bit.
 
@@ -24,14 +32,22 @@ reassembly (process.py:114-254) and its per-batch-size state cache
 from __future__ import annotations
 
 import collections
+import ctypes
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import Dict, List, Optional
 
 import numpy as np
 import torch
 
+from . import _native
 from .decoder import RelayBPDecoder
+
+# pel_serve_unit, as numpy builds a request's units
+_UNIT = np.dtype([("job", "<i4"), ("count", "<i4"), ("syn_row_stride", "<i8"), ("syndromes", "<u8"),
+                  ("errors", "<u8"), ("weights", "<u8"), ("iterations", "<u8"), ("found_count", "<u8"),
+                  ("syn_col_stride", "<i8")])
+assert _UNIT.itemsize == ctypes.sizeof(_native.PelServeUnit) == 64
 
 
 @dataclass
@@ -45,162 +61,311 @@ class Request:
     weights: Optional[torch.Tensor] = None
     iterations: Optional[torch.Tensor] = None
     found_count: Optional[torch.Tensor] = None
-
-
-@dataclass
-class _Slot:
-    stream: torch.cuda.Stream
-    event: torch.cuda.Event
-    workspace: torch.Tensor
-    errors: torch.Tensor
-    weights: torch.Tensor
-    iterations: torch.Tensor
-    found: torch.Tensor
-    busy: bool = False
-    reqs: List[Request] = field(default_factory=list)
-    footprint: int = 0
+    first_unit: int = 0
+    n_units: int = 0
 
 
 class ServerBusy(RuntimeError):
-    """The server's queue holds its maximum number of shots: the request was
+    """The server holds its maximum number of unfinished shots (or a code's
+    result arena has no room next to an unfinished request): the request was
     not admitted (the caller may retry later)."""
 
 
-class DecodeServer:
-    """decoders: code name -> RelayBPDecoder (one device).  max_batch: shots per
-    launch.  slots: concurrent launches (at most the device's hardware queues
-    are useful).  hbm_budget: bytes of device memory in-flight work may hold
-    (workspace, outputs and per-request result copies).  max_queued_shots:
-    shots the queue may hold while they wait for a launch; a request that
-    would exceed it is refused with ServerBusy, so a saturated server answers
-    at once instead of letting its queue (and every later request's latency)
-    grow without bound."""
+class _Arena:
+    """One code's result buffers, allocated once and handed out as contiguous
+    row ranges in ring order: a range is taken at the head, or at row 0 when it
+    does not fit before the end, and never over a request still in flight (the
+    ranges from the oldest unfinished one to the head).  A finished request's
+    rows are reused once the ring comes round to them again."""
 
-    def __init__(self, decoders: Dict[str, RelayBPDecoder], max_batch: int = 2048, slots: int = 4,
-                 hbm_budget: int = 64 << 30, max_queued_shots: int = 16384):
+    def __init__(self, rows: int, V: int, device):
+        self.rows = rows
+        self.errors = torch.empty(rows, V, dtype=torch.uint8, device=device)
+        self.weights = torch.empty(rows, dtype=torch.float32, device=device)
+        self.iterations = torch.empty(rows, dtype=torch.int32, device=device)
+        self.found_count = torch.empty(rows, dtype=torch.int32, device=device)
+        self.head = 0
+        self.live: collections.deque = collections.deque()  # (start, request) in allocation order
+
+    def take(self, b: int, r: "Request") -> Optional[int]:
+        while self.live and self.live[0][1].done >= 0:
+            self.live.popleft()
+        n, h = self.rows, self.head
+        if not self.live:
+            start = h if h + b <= n else 0
+        else:
+            s0 = self.live[0][0]
+            if h > s0:    # in flight: [s0, h); free: [h, n) and [0, s0)
+                start = h if h + b <= n else (0 if b <= s0 else None)
+            elif h < s0:  # in flight: [s0, n) and [0, h); free: [h, s0)
+                start = h if h + b <= s0 else None
+            else:         # the ring is full
+                start = None
+        if start is None or b > n:
+            return None
+        self._undo = self.head
+        self.head = start + b
+        self.live.append((start, r))
+        return start
+
+    def undo(self) -> None:
+        """Return the range the last take handed out (its request was not admitted)."""
+        self.live.pop()
+        self.head = self._undo
+
+
+class DecodeServer:
+    """decoders: code name -> RelayBPDecoder (one device, broadcast gamma draws,
+    shared priors).  max_batch: largest request (shots).  max_queued_shots:
+    shots submitted and not yet finished.  idle_ms: how long the engine's
+    workgroups wait for work before the launch ends.  queue_units: entries of
+    the unit ring; an entry is reused once its unit is done, so a slow shot
+    refuses new work only after queue_units later units (default 2^18: seconds
+    of work at full rate).
+
+    Syndromes are read in place, with their strides: device-resident uint8
+    (or bool) tensors, complete (whatever produced them finished) when
+    submitted; anything else is converted and the current stream synchronised
+    first.  Nothing is allocated per request: a request's
+    result tensors are views of its code's result arena (``result_shots`` rows,
+    default 4 x max_queued_shots), complete once ``poll`` has marked the request
+    done and valid until about ``result_shots`` more shots of that code have
+    been submitted (clone them to keep them longer)."""
+
+    def __init__(self, decoders: Dict[str, RelayBPDecoder], max_batch: int = 2048,
+                 max_queued_shots: int = 16384, idle_ms: float = 2.0, result_shots: Optional[int] = None,
+                 queue_units: int = 1 << 18):
         if not decoders:
             raise ValueError("no decoders")
+        if len(decoders) > _native.PEL_MAX_JOBS:
+            raise ValueError(f"at most {_native.PEL_MAX_JOBS} codes")
         self.decoders = decoders
+        self.names = list(decoders)
         self.device = next(iter(decoders.values())).device
-        self.max_batch = int(max_batch)
-        self.hbm_budget = int(hbm_budget)
-        self.max_queued_shots = int(max_queued_shots)
-        if self.max_queued_shots < self.max_batch:
-            raise ValueError("max_queued_shots must hold at least one full batch")
-        self.queued_shots = 0
-        vmax = max(d.V for d in decoders.values())
-        ws_need = {}
+        if any(d.device != self.device for d in decoders.values()):
+            raise ValueError("all decoders must be on one device")
         for name, d in decoders.items():
-            # plans for every batch size a launch may have are made lazily; size the
-            # workspace for the largest
-            ws_need[name] = d.prepare(self.max_batch)
-        ws_bytes = max(ws_need.values())
-        self.slots: List[_Slot] = []
-        for _ in range(int(slots)):
-            dev = self.device
-            self.slots.append(_Slot(
-                stream=torch.cuda.Stream(device=dev),
-                event=torch.cuda.Event(),
-                workspace=torch.empty(ws_bytes, dtype=torch.uint8, device=dev),
-                errors=torch.empty(self.max_batch * vmax, dtype=torch.uint8, device=dev),
-                weights=torch.empty(self.max_batch, dtype=torch.float32, device=dev),
-                iterations=torch.empty(self.max_batch, dtype=torch.int32, device=dev),
-                found=torch.empty(self.max_batch, dtype=torch.int32, device=dev),
-            ))
-        self.slot_bytes = ws_bytes + self.max_batch * (vmax + 12)
-        self.queues: Dict[str, collections.deque] = {name: collections.deque() for name in decoders}
-        self.in_flight_bytes = 0
+            if d.gamma_table(1).dim() != 2:
+                raise ValueError(f"{name}: serving needs broadcast gamma draws (gamma_draw != 'per_row')")
+        self.max_batch = int(max_batch)
+        self.max_queued_shots = int(max_queued_shots)
+        if self.max_queued_shots < self.max_batch or self.max_batch < 1:
+            raise ValueError("max_queued_shots must hold at least one full request of max_batch >= 1 shots")
+        if queue_units < self.max_queued_shots:
+            raise ValueError("queue_units must be at least max_queued_shots")
+        self.idle_us = int(round(float(idle_ms) * 1000))
+        n = len(self.names)
+        L = _native.lib()
+        self._graphs = (ctypes.c_void_p * n)(*[decoders[k]._handle.h.value for k in self.names])
+        flags = (ctypes.c_int32 * n)(*[
+            _native.PEL_PLAN_LEARNED if (decoders[k].learned_params is not None and decoders[k].learned_legs > 0)
+            else 0 for k in self.names])
+        self.plan = _native.PelJobsPlan()
+        dev = self.device
+        self._queue = ctypes.c_void_p()
+        with torch.cuda.device(dev):
+            _native.check(L.pel_plan_serve(self._graphs, flags, n, ctypes.byref(self.plan)), "pel_plan_serve")
+            self._staging, self._ws, params, batches = {}, {}, [], []
+            self.job = {}
+            for i, name in enumerate(self.names):
+                d = decoders[name]
+                rows = int(self.plan.grid) * (16 if int(self.plan.group[i]) == 64 else 1)
+                d.prepare(rows)
+                st = dict(syndromes=torch.zeros(rows, d.C, dtype=torch.uint8, device=dev),
+                          errors=torch.empty(rows, d.V, dtype=torch.uint8, device=dev),
+                          weights=torch.empty(rows, dtype=torch.float32, device=dev),
+                          iterations=torch.empty(rows, dtype=torch.int32, device=dev),
+                          found_count=torch.empty(rows, dtype=torch.int32, device=dev))
+                self._staging[name] = st
+                self._ws[name] = torch.empty(max(int(self.plan.workspace_bytes[i]), 1), dtype=torch.uint8, device=dev)
+                prm, bat = d.job_args(st["syndromes"], st["errors"], st["weights"], st["iterations"],
+                                      st["found_count"], self._ws[name])
+                bat.shot_clock = None
+                params.append(prm)
+                batches.append(bat)
+                self.job[name] = (i, 1 if int(self.plan.group[i]) != 64 else _native.PEL_SERVE_WAVE_SHOTS)
+            self._params = (_native.PelParams * n)(*params)
+            self._batches = (_native.PelBatch * n)(*batches)
+            self._table = torch.empty(int(self.plan.table_bytes), dtype=torch.uint8, device=dev)
+            _native.check(L.pel_serve_queue_create(int(queue_units), ctypes.byref(self._queue)),
+                          "pel_serve_queue_create")
+            cap = ctypes.c_int32()
+            _native.check(L.pel_serve_queue_info(self._queue, None, ctypes.byref(cap), None),
+                          "pel_serve_queue_info")
+            self._cap = int(cap.value)
+            self._lib = L
+            rs = int(result_shots) if result_shots is not None else 4 * self.max_queued_shots
+            if rs < self.max_batch:
+                raise ValueError("result_shots must hold at least one full request")
+            self._arena = {k: _Arena(rs, decoders[k].V, dev) for k in self.names}
+            self.stream = torch.cuda.Stream(device=dev)
+        self._ended: Optional[torch.cuda.Event] = None
+        self.launches = 0
+        self.queued_shots = 0
+        self.outstanding: collections.deque = collections.deque()
         self.t0 = time.perf_counter()
         self.finished: List[Request] = []
         self._next_id = 0
+
+    def roles(self) -> Dict[str, dict]:
+        """Each code's role in the engine: threads per shot, placement, learned."""
+        p = self.plan
+        return {n: dict(group=int(p.group[i]), placement=int(p.placement[i]), learned=bool(p.learned[i]))
+                for i, n in enumerate(self.names)}
 
     # -- client side -------------------------------------------------------------------------------
     def now(self) -> float:
         return time.perf_counter() - self.t0
 
     def submit(self, code: str, syndromes: torch.Tensor, arrival: Optional[float] = None) -> Request:
+        if self._queue is None:
+            raise RuntimeError("the server is closed")
         if code not in self.decoders:
             raise KeyError(code)
         dec = self.decoders[code]
         if syndromes.dim() != 2 or syndromes.shape[1] != dec.C:
             raise ValueError(f"{code}: syndromes must be [b, {dec.C}]")
-        if syndromes.shape[0] > self.max_batch:
-            raise ValueError(f"request of {syndromes.shape[0]} shots exceeds max_batch={self.max_batch}")
         b = int(syndromes.shape[0])
+        if b > self.max_batch:
+            raise ValueError(f"request of {b} shots exceeds max_batch={self.max_batch}")
         if self.queued_shots + b > self.max_queued_shots:
-            raise ServerBusy(f"{self.queued_shots} shots queued; a request of {b} exceeds "
+            raise ServerBusy(f"{self.queued_shots} shots unfinished; a request of {b} exceeds "
                              f"max_queued_shots={self.max_queued_shots}")
-        r = Request(self._next_id, code, syndromes.to(self.device, torch.uint8).contiguous(),
-                    self.now() if arrival is None else arrival)
+        if syndromes.device == self.device and syndromes.dtype in (torch.uint8, torch.bool):
+            # read in place, strided as it is: no work is put on a stream (a copy kernel would wait
+            # for the engine's workgroups to leave)
+            syn = syndromes.view(torch.uint8) if syndromes.dtype == torch.bool else syndromes
+        else:
+            syn = syndromes.to(self.device, torch.uint8)
+            torch.cuda.current_stream(self.device).synchronize()  # the converted copy is complete
+        r = Request(self._next_id, code, syn, self.now() if arrival is None else arrival)
+        ar = self._arena[code]
+        o = ar.take(b, r) if b > 0 else 0
+        if o is None:
+            raise ServerBusy(f"{code}: no room in the result arena next to unfinished requests")
+        r.errors, r.weights = ar.errors[o:o + b], ar.weights[o:o + b]
+        r.iterations, r.found_count = ar.iterations[o:o + b], ar.found_count[o:o + b]
         self._next_id += 1
-        self.queues[code].append(r)
+        if b == 0:
+            r.done = r.arrival
+            self.finished.append(r)
+            return r
+        job, per_unit = self.job[code]
+        starts = np.arange(0, b, per_unit, dtype=np.int64)
+        u = np.zeros(starts.size, dtype=_UNIT)
+        u["job"] = job
+        u["count"] = np.minimum(per_unit, b - starts)
+        rs, cs = syn.stride()
+        u["syn_row_stride"] = rs
+        u["syn_col_stride"] = cs
+        u["syndromes"] = syn.data_ptr() + starts * rs
+        u["errors"] = r.errors.data_ptr() + starts * dec.V
+        u["weights"] = r.weights.data_ptr() + 4 * starts
+        u["iterations"] = r.iterations.data_ptr() + 4 * starts
+        u["found_count"] = r.found_count.data_ptr() + 4 * starts
+        first = ctypes.c_int64()
+        rc = _native.lib().pel_serve_publish(self._queue, u.ctypes.data_as(ctypes.POINTER(_native.PelServeUnit)),
+                                             int(starts.size), ctypes.byref(first))
+        if rc != _native.PEL_OK:
+            ar.undo()
+            if rc == _native.PEL_EBUSY:
+                raise ServerBusy(_native.lib().pel_last_error().decode(errors="replace"))
+            _native.check(rc, "pel_serve_publish")
+        r.first_unit, r.n_units = int(first.value), int(starts.size)
+        self.outstanding.append(r)
         self.queued_shots += b
+        self._ensure_running()
         return r
 
     # -- server side -------------------------------------------------------------------------------
-    def _launch(self, slot: _Slot, code: str) -> None:
-        dec = self.decoders[code]
-        q = self.queues[code]
-        reqs, shots = [], 0
-        while q and shots + q[0].syndromes.shape[0] <= self.max_batch:
-            r = q.popleft()
-            reqs.append(r)
-            shots += r.syndromes.shape[0]
-        self.queued_shots -= shots
-        # footprint: workspace and buffers of the slot plus the result copies
-        foot = self.slot_bytes + shots * (dec.V + 12)
-        slot.busy, slot.reqs, slot.footprint = True, reqs, foot
-        self.in_flight_bytes += foot
-        # the slot's stream must see the requests' syndromes (written on the submitting stream)
-        slot.stream.wait_stream(torch.cuda.current_stream(self.device))
-        with torch.cuda.stream(slot.stream):
-            syn = reqs[0].syndromes if len(reqs) == 1 else torch.cat([r.syndromes for r in reqs], 0)
-            B = syn.shape[0]
-            err = slot.errors[:B * dec.V].view(B, dec.V)  # contiguous [B, V]
-            w, it, fc = slot.weights[:B], slot.iterations[:B], slot.found[:B]
-            dec.decode_async(syn, err, w, it, fc, slot.workspace, slot.stream)
-            o = 0
-            for r in reqs:
-                b = r.syndromes.shape[0]
-                r.errors = err[o:o + b].clone()
-                r.weights = w[o:o + b].clone()
-                r.iterations = it[o:o + b].clone()
-                r.found_count = fc[o:o + b].clone()
-                o += b
-            slot.event.record(slot.stream)
+    def _ensure_running(self) -> None:
+        """Launch the engine if no launch is running (the last one may have left
+        while units were published)."""
+        if self._ended is not None and not self._ended.query():
+            return
+        with torch.cuda.device(self.device):
+            _native.check(_native.lib().pel_serve_launch(
+                self._graphs, self._params, self._batches, len(self.names), ctypes.byref(self.plan),
+                ctypes.c_void_p(self._table.data_ptr()), self._queue, self.idle_us,
+                ctypes.c_void_p(self.stream.cuda_stream)), "pel_serve_launch")
+            ev = torch.cuda.Event()
+            ev.record(self.stream)
+        self._ended = ev
+        self.launches += 1
+
+    def _is_done(self, r: Request) -> bool:
+        return int(self._lib.pel_serve_done_count(self._queue, r.first_unit, r.n_units)) == r.n_units
 
     def poll(self) -> int:
-        """Retire finished launches and start new ones; returns requests finished."""
-        n = 0
+        """Mark finished requests (and relaunch the engine if it left with work
+        outstanding); returns the number of requests finished."""
+        if not self.outstanding:
+            return 0
         t = self.now()
-        for s in self.slots:
-            if s.busy and s.event.query():
-                for r in s.reqs:
-                    r.done = t
-                    self.finished.append(r)
-                n += len(s.reqs)
-                self.in_flight_bytes -= s.footprint
-                s.busy, s.reqs = False, []
-        for s in self.slots:
-            if s.busy:
-                continue
-            waiting = [(q[0].arrival, name) for name, q in self.queues.items() if q]
-            if not waiting:
-                break
-            code = min(waiting)[1]
-            dec = self.decoders[code]
-            shots = min(self.max_batch, sum(r.syndromes.shape[0] for r in self.queues[code]))
-            if self.in_flight_bytes + self.slot_bytes + shots * (dec.V + 12) > self.hbm_budget:
-                break
-            self._launch(s, code)
+        keep: collections.deque = collections.deque()
+        n = 0
+        for r in self.outstanding:
+            if self._is_done(r):
+                r.done = t
+                self.finished.append(r)
+                self.queued_shots -= int(r.syndromes.shape[0])
+                n += 1
+            else:
+                keep.append(r)
+        self.outstanding = keep
+        if self.outstanding:
+            self._ensure_running()
         return n
 
     def pending(self) -> int:
-        return sum(len(q) for q in self.queues.values()) + sum(len(s.reqs) for s in self.slots if s.busy)
+        return len(self.outstanding)
 
-    def drain(self) -> None:
+    def drain(self, timeout_s: float = 600.0) -> None:
+        t_end = time.perf_counter() + timeout_s
         while self.pending():
             self.poll()
+            if time.perf_counter() > t_end:
+                raise TimeoutError(f"{self.pending()} requests still unfinished after {timeout_s} s")
+
+    def drain_requests(self, reqs: List[Request], timeout_s: float = 600.0) -> None:
+        """Poll until every request in ``reqs`` is finished."""
+        t_end = time.perf_counter() + timeout_s
+        while any(r.done < 0 for r in reqs):
+            self.poll()
+            if time.perf_counter() > t_end:
+                raise TimeoutError(f"requests still unfinished after {timeout_s} s")
+
+    def quiesce(self) -> None:
+        """End the running launch (if any) and wait for it: its workgroups
+        finish every unit published so far, then leave instead of waiting for
+        more.  The next submit launches the engine again."""
+        if self._ended is None:
+            return
+        L = _native.lib()
+        _native.check(L.pel_serve_set_stop(self._queue, 1), "pel_serve_set_stop")
+        try:
+            self._ended.synchronize()
+        finally:
+            _native.check(L.pel_serve_set_stop(self._queue, 0), "pel_serve_set_stop")
+
+    def close(self) -> None:
+        if self._queue is None:
+            return
+        self.quiesce()
+        _native.check(_native.lib().pel_serve_queue_destroy(self._queue), "pel_serve_queue_destroy")
+        self._queue = None
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self) -> "DecodeServer":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
 
 
 def poisson_trace(rate_shots_per_s: float, duration_s: float, codes: List[str], seed: int,

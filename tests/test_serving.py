@@ -80,6 +80,43 @@ def test_admission_policy_accounts_for_every_request(policy):
         assert [r.arrival for r in srv.finished] == sorted(r.arrival for r in srv.finished)
 
 
+def test_result_arena_never_hands_out_rows_of_unfinished_requests():
+    """The result arena's ring allocation, driven by random request sizes and
+    completions: a range it hands out never overlaps a request still in
+    flight, and it refuses (None) only when no such range exists."""
+    import collections
+    import types
+
+    from pelagos_amd.serving import _Arena
+
+    rng = np.random.default_rng(0)
+    a = _Arena.__new__(_Arena)
+    a.rows, a.head, a.live = 64, 0, collections.deque()
+    inflight = []
+    refused = 0
+    for _ in range(5000):
+        if inflight and rng.random() < 0.45:
+            r = inflight.pop(int(rng.integers(len(inflight))))
+            r.done = 1.0
+            continue
+        b = int(rng.integers(1, 20))
+        r = types.SimpleNamespace(done=-1.0)
+        o = a.take(b, r)
+        busy = np.zeros(a.rows, bool)
+        for q in inflight:
+            busy[q.o:q.o + q.b] = True
+        if o is None:
+            refused += 1
+            continue
+        assert 0 <= o and o + b <= a.rows and not busy[o:o + b].any()
+        if rng.random() < 0.1:          # a request refused after its rows were taken gives them back
+            a.undo()
+            continue
+        r.o, r.b = o, b
+        inflight.append(r)
+    assert refused > 0
+
+
 def _decoders():
     from pelagos_amd import RelayBPDecoder
 
@@ -99,31 +136,158 @@ def test_requests_decode_the_same_alone_and_in_the_mix(mode):
     from pelagos_amd.serving import DecodeServer, run_trace
 
     dd = _decoders()
-    server = DecodeServer({k: v[0] for k, v in dd.items()}, max_batch=512, slots=3)
     rng = np.random.default_rng(1)
     pools = {}
     for k, (_, H, p) in dd.items():
         err = codes.sample_errors(p, 1024, seed=int(rng.integers(1 << 30)), scale=1.5)
         pools[k] = torch.from_numpy(codes.syndromes_of(H, err)).cuda()
-    print(f"[gpu] serving {mode}", flush=True)
-    if mode == "burst":
-        reqs = []
-        for _ in range(24):
-            k = ["bb72", "banded4"][int(rng.integers(2))]
-            b = int(rng.integers(1, 97))
-            o = int(rng.integers(0, 1024 - b))
-            reqs.append(server.submit(k, pools[k][o:o + b]))
-        server.drain()
-    else:
-        trace = poisson_trace(20_000, 0.3, ["bb72", "banded4"], seed=2, max_shots=64)
-        stats = run_trace(server, trace, pools)
-        assert stats["requests"] == len(trace) and stats["p99_ms"] >= stats["p50_ms"] > 0
-        reqs = server.finished
     torch.cuda.synchronize()
-    assert reqs and all(r.done >= r.arrival for r in reqs)
-    for r in reqs:
-        dec = dd[r.code][0]
-        alone = dec.decode(r.syndromes)
-        for k, got in (("errors", r.errors), ("weights", r.weights), ("iterations", r.iterations),
-                       ("found_count", r.found_count)):
-            assert torch.equal(got, alone[k]), (r.rid, r.code, k)
+    print(f"[gpu] serving {mode}", flush=True)
+    with DecodeServer({k: v[0] for k, v in dd.items()}, max_batch=512) as server:
+        assert server.roles()["bb72"]["group"] == 64 and server.roles()["banded4"]["group"] == 1024
+        if mode == "burst":
+            reqs = []
+            for _ in range(24):
+                k = ["bb72", "banded4"][int(rng.integers(2))]
+                b = int(rng.integers(1, 97))
+                o = int(rng.integers(0, 1024 - b))
+                reqs.append(server.submit(k, pools[k][o:o + b]))
+            server.drain()
+        else:
+            trace = poisson_trace(20_000, 0.3, ["bb72", "banded4"], seed=2, max_shots=64)
+            stats = run_trace(server, trace, pools)
+            assert stats["requests"] == len(trace) and stats["p99_ms"] >= stats["p50_ms"] > 0
+            reqs = server.finished
+        server.quiesce()
+        assert reqs and all(r.done >= r.arrival for r in reqs)
+        for r in reqs:
+            dec = dd[r.code][0]
+            alone = dec.decode(r.syndromes)
+            for k, got in (("errors", r.errors), ("weights", r.weights), ("iterations", r.iterations),
+                           ("found_count", r.found_count)):
+                assert torch.equal(got, alone[k]), (r.rid, r.code, k)
+
+
+@pytest.mark.gpu
+def test_requests_join_a_running_launch_and_pass_a_slow_shot():
+    """Shots that never converge (the full schedule, hundreds of ms on one
+    workgroup) are submitted first; small requests submitted after them join
+    the same launch and finish while the slow shots still run.  Every result
+    equals the request decoded alone."""
+    from pelagos_amd import RelayBPDecoder
+    from pelagos_amd.serving import DecodeServer
+
+    relay = dict(pre_iter=80, num_sets=100, set_max_iter=60, gamma0=0.65, gamma_dist_interval=(-0.24, 0.66),
+                 stop_nconv=5, data_type="float32", algo="relay", seed=0, device=[0])
+    Hs, ps = codes.banded_circuit_code(seed=0)                 # 936 x 8785
+    Hb, pb = codes.bb_72_12_6(), np.full(72, 0.03)
+    decs = {"slow": RelayBPDecoder(Hs, ps, **relay), "bb72": RelayBPDecoder(Hb, pb, **relay)}
+    # heavy syndromes: about a third of the checks violated, far beyond what the code corrects
+    slow_syn = torch.from_numpy((np.random.default_rng(4).random((4, Hs.shape[0])) < 0.35).astype(np.uint8)).cuda()
+    pool = torch.from_numpy(codes.syndromes_of(Hb, codes.sample_errors(pb, 512, seed=5))).cuda()
+    torch.cuda.synchronize()
+    print("[gpu] serving: small requests past slow shots", flush=True)
+    with DecodeServer(decs, max_batch=512, idle_ms=5.0) as server:
+        slow = server.submit("slow", slow_syn)
+        small = [server.submit("bb72", pool[32 * i:32 * i + 32]) for i in range(16)]
+        server.drain_requests(small)
+        passed = slow.done < 0
+        launches = server.launches
+        server.drain()
+        server.quiesce()
+        full = 80 + 100 * 60
+        print(f"[gpu] slow shots: iterations {slow.iterations.tolist()}, launches {launches}", flush=True)
+        assert int(slow.iterations.max()) == full   # the premise: a shot that runs the whole schedule
+        assert passed, "the slow shots finished before the small requests"
+        assert launches == 1, launches
+        for r in [slow] + small:
+            alone = decs[r.code].decode(r.syndromes)
+            for k, got in (("errors", r.errors), ("weights", r.weights), ("iterations", r.iterations),
+                           ("found_count", r.found_count)):
+                assert torch.equal(got, alone[k]), (r.rid, r.code, k)
+
+
+@pytest.mark.gpu
+def test_a_request_completes_after_its_ring_entries_are_reused():
+    """A request whose first unit finishes at once and whose second runs the
+    whole schedule, while more than the ring's capacity of later units pass
+    through (the first unit's entry is reused meanwhile): the request still
+    completes, with the results of decoding it alone."""
+    from pelagos_amd import RelayBPDecoder
+    from pelagos_amd.serving import DecodeServer, ServerBusy
+
+    relay = dict(pre_iter=80, num_sets=100, set_max_iter=60, gamma0=0.65, gamma_dist_interval=(-0.24, 0.66),
+                 stop_nconv=5, data_type="float32", algo="relay", seed=0, device=[0])
+    Hs, ps = codes.banded_circuit_code(seed=0)
+    Hb, pb = codes.bb_72_12_6(), np.full(72, 0.03)
+    decs = {"slow": RelayBPDecoder(Hs, ps, **relay), "bb72": RelayBPDecoder(Hb, pb, **relay)}
+    heavy = (np.random.default_rng(4).random(Hs.shape[0]) < 0.35).astype(np.uint8)
+    pair = torch.from_numpy(np.stack([np.zeros_like(heavy), heavy])).cuda()
+    pool = torch.from_numpy(codes.syndromes_of(Hb, codes.sample_errors(pb, 256, seed=8))).cuda()
+    torch.cuda.synchronize()
+    print("[gpu] serving: ring entries reused under a pending request", flush=True)
+    with DecodeServer(decs, max_batch=64, max_queued_shots=64, queue_units=64, idle_ms=5.0) as server:
+        r = server.submit("slow", pair)
+        small, i = [], 0
+        while i < 100:
+            try:
+                small.append(server.submit("bb72", pool[i:i + 1]))
+                i += 1
+            except ServerBusy:
+                server.poll()
+        server.drain(timeout_s=60.0)
+        server.quiesce()
+        assert small[-1].first_unit >= 64 + 2
+        for q in [r] + small:
+            alone = decs[q.code].decode(q.syndromes)
+            for k, got in (("errors", q.errors), ("weights", q.weights), ("iterations", q.iterations),
+                           ("found_count", q.found_count)):
+                assert torch.equal(got, alone[k]), (q.rid, q.code, k)
+
+
+@pytest.mark.gpu
+def test_backpressure_and_a_full_queue():
+    """Requests beyond max_queued_shots are refused with ServerBusy and admitted
+    again once earlier ones finish; the C queue refuses (PEL_EBUSY) a unit whose
+    entry still holds an unfinished one."""
+    import ctypes
+
+    from pelagos_amd import _native
+    from pelagos_amd.serving import DecodeServer, ServerBusy
+
+    dd = _decoders()
+    H, p = dd["bb72"][1], dd["bb72"][2]
+    pool = torch.from_numpy(codes.syndromes_of(H, codes.sample_errors(p, 256, seed=6))).cuda()
+    torch.cuda.synchronize()
+    print("[gpu] serving backpressure", flush=True)
+    with DecodeServer({"bb72": dd["bb72"][0]}, max_batch=64, max_queued_shots=128) as server:
+        a = server.submit("bb72", pool[:64])
+        b = server.submit("bb72", pool[64:128])
+        with pytest.raises(ServerBusy):
+            server.submit("bb72", pool[128:129])
+        server.drain()
+        c = server.submit("bb72", pool[128:192])
+        server.drain()
+        server.quiesce()
+        alone = dd["bb72"][0].decode(pool[:192])
+        got = torch.cat([r.errors for r in (a, b, c)])
+        assert torch.equal(got, alone["errors"])
+
+    L = _native.lib()
+    q = ctypes.c_void_p()
+    _native.check(L.pel_serve_queue_create(2, ctypes.byref(q)), "create")
+    try:
+        buf = torch.zeros(4096, dtype=torch.uint8, device="cuda")
+        u = _native.PelServeUnit(job=0, count=1, syn_row_stride=36, syn_col_stride=1, syndromes=buf.data_ptr(),
+                                 errors=buf.data_ptr(),
+                                 weights=buf.data_ptr(), iterations=buf.data_ptr(), found_count=buf.data_ptr())
+        units = (_native.PelServeUnit * 2)(u, u)
+        first = ctypes.c_int64()
+        assert L.pel_serve_publish(q, units, 2, ctypes.byref(first)) == 0 and first.value == 0
+        assert L.pel_serve_publish(q, units, 1, ctypes.byref(first)) == _native.PEL_EBUSY
+        bad = (_native.PelServeUnit * 1)(_native.PelServeUnit(job=0, count=65, syndromes=buf.data_ptr()))
+        assert L.pel_serve_publish(q, bad, 1, None) == _native.PEL_EINVAL
+        pub = ctypes.c_int64()
+        assert L.pel_serve_queue_info(q, None, None, ctypes.byref(pub)) == 0 and pub.value == 2
+    finally:
+        L.pel_serve_queue_destroy(q)

@@ -143,17 +143,19 @@ def test_server_over_three_codes_with_learned_form():
     decs = {k: _decoder(H, p, **(_learned_kw() if k == "banded936" else {})) for k, (H, p) in cs.items()}
     pools = {k: codes.syndromes_of(H, codes.sample_errors(p, 512, seed=7 + i)) for i, (k, (H, p)) in
              enumerate(cs.items())}
-    server = DecodeServer(decs, max_batch=512, slots=3, max_queued_shots=1 << 16)
+    subs = []
     rng = np.random.default_rng(3)
-    reqs = []
-    print("[gpu] three-code server burst", flush=True)
     for i in range(18):
         k = codes.THREE_CODES[i % 3]
         b = int(rng.integers(1, 97))
         o = int(rng.integers(0, 512 - b))
-        reqs.append(server.submit(k, torch.from_numpy(pools[k][o:o + b]).cuda()))
-    server.drain()
+        subs.append((k, torch.from_numpy(pools[k][o:o + b]).cuda()))
     torch.cuda.synchronize()
+    print("[gpu] three-code server burst", flush=True)
+    with DecodeServer(decs, max_batch=512, max_queued_shots=1 << 16) as server:
+        reqs = [server.submit(k, syn) for k, syn in subs]
+        server.drain()
+        server.quiesce()
     for r in reqs:
         alone = decs[r.code].decode(r.syndromes)
         for k, got in (("errors", r.errors), ("weights", r.weights), ("iterations", r.iterations),
