@@ -574,7 +574,7 @@ def main() -> int:
     ap.add_argument("--no-bb72", action="store_true", help="skip the [[72,12,6]] code-capacity runs")
     ap.add_argument("--ler-batches", type=int, default=64,
                     help="4096-shot batches of the paired error-rate comparison")
-    ap.add_argument("--serving-seconds", type=float, default=10.0,
+    ap.add_argument("--serving-seconds", type=float, default=5.0,
                     help="length of each serving run (one per arrival rate)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (3 times), the known-size "
