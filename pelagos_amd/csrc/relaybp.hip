@@ -181,6 +181,19 @@ template <typename T>
 __device__ __forceinline__ __attribute__((address_space(3))) T *lds_at(uint32_t addr) {
     return reinterpret_cast<__attribute__((address_space(3))) T *>(addr);
 }
+
+// Device memory through global-address-space pointers.  A load through a
+// generic pointer compiles to a flat load, which counts against the LDS
+// counter as well as the memory counter, so every wait for an LDS result would
+// also wait for the index loads in flight; global loads let the two overlap.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gmem(const T *p) {
+    return (const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem_w(T *p) {
+    return (__attribute__((address_space(1))) T *)p;
+}
 typedef float vec2f __attribute__((ext_vector_type(2)));
 typedef float vec4f __attribute__((ext_vector_type(4)));
 
@@ -204,13 +217,25 @@ struct ShotMem {
     __device__ __forceinline__ __attribute__((address_space(3))) T *lds(uint32_t addr) const {
         return lds_at<T>(addr + lb);
     }
+    // An array the placement keeps in LDS (its bit set) or in the workspace slot.
+    template <int BIT, typename T>
+    __device__ __forceinline__ T ld(const uint8_t *p) const {
+        if (PLACE & (1 << BIT)) return *(const __attribute__((address_space(3))) T *)p;
+        return *gmem(reinterpret_cast<const T *>(p));
+    }
+    template <int BIT, typename T>
+    __device__ __forceinline__ void st(uint8_t *p, T x) const {
+        if (PLACE & (1 << BIT)) *(__attribute__((address_space(3))) T *)p = x;
+        else *gmem_w(reinterpret_cast<T *>(p)) = x;
+    }
 
     __device__ __forceinline__ float2 ls_ent(uint32_t ent) const {  // ent = lsb + 8 * position
         if (kFast) {
             const vec2f r = *lds<vec2f>(ent);
             return make_float2(r.x, r.y);
         }
-        return *reinterpret_cast<const float2 *>(ls + (ent - lsb));
+        const vec2f r = ld<kLS, vec2f>(ls + (ent - lsb));
+        return make_float2(r.x, r.y);
     }
     // Prior of the variable whose LS entry is ent (placement bit kL0, which
     // only the fast placement 15 has): LS and L0 are parallel arrays.
@@ -219,7 +244,7 @@ struct ShotMem {
     }
     __device__ __forceinline__ void ls_set(int v, float2 x) const {
         if (kFast) *lds<vec2f>(lsb + 8u * (uint32_t)v) = vec2f{x.x, x.y};
-        else *reinterpret_cast<float2 *>(ls + 8u * (uint32_t)v) = x;
+        else st<kLS, vec2f>(ls + 8u * (uint32_t)v, vec2f{x.x, x.y});
     }
     __device__ __forceinline__ float l0_pos(int v) const { return *lds<float>(l0b + 4u * (uint32_t)v); }
     __device__ __forceinline__ void l0_set(int v, float x) const { *lds<float>(l0b + 4u * (uint32_t)v) = x; }
@@ -228,37 +253,38 @@ struct ShotMem {
             const vec2f r = *lds<vec2f>(lsb + 8u * (uint32_t)v);
             return make_float2(r.x, r.y);
         }
-        return *reinterpret_cast<const float2 *>(ls + 8u * (uint32_t)v);
+        const vec2f r = ld<kLS, vec2f>(ls + 8u * (uint32_t)v);
+        return make_float2(r.x, r.y);
     }
     __device__ __forceinline__ void cs_set(int r, float4 x) const {
-        const uint32_t o = 4u * (uint32_t)r, st = 1u << cs_shift;
+        const uint32_t o = 4u * (uint32_t)r, pl = 1u << cs_shift;
         if (kFast) {
             *lds<float>(o) = x.x;
-            *lds<float>(o + st) = x.y;
-            *lds<float>(o + 2u * st) = x.z;
-            *lds<float>(o + 3u * st) = x.w;
+            *lds<float>(o + pl) = x.y;
+            *lds<float>(o + 2u * pl) = x.z;
+            *lds<float>(o + 3u * pl) = x.w;
         } else {
-            *reinterpret_cast<float *>(cs + o) = x.x;
-            *reinterpret_cast<float *>(cs + o + st) = x.y;
-            *reinterpret_cast<float *>(cs + o + 2u * st) = x.z;
-            *reinterpret_cast<float *>(cs + o + 3u * st) = x.w;
+            st<kCS, float>(cs + o, x.x);
+            st<kCS, float>(cs + o + pl, x.y);
+            st<kCS, float>(cs + o + 2u * pl, x.z);
+            st<kCS, float>(cs + o + 3u * pl, x.w);
         }
     }
     __device__ __forceinline__ float cs_msg(uint32_t e) const {  // e = 4 * row position + (code << cs_shift)
         if (kFast) return *lds<float>(e);
-        return *reinterpret_cast<const float *>(cs + e);
+        return ld<kCS, float>(cs + e);
     }
     __device__ __forceinline__ uint32_t sb_get(uint32_t w) const {
         if (kFast) return *lds<uint32_t>(sbb + 4u * w);
-        return *reinterpret_cast<const uint32_t *>(sb + 4u * w);
+        return ld<kSB, uint32_t>(sb + 4u * w);
     }
     __device__ __forceinline__ void sb_set(uint32_t w, uint32_t x) const {
         if (kFast) *lds<uint32_t>(sbb + 4u * w) = x;
-        else *reinterpret_cast<uint32_t *>(sb + 4u * w) = x;
+        else st<kSB, uint32_t>(sb + 4u * w, x);
     }
     __device__ __forceinline__ uint32_t sb_ent(uint32_t e) const {  // e = ((sbb + 4 word) << 5) | bit
         if (kFast) return *lds<uint32_t>(e >> 5);
-        return *reinterpret_cast<const uint32_t *>(sb + ((e >> 5) - sbb));
+        return ld<kSB, uint32_t>(sb + ((e >> 5) - sbb));
     }
 };
 
@@ -274,7 +300,7 @@ struct GammaSrc {
     float scale = 1.f;         // factor on the shifts (the last learned leg's damping)
     __device__ __forceinline__ float base(int v) const {
         if (!row) return konst;
-        return perm ? row[perm[v]] : row[v];
+        return perm ? gmem(row)[gmem(perm)[v]] : gmem(row)[v];
     }
     __device__ __forceinline__ float operator()(int v) const {
         const float u = base(v);
@@ -325,10 +351,15 @@ __device__ __forceinline__ unsigned opaque_tid() {
 // Element u of one ELL chunk: the chunk base is uniform (scalar registers),
 // the lane's byte offset is a loop-invariant 32-bit register and u's offset a
 // compile-time immediate, so the load needs no per-element address arithmetic.
-template <typename T>
-__device__ __forceinline__ T load_ell(const T *__restrict__ chunk, uint32_t lane_off, int u) {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(chunk) + (size_t)lane_off +
-                                        (size_t)(u * kSlice * (int)sizeof(T)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t load_ell(const uint32_t *__restrict__ chunk, uint32_t lane_off, int u) {
+    return *gmem(reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(chunk) + (size_t)lane_off +
+                                                    (size_t)(u * kSlice * 4)));
+}
+__device__ __forceinline__ uint2 load_ell(const uint2 *__restrict__ chunk, uint32_t lane_off, int u) {
+    const u32x2 x = *gmem(reinterpret_cast<const u32x2 *>(reinterpret_cast<const char *>(chunk) + (size_t)lane_off +
+                                                          (size_t)(u * kSlice * 8)));
+    return make_uint2(x.x, x.y);
 }
 
 // A shot runs on a group of BLOCK threads: the whole workgroup (BLOCK is the
@@ -411,7 +442,7 @@ __device__ __forceinline__ void row_info_init(const KArgs &a, const ShotMem<PLAC
     for (int pos = gtid<BLOCK>(); pos < a.g.C; pos += BLOCK) {
         const int deg = a.g.row_deg[pos];
         *m.template lds<uint8_t>(a.rowinfo + (uint32_t)pos) =
-            (uint8_t)((syn[a.g.row_orig[pos]] & 1u) | (deg > 0 ? 2u : 0u) | (deg == 1 ? 4u : 0u));
+            (uint8_t)((gmem(syn)[gmem(a.g.row_orig)[pos]] & 1u) | (deg > 0 ? 2u : 0u) | (deg == 1 ? 4u : 0u));
     }
 }
 
@@ -444,7 +475,7 @@ __device__ __forceinline__ uint32_t check_chunk(const KArgs &a, const ShotMem<PL
                 x[u] = m.l0_ent(e[u]);
             } else {
                 const int v = (int)((e[u] - m.lsb) >> 3);
-                x[u] = v < a.g.V ? m.l0g[a.g.vperm[v]] : INFINITY;
+                x[u] = v < a.g.V ? gmem(m.l0g)[gmem(a.g.vperm)[v]] : INFINITY;
             }
             y[u] = 0.f;
             nuo[u] = 0.f;
@@ -614,7 +645,7 @@ template <int PLACE, bool FLIPS = false>
 __device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> &m, int v, const GammaSrc &gs) {
     const int2 vs = var_slice(a, m, v);  // {ent, width}
     const float g = gs(v);
-    const float l0 = (PLACE & (1 << kL0)) ? m.l0_pos(v) : m.l0g[a.g.vperm[v]];
+    const float l0 = (PLACE & (1 << kL0)) ? m.l0_pos(v) : gmem(m.l0g)[gmem(a.g.vperm)[v]];
     const float2 ls = m.ls_pos(v);
     const float post_prev = ls.x + ls.y;
     const float lm = __fadd_rn(__fmul_rn(1.0f - g, l0), __fmul_rn(g, post_prev));
@@ -882,7 +913,7 @@ struct VarFeatures {
     __device__ __forceinline__ void start(const KArgs &a, const ShotMem<PLACE> &m, int vc) {
         ls = m.ls_pos(vc);
         deg = a.g.var_deg[vc];
-        l0 = (PLACE & (1 << kL0)) ? m.l0_pos(vc) : m.l0g[a.g.vperm[vc]];
+        l0 = (PLACE & (1 << kL0)) ? m.l0_pos(vc) : gmem(m.l0g)[gmem(a.g.vperm)[vc]];
         vs = var_slice(a, m, vc);
         mn = INFINITY;
         uns = 0;
@@ -1090,7 +1121,7 @@ __device__ __forceinline__ void select_solution(const KArgs &a, const ShotMem<PL
 #pragma unroll
             for (int u = 0; u < kBatchU; ++u) {
                 ls[u] = m.ls_pos(min(v0 + u * BLOCK, V));  // position V: the sentinel (+inf, never set)
-                w[u] = a.wj_pos[min(v0 + u * BLOCK, V - 1)];
+                w[u] = gmem(a.wj_pos)[min(v0 + u * BLOCK, V - 1)];
             }
 #pragma unroll
             for (int u = 0; u < kBatchU; ++u) acc += ls[u].x + ls[u].y <= 0.f ? (double)w[u] : 0.0;
@@ -1099,7 +1130,7 @@ __device__ __forceinline__ void select_solution(const KArgs &a, const ShotMem<PL
         // per-shot weights in the caller's order: gathered only for the few set bits
         for (int v = gtid<BLOCK>(); v < V; v += BLOCK) {
             const float2 ls = m.ls_pos(v);
-            if (ls.x + ls.y <= 0.f) acc += (double)wj[a.g.vperm[v]];
+            if (ls.x + ls.y <= 0.f) acc += (double)gmem(wj)[gmem(a.g.vperm)[v]];
         }
     }
     const float w = (float)block_sum<BLOCK>(acc, s_red);
@@ -1120,7 +1151,7 @@ __device__ __forceinline__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m,
                         ShotState &st, double *s_red, int *s_flag) {
     const int C = a.g.C, V = a.g.V;
     for (int t = 0; t < T; ++t) {
-        const float alpha = a.use_alpha ? a.alpha_t[t] : 1.0f;
+        const float alpha = a.use_alpha ? gmem(a.alpha_t)[t] : 1.0f;
         int bad = 0;
         // A leg's first check pass starts from messages reset to the priors
         // (kernels.py:981-1000), so its output (CS and SB) depends only on the
@@ -1130,7 +1161,7 @@ __device__ __forceinline__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m,
         // back.  Each thread restores exactly the words it saved.
         const bool restore = t == 0 && ShotMem<PLACE>::kFast && st.first_ok;
         if (restore) {
-            for (int i = gtid<BLOCK>(); i < m.first_words; i += BLOCK) *m.template lds<vec4f>(16u * (uint32_t)i) = m.first[i];
+            for (int i = gtid<BLOCK>(); i < m.first_words; i += BLOCK) *m.template lds<vec4f>(16u * (uint32_t)i) = gmem(m.first)[i];
         } else if (t == 0) {
             for (int i = gtid<BLOCK>(); i < C; i += BLOCK) check_row<true, PLACE>(a, m, i, alpha, syn);
         } else {
@@ -1140,7 +1171,7 @@ __device__ __forceinline__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m,
         if (t == 0 && ShotMem<PLACE>::kFast && !st.first_ok) {
             // read before the barrier that ends this iteration; the next check
             // pass rewrites CS and SB only after it
-            for (int i = gtid<BLOCK>(); i < m.first_words; i += BLOCK) m.first[i] = *m.template lds<vec4f>(16u * (uint32_t)i);
+            for (int i = gtid<BLOCK>(); i < m.first_words; i += BLOCK) gmem_w(m.first)[i] = *m.template lds<vec4f>(16u * (uint32_t)i);
             st.first_ok = 1;
         }
         if (t > 0 && !bad) {  // iteration t-1 converged
