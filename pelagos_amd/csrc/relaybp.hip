@@ -720,12 +720,31 @@ constexpr int kMlpWords = 37, kMlpStride = 40;
 
 typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 
-// Two fp32 values rounded to bf16 (nearest even; one v_cvt_pk_bf16_f32), low first.
+// Two fp32 values rounded to bf16 (nearest even), low first: one
+// v_cvt_pk_bf16_f32 (two scalar conversions compile to one each plus a v_perm).
 __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
-    bf16x2_t v;
-    v[0] = (__bf16)a;
-    v[1] = (__bf16)b;
-    return __builtin_bit_cast(uint32_t, v);
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(vec2f{a, b}, bf16x2_t));
+}
+
+// max(x, 0) of an fp32 as a signed integer max (one v_max_i32): every
+// negative float (and -0) is a negative integer, and positive floats order as
+// their bit patterns do.  fmaxf compiles to a canonicalize and a max.  For
+// finite x the two agree except fmaxf(-0, 0), which may give -0; the sums it
+// feeds start from +0, where a -0 term and a +0 term leave the same result.
+__device__ __forceinline__ float relu_f32(float x) {
+    return __int_as_float(__builtin_elementwise_max(__float_as_int(x), 0));
+}
+
+// x[l] + x[l ^ 16] and x[l] + x[l ^ 32] in every lane, with gfx950's permlane
+// swaps (VALU, no LDS round trip): swapping a copy of x with itself leaves the
+// two halves (rows) of each pair in the two results.
+__device__ __forceinline__ float sum_xor16(float x) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float sum_xor32(float x) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
 }
 
 // max(x, 0) of both bf16 halves of a word (v_pk_max_i16): the sign bit makes
@@ -772,11 +791,13 @@ __device__ __forceinline__ bf16x4_t unpack_bf16x4(uint32_t lo, uint32_t hi) {
 }
 
 __device__ __forceinline__ void load_mlp(const uint32_t *__restrict__ frag, MlpRegs &r) {
-    const uint4 *__restrict__ base = reinterpret_cast<const uint4 *>(frag + (threadIdx.x & 63) * kMlpStride);
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 *__restrict__ base = reinterpret_cast<const u32x4 *>(frag + (threadIdx.x & 63) * kMlpStride);
+    // global loads (a flat load would also hold up the LDS waits in between)
     uint32_t w[kMlpStride];
 #pragma unroll
     for (int i = 0; i < kMlpStride / 4; ++i) {
-        const uint4 q = base[i];
+        const u32x4 q = *gmem(base + i);
         w[4 * i] = q.x;
         w[4 * i + 1] = q.y;
         w[4 * i + 2] = q.z;
@@ -835,14 +856,12 @@ __device__ __forceinline__ void mlp_logits(const MlpRegs &r, const bf16x4_t (&x)
 #pragma unroll
         for (int g = 0; g < G; ++g) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) part[g] = __builtin_fmaf(r.w3[mb][i], fmaxf(d[g][i], 0.f), part[g]);
+            for (int i = 0; i < 4; ++i) part[g] = __builtin_fmaf(r.w3[mb][i], relu_f32(d[g][i]), part[g]);
         }
     }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        part[g] += __shfl_xor(part[g], 16, 64);
-        part[g] += __shfl_xor(part[g], 32, 64);
-        z[g] = part[g] + r.b3;
+        z[g] = sum_xor32(sum_xor16(part[g])) + r.b3;
     }
 }
 
@@ -1100,6 +1119,25 @@ __device__ __forceinline__ void for_each_caller_index(const KArgs &a, const Shot
     }
 }
 
+// The shot's first check pass back from the workgroup's copy into LDS [0,
+// first_words * 16) (fast placements), with loads that write LDS directly
+// (global_load_lds_dwordx4: no registers, the data lands in LDS when the load
+// completes).  Thread t reloads words t, t + BLOCK, ..., the ones it saved; a
+// wave's 64 words are consecutive, as the instruction writes them (lane l at
+// the wave's base + 16 l).  The barrier after which any wave reads them waits
+// for them (a workgroup barrier waits for the wave's loads to LDS).
+template <int BLOCK, int PLACE>
+__device__ __forceinline__ void restore_first(const ShotMem<PLACE> &m) {
+    const int lane = gtid<BLOCK>() & 63;
+    for (int i = gtid<BLOCK>(); i - lane < m.first_words; i += BLOCK) {
+        if (i < m.first_words)
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(m.first + i),
+                                             (__attribute__((address_space(3))) void *)lds_at<uint8_t>(
+                                                 m.lb + 16u * (uint32_t)(i - lane)),
+                                             16, 0, 0);
+    }
+}
+
 // check_and_select_kernel (kernels.py:657-741) for a shot whose hard decision
 // satisfies every check: weight = sum of wj over bits with post <= 0 (double
 // accumulation, rounded once), strict improvement or first solution wins.  The
@@ -1161,7 +1199,10 @@ __device__ __forceinline__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m,
         // back.  Each thread restores exactly the words it saved.
         const bool restore = t == 0 && ShotMem<PLACE>::kFast && st.first_ok;
         if (restore) {
-            for (int i = gtid<BLOCK>(); i < m.first_words; i += BLOCK) *m.template lds<vec4f>(16u * (uint32_t)i) = gmem(m.first)[i];
+            restore_first<BLOCK, PLACE>(m);
+            // the reloaded words are in LDS before this wave goes on (a one-wave
+            // group's barrier does not wait for loads; a workgroup barrier would)
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         } else if (t == 0) {
             for (int i = gtid<BLOCK>(); i < C; i += BLOCK) check_row<true, PLACE>(a, m, i, alpha, syn);
         } else {
