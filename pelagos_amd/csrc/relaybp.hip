@@ -973,6 +973,9 @@ __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const Shot
     // across the gathers would spill).  Lanes past V, and batch slots past the
     // last slice, compute a copy of variable V - 1 (in the last slice) and store
     // nothing.
+    // the network's operands stay in registers over the wave's slices
+    MlpRegs mr;
+    if (net) load_mlp(a.mlp_frag, mr);
     for (int vb = gtid<BLOCK>() & ~63; vb < V; vb += kFeatBatch * BLOCK) {
         VarFeatures<PLACE, KNOWN_VALID> vf[kFeatBatch];
 #pragma unroll
@@ -1008,10 +1011,6 @@ __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const Shot
             pk[j][3] = pack_bf16x2(f[6], f[7]);
         }
         if (net) {
-            // the operands are reloaded every slice (coalesced, L1-resident)
-            // rather than held across the gathers above
-            MlpRegs mr;
-            load_mlp(a.mlp_frag, mr);
 #pragma unroll
             for (int j = 0; j < kFeatBatch; ++j) {
                 const int v = vb + j * BLOCK + lane;
