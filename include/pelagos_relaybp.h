@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define PEL_ABI_VERSION 9
+#define PEL_ABI_VERSION 10
 
 /* status codes */
 #define PEL_OK 0
@@ -318,6 +318,30 @@ int pel_plan_serve(const pel_graph *const *graphs, const int32_t *flags, int32_t
  * the launch. */
 int pel_serve_launch(const pel_graph *const *graphs, const pel_params *params, const pel_batch *batches, int32_t n,
                      const pel_jobs_plan *plan, void *table, pel_serve_queue *q, int64_t idle_us, void *stream);
+
+/* ---- float16 and float64 decodes ---------------------------------------------
+ * RelayBPDecoder(data_type="float16" | "float64") (decoder.py:402-423; the kernels' dtype_enum
+ * 0 / 2, e.g. kernels.py:1115-1126, 1313-1315, 680-715).  The batch's prior_llr, wj, gamma_legs
+ * (caller's variable order, gamma_by_position = 0), init_gamma, init_posterior, weights and
+ * last_posterior hold elements of that type; pel_params keeps its float32 scalars (gamma0, beta,
+ * alpha_t), as the reference passes them.  The reference's own per-edge formulation: mu and nu of
+ * every edge in a workspace slot per workgroup, one shot per workgroup at a time, the full
+ * Relay-BP-S schedule of pel_decode (no learned strengths, no leg-stepped decodes).  PEL_DTYPE_F32
+ * is accepted too and gives pel_decode's results. */
+#define PEL_DTYPE_F16 0
+#define PEL_DTYPE_F32 1
+#define PEL_DTYPE_F64 2
+
+/* Launch size for B shots of element type dtype: *grid workgroups and the workspace bytes
+ * pel_decode_typed needs (a queue word, then one state slot per workgroup). */
+int pel_plan_typed(const pel_graph *g, int32_t B, int32_t dtype, int32_t *grid, int64_t *workspace_bytes);
+
+/* Decode a batch whose floating-point buffers hold elements of type dtype, with grid workgroups
+ * (1 .. the planned grid) and b->workspace of the planned size.  Replaces the dtype_enum 0 and 2
+ * instantiations of the kernels RelayBPDecoder._decode_loop_single_gpu launches
+ * (decoder.py:3394-3873, kernels.py:446-1350). */
+int pel_decode_typed(const pel_graph *g, const pel_params *p, const pel_batch *b, int32_t dtype, int32_t grid,
+                     void *stream);
 
 /* ---- learned memory strengths (no counterpart in the reference) ----------
  * Training entry points for the MLP above.  x [n][8] features, y [n] soft

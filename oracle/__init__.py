@@ -21,6 +21,10 @@ import numpy as np
 
 _DIR = Path(__file__).resolve().parent
 LIB_PATH = _DIR / "liboracle_relaybp.so"
+# the same restatement per element type (relaybp_oracle.c built with ORC_ELEM = 32 / 64 / 16)
+LIB_PATHS = {"float32": LIB_PATH, "float64": _DIR / "liboracle_relaybp_f64.so",
+             "float16": _DIR / "liboracle_relaybp_f16.so"}
+NP_DTYPES = {"float32": np.float32, "float64": np.float64, "float16": np.float16}
 
 _f32p = ctypes.POINTER(ctypes.c_float)
 _i32p = ctypes.POINTER(ctypes.c_int32)
@@ -45,20 +49,22 @@ class _Params(ctypes.Structure):
 
 
 class _Batch(ctypes.Structure):
+    # element-type arrays (priors, strengths, weights, posteriors) as untyped pointers: float,
+    # double or half bit patterns, per the library's ORC_ELEM
     _fields_ = [
         ("B", ctypes.c_int32),
         ("syndromes", _u8p),
-        ("prior_llr", _f32p),
+        ("prior_llr", ctypes.c_void_p),
         ("prior_batched", ctypes.c_int32),
-        ("gamma_legs", _f32p),
+        ("gamma_legs", ctypes.c_void_p),
         ("gamma_per_row", ctypes.c_int32),
-        ("init_gamma", _f32p),
-        ("init_posterior", _f32p),
+        ("init_gamma", ctypes.c_void_p),
+        ("init_posterior", ctypes.c_void_p),
         ("errors", _u8p),
-        ("weights", _f32p),
+        ("weights", ctypes.c_void_p),
         ("iterations", _i32p),
         ("found_count", _i32p),
-        ("last_posterior", _f32p),
+        ("last_posterior", ctypes.c_void_p),
     ]
 
 
@@ -85,12 +91,18 @@ def load(path: Optional[os.PathLike] = None) -> ctypes.CDLL:
         ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _i32p, _i32p,
         ctypes.POINTER(_Params), ctypes.POINTER(_Batch), ctypes.c_int32,
     ]
-    lib.orc_check_update_row.restype = None
-    lib.orc_check_update_row.argtypes = [
-        ctypes.c_int32, _f32p, ctypes.c_uint8, ctypes.c_int32, ctypes.c_float,
-        ctypes.c_int32, ctypes.c_float, _f32p,
-    ]
+    if hasattr(lib, "orc_check_update_row"):  # the float32 build
+        lib.orc_check_update_row.restype = None
+        lib.orc_check_update_row.argtypes = [
+            ctypes.c_int32, _f32p, ctypes.c_uint8, ctypes.c_int32, ctypes.c_float,
+            ctypes.c_int32, ctypes.c_float, _f32p,
+        ]
+    if hasattr(lib, "orc_to_half"):  # the float16 build
+        for fn in (lib.orc_to_half, lib.orc_from_half):
+            fn.restype = None
+            fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]
     lib.orc_planted.restype = ctypes.c_int
+    lib.orc_elem_bits.restype = ctypes.c_int
     _LIBS[key] = lib
     return lib
 
@@ -143,14 +155,19 @@ def decode(
     want_posterior: bool = False,
     nthreads: int = 0,
     lib_path: Optional[os.PathLike] = None,
+    dtype: str = "float32",
 ) -> Dict[str, np.ndarray]:
     """Decode ``syndromes [B,C]`` on the CPU; returns numpy arrays.
 
     ``gamma_legs`` is the per-leg memory-strength table: ``[num_sets, V]`` or,
     with ``gamma_per_row``, ``[num_sets, B, V]``; row ``l`` is relay leg ``l``.
-    ``prior_llr`` is ``[V]`` or per-shot ``[B, V]``.
+    ``prior_llr`` is ``[V]`` or per-shot ``[B, V]``.  ``dtype`` is the decoder's
+    data_type ("float32", "float64" or "float16"): the element type of the
+    priors, strengths, weights and posteriors, which are converted to it first
+    (as the reference's decoder casts them, decoder.py:724-734, 3684-3700).
     """
-    lib = load(lib_path)
+    lib = load(lib_path if lib_path is not None else LIB_PATHS[dtype])
+    ft = NP_DTYPES[dtype]
     row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int32)
     row_var = np.ascontiguousarray(row_var, dtype=np.int32)
     C = row_ptr.shape[0] - 1
@@ -158,33 +175,36 @@ def decode(
     syn = np.ascontiguousarray(syndromes, dtype=np.uint8)
     B = syn.shape[0]
     assert syn.shape == (B, C)
-    lam = np.ascontiguousarray(prior_llr, dtype=np.float32)
+    lam = np.ascontiguousarray(prior_llr, dtype=ft)
     prior_batched = lam.ndim == 2
     assert lam.shape == ((B, V) if prior_batched else (V,))
-    g = np.ascontiguousarray(gamma_legs, dtype=np.float32)
+    g = np.ascontiguousarray(gamma_legs, dtype=ft)
     if num_sets > 0:
         assert g.shape == ((num_sets, B, V) if gamma_per_row else (num_sets, V)), g.shape
     else:
-        g = np.zeros((1, V), np.float32)
-    ig = None if init_gamma is None else np.ascontiguousarray(init_gamma, dtype=np.float32)
-    ip = None if init_posterior is None else np.ascontiguousarray(init_posterior, dtype=np.float32)
+        g = np.zeros((1, V), ft)
+    ig = None if init_gamma is None else np.ascontiguousarray(init_gamma, dtype=ft)
+    ip = None if init_posterior is None else np.ascontiguousarray(init_posterior, dtype=ft)
     alpha_t = alpha_schedule(alpha, alpha_scaling, max(pre_iter, set_max_iter, 1))
 
     errors = np.zeros((B, V), np.uint8)
-    weights = np.zeros(B, np.float32)
+    weights = np.zeros(B, ft)
     iters = np.zeros(B, np.int32)
     found = np.zeros(B, np.int32)
-    post = np.zeros((B, V), np.float32) if want_posterior else None
+    post = np.zeros((B, V), ft) if want_posterior else None
 
     prm = _Params(
         pre_iter, num_sets, set_max_iter, stop_nconv, int(bool(plain)), float(gamma0),
         int(alpha is not None), int(beta is not None), float(beta or 0.0),
         _ptr(alpha_t, _f32p), int(bool(last_leg)), int(bool(skip_relay_legs)),
     )
+    def vp(a):
+        return None if a is None else a.ctypes.data
+
     bat = _Batch(
-        B, _ptr(syn, _u8p), _ptr(lam, _f32p), int(prior_batched), _ptr(g, _f32p),
-        int(bool(gamma_per_row)), _ptr(ig, _f32p), _ptr(ip, _f32p), _ptr(errors, _u8p),
-        _ptr(weights, _f32p), _ptr(iters, _i32p), _ptr(found, _i32p), _ptr(post, _f32p),
+        B, _ptr(syn, _u8p), vp(lam), int(prior_batched), vp(g),
+        int(bool(gamma_per_row)), vp(ig), vp(ip), _ptr(errors, _u8p),
+        vp(weights), _ptr(iters, _i32p), _ptr(found, _i32p), vp(post),
     )
     rc = lib.orc_decode(C, V, E, _ptr(row_ptr, _i32p), _ptr(row_var, _i32p),
                         ctypes.byref(prm), ctypes.byref(bat), int(nthreads))
@@ -212,3 +232,21 @@ def check_update_row(mu: np.ndarray, syn: int, alpha: Optional[float] = None,
                              float(alpha if alpha is not None else 1.0), int(beta is not None),
                              float(beta or 0.0), _ptr(nu, _f32p))
     return nu
+
+
+def half_round(x: np.ndarray) -> np.ndarray:
+    """The float16 oracle's rounding of doubles to half (bit patterns as np.float16)."""
+    lib = load(LIB_PATHS["float16"])
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    h = np.zeros(x.shape, np.float16)
+    lib.orc_to_half(x.ctypes.data, h.ctypes.data, int(x.size))
+    return h
+
+
+def half_value(h: np.ndarray) -> np.ndarray:
+    """The float16 oracle's value of half bit patterns, as doubles."""
+    lib = load(LIB_PATHS["float16"])
+    h = np.ascontiguousarray(h, dtype=np.float16)
+    x = np.zeros(h.shape, np.float64)
+    lib.orc_from_half(h.ctypes.data, x.ctypes.data, int(h.size))
+    return x

@@ -38,6 +38,19 @@
  * Parity status: pinned by tests/test_oracle_pins.py (closed forms, tree-code
  * exactness, brute-force ML, schedule invariants, relay-leg schedule closed
  * forms, planted-bug mutants 1-10).
+ *
+ * Element type (ORC_ELEM, default 32): the same restatement for data_type
+ * "float64" (64: every stored and computed value a double) and "float16" (16:
+ * mu, nu, posteriors, priors, strengths and weights stored as IEEE half, check
+ * minima and variable sums in fp32, the memory mix in half arithmetic), as the
+ * reference's kernels do for dtype_enum 2 and 0 (kernels.py:1115-1126,
+ * 1299-1350, 676-715).  Half values are held as their 16-bit patterns and
+ * every half operation is computed exactly in double and rounded to nearest
+ * even (to_half).  Parity status of these two: the conversions are pinned
+ * against numpy's float16; the fp64 and fp16 arithmetic against the fp32
+ * restatement on tree codes (same ML answers) and against the GPU; bit-level
+ * agreement with the reference's Triton fp16 kernels is unpinned (they need a
+ * GPU and Triton here).
  */
 #include <math.h>
 #include <stdint.h>
@@ -51,6 +64,79 @@
  * pinning checks reject a wrong oracle.  0 = the real oracle. */
 #ifndef ORC_PLANT
 #define ORC_PLANT 0
+#endif
+
+#ifndef ORC_ELEM
+#define ORC_ELEM 32
+#endif
+
+#if ORC_ELEM == 16
+/* IEEE half as its bit pattern. */
+typedef uint16_t elem_t;
+typedef float real_t;
+static double from_half(uint16_t h)
+{
+    const int s = (h >> 15) & 1, e = (h >> 10) & 31, m = h & 1023;
+    double v;
+    if (e == 0) v = ldexp((double)m, -24);
+    else if (e == 31) v = m ? NAN : INFINITY;
+    else v = ldexp((double)(m | 1024), e - 25);
+    return s ? -v : v;
+}
+/* x rounded to the nearest half (ties to even); beyond the largest finite half
+ * (65504) by half an ulp or more: infinity. */
+static uint16_t to_half(double x)
+{
+    const uint16_t sgn = signbit(x) ? 0x8000u : 0u;
+    const double a = fabs(x);
+    if (isnan(x)) return sgn | 0x7e00u;
+    if (a >= 65520.0) return sgn | 0x7c00u;
+    if (a < ldexp(1.0, -14)) return sgn | (uint16_t)nearbyint(a * 16777216.0); /* units of 2^-24 */
+    int e;
+    frexp(a, &e);                                  /* a in [2^(e-1), 2^e) */
+    double r = nearbyint(ldexp(a, 11 - e));        /* 1024 .. 2048 */
+    if (r == 2048.0) { r = 1024.0; ++e; }
+    const int E = e - 1 + 15;
+    if (E >= 31) return sgn | 0x7c00u;
+    return sgn | (uint16_t)(E << 10) | (uint16_t)((int)r - 1024);
+}
+#define LD(x) ((real_t)from_half(x))
+#define ST(x) to_half((double)(x))
+#define ELEM_ZERO ((elem_t)0)
+#define RABS(x) fabsf(x)
+#define RMAX(a, b) fmaxf(a, b)
+#define NO_SOLUTION 60000.0                        /* kernels.py:713-714 */
+/* (1 - g) lam0 + g post with half operands: each operation rounded to half */
+static elem_t mix(elem_t g, elem_t l0, elem_t post)
+{
+    const double omg = from_half(to_half(1.0 - from_half(g)));
+    const double t1 = from_half(to_half(omg * from_half(l0)));
+    const double t2 = from_half(to_half(from_half(g) * from_half(post)));
+    return to_half(t1 + t2);
+}
+static double weight_round(double acc) { return from_half(to_half((double)(float)acc)); }
+#elif ORC_ELEM == 64
+typedef double elem_t;
+typedef double real_t;
+#define LD(x) (x)
+#define ST(x) ((double)(x))
+#define ELEM_ZERO 0.0
+#define RABS(x) fabs(x)
+#define RMAX(a, b) fmax(a, b)
+#define NO_SOLUTION 1e20
+static elem_t mix(elem_t g, elem_t l0, elem_t post) { return (1.0 - g) * l0 + g * post; }
+static double weight_round(double acc) { return acc; }
+#else
+typedef float elem_t;
+typedef float real_t;
+#define LD(x) (x)
+#define ST(x) ((float)(x))
+#define ELEM_ZERO 0.0f
+#define RABS(x) fabsf(x)
+#define RMAX(a, b) fmaxf(a, b)
+#define NO_SOLUTION 1e20
+static elem_t mix(elem_t g, elem_t l0, elem_t post) { return (1.0f - g) * l0 + g * post; }
+static double weight_round(double acc) { return (double)(float)acc; }
 #endif
 
 typedef struct {
@@ -71,17 +157,17 @@ typedef struct {
 typedef struct {
     int32_t B;
     const uint8_t *syndromes;     /* [B,C] */
-    const float *prior_llr;       /* [V] or [B,V] when prior_batched */
+    const elem_t *prior_llr;      /* [V] or [B,V] when prior_batched */
     int32_t prior_batched;
-    const float *gamma_legs;      /* [R,V] or [R,B,V] when gamma_per_row */
+    const elem_t *gamma_legs;     /* [R,V] or [R,B,V] when gamma_per_row */
     int32_t gamma_per_row;
-    const float *init_gamma;      /* [B,V] or NULL */
-    const float *init_posterior;  /* [B,V] or NULL */
+    const elem_t *init_gamma;     /* [B,V] or NULL */
+    const elem_t *init_posterior; /* [B,V] or NULL */
     uint8_t *errors;              /* [B,V] out */
-    float *weights;               /* [B]   out */
+    elem_t *weights;              /* [B]   out */
     int32_t *iterations;          /* [B]   out */
     int32_t *found_count;         /* [B]   out */
-    float *last_posterior;        /* [B,V] out or NULL */
+    elem_t *last_posterior;       /* [B,V] out or NULL */
 } orc_batch;
 
 typedef struct {
@@ -111,13 +197,13 @@ static int build_var_lists(orc_graph *g)
 }
 
 typedef struct {
-    float *mu, *nu;          /* [E] variable->check and check->variable messages */
-    float *post;             /* [V] posterior_ratios */
+    elem_t *mu, *nu;         /* [E] variable->check and check->variable messages */
+    elem_t *post;            /* [V] posterior_ratios */
     uint8_t *hard;           /* [V] hard_dec */
     uint8_t *best;           /* [V] best_errors */
     uint8_t *first_hd;       /* [V] first_leg_hard_dec */
-    const float *gamma;      /* [V] current leg's memory strengths (or NULL with gamma_const) */
-    float gamma_const;
+    const elem_t *gamma;     /* [V] current leg's memory strengths (or NULL with gamma_const) */
+    elem_t gamma_const;
 } orc_lane;
 
 /* C2V, kernels.py:1128-1215: per check, the parity of negative incoming
@@ -130,15 +216,15 @@ static void check_update(const orc_graph *g, const orc_params *p, const uint8_t 
         if (r1 <= r0) continue;
         int deg_is_one = (r1 - r0) == 1;
         int neg = 0, cnt1 = 0;
-        float min1 = 1e30f, min2 = 1e30f;         /* the reference's start and padding value: magnitudes
-                                                     are clipped at 1e30 (kernels.py:1120-1126, 1146) */
+        real_t min1 = (real_t)1e30, min2 = (real_t)1e30; /* the reference's start and padding value:
+                                                     magnitudes are clipped at 1e30 (kernels.py:1120-1126, 1146) */
 #if ORC_PLANT == 10
         min1 = min2 = INFINITY;                   /* planted: no clip */
 #endif
         for (int32_t e = r0; e < r1; ++e) {
-            float m = L->mu[e];
-            float a = fabsf(m);
-            neg ^= (m < 0.0f);
+            real_t m = LD(L->mu[e]);
+            real_t a = RABS(m);
+            neg ^= (m < 0);
             if (a < min1) { min2 = min1; min1 = a; cnt1 = 1; }
             else if (a == min1) { cnt1++; min2 = a; }
             else if (a < min2) { min2 = a; }
@@ -147,43 +233,43 @@ static void check_update(const orc_graph *g, const orc_params *p, const uint8_t 
 #if ORC_PLANT == 1
         par = neg & 1;                            /* planted: syndrome sign dropped */
 #endif
-        float sign_prod = par == 0 ? 1.0f : -1.0f;
+        real_t sign_prod = par == 0 ? (real_t)1 : (real_t)-1;
         for (int32_t e = r0; e < r1; ++e) {
-            float m = L->mu[e];
-            float a = fabsf(m);
-            float out = (a == min1 && cnt1 == 1) ? min2 : min1;
+            real_t m = LD(L->mu[e]);
+            real_t a = RABS(m);
+            real_t out = (a == min1 && cnt1 == 1) ? min2 : min1;
 #if ORC_PLANT == 2
             out = min1;                           /* planted: own edge not excluded */
 #endif
-            if (deg_is_one) out = 0.0f;
-            if (p->use_alpha) out = alpha * out;
-            if (p->use_beta) out = fmaxf(out - p->beta, 0.0f);
-            float sgn = m >= 0.0f ? 1.0f : -1.0f;
-            L->nu[e] = (sign_prod * sgn) * out;
+            if (deg_is_one) out = 0;
+            if (p->use_alpha) out = (real_t)alpha * out;
+            if (p->use_beta) out = RMAX(out - (real_t)p->beta, (real_t)0);
+            real_t sgn = m >= 0 ? (real_t)1 : (real_t)-1;
+            L->nu[e] = ST((sign_prod * sgn) * out);
         }
     }
 }
 
 /* V2C with memory mixing, kernels.py:1299-1350. */
-static void variable_update(const orc_graph *g, const float *lam0, orc_lane *L)
+static void variable_update(const orc_graph *g, const elem_t *lam0, orc_lane *L)
 {
     for (int32_t j = 0; j < g->V; ++j) {
-        float gj = L->gamma ? L->gamma[j] : L->gamma_const;
-        float lm = (1.0f - gj) * lam0[j] + gj * L->post[j];
+        elem_t gj = L->gamma ? L->gamma[j] : L->gamma_const;
+        elem_t lm = mix(gj, lam0[j], L->post[j]);
 #if ORC_PLANT == 3
         lm = gj * lam0[j] + (1.0f - gj) * L->post[j];   /* planted: memory weights swapped */
 #endif
-        float s = 0.0f;
-        for (int32_t q = g->var_ptr[j]; q < g->var_ptr[j + 1]; ++q) s += L->nu[g->var_edges[q]];
-        float post = lm + s;
-        L->post[j] = post;
-        L->hard[j] = post <= 0.0f;
+        real_t s = 0;
+        for (int32_t q = g->var_ptr[j]; q < g->var_ptr[j + 1]; ++q) s += LD(L->nu[g->var_edges[q]]);
+        real_t post = LD(lm) + s;
+        L->post[j] = ST(post);
+        L->hard[j] = post <= 0;
         for (int32_t q = g->var_ptr[j]; q < g->var_ptr[j + 1]; ++q) {
             int32_t e = g->var_edges[q];
 #if ORC_PLANT == 4
             L->mu[e] = lm + s;                    /* planted: extrinsic subtraction dropped */
 #else
-            L->mu[e] = lm + (s - L->nu[e]);
+            L->mu[e] = ST(LD(lm) + (s - LD(L->nu[e])));
 #endif
         }
     }
@@ -201,13 +287,13 @@ static int all_checks_ok(const orc_graph *g, const uint8_t *syn, const uint8_t *
 }
 
 /* init_messages_with_priors_EB_kernel: nu = 0, mu[e] = lam0[var(e)]. */
-static void init_messages(const orc_graph *g, const float *lam0, orc_lane *L)
+static void init_messages(const orc_graph *g, const elem_t *lam0, orc_lane *L)
 {
-    for (int32_t e = 0; e < g->E; ++e) { L->mu[e] = lam0[g->row_var[e]]; L->nu[e] = 0.0f; }
+    for (int32_t e = 0; e < g->E; ++e) { L->mu[e] = lam0[g->row_var[e]]; L->nu[e] = ELEM_ZERO; }
 }
 
 typedef struct {
-    float best_w;
+    double best_w;           /* a value of the element type (or +inf) */
     int32_t found;
     int32_t iters;
 } orc_shot;
@@ -216,7 +302,7 @@ typedef struct {
  * solution of the leg (freeze_finished_lanes_kernel with in_leg=True and
  * sought = found + 1).  Returns 1 if the lane converged in this leg. */
 static int run_leg(const orc_graph *g, const orc_params *p, const uint8_t *syn,
-                   const float *lam0, const float *wj, int32_t T, orc_lane *L, orc_shot *st)
+                   const elem_t *lam0, const elem_t *wj, int32_t T, orc_lane *L, orc_shot *st)
 {
     for (int32_t t = 0; t < T; ++t) {
         float alpha = p->use_alpha ? p->alpha_t[t] : 1.0f;
@@ -227,17 +313,17 @@ static int run_leg(const orc_graph *g, const orc_params *p, const uint8_t *syn,
             /* check_and_select_kernel: weight over bits (post <= 0) with wj. */
             double acc = 0.0;
             for (int32_t j = 0; j < g->V; ++j)
-                if (L->post[j] <= 0.0f) acc += (double)wj[j];
-            float w = (float)acc;
+                if (LD(L->post[j]) <= 0) acc += (double)LD(wj[j]);
+            double w = weight_round(acc);
 #if ORC_PLANT == 7
-            if (st->best_w >= 1e20f) {                  /* planted: the first solution is never replaced */
+            if (st->best_w >= NO_SOLUTION) {                  /* planted: the first solution is never replaced */
 #elif ORC_PLANT == 9
-            if (st->best_w >= 1e20f || w <= st->best_w) { /* planted: a tie replaces the best */
+            if (st->best_w >= NO_SOLUTION || w <= st->best_w) { /* planted: a tie replaces the best */
 #else
-            if (st->best_w >= 1e20f || w < st->best_w) {
+            if (st->best_w >= NO_SOLUTION || w < st->best_w) {
 #endif
                 st->best_w = w;
-                for (int32_t j = 0; j < g->V; ++j) L->best[j] = L->post[j] <= 0.0f;
+                for (int32_t j = 0; j < g->V; ++j) L->best[j] = LD(L->post[j]) <= 0;
             }
             st->found++;
             /* freeze: hard_dec <- best_errors (kernels.py:806-817) */
@@ -249,13 +335,13 @@ static int run_leg(const orc_graph *g, const orc_params *p, const uint8_t *syn,
 }
 
 static void decode_shot(const orc_graph *g, const orc_params *p, const orc_batch *bt,
-                        int32_t b, orc_lane *L, float *wj_buf)
+                        int32_t b, orc_lane *L, elem_t *wj_buf)
 {
     const int32_t V = g->V;
     const uint8_t *syn = bt->syndromes + (size_t)b * g->C;
-    const float *lam0 = bt->prior_llr + (bt->prior_batched ? (size_t)b * V : 0);
+    const elem_t *lam0 = bt->prior_llr + (bt->prior_batched ? (size_t)b * V : 0);
     /* wj = lam0 with non-finite entries zeroed (decoder.py:736-753) */
-    for (int32_t j = 0; j < V; ++j) wj_buf[j] = isfinite(lam0[j]) ? lam0[j] : 0.0f;
+    for (int32_t j = 0; j < V; ++j) wj_buf[j] = isfinite(LD(lam0[j])) ? lam0[j] : ELEM_ZERO;
 
     /* state reset (decoder.py:3430-3464; _reset_reused_tensors 1557-1592) */
     for (int32_t j = 0; j < V; ++j)
@@ -268,9 +354,9 @@ static void decode_shot(const orc_graph *g, const orc_params *p, const orc_batch
 
     /* pre-iter leg (decoder.py:3470-3593) */
     init_messages(g, lam0, L);
-    if (p->plain) { L->gamma = NULL; L->gamma_const = 0.0f; }
+    if (p->plain) { L->gamma = NULL; L->gamma_const = ELEM_ZERO; }
     else if (bt->init_gamma) { L->gamma = bt->init_gamma + (size_t)b * V; }
-    else { L->gamma = NULL; L->gamma_const = p->gamma0; }
+    else { L->gamma = NULL; L->gamma_const = ST(p->gamma0); }
     if (st.found < S) run_leg(g, p, syn, lam0, wj_buf, p->pre_iter, L, &st);
     memcpy(L->first_hd, L->hard, (size_t)V);            /* decoder.py:3597-3598 */
 
@@ -299,10 +385,10 @@ static void decode_shot(const orc_graph *g, const orc_params *p, const orc_batch
     uint8_t *out = bt->errors + (size_t)b * V;
     const uint8_t *src = isfinite(st.best_w) ? L->best : (p->last_leg ? L->hard : L->first_hd);
     memcpy(out, src, (size_t)V);
-    bt->weights[b] = st.best_w;
+    bt->weights[b] = ST(st.best_w);
     bt->iterations[b] = st.iters;
     bt->found_count[b] = st.found;
-    if (bt->last_posterior) memcpy(bt->last_posterior + (size_t)b * V, L->post, sizeof(float) * V);
+    if (bt->last_posterior) memcpy(bt->last_posterior + (size_t)b * V, L->post, sizeof(elem_t) * V);
 }
 
 /* Decode a batch.  Returns 0 on success, -1 on allocation failure.
@@ -322,15 +408,15 @@ int orc_decode(int32_t C, int32_t V, int32_t E, const int32_t *row_ptr, const in
     {
         orc_lane L;
         size_t e1 = (size_t)(E > 0 ? E : 1);
-        L.mu = (float *)malloc(sizeof(float) * e1);
-        L.nu = (float *)malloc(sizeof(float) * e1);
-        L.post = (float *)malloc(sizeof(float) * (size_t)V);
+        L.mu = (elem_t *)malloc(sizeof(elem_t) * e1);
+        L.nu = (elem_t *)malloc(sizeof(elem_t) * e1);
+        L.post = (elem_t *)malloc(sizeof(elem_t) * (size_t)V);
         L.hard = (uint8_t *)malloc((size_t)V);
         L.best = (uint8_t *)malloc((size_t)V);
         L.first_hd = (uint8_t *)malloc((size_t)V);
-        float *wj = (float *)malloc(sizeof(float) * (size_t)V);
+        elem_t *wj = (elem_t *)malloc(sizeof(elem_t) * (size_t)V);
         L.gamma = NULL;
-        L.gamma_const = 0.0f;
+        L.gamma_const = ELEM_ZERO;
         if (!L.mu || !L.nu || !L.post || !L.hard || !L.best || !L.first_hd || !wj) {
             rc |= 1;
         } else {
@@ -344,6 +430,7 @@ int orc_decode(int32_t C, int32_t V, int32_t E, const int32_t *row_ptr, const in
     return rc ? -1 : 0;
 }
 
+#if ORC_ELEM == 32
 /* A single check update in isolation, for closed-form tests of C2V:
  * mu[deg] in, nu[deg] out (kernels.py:1128-1215). */
 void orc_check_update_row(int32_t deg, const float *mu, uint8_t syn, int32_t use_alpha,
@@ -366,4 +453,13 @@ void orc_check_update_row(int32_t deg, const float *mu, uint8_t syn, int32_t use
     free(row_var);
 }
 
+#endif
+
+#if ORC_ELEM == 16
+/* The half conversions, for their pins. */
+void orc_to_half(const double *x, uint16_t *h, int32_t n) { for (int32_t i = 0; i < n; ++i) h[i] = to_half(x[i]); }
+void orc_from_half(const uint16_t *h, double *x, int32_t n) { for (int32_t i = 0; i < n; ++i) x[i] = from_half(h[i]); }
+#endif
+
 int orc_planted(void) { return ORC_PLANT; }
+int orc_elem_bits(void) { return ORC_ELEM; }

@@ -145,8 +145,8 @@ def sample_errors(priors: np.ndarray, shots: int, seed: int = 0, scale: float = 
 
 
 def syndromes_of(H: csr_matrix, errors: np.ndarray) -> np.ndarray:
-    """s = H e mod 2, [shots, C] uint8."""
-    return ((H.astype(np.int32) @ errors.T.astype(np.int32)) % 2).T.astype(np.uint8)
+    """s = H e mod 2, [shots, C] uint8 (C order, so its buffer can go to the C ABI as is)."""
+    return np.ascontiguousarray(((H.astype(np.int32) @ errors.T.astype(np.int32)) % 2).T, dtype=np.uint8)
 
 
 def synthetic_observables(V: int, k: int = 12, weight: int = 64, seed: int = 0) -> np.ndarray:

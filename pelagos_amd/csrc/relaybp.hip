@@ -1786,6 +1786,267 @@ EngineFn pick_engine(int mask) {
     return engine_kernel<kKindsAll>;
 }
 
+// ---- float16 and float64 decodes: the per-edge formulation ------------------------------------
+// data_type "float16" / "float64" (decoder.py:402-423, the kernels' dtype_enum 0 / 2).  The fast
+// kernels above keep a compressed fp32 state whose LDS addresses are baked into the graph layout;
+// these other element types run the reference's own formulation instead: every edge's mu and nu
+// in the workgroup's workspace slot, a check pass over rows and a variable pass over variables,
+// one workgroup per shot taken from a queue, the schedule of first_leg / relay_legs.  Element
+// type S is what the reference stores (mu, nu, posterior, priors, strengths, weights); the
+// arithmetic follows its kernels for that dtype:
+//   float64  everything in double (kernels.py:1117-1126 min1/min2, 1313-1325 sums);
+//   float16  check minima and variable sums in fp32 (kernels.py:1118-1126, 1315-1325), the memory
+//            mix (1 - g) lam0 + g post in half arithmetic (fp16 operands, kernels.py:1309), every
+//            stored message and posterior rounded to half; the hard decision from the fp32
+//            posterior, the solution bits from the stored one (1328-1337 vs 690-692), and
+//            60000 as "no solution yet" (713-714).
+// float32 through this path gives the fast kernels' results bit for bit (a test checks it).
+
+constexpr int kTypedBlock = 256;
+
+struct TypedArgs {
+    int32_t C, V, E, B;
+    const int32_t *row_ptr, *row_var, *var_ptr, *var_edges;  // CSR (edge id = position); per
+                                                              // variable, edge ids ascending
+    int32_t pre_iter, num_sets, set_max_iter, stop_nconv;
+    int32_t plain, use_alpha, use_beta, last_leg, skip_relay_legs;
+    float gamma0, beta;
+    const float *alpha_t;
+    const uint8_t *syn;
+    const void *prior, *wj;  // S [V] or [B,V]
+    int32_t prior_batched;
+    const void *gamma_legs;  // S [R,V] or [R,B,V] (caller's variable order)
+    int32_t gamma_per_row;
+    const void *init_gamma, *init_post;  // S [B,V] or null
+    uint8_t *errors;
+    void *weights;                       // S [B]
+    int32_t *iters, *found;
+    void *last_post;                     // S [B,V] or null
+    int32_t *ctrl;                       // [0] queue ticket counter (cleared by typed_clear_kernel)
+    uint8_t *slots;
+    int64_t slot_stride;
+};
+
+template <typename S>
+struct TypedMath {                 // float64: double arithmetic throughout
+    typedef double C;
+    static constexpr double kNoSolution = 1e20;
+    // the double sum of the weights as the element type: float32 rounds it once (as the fast
+    // kernels do), float16 rounds that float once more, float64 keeps it
+    __device__ static double weight(double acc) { return acc; }
+    __device__ static S mix(S g, S l0, S post) { return __dadd_rn(__dmul_rn(1.0 - g, l0), __dmul_rn(g, post)); }
+};
+template <>
+struct TypedMath<float> {
+    typedef float C;
+    static constexpr double kNoSolution = 1e20;
+    __device__ static double weight(double acc) { return (double)(float)acc; }
+    __device__ static float mix(float g, float l0, float post) {
+        return __fadd_rn(__fmul_rn(1.0f - g, l0), __fmul_rn(g, post));
+    }
+};
+template <>
+struct TypedMath<_Float16> {
+    typedef float C;
+    static constexpr double kNoSolution = 60000.0;
+    __device__ static double weight(double acc) { return (double)(_Float16)(float)acc; }
+    // fp16 operands give fp16 operations, each rounded (the file is built without contraction)
+    __device__ static _Float16 mix(_Float16 g, _Float16 l0, _Float16 post) {
+        const _Float16 one = (_Float16)1.0f;
+        return (_Float16)((one - g) * l0) + (_Float16)(g * post);
+    }
+};
+
+// Check pass (c2v_min_sum_btile_kernel, kernels.py:1128-1215) over the group's rows.
+template <typename S>
+__device__ __forceinline__ void typed_check_pass(const TypedArgs &a, const uint8_t *syn, float alpha, const S *mu,
+                                                 S *nu) {
+    typedef typename TypedMath<S>::C C;
+    for (int i = threadIdx.x; i < a.C; i += kTypedBlock) {
+        const int r0 = a.row_ptr[i], r1 = a.row_ptr[i + 1];
+        if (r1 <= r0) continue;
+        int neg = 0, cnt1 = 0;
+        C min1 = (C)1e30, min2 = (C)1e30;  // kernels.py:1120-1126: magnitudes clipped at 1e30
+        for (int e = r0; e < r1; ++e) {
+            const C m = (C)mu[e];
+            const C am = m < (C)0 ? -m : m;
+            neg ^= (m < (C)0);
+            if (am < min1) {
+                min2 = min1;
+                min1 = am;
+                cnt1 = 1;
+            } else if (am == min1) {
+                ++cnt1;
+                min2 = am;
+            } else if (am < min2) {
+                min2 = am;
+            }
+        }
+        const C sign_prod = ((neg & 1) ^ (syn[i] & 1)) ? (C)-1 : (C)1;
+        const bool deg_one = r1 - r0 == 1;
+        for (int e = r0; e < r1; ++e) {
+            const C m = (C)mu[e];
+            const C am = m < (C)0 ? -m : m;
+            C out = (am == min1 && cnt1 == 1) ? min2 : min1;
+            if (deg_one) out = (C)0;
+            if (a.use_alpha) out = (C)alpha * out;
+            if (a.use_beta) out = fmax(out - (C)a.beta, (C)0);
+            const C sgn = m >= (C)0 ? (C)1 : (C)-1;
+            nu[e] = (S)((sign_prod * sgn) * out);
+        }
+    }
+}
+
+// Variable pass (v2c_and_gamma_btile_kernel, kernels.py:1299-1350): memory mix, the incoming
+// messages summed in ascending edge id from 0, posterior, hard decision, outgoing messages.
+template <typename S>
+__device__ __forceinline__ void typed_var_pass(const TypedArgs &a, const S *l0, const S *grow, S gconst, S *post,
+                                               uint8_t *hard, S *mu, const S *nu) {
+    typedef typename TypedMath<S>::C C;
+    for (int j = threadIdx.x; j < a.V; j += kTypedBlock) {
+        const S g = grow ? grow[j] : gconst;
+        const S lm = TypedMath<S>::mix(g, l0[j], post[j]);
+        C s = (C)0;
+        for (int q = a.var_ptr[j]; q < a.var_ptr[j + 1]; ++q) s += (C)nu[a.var_edges[q]];
+        const C pc = (C)lm + s;
+        post[j] = (S)pc;
+        hard[j] = pc <= (C)0;
+        for (int q = a.var_ptr[j]; q < a.var_ptr[j + 1]; ++q) {
+            const int e = a.var_edges[q];
+            mu[e] = (S)((C)lm + (s - (C)nu[e]));
+        }
+    }
+}
+
+struct TypedShot {
+    double best_w;
+    int found, iters;
+};
+
+// One leg of up to T iterations; the shot leaves it at its first valid solution
+// (check_and_select_kernel, freeze_finished_lanes_kernel).  Returns true if it converged.
+template <typename S>
+__device__ bool typed_leg(const TypedArgs &a, const uint8_t *syn, const S *l0, const S *wj, const S *grow, S gconst,
+                          int T, S *mu, S *nu, S *post, uint8_t *hard, uint8_t *best, TypedShot &st, int *s_flag,
+                          double *s_red) {
+    typedef typename TypedMath<S>::C C;
+    for (int t = 0; t < T; ++t) {
+        typed_check_pass<S>(a, syn, a.use_alpha ? a.alpha_t[t] : 1.0f, mu, nu);
+        __syncthreads();
+        typed_var_pass<S>(a, l0, grow, gconst, post, hard, mu, nu);
+        if (threadIdx.x == 0) *s_flag = 0;
+        __syncthreads();
+        st.iters += 1;
+        int bad = 0;
+        for (int i = threadIdx.x; i < a.C; i += kTypedBlock) {
+            int par = 0;
+            for (int e = a.row_ptr[i]; e < a.row_ptr[i + 1]; ++e) par ^= hard[a.row_var[e]];
+            bad |= par != (syn[i] & 1);
+        }
+        if (bad) atomicOr(s_flag, 1);
+        __syncthreads();
+        const int any_bad = *s_flag;
+        __syncthreads();
+        if (any_bad) continue;
+        // the weight of the stored posterior's bits, accumulated in double (exact for these inputs)
+        double acc = 0.0;
+        for (int j = threadIdx.x; j < a.V; j += kTypedBlock)
+            if ((C)post[j] <= (C)0) acc += (double)wj[j];
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        double tot = 0.0;
+        for (int w = 0; w < kTypedBlock / 64; ++w) tot += s_red[w];
+        const double w = TypedMath<S>::weight(tot);
+        const bool better = st.best_w >= TypedMath<S>::kNoSolution || w < st.best_w;
+        if (better) st.best_w = w;
+        for (int j = threadIdx.x; j < a.V; j += kTypedBlock) {
+            if (better) best[j] = (C)post[j] <= (C)0;
+            hard[j] = better ? (uint8_t)((C)post[j] <= (C)0) : best[j];  // freeze: hard <- best_errors
+        }
+        st.found += 1;
+        __syncthreads();
+        return true;
+    }
+    return false;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(kTypedBlock) typed_decode_kernel(TypedArgs a) {
+    __shared__ int s_shot, s_flag;
+    __shared__ double s_red[kTypedBlock / 64];
+    const int V = a.V, E = a.E;
+    uint8_t *slot = a.slots + (int64_t)blockIdx.x * a.slot_stride;
+    S *mu = reinterpret_cast<S *>(slot);
+    S *nu = mu + E;
+    S *post = nu + E;
+    uint8_t *hard = reinterpret_cast<uint8_t *>(post + V);
+    uint8_t *best = hard + V;
+    uint8_t *fhd = best + V;
+    for (;;) {
+        if (threadIdx.x == 0) s_shot = atomicAdd(&a.ctrl[0], 1);
+        __syncthreads();
+        const int b = s_shot;
+        __syncthreads();
+        if (b >= a.B) break;
+        const uint8_t *syn = a.syn + (size_t)b * a.C;
+        const size_t pofs = a.prior_batched ? (size_t)b * V : 0;
+        const S *l0 = static_cast<const S *>(a.prior) + pofs;
+        const S *wj = static_cast<const S *>(a.wj) + pofs;
+        // state reset (decoder.py:3430-3464)
+        for (int j = threadIdx.x; j < V; j += kTypedBlock) {
+            post[j] = a.init_post ? static_cast<const S *>(a.init_post)[(size_t)b * V + j] : l0[j];
+            hard[j] = best[j] = fhd[j] = 0;
+        }
+        TypedShot st{(double)INFINITY, 0, 0};
+        // the pre-iter leg (decoder.py:3470-3598): gamma 0 (plain), init_gamma, or gamma0
+        const S *grow = (!a.plain && a.init_gamma) ? static_cast<const S *>(a.init_gamma) + (size_t)b * V : nullptr;
+        const S gconst = a.plain ? (S)0.0f : (S)a.gamma0;
+        for (int e = threadIdx.x; e < E; e += kTypedBlock) {  // init_messages_with_priors_EB_kernel
+            mu[e] = l0[a.row_var[e]];
+            nu[e] = (S)0.0f;
+        }
+        __syncthreads();
+        if (st.found < a.stop_nconv)
+            typed_leg<S>(a, syn, l0, wj, grow, gconst, a.pre_iter, mu, nu, post, hard, best, st, &s_flag, s_red);
+        for (int j = threadIdx.x; j < V; j += kTypedBlock) fhd[j] = hard[j];  // decoder.py:3597-3598
+        __syncthreads();
+        if (!a.skip_relay_legs) {
+            for (int leg = 0; leg < a.num_sets && st.found < a.stop_nconv; ++leg) {
+                const S *row = static_cast<const S *>(a.gamma_legs) +
+                               (a.gamma_per_row ? ((size_t)leg * a.B + b) * V : (size_t)leg * V);
+                for (int e = threadIdx.x; e < E; e += kTypedBlock) {  // message reset (decoder.py:3729-3739)
+                    mu[e] = l0[a.row_var[e]];
+                    nu[e] = (S)0.0f;
+                }
+                __syncthreads();
+                typed_leg<S>(a, syn, l0, wj, row, (S)0.0f, a.set_max_iter, mu, nu, post, hard, best, st, &s_flag,
+                             s_red);
+                if (leg == 0 && a.pre_iter == 0) {  // decoder.py:3809-3810
+                    for (int j = threadIdx.x; j < V; j += kTypedBlock) fhd[j] = hard[j];
+                    __syncthreads();
+                }
+            }
+        }
+        // output assembly (decoder.py:3841-3873)
+        const uint8_t *src = __builtin_isfinite(st.best_w) ? best : (a.last_leg ? hard : fhd);
+        for (int j = threadIdx.x; j < V; j += kTypedBlock) {
+            a.errors[(size_t)b * V + j] = src[j];
+            if (a.last_post) static_cast<S *>(a.last_post)[(size_t)b * V + j] = post[j];
+        }
+        if (threadIdx.x == 0) {
+            static_cast<S *>(a.weights)[b] = (S)st.best_w;
+            a.iters[b] = st.iters;
+            a.found[b] = st.found;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void typed_clear_kernel(int32_t *ctrl) {
+    if (threadIdx.x == 0) ctrl[0] = 0;
+}
+
 // ---- serving: a persistent launch that takes units as they are published -----------------------
 // The host writes units into a ring in host memory, then raises the published
 // count (and fences, so the stores leave the CPU).  A workgroup's leader claims
@@ -2192,6 +2453,9 @@ struct pel_graph {
     int32_t C, V, E, sb_words, max_row_deg, cs_shift;
     void *dev_mem;  // one allocation holding every array of d
     GraphDev d;
+    // plain CSR for the float16 / float64 decodes: rows (edge id = position) and, per
+    // variable, its edge ids ascending
+    const int32_t *row_ptr, *row_var, *var_ptr, *var_edges;
     StateSizes state() const {
         // each array has one sentinel entry past its last variable or check
         return {{align_up(8LL * (V + 1), 16), 4LL << cs_shift, align_up(4LL * std::max(sb_words, 1), 16),
@@ -2605,6 +2869,14 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
             var_ent[vs[v / kSlice].x + (int64_t)fill[v]++ * kSlice + (v % kSlice)] = make_uint2(4u * (uint32_t)p, sbaddr);
         }
     }
+    // Plain CSR for the float16 / float64 decodes.
+    std::vector<int32_t> cvp(V + 1, 0), cve(std::max<int32_t>(E, 1), 0);
+    for (int32_t e = 0; e < E; ++e) cvp[chk_var[e] + 1]++;
+    for (int32_t j = 0; j < V; ++j) cvp[j + 1] += cvp[j];
+    {
+        std::vector<int32_t> cf(cvp.begin(), cvp.end() - 1);
+        for (int32_t e = 0; e < E; ++e) cve[cf[chk_var[e]]++] = e;  // ascending edge id per variable
+    }
     // One device allocation for every array.
     int64_t off = 0;
     auto take = [&](int64_t bytes) {
@@ -2614,7 +2886,9 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
     };
     const int64_t o_ro = take(4LL * C), o_rd = take(4LL * C), o_rs = take(16LL * nrs),
                   o_re = take(4LL * (int64_t)row_ent.size()), o_vp = take(4LL * V), o_vq = take(4LL * V),
-                  o_vd = take(4LL * V), o_vs = take(16LL * nvs), o_ve = take(8LL * (int64_t)var_ent.size());
+                  o_vd = take(4LL * V), o_vs = take(16LL * nvs), o_ve = take(8LL * (int64_t)var_ent.size()),
+                  o_crp = take(4LL * (C + 1)), o_crv = take(4LL * std::max<int32_t>(E, 1)), o_cvp = take(4LL * (V + 1)),
+                  o_cve = take(4LL * std::max<int32_t>(E, 1));
     const int64_t total = off;
     std::vector<uint8_t> host((size_t)total, 0);
     memcpy(host.data() + o_ro, rorig.data(), 4ULL * C);
@@ -2626,6 +2900,10 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
     memcpy(host.data() + o_vd, vdeg.data(), 4ULL * V);
     memcpy(host.data() + o_vs, vs.data(), 16ULL * nvs);
     memcpy(host.data() + o_ve, var_ent.data(), 8ULL * var_ent.size());
+    memcpy(host.data() + o_crp, chk_ptr, 4ULL * (C + 1));
+    if (E > 0) memcpy(host.data() + o_crv, chk_var, 4ULL * E);
+    memcpy(host.data() + o_cvp, cvp.data(), 4ULL * (V + 1));
+    memcpy(host.data() + o_cve, cve.data(), 4ULL * cve.size());
 
     pel_graph *g = new pel_graph();
     hipError_t e = hipGetDevice(&g->device);
@@ -2662,6 +2940,10 @@ int pel_graph_create(const int32_t *chk_ptr, const int32_t *chk_var, int32_t C, 
     g->d.var_deg = reinterpret_cast<const int32_t *>(base + o_vd);
     g->d.vslice = reinterpret_cast<const int4 *>(base + o_vs);
     g->d.var_ent = reinterpret_cast<const uint2 *>(base + o_ve);
+    g->row_ptr = reinterpret_cast<const int32_t *>(base + o_crp);
+    g->row_var = reinterpret_cast<const int32_t *>(base + o_crv);
+    g->var_ptr = reinterpret_cast<const int32_t *>(base + o_cvp);
+    g->var_edges = reinterpret_cast<const int32_t *>(base + o_cve);
     *out = g;
     return PEL_OK;
 }
@@ -3372,6 +3654,106 @@ int64_t pel_serve_done_count(const pel_serve_queue *q, int64_t first, int64_t n)
     for (int64_t k = first; k < first + n; ++k)  // a later unit in the entry implies this one was done
         done += __atomic_load_n(q->done() + k % q->capacity, __ATOMIC_ACQUIRE) >= k + 1;
     return done;
+}
+
+}  // extern "C"
+
+// ---- float16 / float64 decodes -------------------------------------------------------------------
+namespace {
+
+int64_t typed_elem_bytes(int32_t dtype) { return dtype == PEL_DTYPE_F16 ? 2 : dtype == PEL_DTYPE_F32 ? 4 : 8; }
+
+int64_t typed_slot_bytes(const pel_graph *g, int32_t dtype) {
+    return align_up((2LL * g->E + g->V) * typed_elem_bytes(dtype) + 3LL * g->V, 256);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pel_plan_typed(const pel_graph *g, int32_t B, int32_t dtype, int32_t *grid, int64_t *workspace_bytes) {
+    if (!g || !grid || !workspace_bytes) return fail(PEL_EINVAL, "pel_plan_typed: NULL argument");
+    if (dtype != PEL_DTYPE_F16 && dtype != PEL_DTYPE_F32 && dtype != PEL_DTYPE_F64)
+        return fail(PEL_EINVAL, "pel_plan_typed: dtype must be PEL_DTYPE_F16, PEL_DTYPE_F32 or PEL_DTYPE_F64");
+    if (B < 0) return fail(PEL_EINVAL, "pel_plan_typed: negative batch size");
+    int dev = 0, n_cu = 0;
+    PEL_HIP(hipGetDevice(&dev));
+    PEL_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    *grid = std::max(1, std::min<int32_t>(std::max(B, 1), 2 * n_cu));
+    *workspace_bytes = 256 + (int64_t)*grid * typed_slot_bytes(g, dtype);
+    return PEL_OK;
+}
+
+int pel_decode_typed(const pel_graph *g, const pel_params *p, const pel_batch *b, int32_t dtype, int32_t grid,
+                     void *stream) {
+    const std::string w_ = "pel_decode_typed";
+    if (!g || !p || !b) return fail(PEL_EINVAL, w_ + ": NULL argument");
+    int32_t want_grid = 0;
+    int64_t need = 0;
+    int rc = pel_plan_typed(g, b->B, dtype, &want_grid, &need);
+    if (rc != PEL_OK) return rc;
+    if (b->B == 0) return PEL_OK;
+    if (grid < 1 || grid > want_grid) return fail(PEL_EINVAL, w_ + ": grid must lie in 1 .. the planned grid");
+    if (!b->syndromes || !b->prior_llr || !b->wj || !b->errors || !b->weights || !b->iterations || !b->found_count ||
+        !b->workspace || (p->num_sets > 0 && !p->skip_relay_legs && !b->gamma_legs) || (p->use_alpha && !p->alpha_t))
+        return fail(PEL_EINVAL, w_ + ": NULL buffer");
+    if (p->pre_iter < 0 || p->num_sets < 0 || p->set_max_iter < 0 || p->stop_nconv < 1)
+        return fail(PEL_EINVAL, w_ + ": bad schedule parameters");
+    if (b->mlp || b->feat_out || b->hard_out || b->gamma_out || b->learned_legs)
+        return fail(PEL_EINVAL, w_ + ": learned strengths run in float32 only (pel_decode)");
+    if (p->leg_step != 0) return fail(PEL_EINVAL, w_ + ": leg-stepped decodes run in float32 only (pel_decode)");
+    if (b->gamma_by_position) return fail(PEL_EINVAL, w_ + ": gamma rows must be in the caller's variable order");
+    TypedArgs a{};
+    a.C = g->C;
+    a.V = g->V;
+    a.E = g->E;
+    a.B = b->B;
+    a.row_ptr = g->row_ptr;
+    a.row_var = g->row_var;
+    a.var_ptr = g->var_ptr;
+    a.var_edges = g->var_edges;
+    a.pre_iter = p->pre_iter;
+    a.num_sets = p->num_sets;
+    a.set_max_iter = p->set_max_iter;
+    a.stop_nconv = p->stop_nconv;
+    a.plain = p->plain;
+    a.use_alpha = p->use_alpha;
+    a.use_beta = p->use_beta;
+    a.last_leg = p->last_leg;
+    a.skip_relay_legs = p->skip_relay_legs;
+    a.gamma0 = p->gamma0;
+    a.beta = p->beta;
+    a.alpha_t = p->alpha_t;
+    a.syn = b->syndromes;
+    a.prior = b->prior_llr;
+    a.wj = b->wj;
+    a.prior_batched = b->prior_batched;
+    a.gamma_legs = b->gamma_legs;
+    a.gamma_per_row = b->gamma_per_row;
+    a.init_gamma = b->init_gamma;
+    a.init_post = b->init_posterior;
+    a.errors = b->errors;
+    a.weights = b->weights;
+    a.iters = b->iterations;
+    a.found = b->found_count;
+    a.last_post = b->last_posterior;
+    uint8_t *ws = static_cast<uint8_t *>(b->workspace);
+    a.ctrl = reinterpret_cast<int32_t *>(ws);
+    a.slots = ws + 256;
+    a.slot_stride = typed_slot_bytes(g, dtype);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(typed_clear_kernel, dim3(1), dim3(64), 0, s, a.ctrl);
+    PEL_HIP(hipGetLastError());
+    if (b->ev_begin) PEL_HIP(hipEventRecord(static_cast<hipEvent_t>(b->ev_begin), s));
+    if (dtype == PEL_DTYPE_F16)
+        hipLaunchKernelGGL(typed_decode_kernel<_Float16>, dim3(grid), dim3(kTypedBlock), 0, s, a);
+    else if (dtype == PEL_DTYPE_F32)
+        hipLaunchKernelGGL(typed_decode_kernel<float>, dim3(grid), dim3(kTypedBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL(typed_decode_kernel<double>, dim3(grid), dim3(kTypedBlock), 0, s, a);
+    PEL_HIP(hipGetLastError());
+    if (b->ev_end) PEL_HIP(hipEventRecord(static_cast<hipEvent_t>(b->ev_end), s));
+    return PEL_OK;
 }
 
 }  // extern "C"

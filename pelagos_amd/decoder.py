@@ -248,12 +248,15 @@ class RelayBPDecoder:
             raise ValueError(
                 f"data_type must be 'float16', 'float32', or 'float64', got {data_type}"
             )
-        if data_type != "float32":
-            raise NotImplementedError(
-                f"data_type={data_type!r}: the MI355X decoder computes in float32 only"
-            )
         self.dtype_str = data_type
-        self.dtype = torch.float32
+        self.dtype = {"float16": torch.float16, "float32": torch.float32, "float64": torch.float64}[data_type]
+        # float16 / float64 run the per-edge kernel (pel_decode_typed), float32 the fast one
+        self._typed = {"float16": _native.PEL_DTYPE_F16, "float64": _native.PEL_DTYPE_F64}.get(data_type)
+        if self._typed is not None and fast_max_unconverged_pct is not None:
+            raise NotImplementedError(
+                f"fast_max_unconverged_pct with data_type={data_type!r}: the leg-synchronous early exit "
+                "runs in float32 only"
+            )
 
         self.device_ids = self._parse_device_spec(device)
         self.num_gpus = len(self.device_ids)
@@ -359,6 +362,9 @@ class RelayBPDecoder:
             raise ValueError("learned_last_damp must lie in [0, 1]")
         self.learned_params: Optional[torch.Tensor] = None
         if learned_gamma is not None:
+            if self._typed is not None:
+                raise NotImplementedError(f"learned_gamma with data_type={self.dtype_str!r}: the network's "
+                                          "strengths feed the float32 kernel only")
             prm = getattr(learned_gamma, "params", learned_gamma)
             prm = torch.as_tensor(prm, dtype=torch.float32).reshape(-1)
             if prm.numel() != _native.PEL_MLP_PARAMS:
@@ -462,6 +468,11 @@ class RelayBPDecoder:
             t = torch.from_numpy(a).to(self.device)
             self._alpha_cache[n] = t
         return t
+
+    def _float32_only(self, what: str) -> None:
+        if self._typed is not None:
+            raise NotImplementedError(f"{what} runs the float32 kernels; data_type={self.dtype_str!r} "
+                                      "decodes with decode()")
 
     def plan(self, B: int, learned: bool = False, legwise: bool = False) -> _native.PelPlan:
         if legwise:
@@ -641,7 +652,12 @@ class RelayBPDecoder:
                 if self.record_learned_gammas:
                     self.learned_gammas = torch.zeros(B, self.learned_legs, V, dtype=self.dtype, device=dev)
                     learn["gamma_out"] = self.learned_gammas
-            if B > 0 and legwise:
+            if B > 0 and self._typed is not None:
+                if _records is not None:
+                    raise NotImplementedError("traces of the learned network are recorded in float32 only")
+                self._launch_typed(syn, prior, wj, prior_llr is not None, gtab, ig, ip, errors, weights, iters,
+                                   found, last_post, pre_iter, skip_relay_legs, B)
+            elif B > 0 and legwise:
                 self._decode_legwise(syn, prior, wj, prior_llr is not None, gtab, ig, ip, errors, weights,
                                      iters, found, last_post, pre_iter, B, learn)
             elif B > 0:
@@ -659,6 +675,7 @@ class RelayBPDecoder:
         """Build everything a decode of B shots needs (launch plan, gamma rows in
         position order, alpha table) so that decode_async makes no allocation
         and no host synchronisation.  Returns the workspace bytes per launch."""
+        self._float32_only("prepare")
         learned = self.learned_params is not None and self.learned_legs > 0
         plan = self.plan(B, learned)
         gtab = self.gamma_table(B)
@@ -679,6 +696,7 @@ class RelayBPDecoder:
         ``workspace=None`` the launch uses the library's own per-GPU workspace
         (``pel_decode_noalloc``, at most PEL_NOALLOC_MAX_BYTES; such calls
         must not overlap on one GPU)."""
+        self._float32_only("decode_async")
         B = int(syndromes.shape[0])
         # the buffers are handed to the kernel as raw pointers: check what it assumes
         for name, t, shape, dtype in (("syndromes", syndromes, (B, self.C), torch.uint8),
@@ -757,8 +775,34 @@ class RelayBPDecoder:
         if ev is not None:
             ev.append((e0, e1))
 
+    def _launch_typed(self, syn, prior, wj, prior_batched, gtab, ig, ip, errors, weights, iters, found,
+                      last_post, pre_iter, skip_relay_legs, B) -> None:
+        """A float16 / float64 decode (pel_decode_typed): buffers of this decoder's dtype, gamma rows in
+        the caller's variable order."""
+        lib = _native.lib()
+        grid = ctypes.c_int32()
+        nbytes = ctypes.c_int64()
+        _native.check(lib.pel_plan_typed(self._handle.h, B, self._typed, ctypes.byref(grid), ctypes.byref(nbytes)),
+                      "pel_plan_typed")
+        ws = self._workspace_for(int(nbytes.value))
+        params, batch = self._args(syn, prior, wj, prior_batched, gtab, ig, ip, errors, weights, iters, found,
+                                   last_post, pre_iter, skip_relay_legs, B, None, ws, by_position=False)
+        cur = torch.cuda.current_stream(self.device)
+        ev = self.kernel_events
+        if ev is not None:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(cur)
+            e1.record(cur)
+            batch.ev_begin = e0.cuda_event
+            batch.ev_end = e1.cuda_event
+        _native.check(lib.pel_decode_typed(self._handle.h, ctypes.byref(params), ctypes.byref(batch), self._typed,
+                                           grid.value, ctypes.c_void_p(cur.cuda_stream)), "pel_decode_typed")
+        if ev is not None:
+            ev.append((e0, e1))
+
     def _args(self, syn, prior, wj, prior_batched, gtab, ig, ip, errors, weights, iters, found, last_post,
-              pre_iter, skip_relay_legs, B, learn, ws):
+              pre_iter, skip_relay_legs, B, learn, ws, by_position: bool = True):
         """The (pel_params, pel_batch) pair of one decode."""
         per_row = gtab.dim() == 3
         # the kernel reads rows 0 .. num_sets-1 (and, per row, B shots) of the table
@@ -766,7 +810,7 @@ class RelayBPDecoder:
                 or gtab.shape[-1] != self.V:
             raise ValueError(f"gamma table of shape {tuple(gtab.shape)} does not cover num_sets={self.num_sets}, "
                              f"B={B}, V={self.V}")
-        if not per_row:
+        if not per_row and by_position:
             gtab = self._gamma_by_position(gtab)
         params = _native.PelParams(
             pre_iter=pre_iter,
@@ -790,7 +834,7 @@ class RelayBPDecoder:
             prior_batched=int(prior_batched),
             gamma_legs=gtab.data_ptr(),
             gamma_per_row=int(per_row),
-            gamma_by_position=int(not per_row),
+            gamma_by_position=int(not per_row and by_position),
             init_gamma=_ptr(ig),
             init_posterior=_ptr(ip),
             errors=errors.data_ptr(),
@@ -817,6 +861,7 @@ class RelayBPDecoder:
         """(pel_params, pel_batch) of one job of a multi-job launch (``pel_decode_jobs``):
         a decode of device-resident ``syndromes`` into caller-owned buffers, as
         ``decode_async`` makes it.  Call prepare(B) first."""
+        self._float32_only("job_args")
         B = int(syndromes.shape[0])
         learn = None
         if self.learned_params is not None and self.learned_legs > 0:
