@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define PEL_ABI_VERSION 10
+#define PEL_ABI_VERSION 11
 
 /* status codes */
 #define PEL_OK 0
@@ -325,20 +325,24 @@ int pel_serve_launch(const pel_graph *const *graphs, const pel_params *params, c
  * (caller's variable order, gamma_by_position = 0), init_gamma, init_posterior, weights and
  * last_posterior hold elements of that type; pel_params keeps its float32 scalars (gamma0, beta,
  * alpha_t), as the reference passes them.  The reference's own per-edge formulation: mu and nu of
- * every edge in a workspace slot per workgroup, one shot per workgroup at a time, the full
- * Relay-BP-S schedule of pel_decode (no learned strengths, no leg-stepped decodes).  PEL_DTYPE_F32
- * is accepted too and gives pel_decode's results. */
+ * every edge in a workspace slot per workgroup, one shot per workgroup at a time, the Relay-BP-S
+ * schedule of pel_decode, leg-stepped decodes (pel_params.leg_step, plans made with
+ * PEL_PLAN_LEGWISE) included; no learned strengths.  PEL_DTYPE_F32 is accepted too and gives
+ * pel_decode's results. */
 #define PEL_DTYPE_F16 0
 #define PEL_DTYPE_F32 1
 #define PEL_DTYPE_F64 2
 
 /* Launch size for B shots of element type dtype: *grid workgroups and the workspace bytes
- * pel_decode_typed needs (a queue word, then one state slot per workgroup). */
-int pel_plan_typed(const pel_graph *g, int32_t B, int32_t dtype, int32_t *grid, int64_t *workspace_bytes);
+ * pel_decode_typed needs (a queue word, one state slot per workgroup and, with PEL_PLAN_LEGWISE in
+ * flags, every shot's parked state for leg-stepped decodes). */
+int pel_plan_typed(const pel_graph *g, int32_t B, int32_t dtype, int32_t flags, int32_t *grid,
+                   int64_t *workspace_bytes);
 
 /* Decode a batch whose floating-point buffers hold elements of type dtype, with grid workgroups
- * (1 .. the planned grid) and b->workspace of the planned size.  Replaces the dtype_enum 0 and 2
- * instantiations of the kernels RelayBPDecoder._decode_loop_single_gpu launches
+ * (1 .. the planned grid) and b->workspace of the planned size (planned with PEL_PLAN_LEGWISE for
+ * p->leg_step > 0; calls k = 1, 2, ... must use the same buffers, workspace and grid).  Replaces
+ * the dtype_enum 0 and 2 instantiations of the kernels RelayBPDecoder._decode_loop_single_gpu launches
  * (decoder.py:3394-3873, kernels.py:446-1350). */
 int pel_decode_typed(const pel_graph *g, const pel_params *p, const pel_batch *b, int32_t dtype, int32_t grid,
                      void *stream);

@@ -13,7 +13,7 @@ from typing import Optional
 
 LIB_PATH = Path(__file__).resolve().parent / "libpelagos_relaybp.so"
 
-PEL_ABI_VERSION = 10
+PEL_ABI_VERSION = 11
 PEL_MAX_JOBS = 64
 PEL_NOALLOC_MAX_BYTES = 32 << 20
 PEL_PLAN_LEARNED = 1
@@ -157,8 +157,8 @@ SIGNATURES = {
     "pel_plan_serve": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.POINTER(PelJobsPlan)]),
     "pel_serve_launch": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.POINTER(PelJobsPlan), _vp, _vp,
                                         ctypes.c_int64, _vp]),
-    "pel_plan_typed": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
-                                      ctypes.POINTER(ctypes.c_int64)]),
+    "pel_plan_typed": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]),
     "pel_decode_typed": (ctypes.c_int, [_vp, ctypes.POINTER(PelParams), ctypes.POINTER(PelBatch), ctypes.c_int32,
                                         ctypes.c_int32, _vp]),
     "pel_mlp_param_count": (ctypes.c_int, []),

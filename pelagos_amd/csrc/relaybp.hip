@@ -1825,7 +1825,15 @@ struct TypedArgs {
     int32_t *ctrl;                       // [0] queue ticket counter (cleared by typed_clear_kernel)
     uint8_t *slots;
     int64_t slot_stride;
+    int32_t leg_step;                    // 0: the whole schedule; k >= 1: only leg k - 1 (0 the pre-iter
+                                         // leg), for shots short of stop_nconv solutions (as pel_params)
+    uint8_t *park;                       // leg-stepped decodes: [B] shot states between launches
+    int64_t park_stride;
 };
+
+// A parked shot (leg-stepped decodes): best weight, found and iteration counts, then its
+// posterior (V elements) and its hard, best and first-leg bits (V bytes each).
+constexpr int kTypedParkHead = 16;
 
 template <typename S>
 struct TypedMath {                 // float64: double arithmetic throughout
@@ -1999,20 +2007,43 @@ __global__ void __launch_bounds__(kTypedBlock) typed_decode_kernel(TypedArgs a) 
             hard[j] = best[j] = fhd[j] = 0;
         }
         TypedShot st{(double)INFINITY, 0, 0};
-        // the pre-iter leg (decoder.py:3470-3598): gamma 0 (plain), init_gamma, or gamma0
-        const S *grow = (!a.plain && a.init_gamma) ? static_cast<const S *>(a.init_gamma) + (size_t)b * V : nullptr;
-        const S gconst = a.plain ? (S)0.0f : (S)a.gamma0;
-        for (int e = threadIdx.x; e < E; e += kTypedBlock) {  // init_messages_with_priors_EB_kernel
-            mu[e] = l0[a.row_var[e]];
-            nu[e] = (S)0.0f;
+        uint8_t *pk = a.leg_step > 0 ? a.park + (int64_t)b * a.park_stride : nullptr;
+        S *ppost = pk ? reinterpret_cast<S *>(pk + kTypedParkHead) : nullptr;
+        uint8_t *pbits = pk ? reinterpret_cast<uint8_t *>(ppost + V) : nullptr;
+        int leg_begin = 0, leg_end = a.num_sets;
+        if (a.leg_step <= 1) {
+            // the pre-iter leg (decoder.py:3470-3598): gamma 0 (plain), init_gamma, or gamma0
+            const S *grow = (!a.plain && a.init_gamma) ? static_cast<const S *>(a.init_gamma) + (size_t)b * V : nullptr;
+            const S gconst = a.plain ? (S)0.0f : (S)a.gamma0;
+            for (int e = threadIdx.x; e < E; e += kTypedBlock) {  // init_messages_with_priors_EB_kernel
+                mu[e] = l0[a.row_var[e]];
+                nu[e] = (S)0.0f;
+            }
+            __syncthreads();
+            if (st.found < a.stop_nconv)
+                typed_leg<S>(a, syn, l0, wj, grow, gconst, a.pre_iter, mu, nu, post, hard, best, st, &s_flag, s_red);
+            for (int j = threadIdx.x; j < V; j += kTypedBlock) fhd[j] = hard[j];  // decoder.py:3597-3598
+            __syncthreads();
+            if (a.leg_step == 1) leg_end = 0;
+        } else {
+            // relay leg leg_step - 2 of a leg-stepped decode, from the state the last launch parked
+            st.best_w = *reinterpret_cast<const double *>(pk);
+            st.found = *reinterpret_cast<const int32_t *>(pk + 8);
+            st.iters = *reinterpret_cast<const int32_t *>(pk + 12);
+            __syncthreads();  // every thread has read the scalars before any could change them
+            if (st.found >= a.stop_nconv) continue;  // finished in an earlier launch: outputs final
+            for (int j = threadIdx.x; j < V; j += kTypedBlock) {
+                post[j] = ppost[j];
+                hard[j] = pbits[j];
+                best[j] = pbits[V + j];
+                fhd[j] = pbits[2 * V + j];
+            }
+            __syncthreads();
+            leg_begin = a.leg_step - 2;
+            leg_end = leg_begin + 1;
         }
-        __syncthreads();
-        if (st.found < a.stop_nconv)
-            typed_leg<S>(a, syn, l0, wj, grow, gconst, a.pre_iter, mu, nu, post, hard, best, st, &s_flag, s_red);
-        for (int j = threadIdx.x; j < V; j += kTypedBlock) fhd[j] = hard[j];  // decoder.py:3597-3598
-        __syncthreads();
         if (!a.skip_relay_legs) {
-            for (int leg = 0; leg < a.num_sets && st.found < a.stop_nconv; ++leg) {
+            for (int leg = leg_begin; leg < min(a.num_sets, leg_end) && st.found < a.stop_nconv; ++leg) {
                 const S *row = static_cast<const S *>(a.gamma_legs) +
                                (a.gamma_per_row ? ((size_t)leg * a.B + b) * V : (size_t)leg * V);
                 for (int e = threadIdx.x; e < E; e += kTypedBlock) {  // message reset (decoder.py:3729-3739)
@@ -2028,16 +2059,28 @@ __global__ void __launch_bounds__(kTypedBlock) typed_decode_kernel(TypedArgs a) 
                 }
             }
         }
-        // output assembly (decoder.py:3841-3873)
+        // output assembly (decoder.py:3841-3873); a leg-stepped launch leaves them as if the decode
+        // ended after its leg, and parks the state for the next one
         const uint8_t *src = __builtin_isfinite(st.best_w) ? best : (a.last_leg ? hard : fhd);
         for (int j = threadIdx.x; j < V; j += kTypedBlock) {
             a.errors[(size_t)b * V + j] = src[j];
             if (a.last_post) static_cast<S *>(a.last_post)[(size_t)b * V + j] = post[j];
+            if (pk) {
+                ppost[j] = post[j];
+                pbits[j] = hard[j];
+                pbits[V + j] = best[j];
+                pbits[2 * V + j] = fhd[j];
+            }
         }
         if (threadIdx.x == 0) {
             static_cast<S *>(a.weights)[b] = (S)st.best_w;
             a.iters[b] = st.iters;
             a.found[b] = st.found;
+            if (pk) {
+                *reinterpret_cast<double *>(pk) = st.best_w;
+                *reinterpret_cast<int32_t *>(pk + 8) = st.found;
+                *reinterpret_cast<int32_t *>(pk + 12) = st.iters;
+            }
         }
         __syncthreads();
     }
@@ -3667,20 +3710,27 @@ int64_t typed_slot_bytes(const pel_graph *g, int32_t dtype) {
     return align_up((2LL * g->E + g->V) * typed_elem_bytes(dtype) + 3LL * g->V, 256);
 }
 
+int64_t typed_park_bytes(const pel_graph *g, int32_t dtype) {
+    return align_up(kTypedParkHead + (int64_t)g->V * typed_elem_bytes(dtype) + 3LL * g->V, 16);
+}
+
 }  // namespace
 
 extern "C" {
 
-int pel_plan_typed(const pel_graph *g, int32_t B, int32_t dtype, int32_t *grid, int64_t *workspace_bytes) {
+int pel_plan_typed(const pel_graph *g, int32_t B, int32_t dtype, int32_t flags, int32_t *grid,
+                   int64_t *workspace_bytes) {
     if (!g || !grid || !workspace_bytes) return fail(PEL_EINVAL, "pel_plan_typed: NULL argument");
     if (dtype != PEL_DTYPE_F16 && dtype != PEL_DTYPE_F32 && dtype != PEL_DTYPE_F64)
         return fail(PEL_EINVAL, "pel_plan_typed: dtype must be PEL_DTYPE_F16, PEL_DTYPE_F32 or PEL_DTYPE_F64");
     if (B < 0) return fail(PEL_EINVAL, "pel_plan_typed: negative batch size");
+    if (flags & ~PEL_PLAN_LEGWISE) return fail(PEL_EINVAL, "pel_plan_typed: flags other than PEL_PLAN_LEGWISE");
     int dev = 0, n_cu = 0;
     PEL_HIP(hipGetDevice(&dev));
     PEL_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
     *grid = std::max(1, std::min<int32_t>(std::max(B, 1), 2 * n_cu));
-    *workspace_bytes = 256 + (int64_t)*grid * typed_slot_bytes(g, dtype);
+    *workspace_bytes = 256 + (int64_t)*grid * typed_slot_bytes(g, dtype) +
+                       ((flags & PEL_PLAN_LEGWISE) ? (int64_t)B * typed_park_bytes(g, dtype) : 0);
     return PEL_OK;
 }
 
@@ -3690,7 +3740,7 @@ int pel_decode_typed(const pel_graph *g, const pel_params *p, const pel_batch *b
     if (!g || !p || !b) return fail(PEL_EINVAL, w_ + ": NULL argument");
     int32_t want_grid = 0;
     int64_t need = 0;
-    int rc = pel_plan_typed(g, b->B, dtype, &want_grid, &need);
+    int rc = pel_plan_typed(g, b->B, dtype, p->leg_step > 0 ? PEL_PLAN_LEGWISE : 0, &want_grid, &need);
     if (rc != PEL_OK) return rc;
     if (b->B == 0) return PEL_OK;
     if (grid < 1 || grid > want_grid) return fail(PEL_EINVAL, w_ + ": grid must lie in 1 .. the planned grid");
@@ -3701,7 +3751,10 @@ int pel_decode_typed(const pel_graph *g, const pel_params *p, const pel_batch *b
         return fail(PEL_EINVAL, w_ + ": bad schedule parameters");
     if (b->mlp || b->feat_out || b->hard_out || b->gamma_out || b->learned_legs)
         return fail(PEL_EINVAL, w_ + ": learned strengths run in float32 only (pel_decode)");
-    if (p->leg_step != 0) return fail(PEL_EINVAL, w_ + ": leg-stepped decodes run in float32 only (pel_decode)");
+    if (p->leg_step < 0 || p->leg_step > p->num_sets + 1)
+        return fail(PEL_EINVAL, w_ + ": leg_step must lie in 0 .. num_sets + 1");
+    if (p->leg_step > 0 && p->skip_relay_legs)
+        return fail(PEL_EINVAL, w_ + ": leg-stepped decodes run the relay legs");
     if (b->gamma_by_position) return fail(PEL_EINVAL, w_ + ": gamma rows must be in the caller's variable order");
     TypedArgs a{};
     a.C = g->C;
@@ -3741,6 +3794,9 @@ int pel_decode_typed(const pel_graph *g, const pel_params *p, const pel_batch *b
     a.ctrl = reinterpret_cast<int32_t *>(ws);
     a.slots = ws + 256;
     a.slot_stride = typed_slot_bytes(g, dtype);
+    a.leg_step = p->leg_step;
+    a.park = p->leg_step > 0 ? a.slots + (int64_t)grid * a.slot_stride : nullptr;
+    a.park_stride = typed_park_bytes(g, dtype);
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(typed_clear_kernel, dim3(1), dim3(64), 0, s, a.ctrl);
     PEL_HIP(hipGetLastError());

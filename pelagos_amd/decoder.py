@@ -252,11 +252,6 @@ class RelayBPDecoder:
         self.dtype = {"float16": torch.float16, "float32": torch.float32, "float64": torch.float64}[data_type]
         # float16 / float64 run the per-edge kernel (pel_decode_typed), float32 the fast one
         self._typed = {"float16": _native.PEL_DTYPE_F16, "float64": _native.PEL_DTYPE_F64}.get(data_type)
-        if self._typed is not None and fast_max_unconverged_pct is not None:
-            raise NotImplementedError(
-                f"fast_max_unconverged_pct with data_type={data_type!r}: the leg-synchronous early exit "
-                "runs in float32 only"
-            )
 
         self.device_ids = self._parse_device_spec(device)
         self.num_gpus = len(self.device_ids)
@@ -655,8 +650,12 @@ class RelayBPDecoder:
             if B > 0 and self._typed is not None:
                 if _records is not None:
                     raise NotImplementedError("traces of the learned network are recorded in float32 only")
-                self._launch_typed(syn, prior, wj, prior_llr is not None, gtab, ig, ip, errors, weights, iters,
-                                   found, last_post, pre_iter, skip_relay_legs, B)
+                typed = (syn, prior, wj, prior_llr is not None, gtab, ig, ip, errors, weights, iters, found,
+                         last_post, pre_iter, skip_relay_legs, B)
+                if legwise:
+                    self._each_leg(B, found, lambda k: self._launch_typed(*typed, leg_step=k))
+                else:
+                    self._launch_typed(*typed)
             elif B > 0 and legwise:
                 self._decode_legwise(syn, prior, wj, prior_llr is not None, gtab, ig, ip, errors, weights,
                                      iters, found, last_post, pre_iter, B, learn)
@@ -776,17 +775,19 @@ class RelayBPDecoder:
             ev.append((e0, e1))
 
     def _launch_typed(self, syn, prior, wj, prior_batched, gtab, ig, ip, errors, weights, iters, found,
-                      last_post, pre_iter, skip_relay_legs, B) -> None:
+                      last_post, pre_iter, skip_relay_legs, B, leg_step: int = 0) -> None:
         """A float16 / float64 decode (pel_decode_typed): buffers of this decoder's dtype, gamma rows in
-        the caller's variable order."""
+        the caller's variable order; leg_step k > 0 runs leg k - 1 only (as _launch)."""
         lib = _native.lib()
         grid = ctypes.c_int32()
         nbytes = ctypes.c_int64()
-        _native.check(lib.pel_plan_typed(self._handle.h, B, self._typed, ctypes.byref(grid), ctypes.byref(nbytes)),
-                      "pel_plan_typed")
+        flags = _native.PEL_PLAN_LEGWISE if leg_step > 0 else 0
+        _native.check(lib.pel_plan_typed(self._handle.h, B, self._typed, flags, ctypes.byref(grid),
+                                         ctypes.byref(nbytes)), "pel_plan_typed")
         ws = self._workspace_for(int(nbytes.value))
         params, batch = self._args(syn, prior, wj, prior_batched, gtab, ig, ip, errors, weights, iters, found,
                                    last_post, pre_iter, skip_relay_legs, B, None, ws, by_position=False)
+        params.leg_step = int(leg_step)
         cur = torch.cuda.current_stream(self.device)
         ev = self.kernel_events
         if ev is not None:
@@ -895,10 +896,17 @@ class RelayBPDecoder:
         reference's): the decode stops when every shot has stop_nconv solutions,
         or when the fraction without them is below fast_max_unconverged_pct (and
         below 1)."""
+        self._each_leg(B, found, lambda k: self._launch(syn, prior, wj, prior_batched, gtab, ig, ip, errors,
+                                                        weights, iters, found, last_post, pre_iter, False, B,
+                                                        learn, leg_step=k))
+
+    def _each_leg(self, B, found, launch) -> None:
+        """Drive leg-stepped launches launch(1), launch(2), ... (the pre-iter leg, then relay legs
+        0, 1, ...) until the fast_max_unconverged_pct exit on the found counts (decoder.py:2890-2937)
+        or the last leg."""
         pct = float(self.fast_max_unconverged_pct)
         for k in range(self.num_sets + 1):
-            self._launch(syn, prior, wj, prior_batched, gtab, ig, ip, errors, weights, iters, found, last_post,
-                         pre_iter, False, B, learn, leg_step=k + 1)
+            launch(k + 1)
             if k == self.num_sets:
                 break
             unconverged = int((found < self.stop_nconv).sum()) / B

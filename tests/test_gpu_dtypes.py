@@ -147,7 +147,7 @@ def test_float32_per_edge_kernel_equals_the_fast_kernel():
     lib = _native.lib()
     grid = ctypes.c_int32()
     nbytes = ctypes.c_int64()
-    _native.check(lib.pel_plan_typed(dec._handle.h, B, _native.PEL_DTYPE_F32, ctypes.byref(grid),
+    _native.check(lib.pel_plan_typed(dec._handle.h, B, _native.PEL_DTYPE_F32, 0, ctypes.byref(grid),
                                      ctypes.byref(nbytes)), "pel_plan_typed")
     ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device="cuda:0")
     out = {k: torch.empty_like(fast[k]) for k in ("errors", "weights", "iterations", "found_count", "last_posterior")}
@@ -166,8 +166,56 @@ def test_float32_per_edge_kernel_equals_the_fast_kernel():
 def test_typed_decoders_refuse_float32_only_features():
     H = codes.bb_72_12_6()
     p = np.full(72, 0.03)
-    with pytest.raises(NotImplementedError):
-        _decoder(H, p, "float16", fast_max_unconverged_pct=0.1)
     dec = _decoder(H, p, "float64")
     with pytest.raises(NotImplementedError):
         dec.prepare(16)
+
+
+# --- fast_max_unconverged_pct: leg-stepped launches of the per-edge kernel ------------------
+
+
+def _regular_case():
+    H = toy_codes.regular_code(3)
+    V = H.shape[1]
+    priors = np.random.default_rng(4).uniform(0.01, 0.03, V)
+    E = (np.random.default_rng(8).random((96, V)) < priors * np.repeat([1.0, 2.0, 3.0], 32)[:, None])
+    return H, priors, codes.syndromes_of(H, E.astype(np.uint8))
+
+
+def _unconverged(out, S):
+    return float((out["found_count"] < S).float().mean())
+
+
+@pytest.mark.parametrize("correction", ["first_leg", "last_leg"])
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_early_exit_after_relay_leg_one_equals_a_one_leg_decode(dtype, correction):
+    """With the threshold between the unconverged fractions after the pre-iter
+    leg and after relay leg 1, the leg-stepped decode exits after relay leg 1:
+    its results equal a decode with num_sets = 1 and the oracle's, bit for bit
+    (tests/test_unconverged_correction.py:105-121, 177-194)."""
+    H, priors, S = _regular_case()
+    kw = dict(pre_iter=2, set_max_iter=15, stop_nconv=1, seed=7, unconverged_correction=correction)
+    leg0 = _run(_decoder(H, priors, dtype, num_sets=0, **kw), S)
+    one = _decoder(H, priors, dtype, num_sets=1, **kw)
+    leg1 = _run(one, S)
+    u0, u1 = _unconverged(leg0, 1), _unconverged(leg1, 1)
+    assert u1 < u0 < 1.0, (u0, u1)
+    out = _run(_decoder(H, priors, dtype, num_sets=6, fast_max_unconverged_pct=(u0 + u1) / 2, **kw), S)
+    assert len(out["unconverged_indices"]) > 0
+    for k in KEYS + ("valid_mask",):
+        assert torch.equal(out[k], leg1[k]), k
+    _compare(out, _oracle_for(one, S))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_early_exit_threshold_zero_runs_the_whole_schedule(dtype):
+    """A threshold of 0 never exits early: the leg-stepped decode equals the
+    whole-schedule decode and the oracle, bit for bit."""
+    H, priors, S = _regular_case()
+    kw = dict(pre_iter=8, num_sets=6, set_max_iter=8, stop_nconv=3, seed=7)
+    whole_dec = _decoder(H, priors, dtype, **kw)
+    whole = _run(whole_dec, S)
+    stepped = _run(_decoder(H, priors, dtype, fast_max_unconverged_pct=0.0, **kw), S)
+    for k in KEYS:
+        assert torch.equal(stepped[k], whole[k]), k
+    _compare(stepped, _oracle_for(whole_dec, S))
