@@ -389,6 +389,8 @@ def paired_ler(decs: dict, H, p, batches: int, shots: int, seed0: int) -> dict:
             bad = (~conv) | ((O @ (e ^ err).T.astype(np.int64)) % 2).any(0)
             fails[name] = np.concatenate([fails[name], bad])
             iters[name] += int(out["iterations"].long().sum())
+        if (k + 1) % 64 == 0 and k + 1 < batches:
+            print(f"[bench] paired error rates: {k + 1} of {batches} batches", file=sys.stderr, flush=True)
     n = batches * shots
     names = list(decs)
     res = {"shots": n}

@@ -14,7 +14,9 @@ network defined in include/pelagos_relaybp.h:
 Everything runs in float64 so the GPU's fp32 sums can be compared against it
 with a stated tolerance.  Its gradient is pinned by central finite
 differences in tests/test_learned_gamma.py, and ``shift_q8`` (the kernel's
-bf16 inference) by ``forward`` on a network whose every rounding is exact.
+bf16 inference) by ``forward`` on a network whose every rounding is exact.  At
+inference the network runs at the start of relay leg 0, where feature 7 (the
+leg index) is 0: the kernel gives that feature's K slot to the bias.
 """
 
 from __future__ import annotations
@@ -91,17 +93,32 @@ def bf16(a: np.ndarray) -> np.ndarray:
     return r.astype(np.uint32).view(np.float32)
 
 
+def split_bf16(a: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    """The bf16 nearest each value and the bf16 nearest its remainder (the
+    remainder of a float32 and its bf16 rounding is exact in float32)."""
+    a = np.asarray(a, np.float32)
+    hi = bf16(a)
+    return hi, bf16(a - hi)
+
+
 def shift_q8(p: np.ndarray, x: np.ndarray, lo: float = -0.24, hi: float = 0.66) -> np.ndarray:
-    """Per-variable shift of a leg's draw as the decode kernel computes it:
-    bf16 features, weights and first-layer activations, exact accumulation,
-    then 254 (sigmoid(z) - 1/2) in fp32 rounded to the nearest integer (ties
-    to even) and scaled by (hi - lo) / 254.  The strength of the variable is
-    clamp(u + shift, lo, hi) with u the leg's draw."""
+    """Per-variable shift of a leg's draw as the decode kernel computes it at
+    the start of relay leg 0, where feature 7 (the leg index) is 0: features 0-6
+    in bf16 against W1 and b1 carried as bf16 pairs (nearest and remainder),
+    first-layer activations in bf16, W2 in bf16 with b2 in fp32, second-layer
+    activations and w3 in bf16, exact accumulation (rounded to fp32 after each
+    layer), then 254 (sigmoid(z) - 1/2) in fp32 rounded to the nearest integer
+    (ties to even) and scaled by (hi - lo) / 254.  The strength of the variable
+    is clamp(u + shift, lo, hi) with u the leg's draw."""
     q = unpack(np.asarray(p, np.float32))
-    xb = bf16(x).astype(np.float64)
-    h1 = bf16(np.maximum(xb @ bf16(q["W1"]).astype(np.float64).T + q["b1"], 0.0).astype(np.float32))
-    h2 = np.maximum(h1.astype(np.float64) @ bf16(q["W2"]).astype(np.float64).T + q["b2"], 0.0)
-    z = (h2 @ q["w3"] + q["b3"][0]).astype(np.float32)
+    xb = bf16(x[:, :7]).astype(np.float64)
+    w1h, w1l = split_bf16(q["W1"][:, :7])
+    b1h, b1l = split_bf16(q["b1"])
+    w1 = w1h.astype(np.float64) + w1l.astype(np.float64)
+    b1 = b1h.astype(np.float64) + b1l.astype(np.float64)
+    h1 = bf16(np.maximum(xb @ w1.T + b1, 0.0).astype(np.float32)).astype(np.float64)
+    h2 = bf16(np.maximum(h1 @ bf16(q["W2"]).astype(np.float64).T + q["b2"], 0.0).astype(np.float32))
+    z = (h2.astype(np.float64) @ bf16(q["w3"]).astype(np.float64) + q["b3"][0]).astype(np.float32)
     sig = np.float32(1.0) / (np.float32(1.0) + np.exp(-z))
     steps = np.rint(np.float32(254.0) * (sig - np.float32(0.5))).astype(np.float32)
     scale = (np.float32(hi) - np.float32(lo)) * np.float32(1.0 / 254.0)

@@ -703,48 +703,39 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 constexpr int kMW1 = 0, kMB1 = 32 * 8, kMW2 = kMB1 + 32, kMB2 = kMW2 + 32 * 32, kMW3 = kMB2 + 32, kMB3 = kMW3 + 32;
 static_assert(kMB3 + 1 == PEL_MLP_PARAMS, "parameter layout");
 
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+// The network on the matrix cores, 32 variables per product (v_mfma_f32_32x32x16_bf16).
+// A 32x32 product leaves lane l the column l % 32 (a variable) and the rows
+// row32(l, e) = 8 (e / 4) + 4 (l / 32) + e % 4 of its accumulators e = 0..15 (the
+// hidden units); its A operand gives lane l the row l % 32 and K slots 8 (l / 32)
+// .. +7, its B operand the same K slots of column l % 32.  So the hidden units a
+// lane holds after one layer are the K slots it feeds into the next one, in any
+// order the weights of that layer are laid out for: no data moves between layers.
+__host__ __device__ constexpr int row32(int half, int e) { return 8 * (e / 4) + 4 * half + (e % 4); }
+
 // This lane's slices of the network.
 struct MlpRegs {
-    bf16x4_t w1[2];     // layer-1 A operands (hidden block mb, features 4 (l/16) .. +3)
-    bf16x4_t w2[2][2];  // layer-2 A operands [hidden block][input block]
-    f32x4_t b1[2], b2[2];  // biases of this lane's four D rows: the accumulators' start
-    float w3[2][4], b3;
+    bf16x8_t w1;     // layer 1, row h = lane % 32: K slots 0-6 the weights of features 0-6 and
+                     // slot 7 the bias b1[h], as the bf16 nearest each (lanes 0-31) and the bf16
+                     // nearest its remainder (lanes 32-63)
+    bf16x8_t w2[2];  // layer 2, row h: the weights of hidden units row32(lane / 32, 8 i + e)
+    f32x16_t b2;     // layer-2 accumulator start: b2 of rows row32(lane / 32, e)
+    uint32_t w3[8];  // w3 of rows row32(lane / 32, 2 k), row32(lane / 32, 2 k + 1) as bf16 pairs
+    float b3h;       // b3 in lanes 0-31, 0 in lanes 32-63 (the two halves of a column are summed)
 };
 
 // The network as each lane's MFMA operands, laid out [lane][kMlpStride words]
 // so that a lane loads its registers with ten 16-byte loads from one base
-// address (immediate offsets): words 0-3 the two layer-1 A operands, 4-11 the
-// four layer-2 A operands (bf16 pairs), 12-19 b1, 20-27 b2, 28-35 w3 (this
-// lane's four rows of each hidden block), 36 b3, 37-39 zero.
+// address (immediate offsets): words 0-3 w1, 4-11 w2, 12-27 b2, 28-35 w3, 36 b3h,
+// 37-39 zero.
 constexpr int kMlpWords = 37, kMlpStride = 40;
-
-typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 
 // Two fp32 values rounded to bf16 (nearest even), low first: one
 // v_cvt_pk_bf16_f32 (two scalar conversions compile to one each plus a v_perm).
 __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
     return __builtin_bit_cast(uint32_t, __builtin_convertvector(vec2f{a, b}, bf16x2_t));
-}
-
-// max(x, 0) of an fp32 as a signed integer max (one v_max_i32): every
-// negative float (and -0) is a negative integer, and positive floats order as
-// their bit patterns do.  fmaxf compiles to a canonicalize and a max.  For
-// finite x the two agree except fmaxf(-0, 0), which may give -0; the sums it
-// feeds start from +0, where a -0 term and a +0 term leave the same result.
-__device__ __forceinline__ float relu_f32(float x) {
-    return __int_as_float(__builtin_elementwise_max(__float_as_int(x), 0));
-}
-
-// x[l] + x[l ^ 16] and x[l] + x[l ^ 32] in every lane, with gfx950's permlane
-// swaps (VALU, no LDS round trip): swapping a copy of x with itself leaves the
-// two halves (rows) of each pair in the two results.
-__device__ __forceinline__ float sum_xor16(float x) {
-    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
-    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
-}
-__device__ __forceinline__ float sum_xor32(float x) {
-    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
-    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
 }
 
 // max(x, 0) of both bf16 halves of a word (v_pk_max_i16): the sign bit makes
@@ -756,38 +747,36 @@ __device__ __forceinline__ uint32_t relu_bf16x2(uint32_t w) {
     return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(v, z));
 }
 
-// One wave: lane l writes its operand words (weights rounded to bf16 once).
-__global__ void __launch_bounds__(64) mlp_frag_kernel(const float *__restrict__ p, uint32_t *__restrict__ frag) {
-    const int l = threadIdx.x, row = l & 15, kq = 4 * (l >> 4);
-    auto put = [&](int w, uint32_t x) { frag[l * kMlpStride + w] = x; };
-    for (int mb = 0; mb < 2; ++mb) {
-        float a1[4], a2[2][4];
-        for (int j = 0; j < 4; ++j) {
-            const int k = kq + j;
-            a1[j] = k < 8 ? p[kMW1 + (16 * mb + row) * 8 + k] : 0.f;
-            for (int sb = 0; sb < 2; ++sb) a2[sb][j] = p[kMW2 + (16 * mb + row) * 32 + 16 * sb + k];
-        }
-        put(2 * mb, pack_bf16x2(a1[0], a1[1]));
-        put(2 * mb + 1, pack_bf16x2(a1[2], a1[3]));
-        for (int sb = 0; sb < 2; ++sb) {
-            put(4 + 4 * mb + 2 * sb, pack_bf16x2(a2[sb][0], a2[sb][1]));
-            put(4 + 4 * mb + 2 * sb + 1, pack_bf16x2(a2[sb][2], a2[sb][3]));
-        }
-        for (int i = 0; i < 4; ++i) {
-            const int h = 16 * mb + kq + i;
-            put(12 + 4 * mb + i, __float_as_uint(p[kMB1 + h]));
-            put(20 + 4 * mb + i, __float_as_uint(p[kMB2 + h]));
-            put(28 + 4 * mb + i, __float_as_uint(p[kMW3 + h]));
-        }
-    }
-    put(36, __float_as_uint(p[kMB3]));
-    for (int w = kMlpWords; w < kMlpStride; ++w) put(w, 0u);
+// The bf16 nearest x (as a float) and the bf16 nearest the remainder: together
+// they carry x to about 16 significant bits.
+__device__ __forceinline__ void split_bf16(float x, float &hi, float &lo) {
+    hi = (float)(__bf16)x;
+    lo = (float)(__bf16)(x - hi);
 }
 
-// Two words of bf16 pairs as one operand (element 0 in the low half of lo): a
-// reinterpretation, no instructions.
-__device__ __forceinline__ bf16x4_t unpack_bf16x4(uint32_t lo, uint32_t hi) {
-    return __builtin_bit_cast(bf16x4_t, make_uint2(lo, hi));
+// One wave: lane l writes its operand words (weights rounded to bf16 once).
+// The leg-index feature (7) is 0 wherever the network runs (the start of relay
+// leg 0), so its weights never contribute; its K slot carries the bias instead,
+// against a constant 1 in the features.
+__global__ void __launch_bounds__(64) mlp_frag_kernel(const float *__restrict__ p, uint32_t *__restrict__ frag) {
+    const int l = threadIdx.x, h = l & 31, half = l >> 5;
+    auto put = [&](int w, uint32_t x) { frag[l * kMlpStride + w] = x; };
+    float w1[8];
+    for (int k = 0; k < 8; ++k) {
+        float hi, lo;
+        split_bf16(k < 7 ? p[kMW1 + h * 8 + k] : p[kMB1 + h], hi, lo);
+        w1[k] = half ? lo : hi;
+    }
+    for (int k = 0; k < 4; ++k) put(k, pack_bf16x2(w1[2 * k], w1[2 * k + 1]));
+    for (int i = 0; i < 2; ++i)
+        for (int k = 0; k < 4; ++k)
+            put(4 + 4 * i + k, pack_bf16x2(p[kMW2 + h * 32 + row32(half, 8 * i + 2 * k)],
+                                           p[kMW2 + h * 32 + row32(half, 8 * i + 2 * k + 1)]));
+    for (int e = 0; e < 16; ++e) put(12 + e, __float_as_uint(p[kMB2 + row32(half, e)]));
+    for (int k = 0; k < 8; ++k)
+        put(28 + k, pack_bf16x2(p[kMW3 + row32(half, 2 * k)], p[kMW3 + row32(half, 2 * k + 1)]));
+    put(36, __float_as_uint(half ? 0.f : p[kMB3]));
+    for (int w = kMlpWords; w < kMlpStride; ++w) put(w, 0u);
 }
 
 __device__ __forceinline__ void load_mlp(const uint32_t *__restrict__ frag, MlpRegs &r) {
@@ -803,66 +792,63 @@ __device__ __forceinline__ void load_mlp(const uint32_t *__restrict__ frag, MlpR
         w[4 * i + 2] = q.z;
         w[4 * i + 3] = q.w;
     }
+    r.w1 = __builtin_bit_cast(bf16x8_t, u32x4{w[0], w[1], w[2], w[3]});
 #pragma unroll
-    for (int mb = 0; mb < 2; ++mb) {
-        r.w1[mb] = unpack_bf16x4(w[2 * mb], w[2 * mb + 1]);
+    for (int i = 0; i < 2; ++i)
+        r.w2[i] = __builtin_bit_cast(bf16x8_t, u32x4{w[4 + 4 * i], w[5 + 4 * i], w[6 + 4 * i], w[7 + 4 * i]});
 #pragma unroll
-        for (int sb = 0; sb < 2; ++sb) r.w2[mb][sb] = unpack_bf16x4(w[4 + 4 * mb + 2 * sb], w[5 + 4 * mb + 2 * sb]);
+    for (int e = 0; e < 16; ++e) r.b2[e] = __uint_as_float(w[12 + e]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            r.b1[mb][i] = __uint_as_float(w[12 + 4 * mb + i]);
-            r.b2[mb][i] = __uint_as_float(w[20 + 4 * mb + i]);
-            r.w3[mb][i] = __uint_as_float(w[28 + 4 * mb + i]);
-        }
-    }
-    r.b3 = __uint_as_float(w[36]);
+    for (int k = 0; k < 8; ++k) r.w3[k] = w[28 + k];
+    r.b3h = __uint_as_float(w[36]);
 }
 
-// Logits of G groups of 16 variables (group g: lane l holds variable l % 16,
-// features 4 (l/16) .. +3 in x[g]; lanes 32-63 hold the zero padding of K).
-// Every lane gets z[g] of its variable of group g.  The groups are evaluated
-// layer by layer, so their matrix-core chains overlap.  All 64 lanes must be
-// active.
-template <int G>
-__device__ __forceinline__ void mlp_logits(const MlpRegs &r, const bf16x4_t (&x)[G], float (&z)[G]) {
-    bf16x4_t h1[G][2];
+// Half of the logit of column l % 32 of a 32-variable group from the
+// layer-1 input x (B operand): layer 1 (bias through K slot 7), bf16 ReLU,
+// layer 2 from b2, bf16 ReLU, and w3 . h2 over this lane's 16 rows (bf16 pair
+// dot products, v_dot2_f32_bf16) from b3h.
+__device__ __forceinline__ float mlp_half_logit(const MlpRegs &r, bf16x8_t x) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const f32x16_t zero = {};
+    const f32x16_t d1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(r.w1, x, zero, 0, 0, 0);
+    uint32_t h1[8];
 #pragma unroll
-    for (int mb = 0; mb < 2; ++mb) {
+    for (int k = 0; k < 8; ++k) h1[k] = relu_bf16x2(pack_bf16x2(d1[2 * k], d1[2 * k + 1]));
+    f32x16_t d2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+        r.w2[0], __builtin_bit_cast(bf16x8_t, u32x4{h1[0], h1[1], h1[2], h1[3]}), r.b2, 0, 0, 0);
+    d2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+        r.w2[1], __builtin_bit_cast(bf16x8_t, u32x4{h1[4], h1[5], h1[6], h1[7]}), d2, 0, 0, 0);
+    float part = r.b3h;
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            // the accumulator starts at the bias: relu(b1 + W1 x), rounded to bf16
-            const f32x4_t d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(r.w1[mb], x[g], r.b1[mb], 0, 0, 0);
-            // relu after rounding: a bf16 is negative exactly when its 16-bit pattern
-            // is, so a packed signed max with 0 clamps both halves at once
-            h1[g][mb] = unpack_bf16x4(relu_bf16x2(pack_bf16x2(d[0], d[1])), relu_bf16x2(pack_bf16x2(d[2], d[3])));
-        }
+    for (int k = 0; k < 8; ++k)
+        part = __builtin_amdgcn_fdot2_f32_bf16(
+            __builtin_bit_cast(bf16x2_t, relu_bf16x2(pack_bf16x2(d2[2 * k], d2[2 * k + 1]))),
+            __builtin_bit_cast(bf16x2_t, r.w3[k]), part, false);
+    return part;
+}
+
+// Logit of the variable whose features this lane packed (bf16 pairs: features
+// 0-1, 2-3, 4-5, 6 and the constant 1 of the bias slot), for all 64 lanes of
+// the wave at once (all must be active): columns of group A are the wave's
+// variables 0-31, of group B 32-63.  Lane l of a product needs the features of
+// column l % 32 (both halves carry them, against the hi and the lo weights):
+// one permlane32 swap of a word with itself gives group A's word (lanes 32-63
+// take lanes 0-31's) and group B's (lanes 0-31 take lanes 32-63's).  The two
+// halves of a column's sum come together with one more swap.
+__device__ __forceinline__ float mlp_logit(const MlpRegs &r, const uint32_t (&pk)[4]) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    uint32_t xa[4], xb[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const auto t = __builtin_amdgcn_permlane32_swap(pk[w], pk[w], false, false);
+        xa[w] = t[0];
+        xb[w] = t[1];
     }
-    float part[G];
-#pragma unroll
-    for (int g = 0; g < G; ++g) part[g] = 0.f;
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb) {
-        // layer 2 in one K = 32 product: lane group q's eight K slots are hidden
-        // units 4q .. 4q+3 and 16+4q .. 16+4q+3 in both operands (its two layer-1
-        // output rows and the matching weight columns), so the sum runs over
-        // all 32 hidden units whatever order the matrix core takes them in
-        const bf16x8_t wa = __builtin_bit_cast(bf16x8_t, __builtin_shufflevector(r.w2[mb][0], r.w2[mb][1], 0, 1, 2, 3, 4, 5, 6, 7));
-        f32x4_t d[G];
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const bf16x8_t hb = __builtin_bit_cast(bf16x8_t, __builtin_shufflevector(h1[g][0], h1[g][1], 0, 1, 2, 3, 4, 5, 6, 7));
-            d[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, hb, r.b2[mb], 0, 0, 0);
-        }
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) part[g] = __builtin_fmaf(r.w3[mb][i], relu_f32(d[g][i]), part[g]);
-        }
-    }
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-        z[g] = sum_xor32(sum_xor16(part[g])) + r.b3;
-    }
+    const float pa = mlp_half_logit(r, __builtin_bit_cast(bf16x8_t, u32x4{xa[0], xa[1], xa[2], xa[3]}));
+    const float pb = mlp_half_logit(r, __builtin_bit_cast(bf16x8_t, u32x4{xb[0], xb[1], xb[2], xb[3]}));
+    // lanes 0-31: pa (own column) + pa of lane l + 32; lanes 32-63: pb of lane l - 32 + pb (own)
+    const auto t = __builtin_amdgcn_permlane32_swap(__float_as_uint(pa), __float_as_uint(pb), false, false);
+    return __uint_as_float(t[0]) + __uint_as_float(t[1]);
 }
 
 __device__ __forceinline__ float clamp4(float x) { return fminf(fmaxf(x, -4.f), 4.f); }
@@ -876,10 +862,9 @@ __device__ __forceinline__ float clamp4(float x) { return fminf(fmaxf(x, -4.f), 
 //   7 r / 4
 // Every lane computes the eight features of one variable (a wave takes a
 // 64-variable slice).  Then, with parameters, the network gives every variable
-// its strength shift for leg r: the slice is evaluated as four 16-variable
-// MFMA groups, whose B operands (lane l: variable l % 16, features 4 (l / 16)
-// .. +3, zero for l >= 32) are gathered from the owning lanes with
-// ds_bpermute; lane l's own logit comes out of group l / 16.  The shift is
+// its strength shift for leg r: the slice is evaluated as two 32-variable
+// matrix-core groups (mlp_logit), and lane l's own logit comes back to lane l.
+// The shift is
 // (hi - lo) (sigmoid(z) - 1/2), kept in LDS as the nearest multiple of
 // (hi - lo) / 254 (int8), and the leg uses gamma = clamp(u + shift, lo, hi)
 // with u the leg's own random draw, whose disorder lets successive legs
@@ -965,7 +950,7 @@ __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const Shot
             *m.template lds<uint8_t>(a.unsat_lds + (uint32_t)i) = (uint8_t)(i < C ? parity_row<PLACE>(a, m, i, syn) : 0);
         gsync<BLOCK>();
     }
-    const int lane = gtid<BLOCK>() & 63, q = lane >> 4;
+    const int lane = gtid<BLOCK>() & 63;
     const bool net = a.mlp && apply;
     // A wave takes kFeatBatch of its slices at a time (v0 is uniform, so every
     // lane of a wave runs the same rounds; the MFMAs need all 64 lanes), and
@@ -1008,28 +993,14 @@ __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const Shot
             pk[j][0] = pack_bf16x2(f[0], f[1]);
             pk[j][1] = pack_bf16x2(f[2], f[3]);
             pk[j][2] = pack_bf16x2(f[4], f[5]);
-            pk[j][3] = pack_bf16x2(f[6], f[7]);
+            pk[j][3] = pack_bf16x2(f[6], 1.f);  // feature 7 (0 at leg 0) gives its slot to the bias
         }
         if (net) {
 #pragma unroll
             for (int j = 0; j < kFeatBatch; ++j) {
                 const int v = vb + j * BLOCK + lane;
                 if (uniform_i32(vb + j * BLOCK) >= V) break;
-                bf16x4_t xg[4];
-#pragma unroll
-                for (int grp = 0; grp < 4; ++grp) {
-                    const int src = 16 * grp + (lane & 15);
-                    const uint32_t a01 = (uint32_t)__shfl((int)pk[j][0], src, 64);
-                    const uint32_t a23 = (uint32_t)__shfl((int)pk[j][1], src, 64);
-                    const uint32_t b01 = (uint32_t)__shfl((int)pk[j][2], src, 64);
-                    const uint32_t b23 = (uint32_t)__shfl((int)pk[j][3], src, 64);
-                    const uint32_t x01 = q == 0 ? a01 : (q == 1 ? b01 : 0u);
-                    const uint32_t x23 = q == 0 ? a23 : (q == 1 ? b23 : 0u);
-                    xg[grp] = unpack_bf16x4(x01, x23);
-                }
-                float zg[4];
-                mlp_logits<4>(mr, xg, zg);
-                const float z = q == 0 ? zg[0] : (q == 1 ? zg[1] : (q == 2 ? zg[2] : zg[3]));
+                const float z = mlp_logit(mr, pk[j]);
                 // shift of the leg's own draw in steps of (hi - lo) / 254: 254 (sigmoid(z) - 1/2)
                 // rounded to the nearest integer (ties to even), in [-127, 127]
                 // (hardware exp2 and reciprocal: within a few ulp of the correctly rounded

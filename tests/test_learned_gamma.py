@@ -58,18 +58,23 @@ def test_oracle_loss_closed_form_at_zero_network():
 
 
 def _bf16_exact_network(seed):
-    """Features in multiples of 1/4 on [-1, 1], W1 and W2 entries in {-1, 0, 1}/4
-    and /8, dyadic biases: every first-layer activation has at most 8
-    significant bits, so the kernel's bf16 roundings (features, weights, first
-    layer) are all exact and its inference must equal the float64 network."""
+    """Features in multiples of 1/4 on [-1, 1] (feature 7, the leg index, 0 as
+    at relay leg 0), W1 entries in {-1, 0, 1}/4, W2 rows with at most four
+    entries in {-1, 1}/8, w3 in multiples of 1/8, dyadic biases: every
+    activation has at most 8 significant bits, so the kernel's bf16 roundings
+    (features, weights, both layers' activations) are all exact and its
+    inference must equal the float64 network."""
     rng = np.random.default_rng(seed)
     W1 = rng.integers(-1, 2, (32, 8)) / 4.0
     b1 = rng.integers(-4, 5, 32) / 8.0
-    W2 = rng.integers(-1, 2, (32, 32)) / 8.0
+    W2 = np.zeros((32, 32))
+    for r in range(32):
+        W2[r, rng.choice(32, 4, replace=False)] = rng.choice([-1.0, 1.0], 4) / 8.0
     b2 = rng.integers(-4, 5, 32) / 16.0
-    w3 = rng.normal(0, 0.6, 32).astype(np.float32)
+    w3 = rng.integers(-8, 9, 32) / 8.0
     p = np.concatenate([W1.ravel(), b1, W2.ravel(), b2, w3, [0.1]]).astype(np.float32)
     x = (rng.integers(-4, 5, (4000, 8)) / 4.0).astype(np.float32)
+    x[:, 7] = 0.0
     return p, x
 
 
@@ -84,7 +89,8 @@ def test_oracle_bf16_inference_matches_the_network_when_rounding_is_exact():
         p, x = _bf16_exact_network(seed)
         assert np.array_equal(mo.bf16(p[:32 * 8]), p[:32 * 8]) and np.array_equal(mo.bf16(x), x)
         z, c = mo.forward(p.astype(np.float64), x)
-        assert np.array_equal(mo.bf16(c["h1"].astype(np.float32)), c["h1"].astype(np.float32))
+        for k in ("h1", "h2"):
+            assert np.array_equal(mo.bf16(c[k].astype(np.float32)), c[k].astype(np.float32)), k
         t = 254.0 * (1.0 / (1.0 + np.exp(-z)) - 0.5)
         steps_ref = np.rint(t)
         steps = np.rint(mo.shift_q8(p, x, lo, hi) / np.float32((hi - lo) / 254.0))
@@ -95,6 +101,10 @@ def test_oracle_bf16_inference_matches_the_network_when_rounding_is_exact():
         # a plausible slip: W2 transposed gives other steps on this network
         q = p.copy()
         q[32 * 8 + 32:32 * 8 + 32 + 1024] = p[32 * 8 + 32:32 * 8 + 32 + 1024].reshape(32, 32).T.ravel()
+        assert (np.rint(mo.shift_q8(q, x, lo, hi) / np.float32((hi - lo) / 254.0)) != steps).mean() > 0.1
+        # and so does a dropped first-layer bias (the kernel carries it in feature 7's slot)
+        q = p.copy()
+        q[32 * 8:32 * 8 + 32] = 0.0
         assert (np.rint(mo.shift_q8(q, x, lo, hi) / np.float32((hi - lo) / 254.0)) != steps).mean() > 0.1
 
 
