@@ -376,7 +376,8 @@ def paired_ler(decs: dict, H, p, batches: int, shots: int, seed0: int) -> dict:
 
     from pelagos_amd import codes
 
-    O = codes.synthetic_observables(H.shape[1], seed=0).astype(np.int64)
+    # float32 products of 0/1 matrices are exact (counts stay below 2^24) and run on BLAS
+    O = codes.synthetic_observables(H.shape[1], seed=0).astype(np.float32)
     fails = {k: np.zeros(0, bool) for k in decs}
     iters = {k: 0 for k in decs}
     for k in range(batches):
@@ -386,7 +387,7 @@ def paired_ler(decs: dict, H, p, batches: int, shots: int, seed0: int) -> dict:
             out = dec.decode(syn)
             e = out["errors"].cpu().numpy()
             conv = out["found_count"].cpu().numpy() >= dec.stop_nconv
-            bad = (~conv) | ((O @ (e ^ err).T.astype(np.int64)) % 2).any(0)
+            bad = (~conv) | ((O @ (e ^ err).T.astype(np.float32)).astype(np.int64) % 2).any(0)
             fails[name] = np.concatenate([fails[name], bad])
             iters[name] += int(out["iterations"].long().sum())
         if (k + 1) % 64 == 0 and k + 1 < batches:

@@ -39,7 +39,7 @@ def _decoder(H, p, dtype, **kw):
     return dec
 
 
-def _oracle_for(dec, syn, *, prior=None, init_posterior=None, skip_relay_legs=False):
+def _oracle_for(dec, syn, *, prior=None, init_posterior=None, init_gamma=None, skip_relay_legs=False):
     B = syn.shape[0]
     lam = prior if prior is not None else dec.log_prior_ratios_per_gpu[0].reshape(-1).cpu().numpy()
     gtab = dec.gamma_table(B).cpu().numpy()
@@ -50,6 +50,7 @@ def _oracle_for(dec, syn, *, prior=None, init_posterior=None, skip_relay_legs=Fa
         alpha=dec.normalized_min_sum_alpha, alpha_scaling=dec.alpha_iteration_scaling_factor,
         beta=dec.offset_min_sum_beta, last_leg=dec.unconverged_correction == "last_leg",
         skip_relay_legs=skip_relay_legs, gamma_per_row=gtab.ndim == 3, init_posterior=init_posterior,
+        init_gamma=init_gamma,
         want_posterior=True, dtype=dec.dtype_str,
     )
 
@@ -118,7 +119,7 @@ def test_regular_code_options_match_oracle(dtype, case):
 
 @pytest.mark.parametrize("dtype", DTYPES)
 def test_overrides_match_oracle(dtype):
-    """prior_llr, init_posterior and skip_relay_legs in the element type."""
+    """prior_llr, init_posterior, init_gamma and skip_relay_legs in the element type."""
     H = toy_codes.regular_code(3)
     V = H.shape[1]
     priors = np.random.default_rng(4).uniform(0.01, 0.03, V)
@@ -133,6 +134,9 @@ def test_overrides_match_oracle(dtype):
     _compare(out, _oracle_for(dec, S, prior=pl, init_posterior=ip))
     out2 = _run(dec, S, skip_relay_legs=True)
     _compare(out2, _oracle_for(dec, S, skip_relay_legs=True))
+    ig = rng.uniform(-0.24, 0.66, (32, V)).astype(T)   # the pre-iter leg's per-variable strengths
+    out3 = _run(dec, S, init_gamma=torch.from_numpy(ig).to("cuda:0"))
+    _compare(out3, _oracle_for(dec, S, init_gamma=ig))
 
 
 def test_float32_per_edge_kernel_equals_the_fast_kernel():
