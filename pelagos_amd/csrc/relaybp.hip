@@ -1224,9 +1224,18 @@ __device__ __forceinline__ ShotIO shot_io(const KArgs &a, int b) {
 // Returns true when relay legs remain to run.
 template <int BLOCK, int PLACE, bool LEARN>
 __device__ __forceinline__ bool first_leg(const KArgs &a, int b, ShotMem<PLACE> &m, ShotState &st, double *s_red,
-                          int *s_flag) {
+                          int *s_flag, bool l0_ready = false) {
     const int V = a.g.V;
     const ShotIO io = shot_io(a, b);
+    if ((PLACE & (1 << kL0)) && l0_ready) {
+        // The priors in LDS are this shot's already (shared by the batch, and the
+        // group's previous shot had the same graph and priors): the state starts
+        // from them by position, with no global loads.
+        for (int v = gtid<BLOCK>(); v < V; v += BLOCK) {
+            m.ls_set(v, make_float2(m.l0_pos(v), 0.f));
+            if (LEARN) *m.template lds<uint8_t>(a.flips_lds + (uint32_t)v) = 0;
+        }
+    } else
     for (int j0 = gtid<BLOCK>(); j0 < V; j0 += kBatchU * BLOCK) {  // kBatchU per thread at a time
         int v[kBatchU];
         float lj[kBatchU], p0[kBatchU];
@@ -1536,6 +1545,7 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
     // one branch: first-leg tickets in queue order, then re-queued shots.
     int prev = -1;
     ShotState st{0.f, 0, 0, 0, 0, 0, 0};
+    bool l0_loaded = false;  // the priors in LDS are this decode's (a first leg or a resume wrote them)
     for (;;) {
         if (opaque_tid() == 0) {
             const int64_t now = a.shot_clock ? (int64_t)wall_clock64() : 0;
@@ -1572,7 +1582,9 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
         }
         bool more = true;
         if (stage == 1) {
-            more = first_leg<BLOCK, PLACE, LEARN>(a, b, m, st, s_red, s_flag);
+            more = first_leg<BLOCK, PLACE, LEARN>(a, b, m, st, s_red, s_flag,
+                                                  l0_loaded && !a.prior_batched && !a.init_post);
+            l0_loaded = true;
             if (a.requeue_below > 0) {
                 if (more && st.found > 0 && st.iters < a.requeue_below) {
                     // the first leg converged: relay leg 0 starts from a valid solution
@@ -1587,6 +1599,7 @@ __global__ void __launch_bounds__(BLOCK) relay_decode_kernel(KArgs a) {
             }
         } else {
             resume_shot<BLOCK, PLACE, LEARN>(a, b, m);
+            l0_loaded = true;
             const int4 rs = uniform_i4(a.rq_state[2 * b]);
             // parked shots are those whose first leg converged; the network ran before
             // they were parked
@@ -1651,13 +1664,18 @@ __global__ void __launch_bounds__(256) jobs_setup_kernel(SetupArgs) {
 // One shot of a job on a group (the workgroup, or BLOCK = 64: one wave at LDS
 // base lb): its whole schedule, then the leader writes its scalars.
 template <int BLOCK, int PLACE, bool LEARN>
-__device__ __forceinline__ void engine_shot(const KArgs &a, int b, uint8_t *smem, uint32_t lb, uint8_t *slot) {
+__device__ __forceinline__ void engine_shot(const KArgs &a, int b, uint8_t *smem, uint32_t lb, uint8_t *slot,
+                                            bool reuse = false) {
     ShotMem<PLACE> m = shot_mem<PLACE>(a, smem, lb, slot);
     double *s_red = reinterpret_cast<double *>(smem + lb + a.scratch_off);
     int *s_flag = reinterpret_cast<int *>(smem + lb + a.scratch_off + 8u * (BLOCK / 64));
-    group_setup<BLOCK, PLACE>(a, m);
+    // reuse: the group's previous shot had this graph, layout and prior array, so
+    // the slice tables, the sentinels and (when the batch shares its priors) the
+    // priors in LDS are this shot's already
+    if (!reuse) group_setup<BLOCK, PLACE>(a, m);
     ShotState st{0.f, 0, 0, 0, 0, 0, 0};
-    const bool more = first_leg<BLOCK, PLACE, LEARN>(a, b, m, st, s_red, s_flag);
+    const bool more = first_leg<BLOCK, PLACE, LEARN>(a, b, m, st, s_red, s_flag,
+                                                     reuse && !a.prior_batched && !a.init_post);
     relay_legs<BLOCK, PLACE, LEARN>(a, b, m, st, more, s_red, s_flag);
     if (gleader<BLOCK>()) {
         a.weights[b] = st.best_w;
@@ -1700,6 +1718,9 @@ __global__ void __launch_bounds__(kEngineBlock) engine_kernel(EngineArgs e, cons
     extern __shared__ __align__(16) uint8_t smem[];
     int *ctl = reinterpret_cast<int *>(smem + e.ctl_lds);
     int j = 0;
+    // what the workgroup's LDS holds from its last whole-workgroup shot
+    const void *last_graph = nullptr, *last_prior = nullptr;
+    int last_kind = -1;
     for (;;) {
         // The leader finds the first job from j on with a shot left, and takes a
         // whole-workgroup job's ticket (a shot-per-wave job's waves take theirs).
@@ -1728,17 +1749,21 @@ __global__ void __launch_bounds__(kEngineBlock) engine_kernel(EngineArgs e, cons
         uint8_t *slot = a.spill + (int64_t)blockIdx.x * a.spill_stride;
         const int kind = uniform_i32(jobs[j].kind);
         const int b = kind & kKindWave ? 0 : a.order[t];
+        const bool reuse = kind == last_kind && a.g.rslice == last_graph && a.prior == last_prior;
+        last_kind = kind;  // a shot-per-wave job lays out LDS per wave: never reused
+        last_graph = kind & kKindWave ? nullptr : a.g.rslice;
+        last_prior = a.prior;
 #define PEL_ENGINE_CASE(K, CALL) \
     else if ((KINDS & kind_bit(K)) && kind == (K)) CALL;
         if (false) {
         }
-        PEL_ENGINE_CASE(15, (engine_shot<kEngineBlock, 15, false>(a, b, smem, 0, slot)))
-        PEL_ENGINE_CASE(7, (engine_shot<kEngineBlock, 7, false>(a, b, smem, 0, slot)))
-        PEL_ENGINE_CASE(3, (engine_shot<kEngineBlock, 3, false>(a, b, smem, 0, slot)))
-        PEL_ENGINE_CASE(1, (engine_shot<kEngineBlock, 1, false>(a, b, smem, 0, slot)))
-        PEL_ENGINE_CASE(0, (engine_shot<kEngineBlock, 0, false>(a, b, smem, 0, slot)))
-        PEL_ENGINE_CASE(15 | kKindLearn, (engine_shot<kEngineBlock, 15, true>(a, b, smem, 0, slot)))
-        PEL_ENGINE_CASE(7 | kKindLearn, (engine_shot<kEngineBlock, 7, true>(a, b, smem, 0, slot)))
+        PEL_ENGINE_CASE(15, (engine_shot<kEngineBlock, 15, false>(a, b, smem, 0, slot, reuse)))
+        PEL_ENGINE_CASE(7, (engine_shot<kEngineBlock, 7, false>(a, b, smem, 0, slot, reuse)))
+        PEL_ENGINE_CASE(3, (engine_shot<kEngineBlock, 3, false>(a, b, smem, 0, slot, reuse)))
+        PEL_ENGINE_CASE(1, (engine_shot<kEngineBlock, 1, false>(a, b, smem, 0, slot, reuse)))
+        PEL_ENGINE_CASE(0, (engine_shot<kEngineBlock, 0, false>(a, b, smem, 0, slot, reuse)))
+        PEL_ENGINE_CASE(15 | kKindLearn, (engine_shot<kEngineBlock, 15, true>(a, b, smem, 0, slot, reuse)))
+        PEL_ENGINE_CASE(7 | kKindLearn, (engine_shot<kEngineBlock, 7, true>(a, b, smem, 0, slot, reuse)))
         PEL_ENGINE_CASE(15 | kKindWave, (engine_wave_job<15>(a, jobs[j].stride, smem)))
         PEL_ENGINE_CASE(7 | kKindWave, (engine_wave_job<7>(a, jobs[j].stride, smem)))
 #undef PEL_ENGINE_CASE
