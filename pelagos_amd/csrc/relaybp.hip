@@ -689,15 +689,11 @@ __device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> 
 // decoder.py:480-491, 2322).
 //
 // Network evaluation runs on the matrix cores (bf16 operands, fp32
-// accumulation), transposed so that each layer's output tile is already the
-// next layer's B operand: with v_mfma_f32_16x16x16_bf16, lane l holds B
-// column l % 16 (a variable) and rows 4 (l / 16) .. +3, and D rows
-// 4 (l / 16) + i of column l % 16, so H1^T = W1 X^T, H2^T = W2 H1^T need no
-// data movement between layers.
+// accumulation), transposed so that each layer's output is already the next
+// layer's B operand (H1^T = W1 X^T, H2^T = W2 H1^T, 32 variables per product;
+// see row32 below): no data moves between layers.
 
-typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 
 constexpr int kMW1 = 0, kMB1 = 32 * 8, kMW2 = kMB1 + 32, kMB2 = kMW2 + 32 * 32, kMW3 = kMB2 + 32, kMB3 = kMW3 + 32;
