@@ -20,6 +20,14 @@ finished; a request that would exceed that is refused with ServerBusy at once,
 so a saturated server answers immediately instead of letting its queue (and
 every later request's latency) grow without bound.
 
+HBM budget: with ``hbm_budget_bytes`` the server sizes its device memory to the
+budget.  A shot's message state lives in the LDS of the CU decoding it, so the
+device memory a server holds is its fixed part (the engine's staging rows,
+workspaces and job table) plus each code's result arena (error row and three
+counters per shot).  The arenas get the rows the budget leaves after the fixed
+part, and the admission bound follows from them (a quarter of the rows, as by
+default), so the server admits work while its results fit the budget.
+
 Each shot's result does not depend on the other shots decoded with it (the
 decoder's results are batch-independent), so a request decodes the same alone
 and in the mix; tests/test_serving.py checks that bit for bit.
@@ -130,11 +138,17 @@ class DecodeServer:
     result tensors are views of its code's result arena (``result_shots`` rows,
     default 4 x max_queued_shots), complete once ``poll`` has marked the request
     done and valid until about ``result_shots`` more shots of that code have
-    been submitted (clone them to keep them longer)."""
+    been submitted (clone them to keep them longer).
+
+    hbm_budget_bytes: size the device memory to this budget (see the module
+    docstring); result_shots and max_queued_shots, unless given, then follow
+    from it (max_queued_shots at most queue_units), and a budget below the
+    fixed part or below one full request is refused.  ``device_bytes`` reports
+    what the server holds."""
 
     def __init__(self, decoders: Dict[str, RelayBPDecoder], max_batch: int = 2048,
-                 max_queued_shots: int = 16384, idle_ms: float = 2.0, result_shots: Optional[int] = None,
-                 queue_units: int = 1 << 18):
+                 max_queued_shots: Optional[int] = None, idle_ms: float = 2.0, result_shots: Optional[int] = None,
+                 queue_units: int = 1 << 18, hbm_budget_bytes: Optional[int] = None):
         if not decoders:
             raise ValueError("no decoders")
         if len(decoders) > _native.PEL_MAX_JOBS:
@@ -148,11 +162,13 @@ class DecodeServer:
             if d.gamma_table(1).dim() != 2:
                 raise ValueError(f"{name}: serving needs broadcast gamma draws (gamma_draw != 'per_row')")
         self.max_batch = int(max_batch)
-        self.max_queued_shots = int(max_queued_shots)
-        if self.max_queued_shots < self.max_batch or self.max_batch < 1:
-            raise ValueError("max_queued_shots must hold at least one full request of max_batch >= 1 shots")
-        if queue_units < self.max_queued_shots:
-            raise ValueError("queue_units must be at least max_queued_shots")
+        if self.max_batch < 1:
+            raise ValueError("max_batch must be >= 1")
+        budget = None if hbm_budget_bytes is None else int(hbm_budget_bytes)
+        if max_queued_shots is None and budget is None:
+            max_queued_shots = 16384
+        if max_queued_shots is not None:
+            self._set_admission(int(max_queued_shots), queue_units)
         self.idle_us = int(round(float(idle_ms) * 1000))
         n = len(self.names)
         L = _native.lib()
@@ -187,6 +203,23 @@ class DecodeServer:
             self._params = (_native.PelParams * n)(*params)
             self._batches = (_native.PelBatch * n)(*batches)
             self._table = torch.empty(int(self.plan.table_bytes), dtype=torch.uint8, device=dev)
+            fixed = sum(t.numel() * t.element_size() for st in self._staging.values() for t in st.values())
+            fixed += sum(t.numel() for t in self._ws.values()) + self._table.numel()
+            row_bytes = sum(decoders[k].V + 12 for k in self.names)  # error row + weight, iterations, found
+            if budget is not None:
+                room = (budget - fixed) // row_bytes
+                if result_shots is None:
+                    result_shots = room
+                if result_shots < self.max_batch or result_shots > room:
+                    raise ValueError(f"hbm_budget_bytes={budget}: the fixed part takes {fixed} B and each result "
+                                     f"row {row_bytes} B, room for {max(room, 0)} rows; a full request needs "
+                                     f"{self.max_batch}")
+                if max_queued_shots is None:
+                    self._set_admission(max(self.max_batch, min(int(result_shots) // 4, int(queue_units))),
+                                        queue_units)
+            rs = int(result_shots) if result_shots is not None else 4 * self.max_queued_shots
+            if rs < self.max_batch:
+                raise ValueError("result_shots must hold at least one full request")
             _native.check(L.pel_serve_queue_create(int(queue_units), ctypes.byref(self._queue)),
                           "pel_serve_queue_create")
             cap = ctypes.c_int32()
@@ -194,10 +227,9 @@ class DecodeServer:
                           "pel_serve_queue_info")
             self._cap = int(cap.value)
             self._lib = L
-            rs = int(result_shots) if result_shots is not None else 4 * self.max_queued_shots
-            if rs < self.max_batch:
-                raise ValueError("result_shots must hold at least one full request")
             self._arena = {k: _Arena(rs, decoders[k].V, dev) for k in self.names}
+            self.result_shots = rs
+            self._device_bytes = fixed + rs * row_bytes
             self.stream = torch.cuda.Stream(device=dev)
         self._ended: Optional[torch.cuda.Event] = None
         self.launches = 0
@@ -206,6 +238,18 @@ class DecodeServer:
         self.t0 = time.perf_counter()
         self.finished: List[Request] = []
         self._next_id = 0
+
+    def _set_admission(self, max_queued_shots: int, queue_units: int) -> None:
+        self.max_queued_shots = max_queued_shots
+        if self.max_queued_shots < self.max_batch:
+            raise ValueError("max_queued_shots must hold at least one full request of max_batch >= 1 shots")
+        if queue_units < self.max_queued_shots:
+            raise ValueError("queue_units must be at least max_queued_shots")
+
+    def device_bytes(self) -> int:
+        """Device memory the server holds: staging rows, workspaces, the job
+        table and the result arenas (the engine's unit queue is in host memory)."""
+        return self._device_bytes
 
     def roles(self) -> Dict[str, dict]:
         """Each code's role in the engine: threads per shot, placement, learned."""

@@ -291,3 +291,36 @@ def test_backpressure_and_a_full_queue():
         assert L.pel_serve_queue_info(q, None, None, ctypes.byref(pub)) == 0 and pub.value == 2
     finally:
         L.pel_serve_queue_destroy(q)
+
+
+@pytest.mark.gpu
+def test_hbm_budget_sizes_the_server_and_bounds_admission():
+    """With hbm_budget_bytes the server holds at most that much device memory:
+    the result arenas take the rows the budget leaves after the fixed part, the
+    admission bound is a quarter of them, requests beyond it are refused, and a
+    budget that cannot hold one full request is refused."""
+    from pelagos_amd.serving import DecodeServer, ServerBusy
+
+    dd = _decoders()
+    dec, H, p = dd["bb72"]
+    pool = torch.from_numpy(codes.syndromes_of(H, codes.sample_errors(p, 512, seed=8))).cuda()
+    torch.cuda.synchronize()
+    print("[gpu] serving hbm budget", flush=True)
+    with DecodeServer({"bb72": dec}, max_batch=64) as probe:
+        fixed = probe.device_bytes() - probe.result_shots * (dec.V + 12)
+    row = dec.V + 12
+    budget = fixed + 1024 * row + row // 2
+    with DecodeServer({"bb72": dec}, max_batch=64, hbm_budget_bytes=budget) as server:
+        assert server.result_shots == 1024 and server.max_queued_shots == 256
+        assert server.device_bytes() <= budget
+        reqs = [server.submit("bb72", pool[64 * i:64 * (i + 1)]) for i in range(4)]
+        with pytest.raises(ServerBusy):
+            server.submit("bb72", pool[256:257])
+        server.drain()
+        reqs.append(server.submit("bb72", pool[256:320]))
+        server.drain()
+        server.quiesce()
+        alone = dec.decode(pool[:320])
+        assert torch.equal(torch.cat([r.errors for r in reqs]), alone["errors"])
+    with pytest.raises(ValueError):
+        DecodeServer({"bb72": dec}, max_batch=64, hbm_budget_bytes=fixed + 63 * row)
