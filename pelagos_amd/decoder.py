@@ -331,7 +331,8 @@ class RelayBPDecoder:
         if autotune is None:
             autotune = os.environ.get("PELAGOS_AMD_AUTOTUNE", "1") != "0"
         self.autotune = bool(autotune)
-        self._tuned: Dict[Tuple[int, bool], Tuple[int, int]] = {}  # (B, learned) -> (block, requeue_below)
+        # (B, learned) -> (block, requeue_below, placement)
+        self._tuned: Dict[Tuple[int, bool], Tuple[int, int, int]] = {}
         self.tune_runs = 0  # first-sight tunings this decoder timed (cache misses)
         self._workspace: Optional[torch.Tensor] = None
         self._gamma_cache: Dict[Tuple, torch.Tensor] = {}
@@ -480,7 +481,8 @@ class RelayBPDecoder:
         if p is None:
             tuned = self._tuned.get((B, learned))
             block = self._block_size or (tuned[0] if tuned else 0)
-            p = self._handle.plan(B, block, self._placement, learned)
+            place = self._placement if self._placement >= 0 or not tuned else tuned[2]
+            p = self._handle.plan(B, block, place, learned)
             if self.requeue_below is None and tuned is not None and (tuned[1] == 0 or p.requeue_below > 0):
                 p.requeue_below = int(tuned[1])
             if self.requeue_below is not None:
@@ -498,32 +500,42 @@ class RelayBPDecoder:
         return (f"{self.graph.sparsity_hash}|B={B}|learned={int(learned)}|placement={self._placement}"
                 f"|{name}|abi={_native.PEL_ABI_VERSION}")
 
-    def _candidates(self, B: int, learned: bool) -> List[Tuple[int, int]]:
-        """Launch shapes to time: the block sizes around the library's own choice
-        and the re-queue thresholds 0, 16 and 32."""
-        base = self._handle.plan(B, 0, self._placement, learned).block
+    def _candidates(self, B: int, learned: bool) -> List[Tuple[int, int, int]]:
+        """Launch shapes to time: the block sizes around the library's own choice,
+        the re-queue thresholds 0, 16 and 32, and the state placement the library
+        picks (the largest that fits in LDS) beside the next smaller one (for
+        learned strengths, the fast placements only)."""
+        base_plan = self._handle.plan(B, 0, self._placement, learned)
+        base = base_plan.block
         blocks = sorted({b for b in (base // 4, base // 2, base) if 64 <= b <= 1024})
         if self._block_size:
             blocks = [self._block_size]
         requeue = [int(self.requeue_below)] if self.requeue_below is not None else [0, 16, 32]
-        return [(b, r) for b in blocks for r in requeue]
+        if self._placement >= 0:
+            places = [self._placement]
+        else:
+            order = [15, 7, 3, 1, 0]
+            top = int(base_plan.placement)
+            lower = [q for q in order[order.index(top) + 1:] if not learned or (q & 7) == 7]
+            places = [top] + lower[:1]
+        return [(b, r, q) for q in places for b in blocks for r in requeue]
 
     def _tune(self, B: int, learned: bool, run) -> None:
         """First sight of (B, learned): the cached choice, or time every
         candidate with ``run()`` (one decode of the caller's syndromes), keep
         the fastest and save it."""
         key = self._tune_key(B, learned)
-        self._tuned[(B, learned)] = (self._block_size, 0)  # placeholder while timing
+        self._tuned[(B, learned)] = (self._block_size, 0, self._placement)  # placeholder while timing
         with _TUNE_LOCK:
             table = _load_tune_cache()
         hit = table.get(key)
         if hit is not None:
-            self._tuned[(B, learned)] = (int(hit["block"]), int(hit["requeue_below"]))
+            self._tuned[(B, learned)] = (int(hit["block"]), int(hit["requeue_below"]), int(hit.get("placement", -1)))
             self._plans.pop((B, learned), None)
             return
         best = None
-        for blk, rq in self._candidates(B, learned):
-            self._tuned[(B, learned)] = (blk, rq)
+        for blk, rq, pl in self._candidates(B, learned):
+            self._tuned[(B, learned)] = (blk, rq, pl)
             self._plans.pop((B, learned), None)
             try:
                 if self.plan(B, learned).requeue_below != rq:
@@ -541,18 +553,18 @@ class RelayBPDecoder:
                 self.kernel_events = None
                 continue  # a shape the graph does not allow
             t = min(ms)
-            if best is None or t < best[2]:
-                best = (blk, rq, t)
+            if best is None or t < best[3]:
+                best = (blk, rq, pl, t)
         self.tune_runs += 1
         if best is None:
             del self._tuned[(B, learned)]
             self._plans.pop((B, learned), None)
             return
-        self._tuned[(B, learned)] = (best[0], best[1])
+        self._tuned[(B, learned)] = (best[0], best[1], best[2])
         self._plans.pop((B, learned), None)
         with _TUNE_LOCK:
             table = _load_tune_cache()
-            table[key] = {"block": best[0], "requeue_below": best[1], "ms": round(best[2], 4)}
+            table[key] = {"block": best[0], "requeue_below": best[1], "placement": best[2], "ms": round(best[3], 4)}
             _save_tune_cache(table)
 
     def _shot_clock_for(self, B: int) -> Optional[torch.Tensor]:

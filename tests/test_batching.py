@@ -54,7 +54,8 @@ def test_graph_replays_match_eager_decodes():
 
 def test_decode_tunes_its_launch_shape_on_first_sight(tmp_path, monkeypatch):
     """autotune=True: the first decode of a batch size times the candidate
-    launch shapes, keeps the fastest and writes it to the cache; the results
+    launch shapes (block size, re-queue threshold, state placement), keeps the
+    fastest and writes it to the cache; the results
     equal an untuned decoder's, and a second decoder of the same problem takes
     the shape from the cache without timing anything."""
     from pelagos_amd import RelayBPDecoder
@@ -68,14 +69,14 @@ def test_decode_tunes_its_launch_shape_on_first_sight(tmp_path, monkeypatch):
     print("[gpu] autotune", flush=True)
     out = dec.decode(syn)
     assert dec.tune_runs == 1 and tune_cache_path().exists()
-    blk, rq = dec._tuned[(256, False)]
-    assert blk in (256, 512, 1024) and rq in (0, 16, 32)
-    assert dec.plan(256).block == blk and dec.plan(256).requeue_below == rq
+    blk, rq, pl = dec._tuned[(256, False)]
+    assert blk in (256, 512, 1024) and rq in (0, 16, 32) and pl in (15, 7)
+    assert dec.plan(256).block == blk and dec.plan(256).requeue_below == rq and dec.plan(256).placement == pl
     for key in ("errors", "weights", "iterations", "found_count"):
         assert torch.equal(out[key], base[key]), key
     again = RelayBPDecoder(H, p, autotune=True, **RELAY)
     out2 = again.decode(syn)
-    assert again.tune_runs == 0 and again._tuned[(256, False)] == (blk, rq)
+    assert again.tune_runs == 0 and again._tuned[(256, False)] == (blk, rq, pl)
     for key in ("errors", "weights", "iterations", "found_count"):
         assert torch.equal(out2[key], base[key]), key
 
