@@ -1680,18 +1680,30 @@ __device__ __forceinline__ void engine_shot(const KArgs &a, int b, uint8_t *smem
     }
 }
 
-// A shot-per-wave job: every wave of the workgroup takes the job's next shot
-// (lane 0 takes the ticket) until the job has none left.
+// Shot-per-wave jobs from job j on: every wave of the workgroup takes the job's
+// next shot (lane 0 takes the ticket) until the job has none left, then goes
+// on by itself to the next job while the jobs keep job j's kind and per-wave
+// LDS stride (its region of LDS is laid out the same), skipping empty ones.  A
+// wave thus never waits for the workgroup's slower waves between such jobs
+// (a launch of many batches of a small code); the workgroup meets again only
+// at a job of another kind, or at the end.
 template <int PLACE>
-__device__ __forceinline__ void engine_wave_job(const KArgs &a, uint32_t stride, uint8_t *smem) {
+__device__ __forceinline__ void engine_wave_jobs(const JobDesc *__restrict__ jobs, int j, int n_jobs, uint8_t *smem) {
     const uint32_t wave = threadIdx.x >> 6;
-    uint8_t *slot = a.spill + ((int64_t)blockIdx.x * kEngineWaves + wave) * a.spill_stride;
-    for (;;) {
-        int t = 0;
-        if ((opaque_tid() & 63u) == 0) t = atomicAdd(&a.ctrl[kCtrlFirst], 1);
-        t = uniform_i32(t);  // lane 0's ticket (the first active lane)
-        if (t >= a.B) break;
-        engine_shot<64, PLACE, false>(a, a.order[t], smem, wave * stride, slot);
+    const int kind = uniform_i32(jobs[j].kind);
+    const uint32_t stride = (uint32_t)uniform_i32((int)jobs[j].stride);
+    for (int jj = j; jj < n_jobs; ++jj) {
+        if (uniform_i32(jobs[jj].kind) != kind || (uint32_t)uniform_i32((int)jobs[jj].stride) != stride) break;
+        const KArgs &a = jobs[jj].a;
+        if (a.B <= 0) continue;  // an empty job has no workspace
+        uint8_t *slot = a.spill + ((int64_t)blockIdx.x * kEngineWaves + wave) * a.spill_stride;
+        for (;;) {
+            int t = 0;
+            if ((opaque_tid() & 63u) == 0) t = atomicAdd(&a.ctrl[kCtrlFirst], 1);
+            t = uniform_i32(t);  // lane 0's ticket (the first active lane)
+            if (t >= a.B) break;
+            engine_shot<64, PLACE, false>(a, a.order[t], smem, wave * stride, slot);
+        }
     }
 }
 
@@ -1760,8 +1772,8 @@ __global__ void __launch_bounds__(kEngineBlock) engine_kernel(EngineArgs e, cons
         PEL_ENGINE_CASE(0, (engine_shot<kEngineBlock, 0, false>(a, b, smem, 0, slot, reuse)))
         PEL_ENGINE_CASE(15 | kKindLearn, (engine_shot<kEngineBlock, 15, true>(a, b, smem, 0, slot, reuse)))
         PEL_ENGINE_CASE(7 | kKindLearn, (engine_shot<kEngineBlock, 7, true>(a, b, smem, 0, slot, reuse)))
-        PEL_ENGINE_CASE(15 | kKindWave, (engine_wave_job<15>(a, jobs[j].stride, smem)))
-        PEL_ENGINE_CASE(7 | kKindWave, (engine_wave_job<7>(a, jobs[j].stride, smem)))
+        PEL_ENGINE_CASE(15 | kKindWave, (engine_wave_jobs<15>(jobs, j, e.n_jobs, smem)))
+        PEL_ENGINE_CASE(7 | kKindWave, (engine_wave_jobs<7>(jobs, j, e.n_jobs, smem)))
 #undef PEL_ENGINE_CASE
     }
 }

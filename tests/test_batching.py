@@ -170,3 +170,33 @@ def test_one_launch_over_batches_of_one_code():
     jl.launch()
     torch.cuda.synchronize()
     _check_jobs(jl, syn, {f"b{i}": dec for i in range(5)})
+
+
+@pytest.mark.gpu
+def test_waves_go_on_across_batches_of_a_small_code():
+    """Four batches of the [[72,12,6]] code decoded a shot per wave, an empty
+    batch among them and a 936 x 8785 batch after them, in one launch: the
+    waves move from batch to batch on their own, and every batch equals its own
+    decode()."""
+    from pelagos_amd import RelayBPDecoder
+    from pelagos_amd.batching import JobLaunch
+
+    Hb = codes.bb_72_12_6()
+    pb = np.full(72, 0.03)
+    small = RelayBPDecoder(Hb, pb, **RELAY)
+    H, p = codes.banded_circuit_code(seed=0)
+    big = RelayBPDecoder(H, p, **RELAY)
+    jobs = [("s0", small, 700), ("s1", small, 300), ("e", small, 0), ("s2", small, 513), ("s3", small, 64),
+            ("big", big, 40)]
+    jl = JobLaunch(jobs)
+    assert jl.roles()["s0"]["group"] == 64 and jl.roles()["big"]["group"] == 1024
+    syn = {}
+    for i, (name, dec, n) in enumerate(jobs):
+        HH, pp = (Hb, pb) if dec is small else (H, p)
+        err = codes.sample_errors(pp, n, seed=80 + i, scale=1.5)
+        syn[name] = torch.from_numpy(codes.syndromes_of(HH, err)).cuda().contiguous()
+        jl.syndromes[name].copy_(syn[name])
+    print("[gpu] waves across small-code batches", flush=True)
+    jl.launch()
+    torch.cuda.synchronize()
+    _check_jobs(jl, syn, {name: dec for name, dec, _ in jobs})
