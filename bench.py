@@ -264,6 +264,7 @@ def engine_measure(dec, syn, steps: int, warmup: int, jobs: int) -> dict:
     from pelagos_amd.batching import JobLaunch
 
     B = int(syn.shape[0])
+    jobs = max(1, min(jobs, steps))  # fewer steps than batches per launch: one launch of all of them
     full = JobLaunch([(f"s{i}", dec, B) for i in range(jobs)])
     rem = steps % jobs
     part = JobLaunch([(f"s{i}", dec, B) for i in range(rem)]) if rem else None
@@ -434,7 +435,7 @@ def code_capacity_bb72(args) -> dict:
     m = engine_measure(dec, syn, args.steps, args.warmup, args.jobs)
     seq = one_at_a_time(dec, syn, max(3, args.steps // 4))
     shot_iters = m["batch_iterations"]
-    kern = sum(m["kernel_ms"]) / len(m["kernel_ms"]) / args.jobs
+    kern = sum(m["kernel_ms"]) / len(m["kernel_ms"]) / m["jobs_per_launch"]
     lds_b = 20 * E + 20 * V + 16 * C
     dec_l = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0], **RELAY)
     dec_l.requeue_below = 0
@@ -446,7 +447,7 @@ def code_capacity_bb72(args) -> dict:
     ler = ObservableDecoderRunner(dec, L).logical_error_rate(codes.sample_errors(p, 16384, seed=1))
     res = {"code": "[[72,12,6]] BB, H_X 36x72, priors 0.03", "shots": 1024,
            "shots_per_s": round(m["rate"], 1), "kernel_ms_per_batch": round(kern, 4),
-           "role": m["roles"], "jobs_per_launch": args.jobs,
+           "role": m["roles"], "jobs_per_launch": m["jobs_per_launch"],
            "one_at_a_time": seq, "mean_iterations": round(shot_iters / 1024, 3),
            "lds_bytes_per_iteration": lds_b,
            "lds_gb_per_s": round(lds_b * shot_iters / (kern * 1e-3) / 1e9, 1),
@@ -665,7 +666,7 @@ def main() -> int:
             "pre_iter": RELAY["pre_iter"], "num_sets": RELAY["num_sets"],
             "set_max_iter": RELAY["set_max_iter"], "stop_nconv": RELAY["stop_nconv"],
             "gamma0": RELAY["gamma0"], "gamma_dist_interval": list(RELAY["gamma_dist_interval"]),
-            "launch": f"pel_decode_jobs, {args.jobs} batches per launch, one stream",
+            "launch": f"pel_decode_jobs, {m['jobs_per_launch']} batches per launch, one stream",
             "role": m["roles"],
         },
         "roofline": roof_plain,

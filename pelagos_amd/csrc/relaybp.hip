@@ -1163,7 +1163,10 @@ __device__ __forceinline__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m,
         // the same words.  In the fast placements (CS and SB at LDS [0, lsb))
         // the first one is copied to the workgroup's slot and later legs copy it
         // back.  Each thread restores exactly the words it saved.
-        const bool restore = t == 0 && ShotMem<PLACE>::kFast && st.first_ok;
+        // A one-wave shot (a small code) recomputes the pass: a few rows per lane cost
+        // less than the copy's round trip through L2.
+        constexpr bool kCopyFirst = ShotMem<PLACE>::kFast && BLOCK > 64;
+        const bool restore = t == 0 && kCopyFirst && st.first_ok;
         if (restore) {
             restore_first<BLOCK, PLACE>(m);
             // the reloaded words are in LDS before this wave goes on (a one-wave
@@ -1175,7 +1178,7 @@ __device__ __forceinline__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m,
             for (int i = gtid<BLOCK>(); i < C; i += BLOCK) bad |= check_row<false, PLACE>(a, m, i, alpha, syn);
         }
         bad = block_or<BLOCK>(bad, s_flag);
-        if (t == 0 && ShotMem<PLACE>::kFast && !st.first_ok) {
+        if (t == 0 && kCopyFirst && !st.first_ok) {
             // read before the barrier that ends this iteration; the next check
             // pass rewrites CS and SB only after it
             for (int i = gtid<BLOCK>(); i < m.first_words; i += BLOCK) gmem_w(m.first)[i] = *m.template lds<vec4f>(16u * (uint32_t)i);
@@ -1692,6 +1695,8 @@ __device__ __forceinline__ void engine_wave_jobs(const JobDesc *__restrict__ job
     const uint32_t wave = threadIdx.x >> 6;
     const int kind = uniform_i32(jobs[j].kind);
     const uint32_t stride = (uint32_t)uniform_i32((int)jobs[j].stride);
+    // what the wave's region of LDS holds from its last shot (as in engine_kernel)
+    const void *last_graph = nullptr, *last_prior = nullptr;
     for (int jj = j; jj < n_jobs; ++jj) {
         if (uniform_i32(jobs[jj].kind) != kind || (uint32_t)uniform_i32((int)jobs[jj].stride) != stride) break;
         const KArgs &a = jobs[jj].a;
@@ -1702,7 +1707,10 @@ __device__ __forceinline__ void engine_wave_jobs(const JobDesc *__restrict__ job
             if ((opaque_tid() & 63u) == 0) t = atomicAdd(&a.ctrl[kCtrlFirst], 1);
             t = uniform_i32(t);  // lane 0's ticket (the first active lane)
             if (t >= a.B) break;
-            engine_shot<64, PLACE, false>(a, a.order[t], smem, wave * stride, slot);
+            const bool reuse = a.g.rslice == last_graph && a.prior == last_prior;
+            last_graph = a.g.rslice;
+            last_prior = a.prior;
+            engine_shot<64, PLACE, false>(a, a.order[t], smem, wave * stride, slot, reuse);
         }
     }
 }
