@@ -1112,6 +1112,25 @@ template <int BLOCK, int PLACE>
 __device__ __forceinline__ void select_solution(const KArgs &a, const ShotMem<PLACE> &m, const float *__restrict__ wj,
                                 ShotState &st, double *s_red) {
     const int V = a.g.V;
+    if (st.best_w < 1e20f) {
+        // A best solution exists (its bits are the output bits).  A leg that found
+        // the same one again (the common case) cannot replace it: equal weight is
+        // no strict improvement.  Comparing the bits (a ballot per 64 positions) is
+        // cheaper than the weight sum, which needs the weights from L2 and a double
+        // reduction.  The reduction's scratch holds the comparison's flags here; a
+        // barrier separates them from the weight sum that may follow.
+        int diff = 0;
+        for (int vb = (int)(gtid<BLOCK>() & ~63u); vb < V; vb += BLOCK) {
+            const float2 ls = m.ls_pos(min(vb + (int)(gtid<BLOCK>() & 63u), V));  // position V: +inf, a 0
+            const uint64_t bits = __ballot(ls.x + ls.y <= 0.f);
+            diff |= bits != *m.template lds<uint64_t>(a.best_lds + 8u * (uint32_t)(vb >> 6)) ? 1 : 0;
+        }
+        if (!block_or<BLOCK>(diff, reinterpret_cast<int *>(s_red))) {
+            st.found += 1;
+            return;
+        }
+        gsync<BLOCK>();
+    }
     double acc = 0.0;
     // Position order: contiguous state reads.  The double sum of n fp32
     // weights is exact, whatever the order, while the set weights span less
