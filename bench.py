@@ -122,7 +122,7 @@ def cpu_baseline(graph, syn_np, lam, gtab, target_s: float = 10.0, single_sample
         done += B
         reps += 1
         dt = time.perf_counter() - t0
-        if dt >= target_s or reps >= 50:
+        if dt >= target_s or reps >= 10000:
             break
     n1 = min(B, single_sample)
     t1 = time.perf_counter()
