@@ -357,3 +357,40 @@ def test_early_exit_threshold_zero_runs_the_whole_schedule(regular_case):
         torch.cuda.synchronize()
         for k in ("errors", "weights", "iterations", "found_count", "last_posterior"):
             assert torch.equal(stepped[k], whole[k]), (bool(extra), k)
+
+
+def test_bitpack_output_and_llr_column_overrides(regular_case):
+    """bitpack_output=True returns the same errors packed 32 to an int32 word
+    (utils.py:32-60); override_llr_columns writes priors and selection weights
+    of some columns past the clip, the decode follows them as the oracle does
+    with those priors, and the returned callable restores the decoder
+    (decoder.py:786-835)."""
+    from pelagos_amd import utils
+
+    H, priors, S = regular_case
+    kw = dict(pre_iter=8, num_sets=6, set_max_iter=8, stop_nconv=3, seed=7)
+    dec = _decoder(H, priors, **kw)
+    packed = _decoder(H, priors, bitpack_output=True, **kw)
+    syn = torch.from_numpy(S).to("cuda:0")
+    print("[gpu] bitpack and overrides", flush=True)
+    base = dec.decode(syn)
+    outp = packed.decode(syn)
+    torch.cuda.synchronize()
+    V = H.shape[1]
+    assert outp["errors"].dtype == torch.int32 and tuple(outp["errors"].shape) == (S.shape[0], (V + 31) // 32)
+    assert torch.equal(outp["errors"], utils.bitpack_errors(base["errors"]))
+    assert torch.equal(utils.bitunpack_errors(outp["errors"], V), base["errors"])
+    for k in ("weights", "iterations", "found_count"):
+        assert torch.equal(outp[k], base[k]), k
+
+    undo_a = dec.override_llr_columns([0, 5, 17], 25.0)
+    undo_b = dec.override_llr_columns([40, 41], -2.0)
+    out = _run(dec, S)
+    _compare(out, _oracle_for(dec, S))    # the decoder's priors now hold the overrides
+    assert not torch.equal(out["errors"], base["errors"]) or not torch.equal(out["weights"], base["weights"])
+    undo_b()
+    undo_a()
+    again = dec.decode(syn)
+    torch.cuda.synchronize()
+    for k in ("errors", "weights", "iterations", "found_count"):
+        assert torch.equal(again[k], base[k]), k
