@@ -124,6 +124,28 @@ def test_gamma_draw_leaves_global_rng_alone():
     assert torch.equal(a, b)
 
 
+def test_concurrent_gamma_draws_match_sequential():
+    """Seeded draws from several threads at once equal the same draws made one
+    after another: each call has its own generator (the reference's
+    test_gamma_sampling.py checks the same of its samplers)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    intervals = torch.tensor([[-0.24, 0.66], [0.0, 0.5], [0.1, 0.2]] * 20)
+    jobs = [(seed, kind) for seed in range(24) for kind in ("uniform", "per_variable")]
+
+    def draw(job):
+        seed, kind = job
+        if kind == "uniform":
+            return utils.sample_gamma_uniform(4, 60, -0.24, 0.66, "cpu", seed=seed)
+        return utils.sample_gamma_per_variable(4, 60, intervals, "cpu", seed=seed)
+
+    sequential = [draw(j) for j in jobs]
+    with ThreadPoolExecutor(max_workers=8) as pool:
+        concurrent = list(pool.map(draw, jobs))
+    for a, b in zip(sequential, concurrent):
+        assert torch.equal(a, b)
+
+
 def test_per_variable_gamma_respects_intervals():
     V = 40
     iv = torch.stack([torch.linspace(-0.5, 0.0, V), torch.linspace(0.1, 0.9, V)], 1)
