@@ -17,7 +17,9 @@ and of the learned form (trained weights) on the same shots; ``value`` is the
 learned form's rate when its logical error rate, on a paired sample at mean
 prior 0.003 where failures are seen, lies in the plain form's Wilson 95%
 interval and the paired discordance test does not reject equality, else the
-plain rate.  Also: the decode kernel's roofline object (algorithmic bytes per
+plain rate.  Beside it, the same network with one learned leg fewer (one more
+uniform exploration leg): its rate and its paired error rate on the same
+shots.  Also: the decode kernel's roofline object (algorithmic bytes per
 launch over the launch time, with the measured HBM traffic and the shader
 counters of profiles/ beside it), one launch at a time, the [[72,12,6]] code
 at code capacity with its CPU baseline, one launch over the three codes at
@@ -698,7 +700,14 @@ def main() -> int:
         dl3 = RelayBPDecoder(H, p3, data_type="float32", algo="relay", seed=0, device=[0], learned_gamma=net,
                              learned_legs=legs, learned_last_damp=damp, **RELAY)
         ler_w = paired_ler({"plain": dec, "learned": dl}, H, p, args.ler_batches, 4096, 101)
-        ler_3 = paired_ler({"plain": d3, "learned": dl3}, H, p3, args.ler_batches, 4096, 2001)
+        # the same network with one learned leg fewer (one more uniform exploration leg), on the same shots
+        fewer = legs - 1 if legs > 1 else 0
+        decs3 = {"plain": d3, "learned": dl3}
+        if fewer:
+            decs3["learned_fewer_legs"] = RelayBPDecoder(H, p3, data_type="float32", algo="relay", seed=0, device=[0],
+                                                         learned_gamma=net, learned_legs=fewer,
+                                                         learned_last_damp=damp, **RELAY)
+        ler_3 = paired_ler(decs3, H, p3, args.ler_batches, 4096, 2001)
         plain_ci = ler_3["plain"]["ler"]
         matched = (plain_ci[1] <= ler_3["learned"]["ler"][0] <= plain_ci[2]) and ler_3["learned"]["mcnemar_p"] >= 0.05
         result["learned"] = {"shots_per_s": round(ml["rate"], 1),
@@ -708,6 +717,17 @@ def main() -> int:
                              "mean_iterations": round(ml["batch_iterations"] / args.shots, 3),
                              "learned_legs": legs, "learned_last_damp": damp,
                              "weights": str(wpath.relative_to(ROOT))}
+        if fewer:
+            dlf = RelayBPDecoder(H, p, data_type="float32", algo="relay", seed=0, device=[0], learned_gamma=net,
+                                 learned_legs=fewer, learned_last_damp=damp, **RELAY)
+            mf = engine_measure(dlf, syn, args.steps, args.warmup, args.jobs)
+            print(f"[bench] learned, {fewer} legs: {mf['rate']:.0f} shots/s", file=sys.stderr, flush=True)
+            result["learned_fewer_legs"] = {
+                "learned_legs": fewer, "shots_per_s": round(mf["rate"], 1),
+                "speedup_vs_plain": round(mf["rate"] / m["rate"], 3), "matches_decode": mf["matches"],
+                "mean_iterations": round(mf["batch_iterations"] / args.shots, 3),
+                "ler_mean_prior_0.003": ler_3.pop("learned_fewer_legs"),
+                "note": "the shipped network with one more uniform exploration leg (fewer learned legs)"}
         result["ler_paired"] = {"workload_priors": ler_w, "mean_prior_0.003": ler_3,
                                 "matched": bool(matched),
                                 "criterion": "learned LER inside the plain form's Wilson 95% interval at mean "
