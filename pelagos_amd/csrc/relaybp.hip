@@ -162,7 +162,9 @@ struct KArgs {
     uint8_t *hard_out;         // [B][learned_legs][V] hard decision there, or null
     float *gamma_out;          // [B][learned_legs][V] strengths used, or null
     uint32_t flips_lds, unsat_lds, gamma_lds;  // LDS addresses: flip counts u8 [V+1], violated checks
-                                               // u8 [C+1], learned shifts bf16 [V]
+                                               // u8 [C+1], learned shifts i8 [V]
+    uint32_t gamma_slot;       // byte offset in the group's workspace slot of a shifted leg's
+                               // strengths, fp32 [V] by position
     float learned_lo, learned_hi;              // strength interval of the relay legs
     float last_scale;                          // the shift's factor on the last learned leg
     int8_t *rq_gamma;          // [B][V] relay-leg-0 strength shifts of re-queued shots (by position)
@@ -288,8 +290,22 @@ struct ShotMem {
     }
 };
 
-// Memory strength of the variable at position v for one leg: a row of V values
-// (in position order or the caller's order) or a constant.
+// Memory strength of the variable at position v for one leg, as the variable
+// pass reads it: a row of V values (in position order or the caller's order) or
+// a constant.
+struct GammaRow {
+    const float *row;
+    const int32_t *perm;  // non-null: row is in the caller's order
+    float konst;
+    __device__ __forceinline__ float operator()(int v) const {
+        if (!row) return konst;
+        return perm ? gmem(row)[gmem(perm)[v]] : gmem(row)[v];
+    }
+};
+
+// A leg's strengths before learned shifts are applied: the row, and optionally
+// per-variable shifts in LDS (a shifted leg writes clamp(u + shift, lo, hi) to
+// a row of its own before its iterations).
 struct GammaSrc {
     const float *row;
     const int32_t *perm;  // non-null: row is in the caller's order
@@ -642,7 +658,7 @@ __device__ __forceinline__ void var_messages(const ShotMem<PLACE> &m, const uint
 // Variable pass for variable position v (kernels.py:1299-1350).  The incoming
 // messages are added one at a time in ascending edge id.
 template <int PLACE, bool FLIPS = false>
-__device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> &m, int v, const GammaSrc &gs) {
+__device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> &m, int v, const GammaRow &gs) {
     const int2 vs = var_slice(a, m, v);  // {ent, width}
     const float g = gs(v);
     const float l0 = (PLACE & (1 << kL0)) ? m.l0_pos(v) : gmem(m.l0g)[gmem(a.g.vperm)[v]];
@@ -690,8 +706,9 @@ __device__ __forceinline__ void var_update(const KArgs &a, const ShotMem<PLACE> 
 //
 // Network evaluation runs on the matrix cores (bf16 operands, fp32
 // accumulation), transposed so that each layer's output is already the next
-// layer's B operand (H1^T = W1 X^T, H2^T = W2 H1^T, 32 variables per product;
-// see row32 below): no data moves between layers.
+// layer's B operand (H1^T = W1 X^T, H2^T = W2 H1^T, z = w3 H2^T with w3 on every
+// row of its A operand, 32 variables per product; see row32 below): no data moves
+// between layers.
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
@@ -718,13 +735,14 @@ struct MlpRegs {
                      // nearest its remainder (lanes 32-63)
     bf16x8_t w2[2];  // layer 2, row h: the weights of hidden units row32(lane / 32, 8 i + e)
     f32x16_t b2;     // layer-2 accumulator start: b2 of rows row32(lane / 32, e)
-    uint32_t w3[8];  // w3 of rows row32(lane / 32, 2 k), row32(lane / 32, 2 k + 1) as bf16 pairs
-    float b3h;       // b3 in lanes 0-31, 0 in lanes 32-63 (the two halves of a column are summed)
+    bf16x8_t w3[2];  // layer 3 as a 32x32 A operand whose rows all equal w3: the weights of
+                     // hidden units row32(lane / 32, 8 i + e)
+    float b3;
 };
 
 // The network as each lane's MFMA operands, laid out [lane][kMlpStride words]
 // so that a lane loads its registers with ten 16-byte loads from one base
-// address (immediate offsets): words 0-3 w1, 4-11 w2, 12-27 b2, 28-35 w3, 36 b3h,
+// address (immediate offsets): words 0-3 w1, 4-11 w2, 12-27 b2, 28-35 w3, 36 b3,
 // 37-39 zero.
 constexpr int kMlpWords = 37, kMlpStride = 40;
 
@@ -771,7 +789,7 @@ __global__ void __launch_bounds__(64) mlp_frag_kernel(const float *__restrict__ 
     for (int e = 0; e < 16; ++e) put(12 + e, __float_as_uint(p[kMB2 + row32(half, e)]));
     for (int k = 0; k < 8; ++k)
         put(28 + k, pack_bf16x2(p[kMW3 + row32(half, 2 * k)], p[kMW3 + row32(half, 2 * k + 1)]));
-    put(36, __float_as_uint(half ? 0.f : p[kMB3]));
+    put(36, __float_as_uint(p[kMB3]));
     for (int w = kMlpWords; w < kMlpStride; ++w) put(w, 0u);
 }
 
@@ -795,15 +813,16 @@ __device__ __forceinline__ void load_mlp(const uint32_t *__restrict__ frag, MlpR
 #pragma unroll
     for (int e = 0; e < 16; ++e) r.b2[e] = __uint_as_float(w[12 + e]);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) r.w3[k] = w[28 + k];
-    r.b3h = __uint_as_float(w[36]);
+    for (int i = 0; i < 2; ++i)
+        r.w3[i] = __builtin_bit_cast(bf16x8_t, u32x4{w[28 + 4 * i], w[29 + 4 * i], w[30 + 4 * i], w[31 + 4 * i]});
+    r.b3 = __uint_as_float(w[36]);
 }
 
-// Half of the logit of column l % 32 of a 32-variable group from the
-// layer-1 input x (B operand): layer 1 (bias through K slot 7), bf16 ReLU,
-// layer 2 from b2, bf16 ReLU, and w3 . h2 over this lane's 16 rows (bf16 pair
-// dot products, v_dot2_f32_bf16) from b3h.
-__device__ __forceinline__ float mlp_half_logit(const MlpRegs &r, bf16x8_t x) {
+// w3 . h2 of column l % 32 of a 32-variable group (without b3) from the layer-1
+// input x (B operand): layer 1 (bias through K slot 7), bf16 ReLU, layer 2 from
+// b2, bf16 ReLU, and layer 3 as two more products whose A rows all equal w3, so
+// that every accumulator of lane l holds the column's sum.
+__device__ __forceinline__ float mlp_group_logit(const MlpRegs &r, bf16x8_t x) {
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     const f32x16_t zero = {};
     const f32x16_t d1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(r.w1, x, zero, 0, 0, 0);
@@ -814,13 +833,14 @@ __device__ __forceinline__ float mlp_half_logit(const MlpRegs &r, bf16x8_t x) {
         r.w2[0], __builtin_bit_cast(bf16x8_t, u32x4{h1[0], h1[1], h1[2], h1[3]}), r.b2, 0, 0, 0);
     d2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
         r.w2[1], __builtin_bit_cast(bf16x8_t, u32x4{h1[4], h1[5], h1[6], h1[7]}), d2, 0, 0, 0);
-    float part = r.b3h;
+    uint32_t h2[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-        part = __builtin_amdgcn_fdot2_f32_bf16(
-            __builtin_bit_cast(bf16x2_t, relu_bf16x2(pack_bf16x2(d2[2 * k], d2[2 * k + 1]))),
-            __builtin_bit_cast(bf16x2_t, r.w3[k]), part, false);
-    return part;
+    for (int k = 0; k < 8; ++k) h2[k] = relu_bf16x2(pack_bf16x2(d2[2 * k], d2[2 * k + 1]));
+    f32x16_t d3 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+        r.w3[0], __builtin_bit_cast(bf16x8_t, u32x4{h2[0], h2[1], h2[2], h2[3]}), zero, 0, 0, 0);
+    d3 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+        r.w3[1], __builtin_bit_cast(bf16x8_t, u32x4{h2[4], h2[5], h2[6], h2[7]}), d3, 0, 0, 0);
+    return d3[0];
 }
 
 // Logit of the variable whose features this lane packed (bf16 pairs: features
@@ -829,8 +849,8 @@ __device__ __forceinline__ float mlp_half_logit(const MlpRegs &r, bf16x8_t x) {
 // variables 0-31, of group B 32-63.  Lane l of a product needs the features of
 // column l % 32 (both halves carry them, against the hi and the lo weights):
 // one permlane32 swap of a word with itself gives group A's word (lanes 32-63
-// take lanes 0-31's) and group B's (lanes 0-31 take lanes 32-63's).  The two
-// halves of a column's sum come together with one more swap.
+// take lanes 0-31's) and group B's (lanes 0-31 take lanes 32-63's).  Lane l's
+// own column is column l % 32 of group A (l < 32) or of group B.
 __device__ __forceinline__ float mlp_logit(const MlpRegs &r, const uint32_t (&pk)[4]) {
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     uint32_t xa[4], xb[4];
@@ -840,14 +860,26 @@ __device__ __forceinline__ float mlp_logit(const MlpRegs &r, const uint32_t (&pk
         xa[w] = t[0];
         xb[w] = t[1];
     }
-    const float pa = mlp_half_logit(r, __builtin_bit_cast(bf16x8_t, u32x4{xa[0], xa[1], xa[2], xa[3]}));
-    const float pb = mlp_half_logit(r, __builtin_bit_cast(bf16x8_t, u32x4{xb[0], xb[1], xb[2], xb[3]}));
-    // lanes 0-31: pa (own column) + pa of lane l + 32; lanes 32-63: pb of lane l - 32 + pb (own)
-    const auto t = __builtin_amdgcn_permlane32_swap(__float_as_uint(pa), __float_as_uint(pb), false, false);
-    return __uint_as_float(t[0]) + __uint_as_float(t[1]);
+    const float za = mlp_group_logit(r, __builtin_bit_cast(bf16x8_t, u32x4{xa[0], xa[1], xa[2], xa[3]}));
+    const float zb = mlp_group_logit(r, __builtin_bit_cast(bf16x8_t, u32x4{xb[0], xb[1], xb[2], xb[3]}));
+    return ((threadIdx.x & 32u) ? zb : za) + r.b3;
 }
 
 __device__ __forceinline__ float clamp4(float x) { return fminf(fmaxf(x, -4.f), 4.f); }
+
+// 1/deg correctly rounded, for deg >= 1: the reciprocal instruction (within 1
+// ulp) and one Newton step in fused multiply-adds give it for every integer in
+// [1, 65535] (an exhaustive check over every float within 1 ulp of 1/deg); wider
+// columns take the division.
+__device__ __forceinline__ float inv_degree(int deg) {
+    float d = (float)deg;
+    if (deg > 65535) {
+        asm volatile("" : "+v"(d));  // keeps the division in this branch
+        return 1.f / d;
+    }
+    const float r = __builtin_amdgcn_rcpf(d);
+    return __builtin_fmaf(r, __builtin_fmaf(-d, r, 1.f), r);
+}
 
 // Features of every variable at the start of relay leg r (the state the
 // previous leg ended in), scaled to O(1):
@@ -884,11 +916,11 @@ struct VarFeatures {
     float l0, mn;
     int deg, uns;
     int2 vs;
-    // Columns k0 .. k0 + W - 1 of the slice (W <= its width): the W index loads
-    // are issued together (uniform base, the lane's offset, immediates), then
-    // the code words, then the messages; slots past this variable's degree are
-    // not counted.
-    template <int W>
+    // Columns k0 .. k0 + W - 1 of the slice (W <= its width; WHOLE: k0 = 0 and
+    // W is the width): the W index loads are issued together (uniform base, the
+    // lane's offset, immediates), then the code words, then the messages; slots
+    // past this variable's degree are not counted.
+    template <int W, bool WHOLE = false>
     __device__ __forceinline__ void gather(const KArgs &a, const ShotMem<PLACE> &m, int vc, int k0) {
         const uint2 *__restrict__ base = a.g.var_ent + vs.x + (int64_t)k0 * kSlice;
         const uint32_t lane_off = 8u * (uint32_t)(vc & 63);
@@ -903,11 +935,24 @@ struct VarFeatures {
             msg[u] = m.cs_msg(e[u].x + (__builtin_amdgcn_ubfe(w, e[u].y, 2u) << m.cs_shift));
             un[u] = KNOWN_VALID ? 0 : (int)*m.template lds<uint8_t>(a.unsat_lds + (e[u].x >> 2));
         }
+        if (WHOLE && __ballot(deg != W) == 0) {
+            // every variable of the slice has its width (variables sit by decreasing
+            // degree, so this is every slice but a few): every slot counts
 #pragma unroll
-        for (int u = 0; u < W; ++u) {  // selects, not branches: no load is sunk into a branch
-            const bool in = k0 + u < deg;
-            mn = fminf(mn, in ? msg[u] : INFINITY);
-            uns += in ? un[u] : 0;
+            for (int u = 0; u < W; ++u) {
+                mn = fminf(mn, msg[u]);
+                uns += un[u];
+            }
+        } else {
+            // (the opaque copy keeps the slot tests in this branch)
+            int dg = deg;
+            asm volatile("" : "+v"(dg));
+#pragma unroll
+            for (int u = 0; u < W; ++u) {  // selects, not branches: no load is sunk into a branch
+                const bool in = k0 + u < dg;
+                mn = fminf(mn, in ? msg[u] : INFINITY);
+                uns += in ? un[u] : 0;
+            }
         }
     }
     __device__ __forceinline__ void start(const KArgs &a, const ShotMem<PLACE> &m, int vc) {
@@ -919,14 +964,14 @@ struct VarFeatures {
         uns = 0;
         switch (vs.y) {
             case 0: break;
-            case 1: gather<1>(a, m, vc, 0); break;
-            case 2: gather<2>(a, m, vc, 0); break;
-            case 3: gather<3>(a, m, vc, 0); break;
-            case 4: gather<4>(a, m, vc, 0); break;
-            case 5: gather<5>(a, m, vc, 0); break;
-            case 6: gather<6>(a, m, vc, 0); break;
-            case 7: gather<7>(a, m, vc, 0); break;
-            case 8: gather<8>(a, m, vc, 0); break;
+            case 1: gather<1, true>(a, m, vc, 0); break;
+            case 2: gather<2, true>(a, m, vc, 0); break;
+            case 3: gather<3, true>(a, m, vc, 0); break;
+            case 4: gather<4, true>(a, m, vc, 0); break;
+            case 5: gather<5, true>(a, m, vc, 0); break;
+            case 6: gather<6, true>(a, m, vc, 0); break;
+            case 7: gather<7, true>(a, m, vc, 0); break;
+            case 8: gather<8, true>(a, m, vc, 0); break;
             default: {
                 int k0 = 0;
                 for (; k0 + 4 <= vs.y; k0 += 4) gather<4>(a, m, vc, k0);
@@ -950,11 +995,9 @@ __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const Shot
     const bool net = a.mlp && apply;
     // A wave takes kFeatBatch of its slices at a time (v0 is uniform, so every
     // lane of a wave runs the same rounds; the MFMAs need all 64 lanes), and
-    // the network's operands are loaded once per batch (from L1: holding them
-    // across the gathers would spill).  Lanes past V, and batch slots past the
-    // last slice, compute a copy of variable V - 1 (in the last slice) and store
-    // nothing.
-    // the network's operands stay in registers over the wave's slices
+    // the network's operands stay in registers over the wave's slices.  Lanes
+    // past V, and batch slots past the last slice, compute a copy of variable
+    // V - 1 (in the last slice) and store nothing.
     MlpRegs mr;
     if (net) load_mlp(a.mlp_frag, mr);
     for (int vb = gtid<BLOCK>() & ~63; vb < V; vb += kFeatBatch * BLOCK) {
@@ -968,7 +1011,7 @@ __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const Shot
             const int v = vb + j * BLOCK + lane;
             const int vc = min(v, V - 1);
             float f[8];
-            const float inv_deg = x.deg > 0 ? 1.f / (float)x.deg : 0.f;
+            const float inv_deg = x.deg > 0 ? inv_degree(x.deg) : 0.f;
             const float post = x.ls.x + x.ls.y;
             f[0] = x.l0 * 0.125f;
             f[1] = clamp4(post * 0.0625f);
@@ -976,7 +1019,7 @@ __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const Shot
             f[3] = clamp4(x.ls.y * inv_deg * 0.125f);
             f[4] = x.deg > 0 ? clamp4(x.mn * 0.125f) : 0.f;
             f[5] = (float)x.deg * 0.125f;
-            f[6] = (float)x.uns * inv_deg;
+            f[6] = known_valid ? 0.f : (float)x.uns * inv_deg;
             f[7] = (float)r * 0.25f;
             if (v < V && (a.feat_out || a.hard_out)) {
                 const size_t row = ((size_t)b * L + r) * V + a.g.vperm[v];
@@ -1169,7 +1212,7 @@ __device__ __forceinline__ void select_solution(const KArgs &a, const ShotMem<PL
 // count each variable's hard-decision flips (a feature of the learned
 // strengths; only legs whose end state feeds the network count them).
 template <int BLOCK, int PLACE, bool FLIPS = false>
-__device__ __forceinline__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m, int T, const GammaSrc &gs,
+__device__ __forceinline__ bool run_leg(const KArgs &a, const ShotMem<PLACE> &m, int T, const GammaRow &gs,
                         const float *__restrict__ wj, const uint8_t *__restrict__ syn,
                         ShotState &st, double *s_red, int *s_flag) {
     const int C = a.g.C, V = a.g.V;
@@ -1277,7 +1320,7 @@ __device__ __forceinline__ bool first_leg(const KArgs &a, int b, ShotMem<PLACE> 
     gsync<BLOCK>();
 
     st = ShotState{INFINITY, 0, 0, 0, 0, 0, 0};
-    GammaSrc g0{nullptr, nullptr, a.plain ? 0.f : a.gamma0};
+    GammaRow g0{nullptr, nullptr, a.plain ? 0.f : a.gamma0};
     if (!a.plain && a.init_gamma) {
         g0.row = a.init_gamma + (size_t)b * V;
         g0.perm = a.g.vperm;
@@ -1305,6 +1348,19 @@ __device__ __forceinline__ GammaSrc leg_gamma(const KArgs &a, int b, int leg) {
         gl.perm = a.gamma_by_position ? nullptr : a.g.vperm;
     }
     return gl;
+}
+
+// A shifted leg's strengths clamp(u + shift, lo, hi), by position, written once
+// to the group's workspace slot: the leg's iterations then read one float per
+// variable, as a uniform leg reads its row, instead of the row, the shift and
+// the clamp in every variable pass.
+template <int BLOCK, int PLACE>
+__device__ __forceinline__ GammaRow leg_strengths(const KArgs &a, const ShotMem<PLACE> &m, const GammaSrc &gl) {
+    if (!gl.lds) return GammaRow{gl.row, gl.perm, gl.konst};
+    float *g = reinterpret_cast<float *>(reinterpret_cast<uint8_t *>(m.first) + a.gamma_slot);
+    for (int v = gtid<BLOCK>(); v < a.g.V; v += BLOCK) *gmem_w(g + v) = gl(v);
+    gsync<BLOCK>();
+    return GammaRow{g, nullptr, 0.f};
 }
 
 // Stage 2: relay legs until stop_nconv solutions (decoder.py:3655-3835), then
@@ -1344,10 +1400,11 @@ __device__ __forceinline__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE>
                 if (a.gamma_out) record_strengths<BLOCK>(a, b, leg, gl);
                 count = trace && leg + 1 < a.learned_legs;
             }
+            const GammaRow gr = leg_strengths<BLOCK, PLACE>(a, m, gl);
             if (count)
-                st.last_ok = run_leg<BLOCK, PLACE, true>(a, m, a.set_max_iter, gl, io.wj, io.syn, st, s_red, s_flag);
+                st.last_ok = run_leg<BLOCK, PLACE, true>(a, m, a.set_max_iter, gr, io.wj, io.syn, st, s_red, s_flag);
             else
-                st.last_ok = run_leg<BLOCK, PLACE, false>(a, m, a.set_max_iter, gl, io.wj, io.syn, st, s_red, s_flag);
+                st.last_ok = run_leg<BLOCK, PLACE, false>(a, m, a.set_max_iter, gr, io.wj, io.syn, st, s_red, s_flag);
             if (leg == 0 && a.pre_iter == 0 && !a.last_leg && st.found == 0)
                 best_from_state<BLOCK, PLACE>(a, m, st.any_iter != 0);  // decoder.py:3809-3810
         }
@@ -2518,6 +2575,7 @@ struct StateSizes {
 // LDS addresses of the learned-strength arrays (after the slice tables).
 struct LearnLds {
     uint32_t flips, unsat, gamma;
+    uint32_t slot;  // workspace-slot byte offset of a shifted leg's strengths (fp32 [V])
 };
 
 }  // namespace
@@ -2603,7 +2661,7 @@ void place_arrays(const StateSizes &st, int place, int block, bool learn, uint32
     *scratch = (uint32_t)*lds;
     *lds += align_up(12LL * (block / 64) + 8, 16) + 16LL * st.nrs + align_up(8LL * st.nvs, 16) + align_up(st.C, 16) +
             align_up(8LL * st.nvs, 16);
-    LearnLds l{0, 0, 0};
+    LearnLds l{0, 0, 0, 0};
     if (learn) {  // flip counts u8 [V+1], violated checks u8 [C+1], strength shifts i8 [V]
         l.flips = (uint32_t)*lds;
         *lds += align_up(st.V + 1, 16);
@@ -2611,6 +2669,10 @@ void place_arrays(const StateSizes &st, int place, int block, bool learn, uint32
         *lds += align_up(st.C + 1, 16);
         l.gamma = (uint32_t)*lds;
         *lds += align_up(st.V, 16);
+        // the workspace slot: a shifted leg's strengths
+        *spill = align_up(*spill, 256);
+        l.slot = (uint32_t)*spill;
+        *spill += 4 * st.V;
     }
     if (ll) *ll = l;
 }
@@ -3250,6 +3312,7 @@ void fill_args(KArgs &a, const pel_graph *g, const pel_params *p, const pel_batc
     a.flips_lds = L.ll.flips;
     a.unsat_lds = L.ll.unsat;
     a.gamma_lds = L.ll.gamma;
+    a.gamma_slot = L.ll.slot;
     a.learned_lo = b->learned_lo;
     a.learned_hi = b->learned_hi;
     a.last_scale = 1.f - b->learned_last_damp;

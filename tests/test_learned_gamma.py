@@ -108,6 +108,52 @@ def test_oracle_bf16_inference_matches_the_network_when_rounding_is_exact():
         assert (np.rint(mo.shift_q8(q, x, lo, hi) / np.float32((hi - lo) / 254.0)) != steps).mean() > 0.1
 
 
+def _rn32(num: int, exp: int):
+    """num * 2**exp rounded to float32 (nearest, ties to even) as (mantissa, exponent)
+    with a mantissa of at most 24 bits; exact integer arithmetic, normal range."""
+    if num == 0:
+        return 0, 0
+    sign, n = (-1 if num < 0 else 1), abs(num)
+    shift = n.bit_length() - 24
+    if shift > 0:
+        q, rem, half = n >> shift, n & ((1 << shift) - 1), 1 << (shift - 1)
+        if rem > half or (rem == half and q & 1):
+            q += 1
+        n, exp = q, exp + shift
+    return sign * n, exp
+
+
+def test_reciprocal_with_one_newton_step_is_correctly_rounded():
+    """The feature pass takes 1/deg as r1 = fma(r0, fma(-deg, r0, 1), r0) from the
+    hardware reciprocal r0 (within 1 ulp of 1/deg).  For every integer deg in
+    [1, 65535] and every float within 1 ulp of 1/deg (and the float just below
+    a power of two), r1 is 1/deg rounded to the nearest float32: the features
+    equal those of a correctly rounded division."""
+    def eq(a, b):  # (mantissa, exponent) pairs as values
+        (ma, ea), (mb, eb) = a, b
+        e = min(ea, eb)
+        return ma << (ea - e) == mb << (eb - e)
+
+    for d in range(1, 65536):
+        k = 23 + d.bit_length()
+        q, r = divmod(1 << k, d)  # 2**k / d in (2**23, 2**24]
+        if 2 * r > d or (2 * r == d and q & 1):
+            q += 1
+        if q == 1 << 24:
+            q, k = 1 << 23, k - 1
+        rn = (q, -k)
+        cands = [(q - 1, -k), (q, -k), (q + 1, -k)]
+        if q == 1 << 23:
+            cands.append(((1 << 24) - 1, -k - 1))
+        for m0, e0 in cands:
+            # e = RN(1 - d r0): 1 - d m0 2**e0 = ((1 << -e0) - d m0) 2**e0, exact
+            me, ee = _rn32((1 << -e0) - d * m0, e0)
+            # r1 = RN(r0 + r0 e) = m0 (2**-ee + me) 2**(e0 + ee), with ee < 0 (or e = 0)
+            assert me == 0 or ee < 0
+            r1 = (m0, e0) if me == 0 else _rn32(m0 * ((1 << -ee) + me), e0 + ee)
+            assert eq(r1, rn), (d, m0, e0)
+
+
 # --- GPU: training kernels ------------------------------------------------------------------
 
 
