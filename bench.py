@@ -578,6 +578,8 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-latency runs")
     ap.add_argument("--no-batched", action="store_true", help="skip the three-code launch")
     ap.add_argument("--no-bb72", action="store_true", help="skip the [[72,12,6]] code-capacity runs")
+    ap.add_argument("--no-fewer-legs", action="store_true",
+                    help="skip the network with one learned leg fewer (its launches share the learned kernel)")
     ap.add_argument("--ler-batches", type=int, default=64,
                     help="4096-shot batches of the paired error-rate comparison")
     ap.add_argument("--serving-seconds", type=float, default=5.0,
@@ -701,7 +703,7 @@ def main() -> int:
                              learned_legs=legs, learned_last_damp=damp, **RELAY)
         ler_w = paired_ler({"plain": dec, "learned": dl}, H, p, args.ler_batches, 4096, 101)
         # the same network with one learned leg fewer (one more uniform exploration leg), on the same shots
-        fewer = legs - 1 if legs > 1 else 0
+        fewer = legs - 1 if legs > 1 and not args.no_fewer_legs else 0
         decs3 = {"plain": d3, "learned": dl3}
         if fewer:
             decs3["learned_fewer_legs"] = RelayBPDecoder(H, p3, data_type="float32", algo="relay", seed=0, device=[0],
