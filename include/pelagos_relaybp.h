@@ -119,7 +119,8 @@ typedef struct {
                                 bit2: sign words in LDS; a clear bit places it in workspace */
     int32_t lds_bytes;       /* dynamic LDS per workgroup */
     int64_t spill_stride;    /* workspace bytes per workgroup: state not in LDS, or (placements 7 and 15)
-                                the copy of the running shot's first check pass */
+                                the copy of the running shot's first check pass; with learned
+                                strengths also a shifted leg's strengths (4 V bytes) */
     int64_t workspace_bytes; /* total workspace the caller must provide */
     int32_t requeue_below;   /* shots whose first leg converged in fewer iterations are
                                 re-queued behind all first legs (their posterior is parked in
