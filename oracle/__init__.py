@@ -65,6 +65,7 @@ class _Batch(ctypes.Structure):
         ("iterations", _i32p),
         ("found_count", _i32p),
         ("last_posterior", ctypes.c_void_p),
+        ("features0", ctypes.c_void_p),
     ]
 
 
@@ -153,6 +154,7 @@ def decode(
     init_gamma: Optional[np.ndarray] = None,
     init_posterior: Optional[np.ndarray] = None,
     want_posterior: bool = False,
+    want_features0: bool = False,
     nthreads: int = 0,
     lib_path: Optional[os.PathLike] = None,
     dtype: str = "float32",
@@ -165,6 +167,10 @@ def decode(
     data_type ("float32", "float64" or "float16"): the element type of the
     priors, strengths, weights and posteriors, which are converted to it first
     (as the reference's decoder casts them, decoder.py:724-734, 3684-3700).
+    ``want_features0`` (float32 only) also returns ``features0 [B,V,8]``: the
+    learned network's features at the start of relay leg 0 for the shots that
+    reach it (zeros for the others), as the decode kernel's feature pass
+    computes them.
     """
     lib = load(lib_path if lib_path is not None else LIB_PATHS[dtype])
     ft = NP_DTYPES[dtype]
@@ -192,6 +198,9 @@ def decode(
     iters = np.zeros(B, np.int32)
     found = np.zeros(B, np.int32)
     post = np.zeros((B, V), ft) if want_posterior else None
+    if want_features0 and dtype != "float32":
+        raise ValueError("want_features0 needs the float32 oracle (the learned form is float32 only)")
+    feats = np.zeros((B, V, 8), np.float32) if want_features0 else None
 
     prm = _Params(
         pre_iter, num_sets, set_max_iter, stop_nconv, int(bool(plain)), float(gamma0),
@@ -204,7 +213,7 @@ def decode(
     bat = _Batch(
         B, _ptr(syn, _u8p), vp(lam), int(prior_batched), vp(g),
         int(bool(gamma_per_row)), vp(ig), vp(ip), _ptr(errors, _u8p),
-        vp(weights), _ptr(iters, _i32p), _ptr(found, _i32p), vp(post),
+        vp(weights), _ptr(iters, _i32p), _ptr(found, _i32p), vp(post), vp(feats),
     )
     rc = lib.orc_decode(C, V, E, _ptr(row_ptr, _i32p), _ptr(row_var, _i32p),
                         ctypes.byref(prm), ctypes.byref(bat), int(nthreads))
@@ -219,6 +228,8 @@ def decode(
     }
     if post is not None:
         out["last_posterior"] = post
+    if feats is not None:
+        out["features0"] = feats
     return out
 
 

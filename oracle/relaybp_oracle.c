@@ -37,7 +37,12 @@
  *
  * Parity status: pinned by tests/test_oracle_pins.py (closed forms, tree-code
  * exactness, brute-force ML, schedule invariants, relay-leg schedule closed
- * forms, planted-bug mutants 1-10).
+ * forms, planted-bug mutants 1-10).  The learned network's leg-0 features
+ * (leg0_features) are pinned by tests/test_learned_gamma.py against the decode
+ * state (closed forms, the pre-iter posterior, violated-check fractions, the
+ * mean incoming message with a plain pre-iter leg); feature 4 (the smallest
+ * incoming message after one more check pass) only by agreement with the GPU
+ * kernel's gather: parity unpinned on the CPU alone.
  *
  * Element type (ORC_ELEM, default 32): the same restatement for data_type
  * "float64" (64: every stored and computed value a double) and "float16" (16:
@@ -168,6 +173,8 @@ typedef struct {
     int32_t *iterations;          /* [B]   out */
     int32_t *found_count;         /* [B]   out */
     elem_t *last_posterior;       /* [B,V] out or NULL */
+    float *features0;             /* [B,V,8] out or NULL (float32 build): the learned network's
+                                     features at the start of relay leg 0, for shots that reach it */
 } orc_batch;
 
 typedef struct {
@@ -204,6 +211,10 @@ typedef struct {
     uint8_t *first_hd;       /* [V] first_leg_hard_dec */
     const elem_t *gamma;     /* [V] current leg's memory strengths (or NULL with gamma_const) */
     elem_t gamma_const;
+    uint8_t *flips;          /* [V] hard-decision flips during the leg (counted when count_flips) */
+    int32_t count_flips;
+    int32_t leg_t;           /* the in-leg iteration at which the last leg ended */
+    elem_t *nu2;             /* [E] scratch: the messages of the check pass after a converged leg */
 } orc_lane;
 
 /* C2V, kernels.py:1128-1215: per check, the parity of negative incoming
@@ -262,6 +273,7 @@ static void variable_update(const orc_graph *g, const elem_t *lam0, orc_lane *L)
         real_t s = 0;
         for (int32_t q = g->var_ptr[j]; q < g->var_ptr[j + 1]; ++q) s += LD(L->nu[g->var_edges[q]]);
         real_t post = LD(lm) + s;
+        if (L->count_flips && ((LD(L->post[j]) <= 0) != (post <= 0)) && L->flips[j] < 255) L->flips[j]++;
         L->post[j] = ST(post);
         L->hard[j] = post <= 0;
         for (int32_t q = g->var_ptr[j]; q < g->var_ptr[j + 1]; ++q) {
@@ -309,6 +321,7 @@ static int run_leg(const orc_graph *g, const orc_params *p, const uint8_t *syn,
         check_update(g, p, syn, alpha, L);
         variable_update(g, lam0, L);
         st->iters++;
+        L->leg_t = t;
         if (all_checks_ok(g, syn, L->hard)) {
             /* check_and_select_kernel: weight over bits (post <= 0) with wj. */
             double acc = 0.0;
@@ -334,6 +347,63 @@ static int run_leg(const orc_graph *g, const orc_params *p, const uint8_t *syn,
     return 0;
 }
 
+#if ORC_ELEM == 32
+static float clamp4(float x) { return fminf(fmaxf(x, -4.0f), 4.0f); }
+
+/* The learned network's features of every variable at the start of relay leg
+ * 0, from the state the pre-iter leg ended in (no counterpart in the
+ * reference; the decode kernel's feature pass, leg_features_t in
+ * pelagos_amd/csrc/relaybp.hip, computes them from its compressed state):
+ *   0 prior LLR / 8     1 posterior / 16     2 flips in the pre-iter leg / 8
+ *   3 S / deg / 8 with S the sum of the incoming messages of the last variable update
+ *   4 the smallest incoming message / 8 (0 for a variable in no check)
+ *   5 deg / 8           6 fraction of the variable's checks violated     7 leg / 4 = 0
+ * (1, 3, 4 clamped to [-4, 4]).  The incoming messages of feature 4 are the
+ * ones a next variable update would read: after a leg that converged before
+ * its last iteration, those of one more check pass (its alpha, from the final
+ * mu); otherwise the last check pass's. */
+static void leg0_features(const orc_graph *g, const orc_params *p, const uint8_t *syn, const elem_t *lam0,
+                          int converged, orc_lane *L, float *f)
+{
+    const elem_t *nu4 = L->nu;
+    if (converged && L->leg_t + 1 < p->pre_iter) {
+        elem_t *keep = L->nu;
+        L->nu = L->nu2;
+        check_update(g, p, syn, p->use_alpha ? p->alpha_t[L->leg_t + 1] : 1.0f, L);
+        L->nu = keep;
+        nu4 = L->nu2;
+    }
+    for (int32_t j = 0; j < g->V; ++j) {
+        const int32_t q0 = g->var_ptr[j], q1 = g->var_ptr[j + 1], deg = q1 - q0;
+        float sum = 0.0f, mn = INFINITY;
+        int32_t uns = 0;
+        for (int32_t q = q0; q < q1; ++q) {
+            const int32_t e = g->var_edges[q];
+            sum += L->nu[e];
+            mn = fminf(mn, nu4[e]);
+            if (!converged) {
+                /* the edge's check: e lies in [row_ptr[i], row_ptr[i+1]) */
+                int32_t lo = 0, hi = g->C;
+                while (hi - lo > 1) { const int32_t mid = (lo + hi) / 2; if (g->row_ptr[mid] <= e) lo = mid; else hi = mid; }
+                int par = syn[lo] & 1;
+                for (int32_t r = g->row_ptr[lo]; r < g->row_ptr[lo + 1]; ++r) par ^= L->post[g->row_var[r]] <= 0;
+                uns += par;
+            }
+        }
+        const float inv = deg > 0 ? 1.0f / (float)deg : 0.0f;
+        float *o = f + 8 * (size_t)j;
+        o[0] = lam0[j] * 0.125f;
+        o[1] = clamp4(L->post[j] * 0.0625f);
+        o[2] = (float)L->flips[j] * 0.125f;
+        o[3] = clamp4(sum * inv * 0.125f);
+        o[4] = deg > 0 ? clamp4(mn * 0.125f) : 0.0f;
+        o[5] = (float)deg * 0.125f;
+        o[6] = converged ? 0.0f : (float)uns * inv;
+        o[7] = 0.0f;
+    }
+}
+#endif
+
 static void decode_shot(const orc_graph *g, const orc_params *p, const orc_batch *bt,
                         int32_t b, orc_lane *L, elem_t *wj_buf)
 {
@@ -357,7 +427,17 @@ static void decode_shot(const orc_graph *g, const orc_params *p, const orc_batch
     if (p->plain) { L->gamma = NULL; L->gamma_const = ELEM_ZERO; }
     else if (bt->init_gamma) { L->gamma = bt->init_gamma + (size_t)b * V; }
     else { L->gamma = NULL; L->gamma_const = ST(p->gamma0); }
-    if (st.found < S) run_leg(g, p, syn, lam0, wj_buf, p->pre_iter, L, &st);
+    memset(L->flips, 0, (size_t)V);
+    L->count_flips = bt->features0 != NULL;
+    int pre_ok = 0;
+    if (st.found < S) pre_ok = run_leg(g, p, syn, lam0, wj_buf, p->pre_iter, L, &st);
+    L->count_flips = 0;
+#if ORC_ELEM == 32
+    if (bt->features0 && !p->skip_relay_legs && p->num_sets > 0 && st.found < S)
+        leg0_features(g, p, syn, lam0, pre_ok, L, bt->features0 + (size_t)b * V * 8);
+#else
+    (void)pre_ok;
+#endif
     memcpy(L->first_hd, L->hard, (size_t)V);            /* decoder.py:3597-3598 */
 
     if (!p->skip_relay_legs) {
@@ -414,16 +494,21 @@ int orc_decode(int32_t C, int32_t V, int32_t E, const int32_t *row_ptr, const in
         L.hard = (uint8_t *)malloc((size_t)V);
         L.best = (uint8_t *)malloc((size_t)V);
         L.first_hd = (uint8_t *)malloc((size_t)V);
+        L.flips = (uint8_t *)malloc((size_t)V);
+        L.nu2 = (elem_t *)malloc(sizeof(elem_t) * e1);
+        L.count_flips = 0;
+        L.leg_t = 0;
         elem_t *wj = (elem_t *)malloc(sizeof(elem_t) * (size_t)V);
         L.gamma = NULL;
         L.gamma_const = ELEM_ZERO;
-        if (!L.mu || !L.nu || !L.post || !L.hard || !L.best || !L.first_hd || !wj) {
+        if (!L.mu || !L.nu || !L.post || !L.hard || !L.best || !L.first_hd || !L.flips || !L.nu2 || !wj) {
             rc |= 1;
         } else {
 #pragma omp for schedule(dynamic, 1)
             for (int32_t b = 0; b < bt->B; ++b) decode_shot(&g, p, bt, b, &L, wj);
         }
-        free(L.mu); free(L.nu); free(L.post); free(L.hard); free(L.best); free(L.first_hd); free(wj);
+        free(L.mu); free(L.nu); free(L.post); free(L.hard); free(L.best); free(L.first_hd); free(L.flips); free(L.nu2);
+        free(wj);
     }
     free(g.var_ptr);
     free(g.var_edges);

@@ -154,6 +154,55 @@ def test_reciprocal_with_one_newton_step_is_correctly_rounded():
             assert eq(r1, rn), (d, m0, e0)
 
 
+def test_oracle_leg0_features_against_the_decode_state():
+    """The oracle's leg-0 features against what its pinned decode fixes: prior
+    and degree closed forms, the posterior the pre-iter leg ends with, no flips
+    beyond its iterations, violated-check fractions recomputed from that
+    posterior's hard decisions, and no features for shots that stop before
+    relay leg 0."""
+    H, p = codes.banded_circuit_code(n_blocks=4, V=2700, seed=2)
+    err = codes.sample_errors(p, 48, seed=12, scale=2.0)
+    syn = codes.syndromes_of(H, err)
+    Hc = H.tocsr()
+    V = H.shape[1]
+    lam = np.log((1 - p) / p).astype(np.float32)
+    table = np.random.default_rng(0).uniform(-0.24, 0.66, (12, V)).astype(np.float32)
+    kw = dict(pre_iter=30, num_sets=12, set_max_iter=20, stop_nconv=4, gamma0=0.65)
+    f = oracle.decode(Hc.indptr, Hc.indices, V, syn, lam, table, want_features0=True, **kw)["features0"]
+    pre = oracle.decode(Hc.indptr, Hc.indices, V, syn, lam, table, skip_relay_legs=True, want_posterior=True, **kw)
+    r = pre["found_count"] < 4
+    conv = pre["found_count"] >= 1
+    assert r.sum() > 10 and (r & ~conv).sum() > 0 and (r & conv).sum() > 0
+    assert not f[~r].any()
+    deg = np.diff(Hc.tocsc().indptr).astype(np.float32)
+    np.testing.assert_array_equal(f[r][..., 0], np.broadcast_to(lam * np.float32(0.125), (r.sum(), V)))
+    np.testing.assert_array_equal(f[r][..., 5], np.broadcast_to(deg * np.float32(0.125), (r.sum(), V)))
+    assert not f[r][..., 7].any()
+    post = pre["last_posterior"]
+    np.testing.assert_array_equal(f[r][..., 1], np.clip(post[r] * np.float32(0.0625), -4, 4))
+    flips = f[..., 2] * 8
+    assert np.array_equal(flips, np.round(flips)) and (flips <= pre["iterations"][:, None]).all() and flips.max() > 0
+    # violated checks from the posterior's hard decisions (checks x variables incidence)
+    hard = (post <= 0).astype(np.int64)
+    viol = ((hard @ Hc.T.toarray().astype(np.int64)) + syn) % 2         # [B, C]
+    frac = (viol @ Hc.toarray().astype(np.int64)) / np.maximum(deg, 1)   # [B, V]
+    np.testing.assert_allclose(f[r & ~conv][..., 6], frac[r & ~conv], rtol=1e-6)
+    assert not f[r & conv][..., 6].any() and f[r & ~conv][..., 6].max() > 0
+    # the smallest incoming message never exceeds the clamp
+    assert (np.abs(f[r][..., 4]) <= 4).all()
+    # with a plain pre-iter leg (strength 0) the memory term is the prior itself, so the incoming
+    # sum of the last variable update is post - prior (up to the rounding of post)
+    kw0 = dict(kw, plain=True)
+    f0 = oracle.decode(Hc.indptr, Hc.indices, V, syn, lam, table, want_features0=True, **kw0)["features0"]
+    pre0 = oracle.decode(Hc.indptr, Hc.indices, V, syn, lam, table, skip_relay_legs=True, want_posterior=True, **kw0)
+    r0 = pre0["found_count"] < 4
+    post0 = pre0["last_posterior"].astype(np.float64)
+    s_exp = np.clip((post0 - lam[None, :]) / deg / 8, -4, 4)
+    tol = 1e-6 * np.maximum(np.abs(lam)[None, :], np.abs(post0)) / deg + 1e-7
+    assert r0.sum() > 10
+    assert (np.abs(f0[..., 3] - s_exp) <= tol)[r0].all()
+
+
 # --- GPU: training kernels ------------------------------------------------------------------
 
 
@@ -362,6 +411,33 @@ def test_gpu_traces_record_the_state_at_each_leg():
     # a shot that needed no relay leg records nothing
     done = out["found_count"].cpu().numpy() >= 4
     assert ((hard[:, 0, 0] == 255) <= done).all()
+
+
+@pytest.mark.gpu
+def test_gpu_leg0_features_match_the_oracle():
+    """The feature pass at the start of relay leg 0 (gathered from the kernel's
+    compressed check state) equals the oracle's restatement from its explicit
+    per-edge messages, bit for bit, for every shot that reaches the leg: those
+    whose pre-iter leg converged (feature 4 from one more check pass, feature 6
+    zero) and those whose did not (violated-check fractions)."""
+    H, p, err, syn = _banded(64, seed=12)
+    dec = _decoder(H, p)
+    print("[gpu] leg-0 features against the oracle", flush=True)
+    tr = dec.collect_traces(torch.from_numpy(syn).cuda(), legs=1)
+    torch.cuda.synchronize()
+    feats = tr["features"].cpu().numpy()[:, 0]
+    reached = tr["hard"].cpu().numpy()[:, 0, 0] != 255
+    B, V = syn.shape[0], H.shape[1]
+    lam = dec.log_prior_ratios_per_gpu[0].reshape(-1).cpu().numpy()
+    rows = dec.gamma_table(B).cpu().numpy()
+    ref = oracle.decode(dec.graph.chk_ptr, dec.graph.edge_var, V, syn, lam, rows, pre_iter=30, num_sets=12,
+                        set_max_iter=20, stop_nconv=4, gamma0=0.65, want_features0=True)
+    pre = oracle.decode(dec.graph.chk_ptr, dec.graph.edge_var, V, syn, lam, rows, pre_iter=30, num_sets=12,
+                        set_max_iter=20, stop_nconv=4, gamma0=0.65, skip_relay_legs=True)
+    np.testing.assert_array_equal(reached, pre["found_count"] < 4)
+    conv = pre["found_count"] >= 1
+    assert (reached & conv).sum() > 10 and (reached & ~conv).sum() > 0
+    np.testing.assert_array_equal(feats[reached], ref["features0"][reached])
 
 
 @pytest.mark.gpu
