@@ -899,7 +899,9 @@ __device__ __forceinline__ float inv_degree(int deg) {
 // explore differently; a network that outputs z = 0 reproduces the uniform
 // legs exactly.  known_valid: the state satisfies every check (the previous
 // leg converged), so feature 6 is 0 without a parity pass.  apply: evaluate
-// the network and shift the leg (else the leg keeps its uniform row).  The flip counts
+// the network and shift the leg (else the leg keeps its uniform row).  fresh: no
+// check pass has run in this shot (pre_iter 0), so the check state holds no
+// messages of it and feature 4 is 0 (every message is still 0).  The flip counts
 // are cleared for the next leg.  Optionally records features and hard
 // decisions in the caller's order.
 // Slices per wave whose features are gathered before their network evaluation.
@@ -983,7 +985,7 @@ struct VarFeatures {
 
 template <int BLOCK, int PLACE, bool KNOWN_VALID>
 __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const ShotMem<PLACE> &m, int r, const uint8_t *__restrict__ syn,
-                               bool apply) {
+                               bool apply, bool fresh) {
     constexpr bool known_valid = KNOWN_VALID;
     const int C = a.g.C, V = a.g.V, L = a.learned_legs;
     if (!known_valid) {
@@ -1003,7 +1005,10 @@ __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const Shot
     for (int vb = gtid<BLOCK>() & ~63; vb < V; vb += kFeatBatch * BLOCK) {
         VarFeatures<PLACE, KNOWN_VALID> vf[kFeatBatch];
 #pragma unroll
-        for (int j = 0; j < kFeatBatch; ++j) vf[j].start(a, m, min(vb + j * BLOCK + lane, V - 1));
+        for (int j = 0; j < kFeatBatch; ++j) {
+            vf[j].start(a, m, min(vb + j * BLOCK + lane, V - 1));
+            if (fresh) vf[j].mn = 0.f;  // no check pass has run: every message is still 0
+        }
         uint32_t pk[kFeatBatch][4];
 #pragma unroll
         for (int j = 0; j < kFeatBatch; ++j) {
@@ -1060,9 +1065,9 @@ __device__ __forceinline__ void leg_features_t(const KArgs &a, int b, const Shot
 
 template <int BLOCK, int PLACE>
 __device__ __forceinline__ void leg_features(const KArgs &a, int b, const ShotMem<PLACE> &m, int r, const uint8_t *__restrict__ syn,
-                             bool known_valid, bool apply) {
-    if (known_valid) leg_features_t<BLOCK, PLACE, true>(a, b, m, r, syn, apply);
-    else leg_features_t<BLOCK, PLACE, false>(a, b, m, r, syn, apply);
+                             bool known_valid, bool apply, bool fresh = false) {
+    if (known_valid) leg_features_t<BLOCK, PLACE, true>(a, b, m, r, syn, apply, false);
+    else leg_features_t<BLOCK, PLACE, false>(a, b, m, r, syn, apply, fresh);
 }
 
 // The strengths a learned leg uses (shifted or its uniform row), in the
@@ -1387,9 +1392,9 @@ __device__ __forceinline__ void relay_legs(const KArgs &a, int b, ShotMem<PLACE>
                 if (leg == 0) {
                     st.shifted = leg0_ready ? st.shifted : (a.mlp && st.last_ok);
                     if (!leg0_ready && (st.shifted || trace))
-                        leg_features<BLOCK, PLACE>(a, b, m, 0, io.syn, st.last_ok != 0, st.shifted != 0);
+                        leg_features<BLOCK, PLACE>(a, b, m, 0, io.syn, st.last_ok != 0, st.shifted != 0, st.any_iter == 0);
                 } else if (trace) {
-                    leg_features<BLOCK, PLACE>(a, b, m, leg, io.syn, st.last_ok != 0, false);
+                    leg_features<BLOCK, PLACE>(a, b, m, leg, io.syn, st.last_ok != 0, false, st.any_iter == 0);
                 }
                 if (st.shifted && st.last_ok) {
                     gl.lds = a.gamma_lds + m.lb;

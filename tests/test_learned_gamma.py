@@ -414,15 +414,21 @@ def test_gpu_traces_record_the_state_at_each_leg():
 
 
 @pytest.mark.gpu
-def test_gpu_leg0_features_match_the_oracle():
+@pytest.mark.parametrize("pre_iter", [30, 0])
+def test_gpu_leg0_features_match_the_oracle(pre_iter):
     """The feature pass at the start of relay leg 0 (gathered from the kernel's
     compressed check state) equals the oracle's restatement from its explicit
     per-edge messages, bit for bit, for every shot that reaches the leg: those
     whose pre-iter leg converged (feature 4 from one more check pass, feature 6
-    zero) and those whose did not (violated-check fractions)."""
+    zero), those whose did not (violated-check fractions), and with no pre-iter
+    leg at all (every message still 0)."""
+    from pelagos_amd import RelayBPDecoder
+
     H, p, err, syn = _banded(64, seed=12)
-    dec = _decoder(H, p)
-    print("[gpu] leg-0 features against the oracle", flush=True)
+    dec = RelayBPDecoder(H, p, pre_iter=pre_iter, num_sets=12, set_max_iter=20, gamma0=0.65,
+                         gamma_dist_interval=(-0.24, 0.66), stop_nconv=4, data_type="float32", algo="relay",
+                         seed=3, device=[0])
+    print(f"[gpu] leg-0 features against the oracle, pre_iter={pre_iter}", flush=True)
     tr = dec.collect_traces(torch.from_numpy(syn).cuda(), legs=1)
     torch.cuda.synchronize()
     feats = tr["features"].cpu().numpy()[:, 0]
@@ -430,13 +436,15 @@ def test_gpu_leg0_features_match_the_oracle():
     B, V = syn.shape[0], H.shape[1]
     lam = dec.log_prior_ratios_per_gpu[0].reshape(-1).cpu().numpy()
     rows = dec.gamma_table(B).cpu().numpy()
-    ref = oracle.decode(dec.graph.chk_ptr, dec.graph.edge_var, V, syn, lam, rows, pre_iter=30, num_sets=12,
-                        set_max_iter=20, stop_nconv=4, gamma0=0.65, want_features0=True)
-    pre = oracle.decode(dec.graph.chk_ptr, dec.graph.edge_var, V, syn, lam, rows, pre_iter=30, num_sets=12,
-                        set_max_iter=20, stop_nconv=4, gamma0=0.65, skip_relay_legs=True)
+    kw = dict(pre_iter=pre_iter, num_sets=12, set_max_iter=20, stop_nconv=4, gamma0=0.65)
+    ref = oracle.decode(dec.graph.chk_ptr, dec.graph.edge_var, V, syn, lam, rows, want_features0=True, **kw)
+    pre = oracle.decode(dec.graph.chk_ptr, dec.graph.edge_var, V, syn, lam, rows, skip_relay_legs=True, **kw)
     np.testing.assert_array_equal(reached, pre["found_count"] < 4)
     conv = pre["found_count"] >= 1
-    assert (reached & conv).sum() > 10 and (reached & ~conv).sum() > 0
+    if pre_iter:
+        assert (reached & conv).sum() > 10 and (reached & ~conv).sum() > 0
+    else:
+        assert reached.all() and not feats[..., 4].any()
     np.testing.assert_array_equal(feats[reached], ref["features0"][reached])
 
 
