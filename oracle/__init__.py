@@ -66,6 +66,8 @@ class _Batch(ctypes.Structure):
         ("found_count", _i32p),
         ("last_posterior", ctypes.c_void_p),
         ("features0", ctypes.c_void_p),
+        ("mu0", ctypes.c_void_p),
+        ("nu0", ctypes.c_void_p),
     ]
 
 
@@ -155,6 +157,7 @@ def decode(
     init_posterior: Optional[np.ndarray] = None,
     want_posterior: bool = False,
     want_features0: bool = False,
+    want_messages0: bool = False,
     nthreads: int = 0,
     lib_path: Optional[os.PathLike] = None,
     dtype: str = "float32",
@@ -170,7 +173,9 @@ def decode(
     ``want_features0`` (float32 only) also returns ``features0 [B,V,8]``: the
     learned network's features at the start of relay leg 0 for the shots that
     reach it (zeros for the others), as the decode kernel's feature pass
-    computes them.
+    computes them.  ``want_messages0`` returns ``mu0`` and ``nu0 [B,E]``: the
+    variable-to-check and check-to-variable messages when the pre-iter leg ends
+    (edge id = CSR position).
     """
     lib = load(lib_path if lib_path is not None else LIB_PATHS[dtype])
     ft = NP_DTYPES[dtype]
@@ -201,6 +206,8 @@ def decode(
     if want_features0 and dtype != "float32":
         raise ValueError("want_features0 needs the float32 oracle (the learned form is float32 only)")
     feats = np.zeros((B, V, 8), np.float32) if want_features0 else None
+    mu0 = np.zeros((B, max(E, 1)), ft) if want_messages0 else None
+    nu0 = np.zeros((B, max(E, 1)), ft) if want_messages0 else None
 
     prm = _Params(
         pre_iter, num_sets, set_max_iter, stop_nconv, int(bool(plain)), float(gamma0),
@@ -213,7 +220,7 @@ def decode(
     bat = _Batch(
         B, _ptr(syn, _u8p), vp(lam), int(prior_batched), vp(g),
         int(bool(gamma_per_row)), vp(ig), vp(ip), _ptr(errors, _u8p),
-        vp(weights), _ptr(iters, _i32p), _ptr(found, _i32p), vp(post), vp(feats),
+        vp(weights), _ptr(iters, _i32p), _ptr(found, _i32p), vp(post), vp(feats), vp(mu0), vp(nu0),
     )
     rc = lib.orc_decode(C, V, E, _ptr(row_ptr, _i32p), _ptr(row_var, _i32p),
                         ctypes.byref(prm), ctypes.byref(bat), int(nthreads))
@@ -230,6 +237,8 @@ def decode(
         out["last_posterior"] = post
     if feats is not None:
         out["features0"] = feats
+    if mu0 is not None:
+        out["mu0"], out["nu0"] = mu0[:, :E], nu0[:, :E]
     return out
 
 

@@ -40,9 +40,8 @@
  * forms, planted-bug mutants 1-10).  The learned network's leg-0 features
  * (leg0_features) are pinned by tests/test_learned_gamma.py against the decode
  * state (closed forms, the pre-iter posterior, violated-check fractions, the
- * mean incoming message with a plain pre-iter leg); feature 4 (the smallest
- * incoming message after one more check pass) only by agreement with the GPU
- * kernel's gather: parity unpinned on the CPU alone.
+ * mean incoming message with a plain pre-iter leg) and against the pre-iter
+ * leg's final messages (mu0, nu0) through the single-check update.
  *
  * Element type (ORC_ELEM, default 32): the same restatement for data_type
  * "float64" (64: every stored and computed value a double) and "float16" (16:
@@ -175,6 +174,7 @@ typedef struct {
     elem_t *last_posterior;       /* [B,V] out or NULL */
     float *features0;             /* [B,V,8] out or NULL (float32 build): the learned network's
                                      features at the start of relay leg 0, for shots that reach it */
+    elem_t *mu0, *nu0;            /* [B,E] out or NULL: the messages when the pre-iter leg ends */
 } orc_batch;
 
 typedef struct {
@@ -432,6 +432,8 @@ static void decode_shot(const orc_graph *g, const orc_params *p, const orc_batch
     int pre_ok = 0;
     if (st.found < S) pre_ok = run_leg(g, p, syn, lam0, wj_buf, p->pre_iter, L, &st);
     L->count_flips = 0;
+    if (bt->mu0) memcpy(bt->mu0 + (size_t)b * g->E, L->mu, sizeof(elem_t) * (size_t)g->E);
+    if (bt->nu0) memcpy(bt->nu0 + (size_t)b * g->E, L->nu, sizeof(elem_t) * (size_t)g->E);
 #if ORC_ELEM == 32
     if (bt->features0 && !p->skip_relay_legs && p->num_sets > 0 && st.found < S)
         leg0_features(g, p, syn, lam0, pre_ok, L, bt->features0 + (size_t)b * V * 8);

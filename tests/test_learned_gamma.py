@@ -190,6 +190,32 @@ def test_oracle_leg0_features_against_the_decode_state():
     assert not f[r & conv][..., 6].any() and f[r & ~conv][..., 6].max() > 0
     # the smallest incoming message never exceeds the clamp
     assert (np.abs(f[r][..., 4]) <= 4).all()
+    # features 3 and 4 from the messages the pre-iter leg ends with: the sum of the last variable update's
+    # incoming messages in ascending edge order, and the smallest message a next variable update would read
+    # (after a leg that converged before its last iteration, one more check pass over the final mu, by the
+    # separately pinned single-check update; otherwise the last check pass's)
+    msg = oracle.decode(Hc.indptr, Hc.indices, V, syn, lam, table, want_messages0=True, **kw)
+    order = np.argsort(Hc.indices, kind="stable")               # edges by variable, ascending edge id
+    starts = np.concatenate([[0], np.cumsum(deg.astype(np.int64))])
+    inv = np.where(deg > 0, np.float32(1) / np.maximum(deg, 1), np.float32(0)).astype(np.float32)
+    C = Hc.shape[0]
+    early = conv & (pre["iterations"] < kw["pre_iter"])
+    assert (r & early).sum() > 5
+    for b in np.flatnonzero(r)[:16]:
+        nu_last = msg["nu0"][b]
+        nu_next = nu_last
+        if early[b]:
+            nu_next = np.concatenate([oracle.check_update_row(msg["mu0"][b, Hc.indptr[i]:Hc.indptr[i + 1]], syn[b, i])
+                                      for i in range(C)])
+        s_sum = np.zeros(V, np.float32)
+        mn = np.full(V, np.inf, np.float32)
+        for k in range(int(deg.max())):
+            has = deg > k
+            e = order[np.minimum(starts[:-1] + k, len(order) - 1)]
+            s_sum = np.where(has, s_sum + np.where(has, nu_last[e], 0), s_sum).astype(np.float32)
+            mn = np.where(has, np.minimum(mn, nu_next[e]), mn).astype(np.float32)
+        np.testing.assert_array_equal(f[b, :, 3], np.clip(s_sum * inv * np.float32(0.125), -4, 4))
+        np.testing.assert_array_equal(f[b, :, 4], np.where(deg > 0, np.clip(mn * np.float32(0.125), -4, 4), 0))
     # with a plain pre-iter leg (strength 0) the memory term is the prior itself, so the incoming
     # sum of the last variable update is post - prior (up to the rounding of post)
     kw0 = dict(kw, plain=True)
